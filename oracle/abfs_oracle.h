@@ -1,0 +1,89 @@
+/*
+ * abfs_oracle.h -- CPU FP64 restatement of pDynamo's NBModelABFS MM/MM path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * load it.  The product library never links or calls it.
+ *
+ * Every function restates one piece of the reference algorithm and cites the
+ * reference file:line it follows (paths relative to the pDynamo-1.9.0 tree,
+ * pCx = pCore-1.9.0/extensions, pMx = pMolecule-1.9.0/extensions).
+ */
+#ifndef ABFS_ORACLE_H
+#define ABFS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Index of each switching factor inside the 21-double factor block.
+ * Mirrors the field set of PairwiseInteractionABFSFactors
+ * (pMx/cinclude/PairwiseInteraction.h:40-66). */
+enum {
+    ORC_R2DAMP = 0, ORC_R2OFF, ORC_R2ON,
+    ORC_A, ORC_B, ORC_C, ORC_D, ORC_QSHIFT1, ORC_QSHIFT2, ORC_QF0, ORC_QALPHA,
+    ORC_AF6, ORC_AK12, ORC_ASHIFT12, ORC_AF0, ORC_AALPHA,
+    ORC_BF3, ORC_BK6, ORC_BSHIFT6, ORC_BF0, ORC_BALPHA,
+    ORC_NFACTORS
+};
+
+/* Coulomb factor in kJ mol^-1 A e^-2 (pCx/cinclude/Units.h:18-30,48-49). */
+double orc_coulomb_factor(void);
+
+/* Switching factors (pMx/csource/PairwiseInteraction.c:89-141). */
+void orc_make_factors(double damp, double inner, double outer, double *f);
+
+/* One pair: energies and dF = dE/d(r^2) for r2 <= r2Off
+ * (pMx/cinclude/PairwiseInteraction.h:72-119). Returns 0 if r2 > r2Off. */
+int orc_pair_terms(const double *f, double r2, double qij, double aij, double bij,
+                   double *eel, double *elj, double *df);
+
+/* Grid search decision (pCx/csource/PairListGenerator.c:498-525). */
+int orc_determine_method(int n, const double *xyz, double cutoff, double cutoffCellSizeFactor,
+                         int minimumCellExtent, int minimumPoints);
+
+/* Reference cell grid (pCx/csource/Coordinates3.c:1111-1158) and cell of each
+ * point (pCx/csource/RegularGrid.c:240-251); off-grid points get -1.
+ * bins[3], lower[3] out; cell[n] out. Returns the number of cells. */
+long orc_make_grid(int n, const double *xyz, double cellSize, int *bins, double *lower, int *cell);
+
+/* Search stencil (pCx/csource/RegularGrid.c:526-575): offsets[k*3+d], flags[k]
+ * (1 = test distance, 2 = include all).  Returns the number of offsets; cap is
+ * the capacity of offsets/flags in entries. */
+int orc_search_range(double cellSize, double cutoff, int cap, int *offsets, int *flags);
+
+/* Self pair list (pCx/csource/PairListGenerator.c:946-1150).  method: 1 grid
+ * cell/cell, 0 direct.  Exclusions as symmetric CSR sorted ascending per row.
+ * Output records in reference order: rec_i[r], rec_ptr[r..r+1] into js.
+ * Returns npairs, or -1 if capacities are too small (need_* report sizes). */
+long orc_self_pair_list(int n, const double *xyz, const int *xptr, const int *xidx,
+                        double cutoff, double cellSize, int method,
+                        long cap_rec, long cap_pairs, int *rec_i, long *rec_ptr, int *js,
+                        long *nrec_out);
+
+/* Half-skin rebuild rule (pMx/csource/NBModelABFS.c:691-746); fixed may be NULL. */
+int orc_check_for_update(int n, const double *xyz, const double *ref, const uint8_t *fixed,
+                         double listCutoff, double outerCutoff, double *maxdisp);
+
+/* Analytic MM/MM energy over a record list with OpenMP per-thread gradient
+ * copies (pMx/csource/PairwiseInteraction.c:292-429,532-535;
+ * pMx/csource/NBModelABFSState.c:19-38).  grad (n*3) is incremented and may be
+ * NULL.  nthreads <= 0 means the OpenMP default. */
+void orc_mmmm_energy(const double *f, int n, const double *xyz, const double *charge,
+                     const int *ljtype, int ntypes, const double *tableA, const double *tableB,
+                     const int *tableindex, double electrostaticScale, double ljScale,
+                     long nrec, const int *rec_i, const long *rec_ptr, const int *js,
+                     int nthreads, double *eel, double *elj, double *grad);
+
+/* LJ tables from epsilon/sigma (pMx/csource/LJParameterContainer.c:184-210).
+ * geometric: 1 = geometric sigma (OPLS), 0 = arithmetic (AMBER); four_eps: 1 =
+ * A=4 eps s^12 form (OPLS), 0 = B = eps 2 s^6 form (AMBER). */
+void orc_lj_make_table(int ntypes, const double *eps, const double *sigma, int geometric,
+                       int four_eps, double *tableA, double *tableB, int *tableindex);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,183 @@
+/*
+ * ref_driver.c -- thin C driver over the REAL pDynamo 1.9.0 C library, built
+ * from the reference sources in place (see oracle/Makefile, target `ref`).
+ *
+ * TEST INFRASTRUCTURE ONLY: it produces golden vectors (tests/golden/) and can
+ * serve as the timed CPU baseline.  It calls the reference's own entry points
+ * in the order pMolecule.NBModelABFS.pyx uses them:
+ *   NBModelABFSState_SetUp  (pyx:242)        -> state
+ *   NBModelABFSState_Initialize (pyx:268)    -> per call
+ *   NBModelABFS_Update (pyx:271)             -> rebuild decision + lists
+ *   NBModelABFS_MMMMEnergy (pyx:119)         -> energies and gradients
+ * No reference source is copied here; the reference headers are included from
+ * the reference tree at build time.
+ */
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "Coordinates3.h"
+#include "LJParameterContainer.h"
+#include "MMAtomContainer.h"
+#include "NBModelABFS.h"
+#include "NBModelABFSState.h"
+#include "PairList.h"
+#include "PairListGenerator.h"
+#include "PairwiseInteraction.h"
+
+typedef struct {
+    int n;
+    NBModelABFS *model;
+    PairListGenerator *gen;
+    PairwiseInteractionABFS *pw;
+    MMAtomContainer *atoms;
+    LJParameterContainer *lj, *lj14;
+    PairList *excl, *i14;
+    NBModelABFSState *state;
+    Coordinates3 *crd, *grd;
+} RefHandle;
+
+static PairList *make_self_list(int npairs, const int *pairs)
+{
+    if (npairs <= 0) return NULL;
+    Integer *tmp = (Integer *)malloc(sizeof(Integer) * 2 * (size_t)npairs);
+    for (int k = 0; k < 2 * npairs; k++) tmp[k] = pairs[k];
+    PairList *pl = PairList_FromIntegerPairArray(True, npairs, tmp);
+    free(tmp);
+    return pl;
+}
+
+static LJParameterContainer *make_lj(int ntypes, const double *tA, const double *tB)
+{
+    LJParameterContainer *lj = LJParameterContainer_Allocate(ntypes);
+    int size = ntypes * (ntypes + 1) / 2;
+    for (int k = 0; k < size; k++) { lj->tableA[k] = tA[k]; lj->tableB[k] = tB[k]; }
+    return lj;
+}
+
+/* opts: damping, inner, outer, list, dielectric, scale14.
+ * gen: cutoffCellSizeFactor, minimumCellExtent, minimumPoints (the ABFS
+ * default generator of pMolecule.NBModelABFS.pyx:59-71 otherwise). */
+void *ref_create(int n, const double *charges, const int *ljtypes, int ntypes,
+                 const double *tA, const double *tB, const double *tA14, const double *tB14,
+                 int nexcl, const int *excl, int n14, const int *p14,
+                 const double *opts, const double *gen, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    RefHandle *h = (RefHandle *)calloc(1, sizeof(RefHandle));
+    h->n = n;
+    h->model = NBModelABFS_Allocate();
+    h->model->dampingCutoff = opts[0];
+    h->model->innerCutoff = opts[1];
+    h->model->outerCutoff = opts[2];
+    h->model->listCutoff = opts[3];
+    h->model->dielectric = opts[4];
+    h->model->electrostaticScale14 = opts[5];
+    h->gen = PairListGenerator_Allocate();
+    h->gen->cutoff = opts[3];
+    h->gen->cutoffCellSizeFactor = gen[0];
+    h->gen->minimumCellExtent = (Integer)gen[1];
+    h->gen->minimumPoints = (Integer)gen[2];
+    h->gen->minimumCellSize = 3.0;
+    h->gen->minimumExtentFactor = 1.5;
+    h->gen->sortIndices = False;
+    h->gen->useGridByCell = True;
+    h->gen->cellSize = h->gen->cutoffCellSizeFactor * h->gen->cutoff;
+    h->pw = PairwiseInteractionABFS_Allocate();
+    h->pw->dampingCutoff = opts[0];
+    h->pw->innerCutoff = opts[1];
+    h->pw->outerCutoff = opts[2];
+    h->atoms = MMAtomContainer_Allocate(n);
+    for (int i = 0; i < n; i++) {
+        h->atoms->data[i].QACTIVE = True;
+        h->atoms->data[i].atomtype = ljtypes[i];
+        h->atoms->data[i].ljtype = ljtypes[i];
+        h->atoms->data[i].charge = charges[i];
+    }
+    h->lj = make_lj(ntypes, tA, tB);
+    h->lj14 = make_lj(ntypes, tA14 ? tA14 : tA, tB14 ? tB14 : tB);
+    h->excl = make_self_list(nexcl, excl);
+    h->i14 = make_self_list(n14, p14);
+    h->state = NBModelABFSState_SetUp(h->atoms, NULL, NULL, h->excl, h->i14, h->lj, h->lj14,
+                                      NULL, NULL, NULL, NULL, QCMMLinkAtomCoupling_MM);
+    h->crd = Coordinates3_Allocate(n);
+    h->grd = Coordinates3_Allocate(n);
+    return h;
+}
+
+/* One System.Energy-style call.  energies[4] = MM/MM Elect., LJ, 1-4 Elect.,
+ * 1-4 LJ; present[2] = main list / 1-4 list non-empty.  grads may be NULL.
+ * Returns 1 if the lists were rebuilt, -1 on a library error. */
+int ref_energy(void *vh, const double *coords, double *grads, double *energies, int *present)
+{
+    RefHandle *h = (RefHandle *)vh;
+    Status status = Status_Continue;
+    for (int i = 0; i < h->n; i++) {
+        Coordinates3_SetRow(h->crd, i, coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]);
+        if (grads) Coordinates3_SetRow(h->grd, i, 0.0, 0.0, 0.0);
+    }
+    NBModelABFSState_Initialize(h->state, h->crd, NULL, grads ? h->grd : NULL, NULL);
+    Boolean up = NBModelABFS_Update(h->model, h->gen, h->state, &status);
+    if (status != Status_Continue) return -1;
+    NBModelABFS_MMMMEnergy(h->model, h->pw, h->state);
+    energies[0] = h->state->emmel;
+    energies[1] = h->state->emmlj;
+    energies[2] = h->state->emmel14;
+    energies[3] = h->state->emmlj14;
+    present[0] = (h->state->nbmmmm != NULL);
+    present[1] = (h->state->nbmmmm14 != NULL);
+    if (grads) {
+        for (int i = 0; i < h->n; i++) {
+            double x, y, z;
+            Coordinates3_GetRow(h->grd, i, x, y, z);
+            grads[3 * i] = x; grads[3 * i + 1] = y; grads[3 * i + 2] = z;
+        }
+    }
+    return up == True ? 1 : 0;
+}
+
+static long dump_list(PairList *pl, int *oi, int *oj, long cap)
+{
+    if (pl == NULL) return 0;
+    long m = 0;
+    IndexedSelection *rec;
+    List_Iterate_Initialize(pl->pairs);
+    while ((rec = PairList_Iterate(pl)) != NULL) {
+        for (int k = 0; k < rec->nindices; k++) {
+            if (m < cap) { oi[m] = rec->index; oj[m] = rec->indices[k]; }
+            m++;
+        }
+    }
+    return m;
+}
+
+/* Main MM/MM list (which = 0) or 1-4 list (which = 1) in reference order. */
+long ref_pairs(void *vh, int which, int *oi, int *oj, long cap)
+{
+    RefHandle *h = (RefHandle *)vh;
+    return dump_list(which == 0 ? h->state->nbmmmm : h->state->nbmmmm14, oi, oj, cap);
+}
+
+int ref_use_grid(void *vh) { return ((RefHandle *)vh)->state->useGridSearch == True; }
+
+void ref_destroy(void *vh)
+{
+    RefHandle *h = (RefHandle *)vh;
+    if (!h) return;
+    NBModelABFSState_Deallocate(&h->state);
+    PairList_Deallocate(&h->excl);
+    PairList_Deallocate(&h->i14);
+    LJParameterContainer_Deallocate(&h->lj);
+    LJParameterContainer_Deallocate(&h->lj14);
+    MMAtomContainer_Deallocate(&h->atoms);
+    PairwiseInteractionABFS_Deallocate(&h->pw);
+    PairListGenerator_Deallocate(&h->gen);
+    NBModelABFS_Deallocate(&h->model);
+    Coordinates3_Deallocate(&h->crd);
+    Coordinates3_Deallocate(&h->grd);
+    free(h);
+}

@@ -1,0 +1,254 @@
+"""Seeded synthetic MM systems for the NBModelABFS hot path.
+
+Five workloads, all non-periodic, in pDynamo units (A, e, kJ/mol):
+
+* ``water_lattice``      -- rigid 3-site waters on a jittered cubic lattice
+                            (648 atoms in an 18.6 A cube; 200,001 atoms in a
+                            126 A cube).
+* ``solute_in_water``    -- 9,667 waters in a 67 A cube plus 999 random solute
+                            atoms with 8 Lorentz-Berthelot LJ types and 1-2
+                            exclusions along a random chain (30,000 atoms).
+* ``water_droplets``     -- spheres of water at 0.0334 molecules/A^3 with atom
+                            counts log-uniform in [500, 20000].
+* ``perturbed_copies``   -- independent Gaussian perturbations of one system.
+
+Water geometry: O-H 0.9572 A, H-O-H 104.52 deg; charges O -0.834, H +0.417;
+O LJ epsilon 0.1521 kcal/mol, sigma 3.1507 A; H has no LJ.  The LJ table for
+water alone uses the OPLS form (A = 4 eps s^12, B = 4 eps s^6, geometric sigma,
+pMolecule-1.9.0/extensions/csource/LJParameterContainer.c:184-210); the
+solute mixture uses Lorentz-Berthelot coefficients supplied as a table
+(LJParameterContainer_FromTableCoefficients,
+pMolecule-1.9.0/extensions/pyrex/pMolecule.LJParameterContainer.pyx:272-290).
+Tables are packed lower-triangular: index i*(i+1)/2 + j for j <= i.
+"""
+
+from dataclasses import dataclass, field
+import math
+
+import numpy as np
+
+KCAL_TO_KJ = 4.184
+O_H = 0.9572
+HOH_DEG = 104.52
+Q_O, Q_H = -0.834, 0.417
+EPS_O_KCAL, SIGMA_O = 0.1521, 3.1507
+WATER_DENSITY = 0.0334  # molecules / A^3
+
+
+@dataclass
+class MMSystem:
+    """Coordinates plus the MM parameters the ABFS model consumes."""
+    name: str
+    coordinates3: np.ndarray            # (n, 3) float64
+    charges: np.ndarray                 # (n,) float64
+    ljtypes: np.ndarray                 # (n,) int32
+    ntypes: int
+    tableA: np.ndarray                  # packed triangular, float64
+    tableB: np.ndarray
+    exclusions: np.ndarray              # (m, 2) int32
+    interactions14: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))
+
+    @property
+    def natoms(self):
+        return int(self.coordinates3.shape[0])
+
+    def with_coordinates(self, xyz, name=None):
+        return MMSystem(name or self.name, np.ascontiguousarray(xyz, dtype=np.float64), self.charges,
+                        self.ljtypes, self.ntypes, self.tableA, self.tableB, self.exclusions,
+                        self.interactions14)
+
+
+def packed_index(i, j):
+    if i < j:
+        i, j = j, i
+    return i * (i + 1) // 2 + j
+
+
+def _random_rotations(rng, m):
+    """Uniform random rotation matrices from unit quaternions."""
+    q = rng.normal(size=(m, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    w, x, y, z = q.T
+    r = np.empty((m, 3, 3))
+    r[:, 0, 0] = 1 - 2 * (y * y + z * z); r[:, 0, 1] = 2 * (x * y - z * w); r[:, 0, 2] = 2 * (x * z + y * w)
+    r[:, 1, 0] = 2 * (x * y + z * w); r[:, 1, 1] = 1 - 2 * (x * x + z * z); r[:, 1, 2] = 2 * (y * z - x * w)
+    r[:, 2, 0] = 2 * (x * z - y * w); r[:, 2, 1] = 2 * (y * z + x * w); r[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return r
+
+
+def _water_sites(rng, centres):
+    """O at each centre, the two H at random orientations (rigid geometry)."""
+    m = centres.shape[0]
+    half = math.radians(HOH_DEG) / 2.0
+    body = np.array([[0.0, 0.0, 0.0],
+                     [O_H * math.sin(half), 0.0, O_H * math.cos(half)],
+                     [-O_H * math.sin(half), 0.0, O_H * math.cos(half)]])
+    rot = _random_rotations(rng, m)
+    sites = centres[:, None, :] + np.einsum("mij,kj->mki", rot, body)
+    return sites.reshape(-1, 3)
+
+
+def _water_params(nwater):
+    n = 3 * nwater
+    q = np.tile(np.array([Q_O, Q_H, Q_H]), nwater)
+    t = np.tile(np.array([0, 1, 1], np.int32), nwater)
+    base = 3 * np.arange(nwater, dtype=np.int32)
+    excl = np.stack([np.stack([base + 1, base], 1), np.stack([base + 2, base], 1),
+                     np.stack([base + 2, base + 1], 1)], 1).reshape(-1, 2)
+    return n, q, t, excl
+
+
+def opls_table(eps_kj, sigma):
+    """OPLS-form packed table (geometric sigma, A = 4 eps s^12, B = 4 eps s^6)."""
+    nt = len(eps_kj)
+    a = np.zeros(nt * (nt + 1) // 2)
+    b = np.zeros_like(a)
+    for i in range(nt):
+        for j in range(i + 1):
+            e = math.sqrt(eps_kj[i] * eps_kj[j]) * 4.0
+            s = math.sqrt(sigma[i] * sigma[j])
+            s6 = s ** 6
+            a[packed_index(i, j)] = e * s6 * s6
+            b[packed_index(i, j)] = e * s6
+    return a, b
+
+
+def lorentz_berthelot_table(eps_kj, sigma):
+    """Packed table with arithmetic sigma and geometric epsilon, 4 eps form."""
+    nt = len(eps_kj)
+    a = np.zeros(nt * (nt + 1) // 2)
+    b = np.zeros_like(a)
+    for i in range(nt):
+        for j in range(i + 1):
+            e = 4.0 * math.sqrt(eps_kj[i] * eps_kj[j])
+            s6 = (0.5 * (sigma[i] + sigma[j])) ** 6
+            a[packed_index(i, j)] = e * s6 * s6
+            b[packed_index(i, j)] = e * s6
+    return a, b
+
+
+def _lattice_centres(rng, box, per_side, count, jitter):
+    spacing = box / per_side
+    g = (np.arange(per_side) + 0.5) * spacing
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    if count < pts.shape[0]:
+        pts = pts[np.sort(rng.choice(pts.shape[0], size=count, replace=False))]
+    return pts + rng.uniform(-jitter, jitter, size=pts.shape)
+
+
+def water_lattice(nwater=216, box=18.6, seed=20261202, jitter=0.3, name="water_lattice"):
+    """Waters on a (subset of a) jittered cubic lattice filling a cube."""
+    rng = np.random.default_rng(seed)
+    per_side = int(math.ceil(round(nwater ** (1.0 / 3.0), 9)))
+    centres = _lattice_centres(rng, box, per_side, nwater, jitter)
+    xyz = _water_sites(rng, centres)
+    n, q, t, excl = _water_params(nwater)
+    a, b = opls_table([EPS_O_KCAL * KCAL_TO_KJ, 0.0], [SIGMA_O, 0.0])
+    return MMSystem(name, xyz, q, t, 2, a, b, excl.astype(np.int32))
+
+
+def config0(seed=20261202):
+    """648 atoms: 216 waters, 6x6x6 lattice in an 18.6 A cube."""
+    return water_lattice(216, 18.6, seed, name="C0_water648")
+
+
+def config2(seed=20261204):
+    """200,001 atoms: 66,667 waters in a 126 A cube."""
+    return water_lattice(66667, 126.0, seed, name="C2_water200001")
+
+
+def solute_in_water(nwater=9667, nsolute=999, box=67.0, ntypes_solute=8, seed=20261203,
+                    name="solute_in_water"):
+    """Waters on a lattice subset plus random solute atoms (Lorentz-Berthelot)."""
+    rng = np.random.default_rng(seed)
+    per_side = int(math.ceil(nwater ** (1.0 / 3.0)))
+    centres = _lattice_centres(rng, box, per_side, nwater, 0.3)
+    wxyz = _water_sites(rng, centres)
+    nw, qw, tw, exw = _water_params(nwater)
+    sxyz = rng.uniform(0.0, box, size=(nsolute, 3))
+    sq = rng.uniform(-0.5, 0.5, size=nsolute)
+    st = rng.integers(0, ntypes_solute, size=nsolute).astype(np.int32) + 2
+    eps = [EPS_O_KCAL * KCAL_TO_KJ, 0.0] + list(rng.uniform(0.05, 0.2, ntypes_solute) * KCAL_TO_KJ)
+    sig = [SIGMA_O, 0.0] + list(rng.uniform(2.5, 4.0, ntypes_solute))
+    a, b = lorentz_berthelot_table(eps, sig)
+    chain = rng.permutation(nsolute) + nw
+    exs = np.stack([chain[1:], chain[:-1]], 1)
+    xyz = np.concatenate([wxyz, sxyz])
+    q = np.concatenate([qw, sq])
+    t = np.concatenate([tw, st]).astype(np.int32)
+    excl = np.concatenate([exw, exs]).astype(np.int32)
+    return MMSystem(name, xyz, q, t, 2 + ntypes_solute, a, b, excl)
+
+
+def config1(seed=20261203):
+    """30,000 atoms: 9,667 waters plus 999 solute atoms in a 67 A cube."""
+    return solute_in_water(seed=seed, name="C1_solute30000")
+
+
+def water_droplet(natoms, rng, name="droplet"):
+    """A sphere of waters at 0.0334 molecules/A^3 (closest lattice sites)."""
+    nwater = max(1, natoms // 3)
+    spacing = (1.0 / WATER_DENSITY) ** (1.0 / 3.0)
+    radius = (3.0 * nwater / (4.0 * math.pi * WATER_DENSITY)) ** (1.0 / 3.0)
+    k = int(math.ceil(radius / spacing)) + 2
+    g = np.arange(-k, k + 1) * spacing
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    d = np.linalg.norm(pts, axis=1)
+    centres = pts[np.argsort(d, kind="stable")[:nwater]]
+    centres = centres + rng.uniform(-0.3, 0.3, size=centres.shape)
+    xyz = _water_sites(rng, centres)
+    n, q, t, excl = _water_params(nwater)
+    a, b = opls_table([EPS_O_KCAL * KCAL_TO_KJ, 0.0], [SIGMA_O, 0.0])
+    return MMSystem(name, xyz, q, t, 2, a, b, excl.astype(np.int32))
+
+
+def config3(count=64, lo=500, hi=20000, seed=20261205):
+    """64 droplets with atom counts log-uniform in [500, 20000] (multiples of 3)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(lo), math.log(hi), size=count))
+    out = []
+    for k, s in enumerate(sizes):
+        out.append(water_droplet(int(round(s / 3.0)) * 3, rng, name="C3_droplet%02d" % k))
+    return out
+
+
+def perturbed_copies(system, count, sigma, seed):
+    rng = np.random.default_rng(seed)
+    return [system.with_coordinates(system.coordinates3 + rng.normal(0.0, sigma, system.coordinates3.shape),
+                                    name="%s_image%02d" % (system.name, k)) for k in range(count)]
+
+
+def config4(count=16, sigma=0.1, seed=20261206):
+    """16 copies of the 30,000-atom solute system with 0.1 A Gaussian moves."""
+    return perturbed_copies(config1(), count, sigma, seed)
+
+
+def displacement_sequence(system, steps, sigma, seed):
+    """Successive coordinates after cumulative Gaussian moves (sigma per call)."""
+    rng = np.random.default_rng(seed)
+    xyz = system.coordinates3.copy()
+    out = []
+    for _ in range(steps):
+        xyz = xyz + rng.normal(0.0, sigma, xyz.shape)
+        out.append(xyz.copy())
+    return out
+
+
+def random_small_system(natoms, box, seed, ntypes=3, exclusion_fraction=0.3):
+    """Small random mixture for brute-force and edge-case tests."""
+    rng = np.random.default_rng(seed)
+    xyz = rng.uniform(0.0, box, size=(natoms, 3))
+    q = rng.uniform(-0.8, 0.8, size=natoms)
+    t = rng.integers(0, ntypes, size=natoms).astype(np.int32)
+    eps = list(rng.uniform(0.1, 0.9, ntypes))
+    sig = list(rng.uniform(2.5, 3.8, ntypes))
+    a, b = lorentz_berthelot_table(eps, sig)
+    m = int(exclusion_fraction * natoms)
+    if natoms > 1 and m > 0:
+        i = rng.integers(1, natoms, size=m)
+        j = (i - rng.integers(1, 4, size=m)).clip(0)
+        keep = i != j
+        excl = np.unique(np.stack([i[keep], j[keep]], 1), axis=0).astype(np.int32)
+    else:
+        excl = np.zeros((0, 2), np.int32)
+    return MMSystem("random%d" % natoms, xyz, q, t, ntypes, a, b, excl)
