@@ -1,0 +1,175 @@
+"""Benchmark of the NBModelABFS energy + gradient on one MI355X.
+
+One step = NBModelABFS_Update (the half-skin rebuild check, rebuilding when it
+fires) + one energy/gradient evaluation of the whole system, coordinates and
+gradients resident in HBM.  The default workload is the 30,000-atom
+solute-in-water system (9,667 waters + 999 random solute atoms, 67 A cube,
+8 Lorentz-Berthelot solute LJ types), tiled mixed-precision kernel.
+
+Prints ONE JSON line: pair interactions/s (reference list pairs, d <= 13.5 A,
+per second), ms per E+grad, the roofline of the pair kernel (algorithmic
+flops per launch over the launch time measured with HIP events on the launch
+stream) and the reference C library timed on this host's cores.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pdynamo-mirror_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 vector peak
+PEAK_FP64_TFLOPS = 78.6     # MI355X published FP64 vector peak
+
+
+def algorithmic_flops(xyz, keys, n):
+    """8 N_list + 37 N_(d<=8) + 63 N_(8<d<=12): the reference's per-pair operations
+    (displacement and r^2; plain Coulomb+LJ+gradient; switched Coulomb+LJ+gradient)."""
+    i = keys // n
+    j = keys % n
+    d = xyz[i] - xyz[j]
+    r2 = np.einsum("ij,ij->i", d, d)
+    n_list = keys.size
+    n_plain = int(np.count_nonzero(r2 <= 64.0))
+    n_switch = int(np.count_nonzero((r2 > 64.0) & (r2 <= 144.0)))
+    return 8.0 * n_list + 37.0 * n_plain + 63.0 * n_switch, n_list, n_plain, n_switch
+
+
+def make_workload(name):
+    from pdynamo_amd import synthetic
+    if name == "solute30000":
+        return synthetic.config1(), "C1 solute-in-water 30,000 atoms (9,667 waters + 999 solute, 67 A cube)"
+    if name == "water200001":
+        return synthetic.config2(), "C2 water box 200,001 atoms (66,667 waters, 126 A cube)"
+    if name == "water648":
+        return synthetic.config0(), "C0 water 648 atoms (216 waters, 18.6 A cube)"
+    raise SystemExit("unknown workload " + name)
+
+
+def cpu_baseline(mm, calls, threads):
+    """The reference pDynamo C (oracle/_ref, -O3 -fopenmp) or, if absent, the
+    FP64 C restatement, timed for `calls` E+grad calls after the first build."""
+    import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    if kind == "reference":
+        eng = O.ReferenceABFS(mm, nthreads=threads)
+    else:
+        eng = O.OracleABFS(mm, nthreads=threads)
+    eng.energy(mm.coordinates3)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        eng.energy(mm.coordinates3)
+    dt = (time.perf_counter() - t0) / calls
+    if kind == "reference":
+        npairs = eng.canonical(0).size
+        eng.close()
+    else:
+        npairs = eng.records.npairs
+    return {"value": npairs / dt, "unit": "pair interactions/s", "cores": threads, "kind": kind,
+            "sample": "%d E+grad calls on the same workload after the first list build (%.1f ms each, %d list pairs)"
+                      % (calls, dt * 1e3, npairs)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="solute30000")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--entries-per-item", type=int, default=0)
+    ap.add_argument("--waves-per-cu", type=int, default=-1)
+    ap.add_argument("--cpu-calls", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from pdynamo_amd import _native
+    from pdynamo_amd.NBModelABFS import NBModelABFS
+
+    mm, wdesc = make_workload(args.workload)
+    opts = {"precision": args.precision}
+    if args.entries_per_item:
+        opts["entriesPerItem"] = args.entries_per_item
+    if args.waves_per_cu >= 0:
+        opts["wavesPerCU"] = args.waves_per_cu
+    model = NBModelABFS(**opts)
+    st = _native.NativeState(model.NativeOptions(), mm.charges, mm.ljtypes, mm.ntypes, mm.tableA, mm.tableB,
+                             exclusions=mm.exclusions, interactions14=mm.interactions14)
+    x = _native.DeviceArray.from_numpy(mm.coordinates3, st)
+    g = _native.DeviceArray(x.shape)
+    _native.synchronize()
+
+    def step():
+        # System.Energy zeroes the gradients; the model increments them.
+        g.zero(st)
+        st.update(x.ptr)
+        st.energy_async(x.ptr, g.ptr)
+
+    for _ in range(args.warmup):
+        step()
+    _native.synchronize()
+    st.set_timing(True)
+    _native.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    _native.synchronize()
+    t1 = time.perf_counter()
+    stats = st.stats()
+    st.set_timing(False)
+    ms = (t1 - t0) / args.steps * 1e3
+    e, present = st.energies()
+    keys = st.export_pairs(0)
+    n_list = int(keys.size)
+    value = n_list / (ms * 1e-3)
+
+    result = {
+        "metric": "pair interactions/s (reference list pairs d<=13.5 A per E+grad second)",
+        "value": value,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 pair core + f32 rsqrt seed/force vectors + int64 fixed-point sums" if args.precision == "mixed"
+                 else "f64",
+        "data": "synthetic (seeded generator, pdynamo_amd.synthetic)",
+        "config": {"workload": wdesc, "atoms": mm.natoms, "list_pairs": n_list, "precision": args.precision,
+                   "list_cutoff": 13.5, "cutoffs": [0.5, 8.0, 12.0],
+                   "energies_kJ_mol": [float(v) for v in e], "work_items": int(stats["workItems"]),
+                   "cluster_entries": int(stats["entries"])},
+    }
+    if not args.no_roofline:
+        flops, n_list2, n_plain, n_switch = algorithmic_flops(mm.coordinates3, keys, mm.natoms)
+        kms = stats["forceKernelMs"]
+        achieved = flops / (kms * 1e-3) / 1e12 if kms > 0 else None
+        peak = PEAK_FP32_TFLOPS if args.precision == "mixed" else PEAK_FP64_TFLOPS
+        result["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                              "frac": (achieved / peak) if achieved else None, "traffic": None,
+                              "kernel": "k_force_tiled" if args.precision == "mixed" else "k_force_plain",
+                              "kernel_ms": kms, "launches": int(stats["forceKernelLaunches"]),
+                              "flops_per_launch": flops, "pairs_d_le_8": n_plain, "pairs_8_12": n_switch}
+    if not args.no_cpu_baseline:
+        threads = len(os.sched_getaffinity(0))
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+        result["cpu_baseline"] = cpu_baseline(mm, args.cpu_calls, threads)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

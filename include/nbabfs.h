@@ -1,0 +1,158 @@
+/*
+ * nbabfs.h -- C ABI of the gfx950 NBModelABFS MM/MM energy + gradient library
+ * (libnbabfs.so).  Plain pointers and sizes only; no exceptions cross it.
+ *
+ * It stands in for the reference's C seam of this path (pDynamo 1.9.0; paths
+ * relative to the pDynamo tree, pMx = pMolecule-1.9.0/extensions,
+ * pCx = pCore-1.9.0/extensions):
+ *
+ *   NBModelABFSState_SetUp       pMx/cinclude/NBModelABFSState.h:145-156
+ *   NBModelABFSState_Initialize  pMx/cinclude/NBModelABFSState.h:138-142
+ *   NBModelABFS_Update           pMx/cinclude/NBModelABFS.h:46
+ *   NBModelABFS_MMMMEnergy       pMx/cinclude/NBModelABFS.h:42
+ *   NBModelABFSState_Deallocate  pMx/cinclude/NBModelABFSState.h:133
+ *   PairListGenerator_SelfPairListFromCoordinates3
+ *                                pCx/cinclude/PairListGenerator.h:71-79
+ *
+ * Conventions kept from the reference: energies in kJ/mol, lengths in A,
+ * charges in e; gradients (dE/dx) are INCREMENTED, never zeroed; status codes
+ * use the reference's Status enum values (pCx/cinclude/Status.h); one state
+ * per configuration, not re-entrant; calls are synchronous unless named
+ * *_async.  Host arrays are borrowed, device buffers are owned by the state.
+ */
+#ifndef NBABFS_H
+#define NBABFS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status values (pCx/cinclude/Status.h). */
+#define NBABFS_STATUS_CONTINUE          16
+#define NBABFS_STATUS_INVALID_ARGUMENT  22
+#define NBABFS_STATUS_OUT_OF_MEMORY     41
+#define NBABFS_STATUS_VALUE_ERROR       47
+#define NBABFS_STATUS_LOGIC_ERROR       10
+
+/* Arithmetic of the pair kernel. */
+#define NBABFS_MIXED      0   /* FP32 pair math, 64-bit fixed-point sums, FP64 totals */
+#define NBABFS_PLAIN_FP64 1   /* FP64 pair math in the reference expression order    */
+
+typedef struct nbabfs_state nbabfs_state;
+typedef struct nbabfs_batch nbabfs_batch;
+
+/* Options: the NBModelABFS fields (pMx/cinclude/NBModelABFS.h:22-33, defaults
+ * NBModelABFS.c:28-37) and the ABFS default PairListGenerator
+ * (pMx/pyrex/pMolecule.NBModelABFS.pyx:59-71), plus kernel tuning knobs. */
+typedef struct {
+    double dampingCutoff;          /* 0.5  */
+    double innerCutoff;            /* 8.0  */
+    double outerCutoff;            /* 12.0 */
+    double listCutoff;             /* 13.5 */
+    double dielectric;             /* 1.0  */
+    double electrostaticScale14;   /* 1.0  */
+    double cutoffCellSizeFactor;   /* 0.5: cellSize = factor * listCutoff */
+    int    minimumCellExtent;      /* 2    */
+    int    minimumPoints;          /* 500  */
+    int    precision;              /* NBABFS_MIXED or NBABFS_PLAIN_FP64 */
+    int    entriesPerItem;         /* cluster-pair entries per work item (0 = 16) */
+    int    wavesPerCU;             /* persistent-kernel waves per CU; 0 = one workgroup per work item */
+    int    reserved;
+} nbabfs_options;
+
+typedef struct {
+    long long numberOfCalls;       /* NBModelABFSState.numberOfCalls   */
+    long long numberOfUpdates;     /* NBModelABFSState.numberOfUpdates */
+    double    numberOfMMMMPairs;   /* summed over updates (StatisticsAccumulate) */
+    long long npairs;              /* current main list */
+    long long npairs14;            /* 1-4 list */
+    int       useGridSearch;
+    int       clusters;            /* 8-atom clusters (with padding) */
+    long long entries;             /* (super-cluster, j-cluster) entries */
+    long long workItems;
+    double    forceKernelMs;       /* mean duration of the pair kernel (timing on) */
+    long long forceKernelLaunches;
+} nbabfs_stats;
+
+/* Fill *opt with the reference defaults. */
+void nbabfs_default_options(nbabfs_options *opt);
+
+/* Library / device information.  Returns NBABFS_STATUS_CONTINUE when a gfx950
+ * device is usable; the product path never falls back to the CPU. */
+int nbabfs_init(void);
+const char *nbabfs_last_error(void);
+int nbabfs_device_cu_count(void);
+
+/* NBModelABFSState_SetUp: LJ tables are the packed lower triangles of
+ * LJParameterContainer (index i*(i+1)/2+j, j<=i, pMx/cinclude/
+ * LJParameterContainer.h:16-23); tableA14/B14 may be NULL (= main tables).
+ * exclusions / interactions14 are (i,j) integer pairs in any orientation
+ * (PairList_FromIntegerPairArray semantics, pCx/csource/PairList.c:107-189).
+ * fixedAtoms (n bytes, 1 = fixed) may be NULL. */
+int nbabfs_state_create(const nbabfs_options *opt, int natoms, const double *charges, const int *ljtypes,
+                        int ntypes, const double *tableA, const double *tableB, const double *tableA14,
+                        const double *tableB14, int nexclusions, const int *exclusions, int ninteractions14,
+                        const int *interactions14, const unsigned char *fixedAtoms, nbabfs_state **out);
+void nbabfs_state_destroy(nbabfs_state *state);
+
+/* Stream used by every launch of this state (a hipStream_t; NULL = the
+ * library's own stream). */
+int nbabfs_set_stream(nbabfs_state *state, void *hip_stream);
+
+/* NBModelABFSState_Initialize + NBModelABFS_Update + NBModelABFS_MMMMEnergy on
+ * host arrays (coordinates n*3 FP64 row-major).  grad_inout (n*3) is
+ * incremented and may be NULL.  energies[4] = MM/MM Elect., MM/MM LJ,
+ * MM/MM 1-4 Elect., MM/MM 1-4 LJ; present[2] says whether the main / 1-4 list
+ * is non-empty (the reference drops the labels of an empty list,
+ * pMx/pyrex/pMolecule.NBModelABFSState.pyx:41-49).  *rebuilt = list rebuilt. */
+int nbabfs_energy_host(nbabfs_state *state, const double *xyz, double *grad_inout, double *energies,
+                       int *present, int *rebuilt);
+
+/* Device-resident variants.  d_xyz / d_grad are device pointers (n*3 FP64);
+ * d_grad may be NULL.  nbabfs_update: NBModelABFS_Update only.
+ * nbabfs_mmmm_energy_async: enqueue the energy/gradient on the state's stream;
+ * the four energies land in state-owned device memory readable with
+ * nbabfs_energies (which synchronises). */
+int nbabfs_update(nbabfs_state *state, const double *d_xyz, int *rebuilt);
+int nbabfs_mmmm_energy_async(nbabfs_state *state, const double *d_xyz, double *d_grad);
+int nbabfs_energies(nbabfs_state *state, double *energies, int *present);
+
+/* Statistics (NBModelABFSState.pyx:68-97) and kernel timing with HIP events on
+ * the launch stream (enable != 0 turns it on). */
+int nbabfs_stats_get(nbabfs_state *state, nbabfs_stats *out);
+int nbabfs_set_timing(nbabfs_state *state, int enable);
+
+/* The current main (which=0) or 1-4 (which=1) list as canonical int64 keys
+ * max(i,j)*n + min(i,j), sorted ascending.  Returns the pair count; writes at
+ * most cap keys (keys may be NULL to query the count). */
+long long nbabfs_export_pairs(nbabfs_state *state, int which, long long *keys, long long cap);
+
+/* Device memory helpers, so that callers need no second HIP runtime (a
+ * framework that bundles its own libamdhip64 cannot share streams with this
+ * library).  No counterpart in the reference, whose arrays live in host
+ * memory.  Copies and the zero fill are ordered on the state's stream when a
+ * state is given (NULL = synchronous on the device). */
+int nbabfs_device_alloc(size_t bytes, void **out);
+int nbabfs_device_free(void *ptr);
+int nbabfs_memcpy_htod(nbabfs_state *state, void *dst, const void *src, size_t bytes);
+int nbabfs_memcpy_dtoh(nbabfs_state *state, void *dst, const void *src, size_t bytes);
+int nbabfs_zero_async(nbabfs_state *state, void *dst, size_t bytes);
+int nbabfs_synchronize(nbabfs_state *state);
+
+/* Batches of independent systems (reaction-path images, replicas): one
+ * persistent launch over the union of their work items; replays a captured
+ * hipGraph while no list is rebuilt.  states stay owned by the caller. */
+int nbabfs_batch_create(nbabfs_state *const *states, int nsystems, int use_graph, nbabfs_batch **out);
+void nbabfs_batch_destroy(nbabfs_batch *batch);
+/* d_xyz[k], d_grad[k] device pointers per system (d_grad entries may be
+ * NULL); energies (4*nsystems) and rebuilt (nsystems, may be NULL) on host. */
+int nbabfs_batch_energy(nbabfs_batch *batch, const double *const *d_xyz, double *const *d_grad, double *energies,
+                        int *rebuilt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

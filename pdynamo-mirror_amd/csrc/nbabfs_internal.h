@@ -1,0 +1,144 @@
+// nbabfs_internal.h -- shared definitions of the gfx950 NBModelABFS library.
+//
+// Data layout in HBM (per system):
+//   xyz      FP64 n x 3, caller order (the reference Coordinates3 layout).
+//   slots    atoms re-ordered into clusters of 8 (CL) along z inside x-y
+//            columns; 8 clusters form a super-cluster (SC, 64 atoms = one
+//            wavefront).  slot_atom[s] = atom or -1 (padding).
+//   xyzq     double4 per slot: position relative to the system origin and the
+//            charge times sqrt(Coulomb factor / dielectric); refreshed per call.
+//   styp     int per slot: LJ type (0 for padding).
+//   entries  one per (SC, j-cluster) pair: j-cluster index plus one byte per
+//            lane l = 8a+b whose bit c says that i = atom a of i-cluster c and
+//            j = atom b of the j-cluster are on the list.  The set bits are
+//            exactly the reference pair list.
+//   facc     six int64 arrays of n_pad (x,y,z low words, x,y,z high words):
+//            deterministic fixed-point gradient accumulators.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NB_CL 8           // atoms per cluster
+#define NB_SCC 8          // clusters per super-cluster
+#define NB_SC (NB_CL * NB_SCC)
+
+// Fixed point: value = hi * 2^16 + lo * 2^-32.
+#define NB_FX_HI_UNIT 65536.0
+#define NB_FX_LO_SCALE 4294967296.0
+
+// Switching factors, same field set as PairwiseInteractionABFSFactors
+// (pMolecule-1.9.0/extensions/cinclude/PairwiseInteraction.h:40-66).
+struct AbfsFactors {
+    double r2Damp, r2Off, r2On;
+    double a, b, c, d, qShift1, qShift2, qF0, qAlpha;
+    double aF6, aK12, aShift12, aF0, aAlpha;
+    double bF3, bK6, bShift6, bF0, bAlpha;
+};
+
+struct Entry {              // 80 bytes
+    int J;                  // j-cluster (local index)
+    int sc;                 // owning super-cluster
+    int cbits;              // bit c set if i-cluster c has any pair with J
+    int pad;
+    unsigned char bits[NB_SC];  // per lane l = 8a+b: bit c = pair (c,a)-(J,b) listed
+};
+
+struct WorkItem {           // 16 bytes
+    int sys;                // system index in the batch
+    int sc;                 // super-cluster
+    int first;              // first entry
+    int count;              // entries
+};
+
+// Everything a per-call kernel needs to know about one system.
+struct SysDesc {
+    const double *xyz;          // n*3 current coordinates (device)
+    double *grad;               // n*3 gradients to increment (device) or null
+    const int *slot_atom;       // n_pad
+    const int *atom_slot;       // n
+    const float4 *centre;       // ncl
+    double4 *xyzq;              // n_pad
+    int *styp;                  // n_pad: LJ type (0 for padding)
+    const int *atype;           // n: LJ types
+    const double *charge;       // n
+    const double *ljA, *ljB;    // ntypes*ntypes FP64 (main list)
+    const double *ljA14, *ljB14;// ntypes*ntypes FP64 (1-4 list)
+    const Entry *entries;
+    const int *p14;             // 2*n14 (i, j)
+    long long *facc;            // 6*n_pad
+    double *gplain;             // n*3 plain-path gradients (or null)
+    double *epart;              // energy partials (2 per item)
+    double *epart14;            // 2 per 1-4 block
+    double *eplain;             // 2 per plain block
+    double *energies;           // 4 outputs (device)
+    double origin[3];           // FP64 frame origin of the cluster coordinates
+    double escale;              // Coulomb factor / dielectric
+    double escale14;            // times electrostaticScale14
+    double qscale;              // sqrt(escale): charges are stored pre-scaled
+    int n, n_pad, ncl, ntypes, n14;
+    int item_first, item_count; // range of this system's items in the batch
+    int n_plain_blocks, n14_blocks;
+};
+
+// ------------------------------------------------------------------------------------------------
+// Device helpers.
+// ------------------------------------------------------------------------------------------------
+// Pointers loaded from a descriptor are generic; accessing memory through
+// address-space-1 pointers makes the compiler emit global_* operations (vmcnt
+// only) instead of flat_* ones, which would also hold every LDS wait behind
+// outstanding atomics.
+#define GAS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GAS T *as_global(T *p)
+{
+    return (GAS T *)p;
+}
+
+// rint(x) as int64 for |x| < 2^51: x + 1.5*2^52 puts the rounded integer in
+// the low mantissa bits.
+__device__ __forceinline__ long long rint_i64(double x)
+{
+    const double magic = 6755399441055744.0;
+    return __double_as_longlong(x + magic) - __double_as_longlong(magic);
+}
+
+__device__ __forceinline__ void fx_atomic_add(long long *facc, int n_pad, int slot, int comp, double v)
+{
+    // v = hi*2^16 + r with |r| <= 2^15 (exact); r is stored as round(r*2^32).
+    long long hi;
+    double r;
+    if (fabs(v) < 0x1p66) {
+        hi = rint_i64(v * (1.0 / NB_FX_HI_UNIT));
+        r = v - (double)hi * NB_FX_HI_UNIT;
+    } else {   // beyond any physical gradient; keeps the arithmetic defined
+        const double h = rint(v * (1.0 / NB_FX_HI_UNIT));
+        hi = (long long)fmax(fmin(h, 9.0e18), -9.0e18);
+        r = 0.0;
+    }
+    const long long lo = rint_i64(r * NB_FX_LO_SCALE);
+    GAS long long *g = as_global(facc);
+    if (lo != 0) __hip_atomic_fetch_add(&g[(size_t)comp * n_pad + slot], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (hi != 0)
+        __hip_atomic_fetch_add(&g[(size_t)(comp + 3) * n_pad + slot], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double fx_value(const long long *facc, int n_pad, int slot, int comp)
+{
+    return (double)facc[(size_t)(comp + 3) * n_pad + slot] * NB_FX_HI_UNIT +
+           (double)facc[(size_t)comp * n_pad + slot] * (1.0 / NB_FX_LO_SCALE);
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum_f32(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}

@@ -1,0 +1,72 @@
+// nbabfs_kernels.h -- parameter blocks and launch wrappers shared by the host
+// driver (nbabfs_host.hip) and the kernel files.
+#pragma once
+
+#include "nbabfs_internal.h"
+
+// Reference cell grid (Coordinates3_MakeGrid, pCore-1.9.0/extensions/csource/
+// Coordinates3.c:1111-1158) plus the cluster-column grid used for sorting.
+struct BuildGrid {
+    double lower[3];
+    double h;
+    int bins[3];
+    int use_grid;
+    double corg[3];   // cluster-column grid origin (= bounding-box minimum)
+    double cw, dz;    // column width and z-bin height
+    int ncx, ncy, nzb;
+};
+
+struct SearchParams {
+    double cutoff2;                       // listCutoff^2 (FP64 test)
+    const unsigned char *stencil;         // (2u+1)^3 flags
+    int u;
+    int use_grid;
+    const int *xptr, *xidx;               // exclusions CSR
+    const unsigned char *free_flags;      // or null (no fixed atoms)
+    float reach;                          // cull distance for FP32 boxes
+    float cw;
+    int ncx, ncy;
+    int cap;                              // entries per super-cluster
+};
+
+// List construction (nbabfs_build.hip).
+void launch_bbox(const double *xyz, int n, double *part, int nparts, double *out, hipStream_t s);
+void launch_bin(const double *xyz, int n, const BuildGrid &g, int4 *cellv, int *key, int *count, hipStream_t s);
+size_t scan_tmp_size(int m);
+void launch_scan(const int *in, int *out, int m, int *tmp, hipStream_t s);
+void launch_fill_int(int *p, size_t m, int v, hipStream_t s);
+void launch_sort_slots(const int *key, int n, int nkeys, int ncol, int nzb, int *off, int *cursor, int *sorted,
+                       int *padded, hipStream_t s);
+void launch_slots(const int *sorted, const int *key, int n, int nzb, const int *off, const int *colstart,
+                  const int *padded, int ncol, int *slot_atom, int *atom_slot, int2 *colcl, hipStream_t s);
+void launch_cluster_geom(const double *xyz, const int *slot_atom, int ncl, const double *origin, float4 *blo,
+                         float4 *bhi, float4 *centre, hipStream_t s);
+void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4 *cellv, const int *slot_atom,
+                   const float4 *blo, const float4 *bhi, const int2 *colcl, Entry *scratch, int *counts,
+                   int *overflow, hipStream_t s);
+void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, hipStream_t s);
+void launch_compact(const Entry *scratch, int cap, const int *counts, const int *eoff, const int *ioff, int nsc,
+                    int per_item, int sys, Entry *entries, WorkItem *items, hipStream_t s);
+void launch_count_pairs(const Entry *entries, int nent, unsigned long long *total, hipStream_t s);
+void launch_export_pairs(const Entry *entries, int nent, const int *slot_atom, long long n,
+                         unsigned long long *cursor, long long cap, long long *keys, hipStream_t s);
+void launch_csr_degree(const Entry *entries, int nent, const int *slot_atom, int *deg, hipStream_t s);
+void launch_csr_fill(const Entry *entries, int nent, const int *slot_atom, int *cursor, int *cols, hipStream_t s);
+void launch_check_update(const double *xyz, const double *ref, const unsigned char *free_flags, int n, double bufsq,
+                         int *flag, hipStream_t s);
+
+// Per-call kernels (nbabfs_force.hip).  Every launcher takes the batch's
+// descriptor array (device) and the number of systems.
+struct QueueParams {
+    int *heads;            // 8 counters 128 bytes apart (heads[32*q]), reset by launch_prepare;
+                           // null = one work item per workgroup
+    const int *starts;     // 9 item offsets: queue q owns [starts[q], starts[q+1])
+};
+
+void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, int *heads, hipStream_t s);
+void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
+                        int nblocks, int max_ntypes, hipStream_t s);
+void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
+                        const int *cols, hipStream_t s);
+void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s);
+void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s);

@@ -1,0 +1,327 @@
+"""The atom-based force-switching (ABFS) non-bonding model, MI355X edition.
+
+Same classes, option names, validation and labels as pDynamo 1.9.0:
+
+* ``PairListGenerator`` -- pCore-1.9.0/extensions/pyrex/pCore.PairListGenerator.pyx:15-125
+  (defaults pCore-1.9.0/extensions/csource/PairListGenerator.c:360-378).
+* ``NBModelABFS``       -- pMolecule-1.9.0/extensions/pyrex/pMolecule.NBModelABFS.pyx:15-302.
+* ``NBModelABFSState``  -- pMolecule-1.9.0/extensions/pyrex/pMolecule.NBModelABFSState.pyx:13-127.
+
+Energies and gradients are computed by libnbabfs.so on the GPU (cluster-pair
+list + FP32 pair kernel with fixed-point sums by default, or the plain FP64
+kernel with ``precision="fp64"``).  There is no CPU compute path.
+"""
+
+import copy
+
+import numpy as np
+
+from . import _native
+from ._native import CLibraryError
+from .LogFile import LogFileActive, logFile
+
+_STATENAMES = ("nbState", "qcmmstate")
+
+
+class PairListGenerator:
+    """Options of the cell-grid self pair list generator."""
+
+    _DEFAULTS = {"sortIndices": True, "useGridByCell": False, "minimumCellExtent": 2, "minimumPoints": 500,
+                 "cellSize": 0.0, "cutoff": 0.0, "cutoffCellSizeFactor": 0.5, "minimumCellSize": 1.0,
+                 "minimumExtentFactor": 1.5}
+
+    def __init__(self, **options):
+        for k, v in self._DEFAULTS.items():
+            setattr(self, k, v)
+        self.SetOptions(**options)
+
+    @classmethod
+    def FromOptions(cls, **options):
+        return cls(**options)
+
+    def SetOptions(self, **keywordArguments):
+        for k in ("sortIndices", "useGridByCell"):
+            if k in keywordArguments:
+                setattr(self, k, bool(keywordArguments.pop(k)))
+        for k in ("minimumCellExtent", "minimumPoints"):
+            if k in keywordArguments:
+                setattr(self, k, int(keywordArguments.pop(k)))
+        for k in ("cellSize", "cutoff", "cutoffCellSizeFactor", "minimumCellSize", "minimumExtentFactor"):
+            if k in keywordArguments:
+                setattr(self, k, float(keywordArguments.pop(k)))
+        if len(keywordArguments) > 0:
+            raise ValueError("Invalid options: " + ", ".join(sorted(keywordArguments.keys())) + ".")
+        # The cell size always follows the cutoff (pCore.PairListGenerator.pyx:104).
+        self.cellSize = self.cutoffCellSizeFactor * self.cutoff
+
+    def __getstate__(self):
+        return {k: getattr(self, k) for k in self._DEFAULTS}
+
+    def __setstate__(self, state):
+        self.__init__(**state)
+
+    def __copy__(self):
+        return self.__class__(**self.__getstate__())
+
+    def __deepcopy__(self, memo):
+        return self.__copy__()
+
+    def Summary(self, log=logFile):
+        if LogFileActive(log):
+            s = log.GetSummary()
+            s.Start("Pairlist Generator Summary")
+            s.Entry("Cell Size", "{:.3f}".format(self.cellSize))
+            s.Entry("Cutoff", "{:.3f}".format(self.cutoff))
+            s.Entry("Cutoff Cell Size Factor", "{:.3f}".format(self.cutoffCellSizeFactor))
+            s.Entry("Grid Cell/Cell Method", "{!r}".format(self.useGridByCell))
+            s.Entry("Minimum Cell Extent", "{:d}".format(self.minimumCellExtent))
+            s.Entry("Minimum Cell Size", "{:.3f}".format(self.minimumCellSize))
+            s.Entry("Minimum Extent Factor", "{:.3f}".format(self.minimumExtentFactor))
+            s.Entry("Minimum Points", "{:d}".format(self.minimumPoints))
+            s.Entry("Sort Indices", "{!r}".format(self.sortIndices))
+            s.Stop()
+
+
+class NBModelABFSState:
+    """Per-configuration cache: the device lists and the energies of the last call."""
+
+    def __init__(self, native, mmAtoms):
+        self.native = native
+        self.natoms = mmAtoms.size
+        self.emmel = self.emmlj = self.emmel14 = self.emmlj14 = 0.0
+        self.present = (False, False)
+        self.lastRebuilt = False
+
+    def GetEnergies(self, energies):
+        """Append the energy terms; a term is present only for a non-empty list."""
+        if self.present[0]:
+            energies.append(("MM/MM Elect.", self.emmel))
+            energies.append(("MM/MM LJ", self.emmlj))
+        if self.present[1]:
+            energies.append(("MM/MM 1-4 Elect.", self.emmel14))
+            energies.append(("MM/MM 1-4 LJ", self.emmlj14))
+
+    def Statistics(self):
+        return self.native.stats()
+
+    def Summary(self, log=logFile):
+        if LogFileActive(log):
+            st = self.native.stats()
+            s = log.GetSummary()
+            s.Start("ABFS NB Model State Summary")
+            if st["npairs"] > 0:
+                s.Entry("MM/MM Pairs", "{:d}".format(st["npairs"]))
+            if st["npairs14"] > 0:
+                s.Entry("MM/MM 1-4 Pairs", "{:d}".format(st["npairs14"]))
+            s.Stop()
+
+    def StatisticsSummary(self, log=logFile):
+        if LogFileActive(log):
+            st = self.native.stats()
+            s = log.GetSummary()
+            s.Start("ABFS NB Model State Statistics Summary")
+            s.Entry("Number of Calls", "{:d}".format(st["numberOfCalls"]))
+            s.Entry("Number of Updates", "{:d}".format(st["numberOfUpdates"]))
+            n = st["numberOfUpdates"]
+            if n > 0:
+                if st["numberOfMMMMPairs"] != 0.0:
+                    s.Entry("<MM/MM Pairs>", "{:.1f}".format(st["numberOfMMMMPairs"] / n))
+                if st["npairs14"] > 0:
+                    s.Entry("MM/MM 1-4 Pairs", "{:d}".format(st["npairs14"]))
+                s.Entry("Calls per Update", "{:.1f}".format(float(st["numberOfCalls"]) / n))
+                s.Entry("Grid Updating", "{!r}".format(bool(st["useGridSearch"])))
+            s.Stop()
+
+    def Pairs(self, which=0):
+        """The list as sorted canonical keys max(i,j)*n + min(i,j)."""
+        return self.native.export_pairs(which)
+
+
+class NBModelABFS:
+    """Define an ABFS NB model."""
+
+    _PRECISIONS = {"mixed": _native.MIXED, "fp64": _native.PLAIN_FP64}
+
+    def __init__(self, **options):
+        self.label = "ABFS"
+        self.checkForInverses = True
+        self.dampingCutoff = 0.5
+        self.dielectric = 1.0
+        self.electrostaticScale14 = 1.0
+        self.imageExpandFactor = 0
+        self.innerCutoff = 8.0
+        self.listCutoff = 13.5
+        self.outerCutoff = 12.0
+        self.qcmmCoupling = "RC"
+        self.useCentering = False
+        self.generator = None
+        self.mmmmPairwiseInteraction = None
+        self.qcmmPairwiseInteraction = None
+        self.qcqcPairwiseInteraction = None
+        # Extensions of this implementation (tuning; reference defaults retained above).
+        self.precision = "mixed"
+        self.entriesPerItem = 16
+        self.wavesPerCU = 0
+        self.SetOptions(**options)
+
+    @classmethod
+    def FromOptions(cls, **options):
+        return cls(**options)
+
+    def SetOptions(self, **keywordArguments):
+        """Set options for the model (pMolecule.NBModelABFS.pyx:140-179)."""
+        for k in ("dampingCutoff", "dielectric", "electrostaticScale14", "innerCutoff", "listCutoff", "outerCutoff"):
+            if k in keywordArguments:
+                setattr(self, k, float(keywordArguments.pop(k)))
+        for k in ("generator", "mmmmPairwiseInteraction", "qcmmPairwiseInteraction", "qcqcPairwiseInteraction"):
+            if k in keywordArguments:
+                setattr(self, k, keywordArguments.pop(k))
+        if "imageExpandFactor" in keywordArguments:
+            self.imageExpandFactor = int(keywordArguments.pop("imageExpandFactor"))
+        for k in ("checkForInverses", "useCentering"):
+            if k in keywordArguments:
+                setattr(self, k, bool(keywordArguments.pop(k)))
+        if "qcmmCoupling" in keywordArguments:
+            coupling = keywordArguments.pop("qcmmCoupling")
+            if coupling not in ("MM", "RC", "RD"):
+                raise TypeError("Unrecognized QC/MM coupling option: " + str(coupling) + ".")
+            self.qcmmCoupling = coupling
+        if "precision" in keywordArguments:
+            p = keywordArguments.pop("precision")
+            if p not in self._PRECISIONS:
+                raise ValueError("Unrecognized precision: {!r} (use 'mixed' or 'fp64').".format(p))
+            self.precision = p
+        if "entriesPerItem" in keywordArguments:
+            v = int(keywordArguments.pop("entriesPerItem"))
+            if v <= 0:
+                raise ValueError("entriesPerItem must be positive.")
+            self.entriesPerItem = v
+        if "wavesPerCU" in keywordArguments:
+            v = int(keywordArguments.pop("wavesPerCU"))
+            if v < 0:
+                raise ValueError("wavesPerCU must be >= 0 (0 = one workgroup per work item).")
+            self.wavesPerCU = v
+        if len(keywordArguments) > 0:
+            raise ValueError("Invalid options: " + ", ".join(sorted(keywordArguments.keys())) + ".")
+        if (self.dampingCutoff < 0.0) or (self.innerCutoff < self.dampingCutoff) or \
+           (self.outerCutoff < self.innerCutoff) or (self.listCutoff < self.outerCutoff):
+            raise TypeError("Invalid cutoff values: damping - {:.3f}; inner - {:.3f}; outer - {:.3f}; list - {:.3f}.".format(
+                self.dampingCutoff, self.innerCutoff, self.outerCutoff, self.listCutoff))
+        self.CheckGenerator()
+
+    def CheckGenerator(self):
+        """The ABFS default generator (pMolecule.NBModelABFS.pyx:59-71)."""
+        if self.generator is None:
+            self.generator = PairListGenerator.FromOptions(cutoff=self.listCutoff, cutoffCellSizeFactor=0.5,
+                                                           minimumCellExtent=2, minimumCellSize=3.0,
+                                                           minimumExtentFactor=1.5, minimumPoints=500,
+                                                           sortIndices=False, useGridByCell=True)
+        else:
+            self.generator.SetOptions(cutoff=self.listCutoff)
+
+    def __getstate__(self):
+        state = {k: getattr(self, k) for k in ("checkForInverses", "imageExpandFactor", "dampingCutoff", "dielectric",
+                                                "electrostaticScale14", "innerCutoff", "listCutoff", "outerCutoff",
+                                                "qcmmCoupling", "useCentering")}
+        if self.generator is not None:
+            state["generator"] = self.generator
+        if self.precision != "mixed":
+            state["precision"] = self.precision
+        return state
+
+    def __setstate__(self, state):
+        self.__init__(**state)
+
+    def __copy__(self):
+        return self.__class__(**copy.deepcopy(self.__getstate__()))
+
+    def __deepcopy__(self, memo):
+        return self.__copy__()
+
+    def NativeOptions(self):
+        o = _native.default_options()
+        o.dampingCutoff = self.dampingCutoff
+        o.innerCutoff = self.innerCutoff
+        o.outerCutoff = self.outerCutoff
+        o.listCutoff = self.listCutoff
+        o.dielectric = self.dielectric
+        o.electrostaticScale14 = self.electrostaticScale14
+        o.cutoffCellSizeFactor = self.generator.cutoffCellSizeFactor
+        o.minimumCellExtent = self.generator.minimumCellExtent
+        o.minimumPoints = self.generator.minimumPoints
+        o.precision = self._PRECISIONS[self.precision]
+        o.entriesPerItem = self.entriesPerItem
+        o.wavesPerCU = self.wavesPerCU
+        return o
+
+    def Clear(self, configuration):
+        """Clear up temporary data (pMolecule.NBModel.pyx:74-78)."""
+        if configuration is not None:
+            for name in _STATENAMES:
+                if hasattr(configuration, name):
+                    state = getattr(configuration, name)
+                    if hasattr(state, "native"):
+                        state.native.close()
+                    delattr(configuration, name)
+
+    def SetUp(self, mmAtoms, qcAtoms, ljParameters, ljParameters14, fixedAtoms, interactions14, exclusions,
+              symmetry, isolates, configuration, log=logFile):
+        """Create the state on first use, then run the list update (pyx:181-273)."""
+        if configuration is None:
+            return
+        if qcAtoms is not None and getattr(qcAtoms, "size", 0) > 0:
+            raise NotImplementedError("QC/MM interactions are not part of this implementation")
+        if symmetry is not None:
+            raise NotImplementedError("periodic images are not part of this implementation")
+        if not hasattr(configuration, "nbState"):
+            if ljParameters is None:
+                raise CLibraryError("Unable to create NB state: LJ parameters are required.")
+            fixed = None
+            if fixedAtoms is not None:
+                fixed = np.zeros(mmAtoms.size, np.uint8)
+                fixed[np.asarray(list(fixedAtoms), dtype=np.int64)] = 1
+            lj14 = ljParameters14 if ljParameters14 is not None else ljParameters
+            native = _native.NativeState(self.NativeOptions(), mmAtoms.charges, mmAtoms.ljtypes, ljParameters.ntypes,
+                                         ljParameters.tableA, ljParameters.tableB, lj14.tableA, lj14.tableB,
+                                         None if exclusions is None else exclusions.pairs,
+                                         None if interactions14 is None else interactions14.pairs, fixed)
+            configuration.nbState = NBModelABFSState(native, mmAtoms)
+        nbState = configuration.nbState
+        xyz = configuration.coordinates3
+        grad = getattr(configuration, "gradients3", None)
+        e, present, rebuilt = nbState.native.energy_host(xyz, grad)
+        nbState.emmel, nbState.emmlj, nbState.emmel14, nbState.emmlj14 = (float(x) for x in e)
+        nbState.present = (bool(present[0]), bool(present[1]))
+        nbState.lastRebuilt = rebuilt
+        if rebuilt:
+            nbState.Summary(log=log)
+
+    def Energy(self, configuration):
+        """Energy terms of the last set-up call (pyx:108-122)."""
+        energies = []
+        if hasattr(configuration, "nbState"):
+            configuration.nbState.GetEnergies(energies)
+        return energies
+
+    def QCMMGradients(self, configuration):
+        pass
+
+    def QCMMPotentials(self, configuration):
+        pass
+
+    def Summary(self, log=logFile):
+        if LogFileActive(log):
+            s = log.GetSummary()
+            s.Start("ABFS NB Model Summary")
+            s.Entry("Dielectric", "{:.6f}".format(self.dielectric))
+            s.Entry("El. 1-4 Scaling", "{:.6f}".format(self.electrostaticScale14))
+            s.Entry("Damping Cutoff", "{:.6f}".format(self.dampingCutoff))
+            s.Entry("Inner Cutoff", "{:.6f}".format(self.innerCutoff))
+            s.Entry("List Cutoff", "{:.6f}".format(self.listCutoff))
+            s.Entry("Outer Cutoff", "{:.6f}".format(self.outerCutoff))
+            s.Entry("QC/MM Coupling", self.qcmmCoupling)
+            s.Entry("Use Centering", "{!r}".format(self.useCentering))
+            s.Entry("Precision", self.precision)
+            s.Stop()
+            if self.generator is not None:
+                self.generator.Summary(log=log)

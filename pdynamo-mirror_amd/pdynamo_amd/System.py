@@ -1,0 +1,216 @@
+"""``System`` / ``Configuration`` / ``EnergyTerms`` for the ABFS path.
+
+Mirrors the parts of pDynamo's object model the path goes through:
+
+* ``System.DefineNBModel`` -- pMolecule-1.9.0/pMolecule/System.py:182-197
+* ``System.Energy``        -- pMolecule-1.9.0/pMolecule/System.py:272-318
+* ``Configuration``        -- pMolecule-1.9.0/pMolecule/Configuration.py (permanent
+                              coordinates3, persistent nbState, temporary rest)
+* ``EnergyTerms``          -- pMolecule-1.9.0/pMolecule/EnergyTerms.py:13-69
+
+The MM model (typing, bonded terms) is not built here: a system is defined by
+coordinates plus the MM parameters the non-bonding model consumes.
+"""
+
+import math
+
+import numpy as np
+
+from .LogFile import LogFileActive, logFile
+from .NBModelABFS import NBModelABFS
+from .Parameters import MMAtomContainer, SelfPairList
+
+
+class EnergyTerms:
+    """Named potential-energy terms and their total."""
+
+    def __init__(self):
+        self._grms = None
+        self._terms = []
+        self._total = 0.0
+
+    def Append(self, term):
+        if isinstance(term, tuple) and len(term) == 2 and isinstance(term[0], str) and isinstance(term[1], float):
+            self._terms.append(term)
+
+    def Extend(self, terms):
+        for t in terms:
+            self.Append(t)
+
+    def RMSGradient(self, value):
+        self._grms = value if isinstance(value, float) else None
+
+    def Terms(self, asDictionary=False):
+        if asDictionary:
+            return {k: v for k, v in self._terms}
+        return self._terms
+
+    def Total(self):
+        self._total = 0.0
+        for _, v in self._terms:
+            self._total += v
+
+    @property
+    def potentialEnergy(self):
+        return self._total
+
+    @property
+    def rmsGradient(self):
+        return self._grms
+
+    def Summary(self, log=logFile):
+        if LogFileActive(log):
+            s = log.GetSummary(pageWidth=90, valueWidth=16)
+            s.Start("Summary of Energy Terms")
+            s.Entry("Potential Energy", "None" if self._total is None else "{:16.4f}".format(self._total))
+            s.Entry("RMS Gradient", "None" if self._grms is None else "{:16.4f}".format(self._grms))
+            for name, value in self._terms:
+                s.Entry(name, "{:16.4f}".format(value))
+            s.Stop()
+
+
+class Configuration:
+    """Permanent coordinates3; everything else is persistent or temporary."""
+
+    permanentAttributes = ("coordinates3", "symmetryParameters")
+
+    def __init__(self):
+        self.temporaryAttributes = set()
+
+    def __getstate__(self):
+        return {k: getattr(self, k) for k in self.permanentAttributes if getattr(self, k, None) is not None}
+
+    def __setstate__(self, state):
+        self.__init__()
+        self.__dict__.update(state)
+
+    def Clear(self):
+        for key in list(self.__dict__.keys()):
+            if key not in self.permanentAttributes and key != "temporaryAttributes":
+                delattr(self, key)
+        self.temporaryAttributes.clear()
+
+    def ClearTemporaryAttributes(self):
+        for key in self.temporaryAttributes:
+            if hasattr(self, key):
+                delattr(self, key)
+        self.temporaryAttributes.clear()
+
+    def SetTemporaryAttribute(self, label, value):
+        setattr(self, label, value)
+        self.temporaryAttributes.add(label)
+
+
+class EnergyModel:
+    """The MM data the NB model consumes plus the NB model itself."""
+
+    def __init__(self):
+        self.nbModel = None
+        self.mmAtoms = None
+        self.qcAtoms = None
+        self.ljParameters = None
+        self.ljParameters14 = None
+        self.exclusions = None
+        self.interactions14 = None
+        self.electrostaticScale14 = None
+
+    def ClearNBModel(self, configuration):
+        if self.nbModel is not None:
+            self.nbModel.Clear(configuration)
+        self.nbModel = None
+
+
+class System:
+    """A set of atoms with MM non-bonding parameters and a configuration."""
+
+    def __init__(self, natoms=0, label=""):
+        self.label = label
+        self.natoms = int(natoms)
+        self.configuration = Configuration()
+        self.energyModel = None
+        self.fixedAtoms = None
+        self.symmetry = None
+
+    @classmethod
+    def FromMMParameters(cls, coordinates3, mmAtoms, ljParameters, exclusions=None, interactions14=None,
+                         ljParameters14=None, electrostaticScale14=None, fixedAtoms=None, label=""):
+        """A system from coordinates and MM parameters (what MMModel.BuildModel produces in pDynamo)."""
+        xyz = np.ascontiguousarray(coordinates3, dtype=np.float64)
+        if xyz.ndim != 2 or xyz.shape[1] != 3 or xyz.shape[0] != mmAtoms.size:
+            raise ValueError("coordinates3 must be (natoms, 3)")
+        self = cls(mmAtoms.size, label)
+        self.configuration.coordinates3 = xyz.copy()
+        em = EnergyModel()
+        em.mmAtoms = mmAtoms
+        em.ljParameters = ljParameters
+        em.ljParameters14 = ljParameters14
+        em.exclusions = exclusions if (exclusions is None or isinstance(exclusions, SelfPairList)) else SelfPairList(exclusions, mmAtoms.size)
+        em.interactions14 = interactions14 if (interactions14 is None or isinstance(interactions14, SelfPairList)) else SelfPairList(interactions14, mmAtoms.size)
+        em.electrostaticScale14 = electrostaticScale14
+        self.energyModel = em
+        self.fixedAtoms = None if fixedAtoms is None else sorted(int(i) for i in fixedAtoms)
+        return self
+
+    @classmethod
+    def FromSynthetic(cls, mmsystem, fixedAtoms=None):
+        """Wrap a pdynamo_amd.synthetic.MMSystem."""
+        from .Parameters import LJParameterContainer_FromTableCoefficients
+        lj = LJParameterContainer_FromTableCoefficients(mmsystem.ntypes, mmsystem.tableA, mmsystem.tableB)
+        return cls.FromMMParameters(mmsystem.coordinates3, MMAtomContainer(mmsystem.charges, mmsystem.ljtypes), lj,
+                                    exclusions=mmsystem.exclusions, interactions14=mmsystem.interactions14,
+                                    fixedAtoms=fixedAtoms, label=mmsystem.name)
+
+    @property
+    def coordinates3(self):
+        return getattr(self.configuration, "coordinates3", None)
+
+    @coordinates3.setter
+    def coordinates3(self, value):
+        xyz = np.ascontiguousarray(value, dtype=np.float64)
+        if xyz.shape != (self.natoms, 3):
+            raise ValueError("coordinates3 must be (natoms, 3)")
+        self.configuration.coordinates3 = xyz.copy()
+
+    @property
+    def nbModel(self):
+        return None if self.energyModel is None else self.energyModel.nbModel
+
+    def DefineNBModel(self, nbModel):
+        """Define a non-bonding model (System.py:182-197)."""
+        if isinstance(nbModel, NBModelABFS):
+            if self.energyModel is None:
+                self.energyModel = EnergyModel()
+            else:
+                self.energyModel.ClearNBModel(self.configuration)
+            em = self.energyModel
+            em.nbModel = nbModel
+            if em.electrostaticScale14 is not None:
+                try:
+                    em.nbModel.SetOptions(electrostaticScale14=float(em.electrostaticScale14))
+                except Exception:
+                    pass
+
+    def Energy(self, log=logFile, doGradients=False):
+        """Energy and, optionally, gradients (System.py:272-318, MM non-bonding part)."""
+        if self.coordinates3 is None or self.energyModel is None:
+            return None
+        cfg = self.configuration
+        cfg.ClearTemporaryAttributes()
+        cfg.SetTemporaryAttribute("energyTerms", EnergyTerms())
+        if doGradients:
+            cfg.SetTemporaryAttribute("gradients3", np.zeros((self.natoms, 3)))
+        em = self.energyModel
+        et = cfg.energyTerms
+        if em.nbModel is not None:
+            em.nbModel.SetUp(em.mmAtoms, em.qcAtoms, em.ljParameters, em.ljParameters14, self.fixedAtoms,
+                             em.interactions14, em.exclusions, self.symmetry, None, cfg, log=log)
+            et.Extend(em.nbModel.Energy(cfg))
+        et.Total()
+        grd = getattr(cfg, "gradients3", None)
+        if self.fixedAtoms is not None and doGradients:
+            grd[np.asarray(self.fixedAtoms, dtype=np.int64)] = 0.0
+        if LogFileActive(log):
+            if doGradients:
+                et.RMSGradient(float(math.sqrt(np.mean(grd * grd))) if grd.size else 0.0)
+            et.Summary(log=log)
+        return et.potentialEnergy

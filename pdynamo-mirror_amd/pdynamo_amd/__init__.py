@@ -1,0 +1,22 @@
+"""pdynamo_amd -- pDynamo's NBModelABFS MM/MM energy + gradient and its cell-grid
+pair list, rebuilt for AMD Instinct MI355X (gfx950).
+
+Drop-in names: ``System``, ``NBModelABFS``, ``NBModelABFSState``,
+``PairListGenerator``, ``LJParameterContainer`` (+ ``_FromEpsilonSigma`` /
+``_FromTableCoefficients``), ``MMAtomContainer``, ``SelfPairList``,
+``EnergyTerms``.  Batches of images / replicas: ``BatchEnergy``.
+"""
+
+from ._native import CLibraryError
+from .LogFile import LogFileActive, TextLogFile, logFile
+from .NBModelABFS import NBModelABFS, NBModelABFSState, PairListGenerator
+from .Parameters import (LJParameterContainer, LJParameterContainer_FromEpsilonSigma,
+                         LJParameterContainer_FromTableCoefficients, MMAtomContainer, SelfPairList,
+                         SelfPairList_FromIntegerPairs, UNITS_ENERGY_E2ANGSTROMS_TO_KILOJOULES_PER_MOLE)
+from .System import Configuration, EnergyTerms, System
+
+__all__ = ["CLibraryError", "LogFileActive", "TextLogFile", "logFile", "NBModelABFS", "NBModelABFSState",
+           "PairListGenerator", "LJParameterContainer", "LJParameterContainer_FromEpsilonSigma",
+           "LJParameterContainer_FromTableCoefficients", "MMAtomContainer", "SelfPairList",
+           "SelfPairList_FromIntegerPairs", "UNITS_ENERGY_E2ANGSTROMS_TO_KILOJOULES_PER_MOLE", "Configuration",
+           "EnergyTerms", "System"]

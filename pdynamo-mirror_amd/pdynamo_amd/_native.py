@@ -1,0 +1,275 @@
+"""ctypes binding of libnbabfs.so (the C ABI declared in include/nbabfs.h).
+
+The library is the only compute path: there is no CPU fallback.  Loading it on
+a machine without a gfx950 GPU succeeds (so the symbol table can be checked),
+but every entry point that needs the device raises ``CLibraryError``.
+"""
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .build import LIB
+
+STATUS_CONTINUE = 16
+MIXED, PLAIN_FP64 = 0, 1
+
+# Every symbol include/nbabfs.h declares.
+EXPORTED = ["nbabfs_default_options", "nbabfs_init", "nbabfs_last_error", "nbabfs_device_cu_count",
+            "nbabfs_state_create", "nbabfs_state_destroy", "nbabfs_set_stream", "nbabfs_energy_host",
+            "nbabfs_update", "nbabfs_mmmm_energy_async", "nbabfs_energies", "nbabfs_stats_get",
+            "nbabfs_set_timing", "nbabfs_export_pairs", "nbabfs_batch_create", "nbabfs_batch_destroy",
+            "nbabfs_batch_energy", "nbabfs_device_alloc", "nbabfs_device_free", "nbabfs_memcpy_htod",
+            "nbabfs_memcpy_dtoh", "nbabfs_zero_async", "nbabfs_synchronize"]
+
+
+class CLibraryError(Exception):
+    """Raised when the native library reports a failure (pCore.CLibraryError)."""
+
+
+class Options(C.Structure):
+    _fields_ = [("dampingCutoff", C.c_double), ("innerCutoff", C.c_double), ("outerCutoff", C.c_double),
+                ("listCutoff", C.c_double), ("dielectric", C.c_double), ("electrostaticScale14", C.c_double),
+                ("cutoffCellSizeFactor", C.c_double), ("minimumCellExtent", C.c_int), ("minimumPoints", C.c_int),
+                ("precision", C.c_int), ("entriesPerItem", C.c_int), ("wavesPerCU", C.c_int),
+                ("reserved", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("numberOfCalls", C.c_longlong), ("numberOfUpdates", C.c_longlong),
+                ("numberOfMMMMPairs", C.c_double), ("npairs", C.c_longlong), ("npairs14", C.c_longlong),
+                ("useGridSearch", C.c_int), ("clusters", C.c_int), ("entries", C.c_longlong),
+                ("workItems", C.c_longlong), ("forceKernelMs", C.c_double), ("forceKernelLaunches", C.c_longlong)]
+
+
+_lib = None
+
+
+def library_path():
+    return LIB
+
+
+def load():
+    """Load libnbabfs.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        raise CLibraryError("libnbabfs.so is not built (run __graft_entry__.build() or python -m pdynamo_amd.build)")
+    L = C.CDLL(LIB)
+    vp = C.c_void_p
+    L.nbabfs_default_options.argtypes = [C.POINTER(Options)]
+    L.nbabfs_default_options.restype = None
+    L.nbabfs_init.restype = C.c_int
+    L.nbabfs_last_error.restype = C.c_char_p
+    L.nbabfs_device_cu_count.restype = C.c_int
+    L.nbabfs_state_create.argtypes = [C.POINTER(Options), C.c_int, vp, vp, C.c_int, vp, vp, vp, vp, C.c_int, vp,
+                                      C.c_int, vp, vp, C.POINTER(vp)]
+    L.nbabfs_state_destroy.argtypes = [vp]
+    L.nbabfs_state_destroy.restype = None
+    L.nbabfs_set_stream.argtypes = [vp, vp]
+    L.nbabfs_energy_host.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.nbabfs_update.argtypes = [vp, vp, vp]
+    L.nbabfs_mmmm_energy_async.argtypes = [vp, vp, vp]
+    L.nbabfs_energies.argtypes = [vp, vp, vp]
+    L.nbabfs_stats_get.argtypes = [vp, C.POINTER(Stats)]
+    L.nbabfs_set_timing.argtypes = [vp, C.c_int]
+    L.nbabfs_export_pairs.argtypes = [vp, C.c_int, vp, C.c_longlong]
+    L.nbabfs_export_pairs.restype = C.c_longlong
+    L.nbabfs_batch_create.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
+    L.nbabfs_batch_destroy.argtypes = [vp]
+    L.nbabfs_batch_destroy.restype = None
+    L.nbabfs_batch_energy.argtypes = [vp, vp, vp, vp, vp]
+    L.nbabfs_device_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
+    L.nbabfs_device_free.argtypes = [vp]
+    L.nbabfs_memcpy_htod.argtypes = [vp, vp, vp, C.c_size_t]
+    L.nbabfs_memcpy_dtoh.argtypes = [vp, vp, vp, C.c_size_t]
+    L.nbabfs_zero_async.argtypes = [vp, vp, C.c_size_t]
+    L.nbabfs_synchronize.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != STATUS_CONTINUE:
+        msg = load().nbabfs_last_error()
+        raise CLibraryError("libnbabfs status %d: %s" % (status, msg.decode() if msg else ""))
+
+
+def init():
+    check(load().nbabfs_init())
+
+
+def default_options():
+    o = Options()
+    load().nbabfs_default_options(C.byref(o))
+    return o
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class DeviceArray:
+    """A float64 device buffer owned through libnbabfs (one HIP runtime in the process)."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(x) for x in shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        p = C.c_void_p()
+        check(load().nbabfs_device_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a, state=None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.shape)
+        d.upload(a, state)
+        return d
+
+    def upload(self, a, state=None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        check(load().nbabfs_memcpy_htod(None if state is None else state.h, self.ptr, a.ctypes.data, self.nbytes))
+
+    def numpy(self, state=None):
+        out = np.empty(self.shape)
+        check(load().nbabfs_memcpy_dtoh(None if state is None else state.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def zero(self, state=None):
+        check(load().nbabfs_zero_async(None if state is None else state.h, self.ptr, self.nbytes))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            load().nbabfs_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize(state=None):
+    check(load().nbabfs_synchronize(None if state is None else state.h))
+
+
+class NativeState:
+    """Owns one nbabfs_state (the NBModelABFSState of one configuration)."""
+
+    def __init__(self, options, charges, ljtypes, ntypes, tableA, tableB, tableA14=None, tableB14=None,
+                 exclusions=None, interactions14=None, fixed=None):
+        L = load()
+        init()
+        self.n = int(len(charges))
+        q = np.ascontiguousarray(charges, dtype=np.float64)
+        t = np.ascontiguousarray(ljtypes, dtype=np.int32)
+        tA = np.ascontiguousarray(tableA, dtype=np.float64)
+        tB = np.ascontiguousarray(tableB, dtype=np.float64)
+        tA14 = None if tableA14 is None else np.ascontiguousarray(tableA14, dtype=np.float64)
+        tB14 = None if tableB14 is None else np.ascontiguousarray(tableB14, dtype=np.float64)
+        ex = np.zeros((0, 2), np.int32) if exclusions is None else np.ascontiguousarray(exclusions, np.int32).reshape(-1, 2)
+        p14 = np.zeros((0, 2), np.int32) if interactions14 is None else np.ascontiguousarray(interactions14, np.int32).reshape(-1, 2)
+        fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        h = C.c_void_p()
+        check(L.nbabfs_state_create(C.byref(options), self.n, _ptr(q), _ptr(t), int(ntypes), _ptr(tA), _ptr(tB),
+                                    _ptr(tA14), _ptr(tB14), ex.shape[0], _ptr(ex) if ex.size else None,
+                                    p14.shape[0], _ptr(p14) if p14.size else None, _ptr(fx), C.byref(h)))
+        self.h = h.value
+        self.options = options
+
+    def energy_host(self, xyz, grad=None):
+        """One Initialize + Update + MMMMEnergy on host arrays; grad is incremented."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        if xyz.shape != (self.n, 3):
+            raise ValueError("coordinates must be (%d, 3)" % self.n)
+        if grad is not None and (grad.dtype != np.float64 or not grad.flags.c_contiguous or grad.shape != (self.n, 3)):
+            raise ValueError("gradients must be a C-contiguous (%d, 3) float64 array" % self.n)
+        e = np.zeros(4)
+        present = np.zeros(2, np.int32)
+        rebuilt = C.c_int()
+        check(load().nbabfs_energy_host(self.h, _ptr(xyz), _ptr(grad), _ptr(e), _ptr(present), C.byref(rebuilt)))
+        return e, present, bool(rebuilt.value)
+
+    def update(self, d_xyz):
+        rebuilt = C.c_int()
+        check(load().nbabfs_update(self.h, C.c_void_p(d_xyz), C.byref(rebuilt)))
+        return bool(rebuilt.value)
+
+    def energy_async(self, d_xyz, d_grad=None):
+        check(load().nbabfs_mmmm_energy_async(self.h, C.c_void_p(d_xyz), None if d_grad is None else C.c_void_p(d_grad)))
+
+    def energies(self):
+        e = np.zeros(4)
+        present = np.zeros(2, np.int32)
+        check(load().nbabfs_energies(self.h, _ptr(e), _ptr(present)))
+        return e, present
+
+    def set_stream(self, stream_handle):
+        check(load().nbabfs_set_stream(self.h, None if not stream_handle else C.c_void_p(stream_handle)))
+
+    def stats(self):
+        s = Stats()
+        check(load().nbabfs_stats_get(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def set_timing(self, enable):
+        check(load().nbabfs_set_timing(self.h, 1 if enable else 0))
+
+    def export_pairs(self, which=0):
+        L = load()
+        m = L.nbabfs_export_pairs(self.h, which, None, 0)
+        if m < 0:
+            check(int(-m))
+        keys = np.zeros(max(m, 1), np.int64)
+        m2 = L.nbabfs_export_pairs(self.h, which, _ptr(keys), m)
+        if m2 < 0:
+            check(int(-m2))
+        return keys[:m]
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().nbabfs_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class NativeBatch:
+    """One persistent launch over many states (images / replicas)."""
+
+    def __init__(self, states, use_graph=True):
+        L = load()
+        self.states = list(states)
+        arr = (C.c_void_p * len(self.states))(*[s.h for s in self.states])
+        h = C.c_void_p()
+        check(L.nbabfs_batch_create(arr, len(self.states), 1 if use_graph else 0, C.byref(h)))
+        self.h = h.value
+
+    def energy(self, d_xyz_ptrs, d_grad_ptrs=None):
+        ns = len(self.states)
+        xa = (C.c_void_p * ns)(*d_xyz_ptrs)
+        ga = None if d_grad_ptrs is None else (C.c_void_p * ns)(*d_grad_ptrs)
+        e = np.zeros(4 * ns)
+        rebuilt = np.zeros(ns, np.int32)
+        check(load().nbabfs_batch_energy(self.h, xa, ga, _ptr(e), _ptr(rebuilt)))
+        return e.reshape(ns, 4), rebuilt.astype(bool)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().nbabfs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,173 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle.
+
+Tolerances: mixed precision -- relative total energy <= 1e-6 and relative RMS
+gradient <= 1e-5 (per term: |dE| <= 1e-6 * |E_total|); plain FP64 -- <= 1e-12.
+Pair sets are compared bit for bit as sorted canonical keys max*n+min.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(mm, **opts):
+    from pdynamo_amd import NBModelABFS, System
+    s = System.FromSynthetic(mm)
+    s.DefineNBModel(NBModelABFS(**opts))
+    return s
+
+
+def _run(s, xyz=None):
+    if xyz is not None:
+        s.coordinates3 = xyz
+    e = s.Energy(log=None, doGradients=True)
+    terms = s.configuration.energyTerms.Terms(asDictionary=True)
+    return e, terms, s.configuration.gradients3.copy()
+
+
+def _check(terms, grad, rterms, rgrad, etol, gtol):
+    rterms = dict(rterms)
+    assert set(terms) == set(rterms)
+    etot = sum(rterms.values())
+    scale = max(abs(etot), 1.0)
+    assert abs(sum(terms.values()) - etot) <= etol * scale
+    for k in rterms:
+        assert abs(terms[k] - rterms[k]) <= etol * scale, (k, terms[k], rterms[k])
+    rel = np.linalg.norm(grad - rgrad) / max(np.linalg.norm(rgrad), 1e-300)
+    assert rel <= gtol, rel
+
+
+@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("fp64", 1e-12, 1e-12)])
+def test_water648(oracle_mod, precision, etol, gtol):
+    from pdynamo_amd import synthetic
+    mm = synthetic.config0()
+    s = _system(mm, precision=precision)
+    _, terms, grad = _run(s)
+    o = oracle_mod.OracleABFS(mm)
+    rterms, rgrad, _ = o.energy(mm.coordinates3)
+    assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
+    _check(terms, grad, rterms, rgrad, etol, gtol)
+
+
+@pytest.mark.parametrize("natoms,box,seed", [(300, 30.0, 1), (900, 45.0, 2), (2000, 60.0, 3), (7, 10.0, 4)])
+@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("fp64", 1e-12, 1e-12)])
+def test_random_mixtures(oracle_mod, natoms, box, seed, precision, etol, gtol):
+    from pdynamo_amd import synthetic
+    mm = synthetic.random_small_system(natoms, box, seed)
+    s = _system(mm, precision=precision)
+    _, terms, grad = _run(s)
+    o = oracle_mod.OracleABFS(mm)
+    rterms, rgrad, _ = o.energy(mm.coordinates3)
+    assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
+    _check(terms, grad, rterms, rgrad, etol, gtol)
+
+
+def test_options_14_dielectric(oracle_mod):
+    """1-4 list with its own LJ table, scale14 and a dielectric (NBModelABFS.c:228-301)."""
+    from pdynamo_amd import (LJParameterContainer_FromTableCoefficients, MMAtomContainer, NBModelABFS, System,
+                             synthetic)
+    mm = synthetic.random_small_system(1200, 50.0, 11)
+    rng = np.random.default_rng(5)
+    i = rng.integers(10, mm.natoms, 300)
+    p14 = np.stack([i, i - rng.integers(3, 9, 300)], 1)
+    mm.interactions14 = p14.astype(np.int32)
+    a14, b14 = mm.tableA * 0.5, mm.tableB * 0.5
+    lj = LJParameterContainer_FromTableCoefficients(mm.ntypes, mm.tableA, mm.tableB)
+    lj14 = LJParameterContainer_FromTableCoefficients(mm.ntypes, a14, b14)
+    s = System.FromMMParameters(mm.coordinates3, MMAtomContainer(mm.charges, mm.ljtypes), lj, mm.exclusions,
+                                mm.interactions14, lj14)
+    s.DefineNBModel(NBModelABFS(dielectric=2.5, electrostaticScale14=0.5, innerCutoff=7.0, outerCutoff=11.0,
+                                listCutoff=12.5, dampingCutoff=0.75))
+    _, terms, grad = _run(s)
+    o = oracle_mod.OracleABFS(mm, damping=0.75, inner=7.0, outer=11.0, list_cutoff=12.5, dielectric=2.5,
+                              scale14=0.5, tableA14=a14, tableB14=b14)
+    rterms, rgrad, _ = o.energy(mm.coordinates3)
+    assert "MM/MM 1-4 Elect." in terms
+    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)
+
+
+def test_rebuild_sequence(oracle_mod):
+    """Rebuild decisions of the half-skin rule and energies over moving coordinates."""
+    from pdynamo_amd import synthetic
+    mm = synthetic.water_lattice(1000, 31.0, seed=7)
+    seq = synthetic.displacement_sequence(mm, 8, 0.2, seed=3)
+    s = _system(mm)
+    o = oracle_mod.OracleABFS(mm)
+    flags = []
+    for xyz in [mm.coordinates3] + seq:
+        _, terms, grad = _run(s, xyz)
+        rterms, rgrad, rb = o.energy(xyz)
+        flags.append((s.configuration.nbState.lastRebuilt, rb))
+        _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)
+        assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
+    assert all(a == b for a, b in flags), flags
+    assert sum(b for _, b in flags) >= 2
+
+
+def test_deterministic_repeat():
+    """Fixed-point accumulation: identical inputs give bit-identical results."""
+    from pdynamo_amd import synthetic
+    mm = synthetic.random_small_system(3000, 60.0, 21)
+    s = _system(mm)
+    e1, t1, g1 = _run(s)
+    e2, t2, g2 = _run(s)
+    assert e1 == e2
+    assert np.array_equal(g1, g2)
+
+
+@pytest.mark.parametrize("factor", [0.25, 0.5, 1.0, 2.75])
+def test_cell_size_invariance(oracle_mod, factor):
+    """GridUpdating.py analogue: any cell size gives the reference set for that generator."""
+    from pdynamo_amd import NBModelABFS, PairListGenerator, System, synthetic
+    mm = synthetic.water_lattice(900, 28.0, seed=9)
+    gen = PairListGenerator(cutoff=13.5, cutoffCellSizeFactor=factor, useGridByCell=True, sortIndices=False,
+                            minimumPoints=500, minimumCellExtent=2)
+    s = System.FromSynthetic(mm)
+    s.DefineNBModel(NBModelABFS(generator=gen))
+    _, terms, grad = _run(s)
+    o = oracle_mod.OracleABFS(mm, cell_factor=factor)
+    rterms, rgrad, _ = o.energy(mm.coordinates3)
+    assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
+    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)
+
+
+def test_solute30000(oracle_mod):
+    """The 30,000-atom solute-in-water system (damped-region pairs included)."""
+    from pdynamo_amd import synthetic
+    mm = synthetic.config1()
+    s = _system(mm)
+    _, terms, grad = _run(s)
+    o = oracle_mod.OracleABFS(mm, nthreads=8)
+    rterms, rgrad, _ = o.energy(mm.coordinates3)
+    assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
+    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)
+
+
+def test_batch_matches_single():
+    from pdynamo_amd import _native, synthetic
+    from pdynamo_amd.NBModelABFS import NBModelABFS
+    systems = [synthetic.random_small_system(n, b, sd) for n, b, sd in [(600, 35.0, 1), (2500, 55.0, 2), (1200, 40.0, 3)]]
+    model = NBModelABFS()
+    singles, batch_states = [], []
+    for mm in systems:
+        for bucket in (singles, batch_states):
+            bucket.append(_native.NativeState(model.NativeOptions(), mm.charges, mm.ljtypes, mm.ntypes, mm.tableA,
+                                              mm.tableB, exclusions=mm.exclusions))
+    ref = []
+    for st, mm in zip(singles, systems):
+        g = np.zeros((mm.natoms, 3))
+        e, _, _ = st.energy_host(mm.coordinates3, g)
+        ref.append((e, g))
+    xs = [_native.DeviceArray.from_numpy(mm.coordinates3) for mm in systems]
+    gs = [_native.DeviceArray(x.shape) for x in xs]
+    batch = _native.NativeBatch(batch_states, use_graph=True)
+    for rep in range(3):
+        for g in gs:
+            g.zero()
+        e, rebuilt = batch.energy([x.ptr for x in xs], [g.ptr for g in gs])
+        assert rebuilt.all() == (rep == 0)
+        for k, (er, gr) in enumerate(ref):
+            assert np.array_equal(e[k], er)
+            assert np.array_equal(gs[k].numpy(), gr)
+    batch.close()
