@@ -27,7 +27,7 @@ import oracle as O  # noqa: E402
 from pdynamo_amd import synthetic as S  # noqa: E402
 
 
-def run_case(name, mm, coords_seq, opts=None, lj14=None):
+def run_case(name, mm, coords_seq, opts=None, lj14=None, grad_calls=None):
     opts = dict(opts or {})
     kw = dict(damping=opts.get("damping", 0.5), inner=opts.get("inner", 8.0), outer=opts.get("outer", 12.0),
               list_cutoff=opts.get("list_cutoff", 13.5), dielectric=opts.get("dielectric", 1.0),
@@ -55,7 +55,8 @@ def run_case(name, mm, coords_seq, opts=None, lj14=None):
                options=np.array([kw["damping"], kw["inner"], kw["outer"], kw["list_cutoff"], kw["dielectric"],
                                  kw["scale14"], kw["cell_factor"]]),
                energies=np.array(energies), present=np.array(present), rebuilt=np.array(rebuilt),
-               grads=np.array(grads), npairs=np.array(npairs), use_grid=use_grid,
+               grads=np.array([grads[k] for k in (grad_calls or range(len(grads)))]),
+               grad_calls=np.array(list(grad_calls or range(len(grads)))), npairs=np.array(npairs), use_grid=use_grid,
                keys_first=np.diff(np.concatenate([[0], keys0])).astype(np.uint32) if keys0.size else np.zeros(0, np.uint32),
                keys_base=np.int64(keys0[0]) if keys0.size else np.int64(0))
     if lj14 is not None:
@@ -83,11 +84,15 @@ def main():
              lj14=(m14.tableA * 0.5, m14.tableB * 0.5))
     # Rebuild decisions over successive moves (half-skin rule).
     w = S.water_lattice(1000, 31.0, seed=7)
-    run_case("water3000_moves", w, [w.coordinates3] + S.displacement_sequence(w, 8, 0.2, seed=3))
+    run_case("water3000_moves", w, [w.coordinates3] + S.displacement_sequence(w, 8, 0.08, seed=3),
+             grad_calls=[0, 4, 8])
     # Cell-size invariance (GridUpdating.py analogue).
     wg = S.water_lattice(900, 28.0, seed=9)
     for f in (0.25, 1.0, 2.75):
         run_case("water2700_cell%.2f" % f, wg, [wg.coordinates3], opts=dict(cell_factor=f))
+    # Large cells on a grid: same-cell pairs are distance-tested (offset 0 not fully in range).
+    big = S.random_small_system(1500, 90.0, 13)
+    run_case("random1500_cell2.75", big, [big.coordinates3], opts=dict(cell_factor=2.75))
 
 
 if __name__ == "__main__":

@@ -91,7 +91,7 @@ def test_rebuild_sequence(oracle_mod):
     """Rebuild decisions of the half-skin rule and energies over moving coordinates."""
     from pdynamo_amd import synthetic
     mm = synthetic.water_lattice(1000, 31.0, seed=7)
-    seq = synthetic.displacement_sequence(mm, 8, 0.2, seed=3)
+    seq = synthetic.displacement_sequence(mm, 8, 0.08, seed=3)
     s = _system(mm)
     o = oracle_mod.OracleABFS(mm)
     flags = []
@@ -102,7 +102,7 @@ def test_rebuild_sequence(oracle_mod):
         _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)
         assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
     assert all(a == b for a, b in flags), flags
-    assert sum(b for _, b in flags) >= 2
+    assert 2 <= sum(b for _, b in flags) < len(flags)
 
 
 def test_deterministic_repeat():
