@@ -85,7 +85,10 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--entries-per-item", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=-1)
-    ap.add_argument("--cpu-calls", type=int, default=10)
+    ap.add_argument("--cpu-calls", type=int, default=100)
+    ap.add_argument("--moves", type=float, default=0.0,
+                    help="Gaussian displacement (A) applied before every call, cumulative; rebuilds follow the "
+                         "half-skin rule (the 200,001-atom protocol uses 0.05)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     args = ap.parse_args()
@@ -106,18 +109,28 @@ def main():
     model = NBModelABFS(**opts)
     st = _native.NativeState(model.NativeOptions(), mm.charges, mm.ljtypes, mm.ntypes, mm.tableA, mm.tableB,
                              exclusions=mm.exclusions, interactions14=mm.interactions14)
-    x = _native.DeviceArray.from_numpy(mm.coordinates3, st)
-    g = _native.DeviceArray(x.shape)
+    g = _native.DeviceArray(mm.coordinates3.shape)
+    if args.moves > 0.0:
+        from pdynamo_amd import synthetic
+        seq = synthetic.displacement_sequence(mm, args.warmup + args.steps, args.moves, seed=7)
+        xs = [_native.DeviceArray.from_numpy(c, st) for c in seq]
+    else:
+        xs = [_native.DeviceArray.from_numpy(mm.coordinates3, st)]
     _native.synchronize()
+    counter = [0]
+    rebuilds = [0]
 
     def step():
         # System.Energy zeroes the gradients; the model increments them.
+        x = xs[counter[0] % len(xs)]
+        counter[0] += 1
         g.zero(st)
-        st.update(x.ptr)
+        rebuilds[0] += st.update(x.ptr)
         st.energy_async(x.ptr, g.ptr)
 
     for _ in range(args.warmup):
         step()
+    rebuilds[0] = 0
     _native.synchronize()
     st.set_timing(True)
     _native.synchronize()
@@ -131,6 +144,7 @@ def main():
     ms = (t1 - t0) / args.steps * 1e3
     e, present = st.energies()
     keys = st.export_pairs(0)
+    x_last = xs[(counter[0] - 1) % len(xs)].numpy(st)
     n_list = int(keys.size)
     value = n_list / (ms * 1e-3)
 
@@ -151,10 +165,11 @@ def main():
         "config": {"workload": wdesc, "atoms": mm.natoms, "list_pairs": n_list, "precision": args.precision,
                    "list_cutoff": 13.5, "cutoffs": [0.5, 8.0, 12.0],
                    "energies_kJ_mol": [float(v) for v in e], "work_items": int(stats["workItems"]),
-                   "cluster_entries": int(stats["entries"])},
+                   "cluster_entries": int(stats["entries"]), "moves_sigma_A": args.moves,
+                   "rebuilds_in_timed_steps": int(rebuilds[0])},
     }
     if not args.no_roofline:
-        flops, n_list2, n_plain, n_switch = algorithmic_flops(mm.coordinates3, keys, mm.natoms)
+        flops, n_list2, n_plain, n_switch = algorithmic_flops(x_last, keys, mm.natoms)
         kms = stats["forceKernelMs"]
         achieved = flops / (kms * 1e-3) / 1e12 if kms > 0 else None
         peak = PEAK_FP32_TFLOPS if args.precision == "mixed" else PEAK_FP64_TFLOPS
