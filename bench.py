@@ -52,6 +52,81 @@ def make_workload(name):
     raise SystemExit("unknown workload " + name)
 
 
+def make_batch(name):
+    from pdynamo_amd import synthetic
+    if name == "droplets64":
+        return synthetic.config3(), "C3 64 water droplets, log-uniform 500-20,000 atoms, one batched launch"
+    if name == "images16":
+        return synthetic.config4(), "C4 16 images of the 30,000-atom solute system (0.1 A perturbations), one batched launch"
+    return None, None
+
+
+def run_batch(args, systems, wdesc):
+    """One step = rebuild checks of every system + one batched E+grad launch (hipGraph replay)."""
+    from pdynamo_amd import _native
+    from pdynamo_amd.NBModelABFS import NBModelABFS
+
+    def states_for(precision):
+        model = NBModelABFS(precision=precision, **({"entriesPerItem": args.entries_per_item}
+                                                    if args.entries_per_item else {}))
+        return [_native.NativeState(model.NativeOptions(), m.charges, m.ljtypes, m.ntypes, m.tableA, m.tableB,
+                                    exclusions=m.exclusions) for m in systems]
+
+    sizes = [m.natoms for m in systems]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    allx = np.concatenate([m.coordinates3 for m in systems])
+    X = _native.DeviceArray.from_numpy(allx)
+    G = _native.DeviceArray(allx.shape)
+    xp = [X.ptr + 24 * int(o) for o in offs[:-1]]
+    gp = [G.ptr + 24 * int(o) for o in offs[:-1]]
+
+    states = states_for(args.precision)
+    batch = _native.NativeBatch(states, use_graph=True)
+    for _ in range(args.warmup):
+        G.zero()
+        batch.energy(xp, gp)
+    _native.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        G.zero()
+        e, rebuilt = batch.energy(xp, gp)
+    _native.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    npairs = int(sum(st.stats()["npairs"] for st in states))
+
+    def sequential(precision):
+        # The same systems as separate single-state calls, one after another, on one stream.
+        seq = states_for(precision)
+        for st, x, g in zip(seq, xp, gp):
+            st.update(x)
+            st.energy_async(x, g)
+        _native.synchronize()
+        reps = max(1, args.steps // 5)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            G.zero()
+            for st, x, g in zip(seq, xp, gp):
+                st.update(x)
+                st.energy_async(x, g)
+            _native.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    ms_plain = sequential("fp64")
+    ms_seq_mixed = sequential("mixed")
+    result = {
+        "metric": "pair interactions/s (reference list pairs d<=13.5 A per E+grad second)",
+        "value": npairs / (ms * 1e-3), "unit": "pair interactions/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 pair core + f32 rsqrt seed/force vectors + int64 fixed-point sums",
+        "data": "synthetic (seeded generator, pdynamo_amd.synthetic)",
+        "config": {"workload": wdesc, "systems": len(systems), "atoms": int(offs[-1]), "list_pairs": npairs,
+                   "min_atoms": int(min(sizes)), "max_atoms": int(max(sizes)), "precision": args.precision,
+                   "sequential_plain_fp64_ms": ms_plain, "sequential_mixed_ms": ms_seq_mixed,
+                   "speedup_vs_sequential_plain_fp64": ms_plain / ms},
+    }
+    print(json.dumps(result))
+
+
 def cpu_baseline(mm, calls, threads):
     """The reference pDynamo C (oracle/_ref, -O3 -fopenmp) or, if absent, the
     FP64 C restatement, timed for `calls` E+grad calls after the first build."""
@@ -100,6 +175,10 @@ def main():
     from pdynamo_amd import _native
     from pdynamo_amd.NBModelABFS import NBModelABFS
 
+    systems, bdesc = make_batch(args.workload)
+    if systems is not None:
+        run_batch(args, systems, bdesc)
+        return
     mm, wdesc = make_workload(args.workload)
     opts = {"precision": args.precision}
     if args.entries_per_item:
