@@ -458,26 +458,6 @@ __global__ __launch_bounds__(64) void k_csr_fill(const Entry *__restrict__ entri
     }
 }
 
-// Half-skin rebuild test (CheckForUpdate, pMolecule-1.9.0/extensions/csource/
-// NBModelABFS.c:691-746): any free atom displaced by more than
-// 0.5*(listCutoff - outerCutoff) since the last build.
-__global__ __launch_bounds__(256) void k_check_update(const double *__restrict__ xyz, const double *__restrict__ ref,
-                                                      const unsigned char *__restrict__ free_flags, int n, double bufsq,
-                                                      int *__restrict__ flag)
-{
-#pragma clang fp contract(off)
-    int i = blockIdx.x * 256 + threadIdx.x;
-    bool up = false;
-    if (i < n && (!free_flags || free_flags[i])) {
-        double dx = xyz[3 * (size_t)i] - ref[3 * (size_t)i];
-        double dy = xyz[3 * (size_t)i + 1] - ref[3 * (size_t)i + 1];
-        double dz = xyz[3 * (size_t)i + 2] - ref[3 * (size_t)i + 2];
-        double r2 = dx * dx + dy * dy + dz * dz;
-        up = r2 > bufsq;
-    }
-    if (__any(up) && threadIdx.x % 64 == 0) atomicOr(flag, 1);
-}
-
 // ------------------------------------------------------------------------------------------------
 // Host-side launch wrappers.
 // ------------------------------------------------------------------------------------------------
@@ -591,9 +571,3 @@ void launch_csr_fill(const Entry *entries, int nent, const int *slot_atom, int *
     if (nent > 0) hipLaunchKernelGGL(k_csr_fill, dim3(nent), dim3(64), 0, s, entries, nent, slot_atom, cursor, cols);
 }
 
-void launch_check_update(const double *xyz, const double *ref, const unsigned char *free_flags, int n, double bufsq,
-                         int *flag, hipStream_t s)
-{
-    if (n > 0)
-        hipLaunchKernelGGL(k_check_update, dim3(nblk(n, 256)), dim3(256), 0, s, xyz, ref, free_flags, n, bufsq, flag);
-}

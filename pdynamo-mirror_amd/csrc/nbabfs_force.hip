@@ -15,6 +15,8 @@
 #include "nbabfs_internal.h"
 #include "nbabfs_kernels.h"
 
+#include <algorithm>
+
 static inline int nblk(size_t m, int b) { return (int)((m + b - 1) / b); }
 
 // ------------------------------------------------------------------------------------------------
@@ -102,26 +104,50 @@ __device__ __forceinline__ void pair_tiled(const AbfsFactors &F, double r2, doub
 
 // ------------------------------------------------------------------------------------------------
 // Prepare: slot records (origin-relative FP64 position, scaled charge), slot
-// types; zero the fixed-point accumulators; reset the work queues.
+// types; zero the fixed-point accumulators; reset the work queues.  With
+// chk.ref set it also applies CheckForUpdate as NBModelABFS_Update calls it
+// (pMolecule-1.9.0/extensions/csource/NBModelABFS.c:538-545, 691-746: some
+// free atom moved more than half the buffer since the list was built) and
+// stores chk.epoch into *chk.flag when it holds.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ descs, int tiled, int *__restrict__ heads)
+__global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ descs, int tiled, int *__restrict__ heads,
+                                                 PrepareCheck chk)
 {
     const SysDesc &D = descs[blockIdx.y];
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (heads && blockIdx.y == 0 && blockIdx.x == 0 && threadIdx.x < 8) heads[32 * threadIdx.x] = 0;
-    if (s >= D.n_pad) return;
-    for (int k = 0; k < 6; k++) D.facc[(size_t)k * D.n_pad + s] = 0;
-    if (!tiled) return;
-    const int a = D.slot_atom[s];
-    double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
-    int t = 0;
-    if (a >= 0) {
-        v = make_double4(D.xyz[3 * (size_t)a] - D.origin[0], D.xyz[3 * (size_t)a + 1] - D.origin[1],
-                         D.xyz[3 * (size_t)a + 2] - D.origin[2], D.charge[a] * D.qscale);
-        t = D.atype[a];
+    if (blockIdx.x == 0 && D.eacc)
+        for (int k = threadIdx.x; k < NB_EACC_WORDS; k += 256) D.eacc[k] = 0;
+    bool up = false;
+    if (s < D.n_pad) {
+        for (int k = 0; k < 6; k++) D.facc[(size_t)k * D.n_pad + s] = 0;
+        const double *xyz = chk.xyz ? chk.xyz : D.xyz;
+        const int a = D.slot_atom[s];
+        double x = 0.0, y = 0.0, z = 0.0;
+        if (a >= 0) {
+            x = xyz[3 * (size_t)a];
+            y = xyz[3 * (size_t)a + 1];
+            z = xyz[3 * (size_t)a + 2];
+            if (chk.ref && (!chk.free_flags || chk.free_flags[a])) {
+#pragma clang fp contract(off)
+                const double dx = x - chk.ref[3 * (size_t)a];
+                const double dy = y - chk.ref[3 * (size_t)a + 1];
+                const double dz = z - chk.ref[3 * (size_t)a + 2];
+                up = dx * dx + dy * dy + dz * dz > chk.bufsq;
+            }
+        }
+        if (tiled) {
+            double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+            int t = 0;
+            if (a >= 0) {
+                v = make_double4(x - D.origin[0], y - D.origin[1], z - D.origin[2], D.charge[a] * D.qscale);
+                t = D.atype[a];
+            }
+            D.xyzq[s] = v;
+            D.styp[s] = t;
+        }
     }
-    D.xyzq[s] = v;
-    D.styp[s] = t;
+    if (__any(up) && (threadIdx.x & 63) == 0) *(volatile int *)chk.flag = chk.epoch;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -151,16 +177,46 @@ __device__ __forceinline__ int entry_bits(const GAS unsigned char *__restrict__ 
     return eb[(size_t)e * sizeof(Entry) + 16 + lane];
 }
 
+// j-force partials (gx, gy, gz) of lane 8a+b are summed over a by a
+// reduce-scatter: after three exchanges lanes a = 0, 2, 4 hold the x, y and z
+// sums of j-atom b, so one fixed-point conversion serves all three components.
+__device__ __forceinline__ float jforce_reduce_scatter(int a, float gx, float gy, float gz)
+{
+    const bool lo4 = a < 4;
+    const float r1 = __shfl_xor(lo4 ? gz : gx, 32, 64);
+    const float r2 = __shfl_xor(gy, 32, 64);
+    const float k1 = (lo4 ? gx : gz) + r1;   // a < 4: x; a >= 4: z
+    const float k2 = gy + r2;                // a < 4: y
+    const bool hi2 = (a & 2) != 0;
+    const float snd = lo4 ? (hi2 ? k1 : k2) : k1;
+    float kp = lo4 ? (hi2 ? k2 : k1) : k1;
+    kp += __shfl_xor(snd, 16, 64);
+    kp += __shfl_xor(kp, 8, 64);
+    return kp;
+}
+
+// LDS of the tiled kernel.  The i-force transpose at the end of an item
+// reuses the space of the item's i-cluster records and LJ table.
+template <bool LDS_LJ>
+union TiledShared {
+    struct {
+        double4 sI[NB_SC];
+        int sT[NB_SC];
+        double2 sLJ[LDS_LJ ? NB_LDS_TYPES * NB_LDS_TYPES : 1];
+    } in;
+    float sF[NB_SCC * 3 * NB_SC];   // [c][component][lane]
+};
+
 template <bool LDS_LJ>
 __global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ descs,
                                                     const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F)
 {
     const int lane = threadIdx.x;
     const int a = lane >> 3, b = lane & 7;
-    __shared__ double4 sI[NB_SC];
-    __shared__ int sT[NB_SC];
-    __shared__ double2 sLJ[LDS_LJ ? NB_LDS_TYPES * NB_LDS_TYPES : 1];
-    int loaded_sys = -1;
+    __shared__ TiledShared<LDS_LJ> sh;
+    double4 *sI = sh.in.sI;
+    int *sT = sh.in.sT;
+    double2 *sLJ = sh.in.sLJ;
     int q = blockIdx.x & 7;
     int tries = 0;
     // heads == null: one work item per workgroup (the hardware dispatcher
@@ -191,17 +247,14 @@ __global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ 
         const GAS double *__restrict__ ljA = as_global(D->ljA);
         const GAS double *__restrict__ ljB = as_global(D->ljB);
         const GAS unsigned char *__restrict__ eb = (const GAS unsigned char *)D->entries;
-        GAS double *__restrict__ epart = as_global(D->epart);
         long long *facc = D->facc;
         const int n_pad = D->n_pad;
         const int nt = D->ntypes;
         const int I0 = w.sc * NB_SCC;
 
         __syncthreads();
-        if (LDS_LJ && w.sys != loaded_sys) {
+        if (LDS_LJ)
             for (int k = lane; k < nt * nt; k += 64) sLJ[k] = make_double2(ljA[k], ljB[k]);
-            loaded_sys = w.sys;
-        }
         sI[lane] = load_slot(xq, I0 * NB_CL + lane);
         sT[lane] = ty[I0 * NB_CL + lane] * nt;
         __syncthreads();
@@ -272,18 +325,9 @@ __global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ 
                     }
                 }
             }
-#pragma unroll
-            for (int o = 8; o <= 32; o <<= 1) {
-                gx += __shfl_xor(gx, o, 64);
-                gy += __shfl_xor(gy, o, 64);
-                gz += __shfl_xor(gz, o, 64);
-            }
-            if (a == 0) {
-                const int sj = J * NB_CL + b;
-                fx_atomic_add(facc, n_pad, sj, 0, (double)gx);
-                fx_atomic_add(facc, n_pad, sj, 1, (double)gy);
-                fx_atomic_add(facc, n_pad, sj, 2, (double)gz);
-            }
+            const float gj = jforce_reduce_scatter(a, gx, gy, gz);
+            if (a == 0 || a == 2 || a == 4)
+                fx_atomic_add(facc, n_pad, J * NB_CL + b, a >> 1, (double)gj);
             jc0 = jc1;
             lb0 = lb1;
             jc1 = jc2;
@@ -292,27 +336,31 @@ __global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ 
             xj = xjn;
             tj = tjn;
         }
+        // i-forces: transpose through LDS so that lane 8a+b sums atom a of
+        // i-cluster c = b over the eight lanes that held its partials.
+        __syncthreads();
 #pragma unroll
         for (int c = 0; c < NB_SCC; c++) {
-            float vx = fx[c], vy = fy[c], vz = fz[c];
-#pragma unroll
-            for (int o = 1; o <= 4; o <<= 1) {
-                vx += __shfl_xor(vx, o, 64);
-                vy += __shfl_xor(vy, o, 64);
-                vz += __shfl_xor(vz, o, 64);
-            }
-            if (b == 0) {
-                const int si = (I0 + c) * NB_CL + a;
-                fx_atomic_add(facc, n_pad, si, 0, (double)vx);
-                fx_atomic_add(facc, n_pad, si, 1, (double)vy);
-                fx_atomic_add(facc, n_pad, si, 2, (double)vz);
-            }
+            sh.sF[(3 * c + 0) * NB_SC + lane] = fx[c];
+            sh.sF[(3 * c + 1) * NB_SC + lane] = fy[c];
+            sh.sF[(3 * c + 2) * NB_SC + lane] = fz[c];
         }
+        __syncthreads();
+        float ig[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float4 *row = (const float4 *)&sh.sF[(3 * b + k) * NB_SC + NB_CL * a];
+            const float4 u = row[0], v = row[1];
+            ig[k] = ((u.x + u.y) + (u.z + u.w)) + ((v.x + v.y) + (v.z + v.w));
+        }
+        const int si = (I0 + b) * NB_CL + a;
+#pragma unroll
+        for (int k = 0; k < 3; k++) fx_atomic_add(facc, n_pad, si, k, (double)ig[k]);
         eel = wave_sum_f64(eel);
         elj = wave_sum_f64(elj);
-        if (lane == 0) {
-            epart[2 * (size_t)it] = eel;
-            epart[2 * (size_t)it + 1] = elj;
+        if (lane < 2) {
+            const int slot = it & (NB_ESLOTS - 1);
+            fx_add(eacc_word(D->eacc, lane, 0, slot), eacc_word(D->eacc, lane, 1, slot), lane == 0 ? eel : elj);
         }
     }
 }
@@ -419,21 +467,9 @@ __global__ __launch_bounds__(256) void k_pairs14(const SysDesc *__restrict__ des
 
 // ------------------------------------------------------------------------------------------------
 // Finalize: gradients (fixed point -> FP64, plus the plain-path rows) added
-// into the caller's array; energies summed in a fixed order.
+// into the caller's array; block 0 of each system also sums the energies in a
+// fixed order.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_finalize_grad(const SysDesc *__restrict__ descs, int plain)
-{
-    const SysDesc &D = descs[blockIdx.y];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= D.n || D.grad == nullptr) return;
-    const int s = D.atom_slot[i];
-    for (int k = 0; k < 3; k++) {
-        double g = fx_value(D.facc, D.n_pad, s, k);
-        if (plain) g += D.gplain[3 * (size_t)i + k];
-        D.grad[3 * (size_t)i + k] += g;
-    }
-}
-
 __device__ void sum_range2(const double *__restrict__ p, int m, double &x, double &y)
 {
     double sx = 0.0, sy = 0.0;
@@ -447,16 +483,35 @@ __device__ void sum_range2(const double *__restrict__ p, int m, double &x, doubl
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_finalize_energy(const SysDesc *__restrict__ descs, int plain)
+__global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ descs, int plain)
 {
-    const SysDesc &D = descs[blockIdx.x];
+    const SysDesc &D = descs[blockIdx.y];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < D.n && D.grad != nullptr) {
+        const int s = D.atom_slot[i];
+        for (int k = 0; k < 3; k++) {
+            double g = fx_value(D.facc, D.n_pad, s, k);
+            if (plain) g += D.gplain[3 * (size_t)i + k];
+            D.grad[3 * (size_t)i + k] += g;
+        }
+    }
+    if (blockIdx.x != 0) return;
     double el, lj, el14, lj14;
     if (plain) {
         sum_range2(D.eplain, D.n_plain_blocks, el, lj);
         el *= 0.5;
         lj *= 0.5;
     } else {
-        sum_range2(D.epart + 2 * (size_t)D.item_first, D.item_count, el, lj);
+        double ex = 0.0, ey = 0.0;
+        if (threadIdx.x < NB_ESLOTS) {
+            const int t = threadIdx.x;
+            ex = (double)*eacc_word(D.eacc, 0, 1, t) * NB_FX_HI_UNIT + (double)*eacc_word(D.eacc, 0, 0, t) / NB_FX_LO_SCALE;
+            ey = (double)*eacc_word(D.eacc, 1, 1, t) * NB_FX_HI_UNIT + (double)*eacc_word(D.eacc, 1, 0, t) / NB_FX_LO_SCALE;
+        }
+        block_sum2<256>(ex, ey);
+        el = ex;
+        lj = ey;
+        __syncthreads();
     }
     sum_range2(D.epart14, D.n14_blocks, el14, lj14);
     if (threadIdx.x == 0) {
@@ -470,10 +525,12 @@ __global__ __launch_bounds__(256) void k_finalize_energy(const SysDesc *__restri
 // ------------------------------------------------------------------------------------------------
 // Launchers.
 // ------------------------------------------------------------------------------------------------
-void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, int *heads, hipStream_t s)
+void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, int *heads, const PrepareCheck &chk,
+                    hipStream_t s)
 {
-    if (nsys <= 0 || max_npad <= 0) return;
-    hipLaunchKernelGGL(k_prepare, dim3(nblk(max_npad, 256), nsys), dim3(256), 0, s, d_desc, tiled, heads);
+    if (nsys <= 0) return;
+    hipLaunchKernelGGL(k_prepare, dim3(std::max(nblk(max_npad, 256), 1), nsys), dim3(256), 0, s, d_desc, tiled, heads,
+                       chk);
 }
 
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
@@ -501,6 +558,5 @@ void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const A
 void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s)
 {
     if (nsys <= 0) return;
-    if (max_n > 0) hipLaunchKernelGGL(k_finalize_grad, dim3(nblk(max_n, 256), nsys), dim3(256), 0, s, d_desc, plain);
-    hipLaunchKernelGGL(k_finalize_energy, dim3(nsys), dim3(256), 0, s, d_desc, plain);
+    hipLaunchKernelGGL(k_finalize, dim3(std::max(nblk(max_n, 256), 1), nsys), dim3(256), 0, s, d_desc, plain);
 }

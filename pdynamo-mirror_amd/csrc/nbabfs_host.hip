@@ -105,6 +105,39 @@ struct PinBuf {
     ~PinBuf() { release(); }
 };
 
+// Coherent pinned host memory that kernels write directly (rebuild flags):
+// the host reads it after the stream has synchronised, with no copy.
+struct MappedInts {
+    int *h = nullptr;
+    int *d = nullptr;
+    size_t cap = 0;
+    int ensure(size_t m)
+    {
+        if (h && m <= cap) return NBABFS_STATUS_CONTINUE;
+        release();
+        size_t mm = std::max<size_t>(m, 16);
+        if (hipHostMalloc((void **)&h, mm * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+            hipHostGetDevicePointer((void **)&d, h, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            release();
+            return fail(NBABFS_STATUS_OUT_OF_MEMORY, "mapped host allocation of %zu bytes failed", mm * sizeof(int));
+        }
+        std::memset(h, 0, mm * sizeof(int));
+        cap = mm;
+        return NBABFS_STATUS_CONTINUE;
+    }
+    void release()
+    {
+        if (h) (void)hipHostFree(h);
+        h = d = nullptr;
+        cap = 0;
+    }
+    ~MappedInts() { release(); }
+};
+
+// Epochs written by the update test; never 0, the initial flag value.
+static inline int next_epoch(int e) { return e == 0x7fffffff ? 1 : e + 1; }
+
 static inline int nblk(size_t m, int b) { return (int)((m + b - 1) / b); }
 
 // ------------------------------------------------------------------------------------------------
@@ -218,7 +251,7 @@ struct CheckDesc {
     double bufsq;
 };
 
-__global__ __launch_bounds__(256) void k_check_batch(const CheckDesc *__restrict__ cd, int *__restrict__ flags)
+__global__ __launch_bounds__(256) void k_check_batch(const CheckDesc *__restrict__ cd, int *__restrict__ flags, int epoch)
 {
 #pragma clang fp contract(off)
     const CheckDesc &C = cd[blockIdx.y];
@@ -230,7 +263,7 @@ __global__ __launch_bounds__(256) void k_check_batch(const CheckDesc *__restrict
         double dz = C.xyz[3 * (size_t)i + 2] - C.ref[3 * (size_t)i + 2];
         up = dx * dx + dy * dy + dz * dz > C.bufsq;
     }
-    if (__any(up) && threadIdx.x % 64 == 0) atomicOr(&flags[blockIdx.y], 1);
+    if (__any(up) && threadIdx.x % 64 == 0) *(volatile int *)&flags[blockIdx.y] = epoch;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -276,7 +309,8 @@ struct nbabfs_state {
     DBuf<double4> xyzq;
     DBuf<int> styp;
     DBuf<long long> facc;
-    DBuf<double> epart, epart14, eplain, gplain, energies;
+    DBuf<double> epart14, eplain, gplain, energies;
+    DBuf<long long> eacc;
     DBuf<SysDesc> d_desc;
     DBuf<int> heads, qstarts;
     PinBuf<SysDesc> h_desc;
@@ -284,6 +318,11 @@ struct nbabfs_state {
     long desc_version = -1;
     const double *last_xyz = nullptr;
     double *last_grad = nullptr;
+    // Coordinates whose slot records the last update already prepared (the
+    // next energy call on the same array skips its own prepare pass).
+    const double *prepared_xyz = nullptr;
+    MappedInts upflag;
+    int epoch = 0;
 
     // Build temporaries.
     DBuf<int> key, count, off, cursor, sorted, padded, colstart, scantmp, counts, eoff, nits, ioff, flag;
@@ -558,7 +597,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     CHK(S->xyzq.ensure(S->n_pad));
     CHK(S->styp.ensure(S->n_pad));
     CHK(S->facc.ensure(6 * (size_t)S->n_pad));
-    CHK(S->epart.ensure(2 * (size_t)std::max(S->nitems, 1)));
+    CHK(S->eacc.ensure(NB_EACC_WORDS));
     CHK(S->energies.ensure(4));
     std::vector<int> qs(9);
     for (int q = 0; q <= 8; q++) qs[q] = (int)((long long)S->nitems * q / 8);
@@ -583,20 +622,54 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     return NBABFS_STATUS_CONTINUE;
 }
 
-// NBModelABFS_Update (NBModelABFS.c:508-623) for device coordinates.
+static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *d_grad);
+
+// Upload the state's descriptor when allocations or the coordinate/gradient
+// arrays changed.  The staging copy is reused only after the stream drained.
+static int ensure_desc(nbabfs_state *S, const double *d_xyz, double *d_grad)
+{
+    CHK(S->d_desc.ensure(1));
+    CHK(S->h_desc.ensure(1));
+    if (S->desc_version == g_alloc_version && S->last_xyz == d_xyz && S->last_grad == d_grad)
+        return NBABFS_STATUS_CONTINUE;
+    HIPCHK(hipStreamSynchronize(S->stream));
+    fill_desc(S, S->h_desc.p[0], d_xyz, d_grad);
+    HIPCHK(hipMemcpyAsync(S->d_desc.p, S->h_desc.p, sizeof(SysDesc), hipMemcpyHostToDevice, S->stream));
+    S->desc_version = g_alloc_version;
+    S->last_xyz = d_xyz;
+    S->last_grad = d_grad;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+static int ensure_call_buffers(nbabfs_state *S)
+{
+    CHK(S->energies.ensure(4));
+    CHK(S->eacc.ensure(NB_EACC_WORDS));
+    CHK(S->epart14.ensure(2 * (size_t)std::max(nblk(S->n14, 256), 1)));
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// NBModelABFS_Update (NBModelABFS.c:508-623) for device coordinates.  When a
+// list exists, the displacement test runs inside the prepare pass of the
+// coming energy call (same kernel, same read of the coordinates); only if it
+// fires is the list rebuilt and the slot records prepared again.
 static int update(nbabfs_state *S, const double *d_xyz, int *rebuilt)
 {
     hipStream_t st = S->stream;
-    bool doUpdate = S->isNew || !S->have_ref;
+    bool doUpdate = S->isNew || !S->have_ref || !S->listValid;
+    S->prepared_xyz = nullptr;
     if (!doUpdate) {
         const double buf = 0.5 * (S->opt.listCutoff - S->opt.outerCutoff);
-        CHK(S->flag.ensure(1));
-        CHK(S->h_int.ensure(16));
-        launch_fill_int(S->flag.p, 1, 0, st);
-        launch_check_update(d_xyz, S->xyz_ref.p, S->has_fixed ? S->freef.p : nullptr, S->n, buf * buf, S->flag.p, st);
-        HIPCHK(hipMemcpyAsync(S->h_int.p, S->flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        CHK(S->upflag.ensure(1));
+        CHK(ensure_call_buffers(S));
+        CHK(ensure_desc(S, d_xyz, S->last_grad));
+        S->epoch = next_epoch(S->epoch);
+        PrepareCheck chk{d_xyz, S->xyz_ref.p, S->has_fixed ? S->freef.p : nullptr, buf * buf, S->upflag.d, S->epoch};
+        launch_prepare(S->d_desc.p, 1, S->n_pad, S->opt.precision == NBABFS_PLAIN_FP64 ? 0 : 1, S->heads.p, chk, st);
+        HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
-        doUpdate = S->h_int.p[0] != 0;
+        doUpdate = *(volatile int *)S->upflag.h == S->epoch;
+        if (!doUpdate) S->prepared_xyz = d_xyz;
     }
     if (doUpdate) {
         CHK(build_list(S, d_xyz));
@@ -631,7 +704,7 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.p14 = S->p14.p;
     D.facc = S->facc.p;
     D.gplain = S->gplain.p;
-    D.epart = S->epart.p;
+    D.eacc = S->eacc.p;
     D.epart14 = S->epart14.p;
     D.eplain = S->eplain.p;
     D.energies = S->energies.p;
@@ -644,8 +717,6 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.ncl = S->ncl;
     D.ntypes = S->ntypes;
     D.n14 = S->n14;
-    D.item_first = 0;
-    D.item_count = S->nitems;
     D.n_plain_blocks = nblk(S->n, 256);
     D.n14_blocks = nblk(S->n14, 256);
 }
@@ -665,23 +736,18 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
     if (!S->listValid) return fail(NBABFS_STATUS_LOGIC_ERROR, "no pair list: call the update first");
     hipStream_t st = S->stream;
     const bool plain = S->opt.precision == NBABFS_PLAIN_FP64;
-    CHK(S->energies.ensure(4));
-    CHK(S->epart14.ensure(2 * (size_t)std::max(nblk(S->n14, 256), 1)));
-    CHK(S->d_desc.ensure(1));
-    CHK(S->h_desc.ensure(1));
-    if (S->desc_version != g_alloc_version || S->last_xyz != d_xyz || S->last_grad != d_grad) {
-        fill_desc(S, S->h_desc.p[0], d_xyz, d_grad);
-        HIPCHK(hipMemcpyAsync(S->d_desc.p, S->h_desc.p, sizeof(SysDesc), hipMemcpyHostToDevice, st));
-        S->desc_version = g_alloc_version;
-        S->last_xyz = d_xyz;
-        S->last_grad = d_grad;
-    }
+    CHK(ensure_call_buffers(S));
+    CHK(ensure_desc(S, d_xyz, d_grad));
     if (S->n == 0) {
         launch_fill_int((int *)S->energies.p, 8, 0, st);
         HIPCHK(hipGetLastError());
         return NBABFS_STATUS_CONTINUE;
     }
-    launch_prepare(S->d_desc.p, 1, S->n_pad, plain ? 0 : 1, S->heads.p, st);
+    if (S->prepared_xyz != d_xyz) {
+        PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, 0};
+        launch_prepare(S->d_desc.p, 1, S->n_pad, plain ? 0 : 1, S->heads.p, none, st);
+    }
+    S->prepared_xyz = nullptr;
     if (S->timing) CHK(record_event(S));
     if (!plain) {
         if (S->nitems > 0) {
@@ -1050,11 +1116,11 @@ struct nbabfs_batch {
     DBuf<SysDesc> d_desc;
     PinBuf<SysDesc> h_desc;
     DBuf<WorkItem> items;
-    DBuf<double> epart;
-    DBuf<int> heads, qstarts, flags;
+    DBuf<int> heads, qstarts;
+    MappedInts flags;
+    int epoch = 0;
     DBuf<CheckDesc> d_check;
     PinBuf<CheckDesc> h_check;
-    PinBuf<int> h_flags;
     PinBuf<double> h_energies;
     DBuf<double> energies;
     int nitems = 0, max_npad = 0, max_n = 0, max_n14b = 0;
@@ -1123,7 +1189,6 @@ static int batch_layout(nbabfs_batch *B, const double *const *d_xyz, double *con
     }
     B->nitems = total;
     CHK(B->items.ensure(std::max(total, 1)));
-    CHK(B->epart.ensure(2 * (size_t)std::max(total, 1)));
     CHK(B->d_desc.ensure(ns));
     CHK(B->h_desc.ensure(ns));
     CHK(B->energies.ensure(4 * (size_t)ns));
@@ -1131,14 +1196,10 @@ static int batch_layout(nbabfs_batch *B, const double *const *d_xyz, double *con
     int first = 0;
     for (int k = 0; k < ns; k++) {
         nbabfs_state *s = B->st[k];
-        CHK(s->energies.ensure(4));
-        CHK(s->epart14.ensure(2 * (size_t)std::max(nblk(s->n14, 256), 1)));
+        CHK(ensure_call_buffers(s));
         SysDesc &D = B->h_desc.p[k];
         fill_desc(s, D, d_xyz[k], d_grad ? d_grad[k] : nullptr);
-        D.epart = B->epart.p;
         D.energies = B->energies.p + 4 * (size_t)k;
-        D.item_first = first;
-        D.item_count = s->nitems;
         if (s->nitems > 0)
             hipLaunchKernelGGL(k_remap_items, dim3(nblk(s->nitems, 256)), dim3(256), 0, st, s->items.p, s->nitems, k,
                                B->items.p + first);
@@ -1165,7 +1226,8 @@ static int batch_enqueue(nbabfs_batch *B)
     hipStream_t st = B->stream;
     const int ns = (int)B->st.size();
     nbabfs_state *s0 = B->st[0];
-    launch_prepare(B->d_desc.p, ns, B->max_npad, B->mixed ? 1 : 0, B->heads.p, st);
+    PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, 0};
+    launch_prepare(B->d_desc.p, ns, B->max_npad, B->mixed ? 1 : 0, B->heads.p, none, st);
     if (B->mixed) {
         if (B->nitems > 0) {
             QueueParams q{s0->opt.wavesPerCU > 0 ? B->heads.p : nullptr, B->qstarts.p};
@@ -1196,7 +1258,6 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
     CHK(B->d_check.ensure(ns));
     CHK(B->h_check.ensure(ns));
     CHK(B->flags.ensure(ns));
-    CHK(B->h_flags.ensure(ns));
     CHK(B->h_energies.ensure(4 * (size_t)ns));
     int maxn = 0;
     for (int k = 0; k < ns; k++) {
@@ -1207,15 +1268,15 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
         maxn = std::max(maxn, s->n);
     }
     HIPCHK(hipMemcpyAsync(B->d_check.p, B->h_check.p, ns * sizeof(CheckDesc), hipMemcpyHostToDevice, st));
-    launch_fill_int(B->flags.p, ns, 0, st);
+    B->epoch = next_epoch(B->epoch);
     if (maxn > 0)
-        hipLaunchKernelGGL(k_check_batch, dim3(nblk(maxn, 256), ns), dim3(256), 0, st, B->d_check.p, B->flags.p);
-    HIPCHK(hipMemcpyAsync(B->h_flags.p, B->flags.p, ns * sizeof(int), hipMemcpyDeviceToHost, st));
+        hipLaunchKernelGGL(k_check_batch, dim3(nblk(maxn, 256), ns), dim3(256), 0, st, B->d_check.p, B->flags.d, B->epoch);
     HIPCHK(hipStreamSynchronize(st));
     bool any = false;
     for (int k = 0; k < ns; k++) {
         nbabfs_state *s = B->st[k];
-        const bool need = s->isNew || !s->have_ref || B->h_flags.p[k] != 0;
+        s->prepared_xyz = nullptr;
+        const bool need = s->isNew || !s->have_ref || !s->listValid || ((volatile int *)B->flags.h)[k] == B->epoch;
         s->calls++;
         if (need) {
             HIPCHK(hipStreamSynchronize(s->stream));

@@ -14,6 +14,7 @@
 //            exactly the reference pair list.
 //   facc     six int64 arrays of n_pad (x,y,z low words, x,y,z high words):
 //            deterministic fixed-point gradient accumulators.
+//   eacc     fixed-point Coulomb and LJ energy accumulators, 64 slots each.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -68,7 +69,7 @@ struct SysDesc {
     const int *p14;             // 2*n14 (i, j)
     long long *facc;            // 6*n_pad
     double *gplain;             // n*3 plain-path gradients (or null)
-    double *epart;              // energy partials (2 per item)
+    long long *eacc;            // NB_EACC_WORDS fixed-point energy accumulators (tiled path)
     double *epart14;            // 2 per 1-4 block
     double *eplain;             // 2 per plain block
     double *energies;           // 4 outputs (device)
@@ -103,30 +104,46 @@ __device__ __forceinline__ long long rint_i64(double x)
     return __double_as_longlong(x + magic) - __double_as_longlong(magic);
 }
 
+// v = hi*2^16 + lo*2^-32 with hi = round(v/2^16) and the remainder |r| <= 2^15
+// (exact) stored as round(r*2^32).  |v| is clamped to 2^66, far beyond any
+// physical gradient or energy, which also maps NaN to a finite value.
+__device__ __forceinline__ void fx_split(double v, long long &hi, long long &lo)
+{
+    const double magic = 6755399441055744.0;
+    v = fmin(fmax(v, -0x1p66), 0x1p66);
+    const double t = v * (1.0 / NB_FX_HI_UNIT) + magic;
+    hi = __double_as_longlong(t) - __double_as_longlong(magic);
+    const double r = fma(-(t - magic), NB_FX_HI_UNIT, v);
+    lo = rint_i64(r * NB_FX_LO_SCALE);
+}
+
+__device__ __forceinline__ void fx_add(long long *lo_word, long long *hi_word, double v)
+{
+    long long hi, lo;
+    fx_split(v, hi, lo);
+    if (lo != 0) __hip_atomic_fetch_add(as_global(lo_word), lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (hi != 0) __hip_atomic_fetch_add(as_global(hi_word), hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void fx_atomic_add(long long *facc, int n_pad, int slot, int comp, double v)
 {
-    // v = hi*2^16 + r with |r| <= 2^15 (exact); r is stored as round(r*2^32).
-    long long hi;
-    double r;
-    if (fabs(v) < 0x1p66) {
-        hi = rint_i64(v * (1.0 / NB_FX_HI_UNIT));
-        r = v - (double)hi * NB_FX_HI_UNIT;
-    } else {   // beyond any physical gradient; keeps the arithmetic defined
-        const double h = rint(v * (1.0 / NB_FX_HI_UNIT));
-        hi = (long long)fmax(fmin(h, 9.0e18), -9.0e18);
-        r = 0.0;
-    }
-    const long long lo = rint_i64(r * NB_FX_LO_SCALE);
-    GAS long long *g = as_global(facc);
-    if (lo != 0) __hip_atomic_fetch_add(&g[(size_t)comp * n_pad + slot], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (hi != 0)
-        __hip_atomic_fetch_add(&g[(size_t)(comp + 3) * n_pad + slot], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    fx_add(&facc[(size_t)comp * n_pad + slot], &facc[(size_t)(comp + 3) * n_pad + slot], v);
 }
 
 __device__ __forceinline__ double fx_value(const long long *facc, int n_pad, int slot, int comp)
 {
     return (double)facc[(size_t)(comp + 3) * n_pad + slot] * NB_FX_HI_UNIT +
            (double)facc[(size_t)comp * n_pad + slot] * (1.0 / NB_FX_LO_SCALE);
+}
+
+// Energy accumulators: [kind][lo word, hi word][slot], kinds 0 = Coulomb and
+// 1 = LJ of the main list; a work item adds into slot (item % NB_ESLOTS).
+#define NB_ESLOTS 64
+#define NB_EACC_WORDS (2 * 2 * NB_ESLOTS)
+
+__device__ __forceinline__ long long *eacc_word(long long *eacc, int kind, int hi, int slot)
+{
+    return &eacc[(2 * kind + hi) * NB_ESLOTS + slot];
 }
 
 __device__ __forceinline__ double wave_sum_f64(double v)

@@ -52,8 +52,6 @@ void launch_export_pairs(const Entry *entries, int nent, const int *slot_atom, l
                          unsigned long long *cursor, long long cap, long long *keys, hipStream_t s);
 void launch_csr_degree(const Entry *entries, int nent, const int *slot_atom, int *deg, hipStream_t s);
 void launch_csr_fill(const Entry *entries, int nent, const int *slot_atom, int *cursor, int *cols, hipStream_t s);
-void launch_check_update(const double *xyz, const double *ref, const unsigned char *free_flags, int n, double bufsq,
-                         int *flag, hipStream_t s);
 
 // Per-call kernels (nbabfs_force.hip).  Every launcher takes the batch's
 // descriptor array (device) and the number of systems.
@@ -63,7 +61,17 @@ struct QueueParams {
     const int *starts;     // 9 item offsets: queue q owns [starts[q], starts[q+1])
 };
 
-void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, int *heads, hipStream_t s);
+// Optional list-update test fused into the prepare pass (single system).
+struct PrepareCheck {
+    const double *xyz;                 // coordinates to prepare from (null: the descriptor's)
+    const double *ref;                 // coordinates of the last list build (null: no test)
+    const unsigned char *free_flags;   // null: every atom is free
+    double bufsq;                      // (half the list buffer)^2
+    int *flag;                         // receives epoch when some free atom moved too far
+    int epoch;
+};
+void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, int *heads, const PrepareCheck &chk,
+                    hipStream_t s);
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
                         int nblocks, int max_ntypes, hipStream_t s);
 void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
