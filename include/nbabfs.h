@@ -41,6 +41,9 @@ extern "C" {
 #define NBABFS_MIXED      0   /* FP32 pair math, 64-bit fixed-point sums, FP64 totals */
 #define NBABFS_PLAIN_FP64 1   /* FP64 pair math in the reference expression order    */
 
+/* Upper bound of nbabfs_options.entriesPerItem (per-item j-force staging in LDS). */
+#define NBABFS_MAX_ENTRIES_PER_ITEM 128
+
 typedef struct nbabfs_state nbabfs_state;
 typedef struct nbabfs_batch nbabfs_batch;
 
@@ -58,7 +61,8 @@ typedef struct {
     int    minimumCellExtent;      /* 2    */
     int    minimumPoints;          /* 500  */
     int    precision;              /* NBABFS_MIXED or NBABFS_PLAIN_FP64 */
-    int    entriesPerItem;         /* cluster-pair entries per work item (0 = 16) */
+    int    entriesPerItem;         /* cluster-pair entries per work item (0 = 16, at most
+                                      NBABFS_MAX_ENTRIES_PER_ITEM) */
     int    wavesPerCU;             /* persistent-kernel waves per CU; 0 = one workgroup per work item */
     int    reserved;
 } nbabfs_options;

@@ -59,15 +59,22 @@ __device__ __forceinline__ bool pair_fp64(const AbfsFactors &F, double r2, doubl
 }
 
 // Tiled path: one pair with 1/r from the FP32 v_rsq seed refined by one
-// Newton step in FP64 (relative error ~1e-14; no FP64 divide or square root),
-// the three regions evaluated branch-free and selected per lane, the damped
-// region (r < dampingCutoff) as a wave-uniform rare path.  On gfx950 FP64
-// FMA issues at the FP32 rate, so the scalar core stays in FP64 and only the
-// rsqrt seed and the force vectors are FP32.
-__device__ __forceinline__ void pair_tiled(const AbfsFactors &F, double r2, double qq, double A, double B, bool act,
-                                           double &fs, double &eel, double &elj)
+// Newton step in FP64 (relative error ~1e-14; no FP64 divide or square root).
+// On gfx950 FP64 FMA issues at the FP32 rate, so the scalar core stays in FP64
+// and only the rsqrt seed and the force vectors are FP32.
+//
+// Lennard-Jones in both undamped regions is one polynomial in s = 1/r,
+//   E = C12 s^12 + C6 s^6 + C3 s^3 + C0,
+// whose coefficients per type pair and region come from a row table (plain:
+// A, -B, 0, B*bShift6 - A*aShift12; switched: the expanded squares
+// A*aK12*(s^6-aF6)^2 - B*bK6*(s^3-bF3)^2), so the region choice is a row
+// index instead of two evaluated branches.  The Coulomb switch keeps the
+// reference polynomial with the region selected per lane.  The damped region
+// (r < dampingCutoff) is a wave-uniform rare path on the raw A, B.
+__device__ __forceinline__ void pair_rows(const AbfsFactors &F, double r2, double qq, double4 C, bool sw, bool act,
+                                          const GAS double *__restrict__ ljA, const GAS double *__restrict__ ljB,
+                                          int tij, double &fs, double &eel, double &elj)
 {
-    const bool sw = r2 > F.r2On;
     const double y = (double)__builtin_amdgcn_rsqf((float)r2);
     const double h = fma(-(r2 * y), y, 1.0);
     const double s = fma(0.5 * y, h, y);
@@ -78,19 +85,15 @@ __device__ __forceinline__ void pair_tiled(const AbfsFactors &F, double r2, doub
     const double Q = sw ? fma(r2, fma(r2, fma(5.0 * F.d, r2, 3.0 * F.c), F.b), F.a) : 1.0;
     const double K = sw ? F.qShift2 : F.qShift1;
     double e1 = qq * fma(s, P, K);
-    double d1 = -0.5 * qq * s3 * Q;
-    const double l1 = s6 - F.aF6, l2 = s3 - F.bF3;
-    const double Aa = A * F.aK12 * l1;
-    const double Bb = B * F.bK6 * l2;
-    const double e_sw = fma(Aa, l1, -Bb * l2);
-    const double d_sw = -3.0 * s6 * fma(2.0 * Aa, s2, -Bb * (r2 * s));
-    const double e_pl = fma(A, s6 * s6 - F.aShift12, -B * (s6 - F.bShift6));
-    const double d_pl = -3.0 * s6 * s2 * fma(2.0 * A, s6, -B);
-    double e2 = sw ? e_sw : e_pl;
-    double d2 = sw ? d_sw : d_pl;
+    double d1 = (-0.5 * qq) * s3 * Q;
+    // 0.5 (1/r) dE/dr of the polynomial = -1.5 s^5 (s^3 (4 C12 s^6 + 2 C6) + C3).
+    double e2 = fma(fma(C.x, s6, C.y), s6, fma(C.z, s3, C.w));
+    const double inner = fma(fma(4.0 * C.x, s6, 2.0 * C.y), s3, C.z);
+    double d2 = (-1.5 * s2) * s3 * inner;
     const bool dmp = r2 < F.r2Damp;
     if (__any(act && dmp)) {
         if (dmp) {
+            const double A = ljA[tij], B = ljB[tij];
             e1 = qq * fma(-F.qAlpha, r2, F.qF0);
             d1 = -qq * F.qAlpha;
             e2 = A * fma(-F.aAlpha, r2, F.aF0) - B * fma(-F.bAlpha, r2, F.bF0);
@@ -151,12 +154,11 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
 }
 
 // ------------------------------------------------------------------------------------------------
-// Tiled force kernel (persistent; one queue per XCD with stealing).  The LJ
-// table of the item's system sits in LDS (ntypes <= NB_LDS_TYPES); the next
-// entry's j-cluster record and masks are fetched while the current one is
-// evaluated.
+// Tiled force kernel: one work item per workgroup, or a persistent loop over
+// per-XCD queues with stealing.  The LJ row table of the item's system sits
+// in LDS when it fits; the next entry's j-cluster record and masks are
+// fetched while the current one is evaluated.
 // ------------------------------------------------------------------------------------------------
-#define NB_LDS_TYPES 16
 
 __device__ __forceinline__ double4 load_slot(const GAS double *__restrict__ xr, int k)
 {
@@ -177,46 +179,58 @@ __device__ __forceinline__ int entry_bits(const GAS unsigned char *__restrict__ 
     return eb[(size_t)e * sizeof(Entry) + 16 + lane];
 }
 
-// j-force partials (gx, gy, gz) of lane 8a+b are summed over a by a
-// reduce-scatter: after three exchanges lanes a = 0, 2, 4 hold the x, y and z
-// sums of j-atom b, so one fixed-point conversion serves all three components.
-__device__ __forceinline__ float jforce_reduce_scatter(int a, float gx, float gy, float gz)
+// Sum over the eight consecutive lanes 8b..8b+7 with DPP moves: quad
+// exchanges (xor 1, xor 2) then the half-row mirror (lane i <-> 7-i).  Every
+// lane of the group ends with the total, in the same order on every lane.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v)
 {
-    const bool lo4 = a < 4;
-    const float r1 = __shfl_xor(lo4 ? gz : gx, 32, 64);
-    const float r2 = __shfl_xor(gy, 32, 64);
-    const float k1 = (lo4 ? gx : gz) + r1;   // a < 4: x; a >= 4: z
-    const float k2 = gy + r2;                // a < 4: y
-    const bool hi2 = (a & 2) != 0;
-    const float snd = lo4 ? (hi2 ? k1 : k2) : k1;
-    float kp = lo4 ? (hi2 ? k2 : k1) : k1;
-    kp += __shfl_xor(snd, 16, 64);
-    kp += __shfl_xor(kp, 8, 64);
-    return kp;
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-// LDS of the tiled kernel.  The i-force transpose at the end of an item
-// reuses the space of the item's i-cluster records and LJ table.
-template <bool LDS_LJ>
-union TiledShared {
-    struct {
-        double4 sI[NB_SC];
-        int sT[NB_SC];
-        double2 sLJ[LDS_LJ ? NB_LDS_TYPES * NB_LDS_TYPES : 1];
-    } in;
-    float sF[NB_SCC * 3 * NB_SC];   // [c][component][lane]
-};
-
-template <bool LDS_LJ>
-__global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ descs,
-                                                    const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F)
+__device__ __forceinline__ float sum8(float v)
 {
+    v += dpp_f32<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_f32<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f32<0x141>(v);   // row_half_mirror
+    return v;
+}
+
+// LDS of the tiled kernel (dynamic): the item's i-cluster records and slot
+// types, the LJ row table of its system when it fits (ROWS_LDS), and the
+// per-entry j-force sums with their j-clusters, flushed to the fixed-point
+// accumulators once at the end of the item.  The i-force transpose at the end
+// of an item reuses the space of the records and the row table.
+#define NB_LDS_I_BYTES (NB_SC * 32 + NB_SC * 4)
+#define NB_LDS_F_BYTES (NB_SCC * 3 * NB_SC * 4)
+#define NB_LDS_ROW_LIMIT 16384
+
+static inline size_t tiled_lds_head(int ntypes, bool rows_lds)
+{
+    size_t h = NB_LDS_I_BYTES + (rows_lds ? (size_t)2 * ntypes * ntypes * 32 : 0);
+    return h > NB_LDS_F_BYTES ? h : NB_LDS_F_BYTES;
+}
+
+static inline size_t tiled_lds_bytes(int ntypes, bool rows_lds, int max_count)
+{
+    return tiled_lds_head(ntypes, rows_lds) + (size_t)max_count * (3 * NB_CL * 4 + 4);
+}
+
+template <bool ROWS_LDS>
+__global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ descs,
+                                                    const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F,
+                                                    int lds_head)
+{
+    // Lane l = 8b + a: a is the i-atom within an i-cluster, b the j-atom of
+    // the entry's j-cluster.  The entry masks are stored per l' = 8a + b.
     const int lane = threadIdx.x;
-    const int a = lane >> 3, b = lane & 7;
-    __shared__ TiledShared<LDS_LJ> sh;
-    double4 *sI = sh.in.sI;
-    int *sT = sh.in.sT;
-    double2 *sLJ = sh.in.sLJ;
+    const int a = lane & 7, b = lane >> 3;
+    const int bl = 8 * a + b;
+    extern __shared__ __align__(32) unsigned char lds[];
+    double4 *sI = (double4 *)lds;
+    int *sT = (int *)(lds + NB_SC * 32);
+    double4 *sR = (double4 *)(lds + NB_LDS_I_BYTES);
+    float *sF = (float *)lds;
     int q = blockIdx.x & 7;
     int tries = 0;
     // heads == null: one work item per workgroup (the hardware dispatcher
@@ -246,52 +260,44 @@ __global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ 
         const GAS int *__restrict__ ty = as_global(D->styp);
         const GAS double *__restrict__ ljA = as_global(D->ljA);
         const GAS double *__restrict__ ljB = as_global(D->ljB);
+        const GAS double *__restrict__ rows = as_global(D->ljrow);
         const GAS unsigned char *__restrict__ eb = (const GAS unsigned char *)D->entries;
         long long *facc = D->facc;
         const int n_pad = D->n_pad;
         const int nt = D->ntypes;
         const int I0 = w.sc * NB_SCC;
+        float *sJF = (float *)(lds + lds_head);           // [entry][3 * NB_CL]
+        int *sJ = (int *)(sJF + 3 * NB_CL * w.count);      // [entry]
 
         __syncthreads();
-        if (LDS_LJ)
-            for (int k = lane; k < nt * nt; k += 64) sLJ[k] = make_double2(ljA[k], ljB[k]);
+        if (ROWS_LDS)
+            for (int k = lane; k < 2 * nt * nt; k += 64) sR[k] = load_slot(rows, k);
         sI[lane] = load_slot(xq, I0 * NB_CL + lane);
-        sT[lane] = ty[I0 * NB_CL + lane] * nt;
+        sT[lane] = 2 * nt * ty[I0 * NB_CL + lane];
         __syncthreads();
 
         float fx[NB_SCC], fy[NB_SCC], fz[NB_SCC];
 #pragma unroll
         for (int c = 0; c < NB_SCC; c++) fx[c] = fy[c] = fz[c] = 0.0f;
         double eel = 0.0, elj = 0.0;
-        const int e_end = w.first + w.count;
-        // Two-deep pipeline: entry e+2's header and mask byte and entry e+1's
-        // j-cluster record are in flight while entry e is evaluated.
+        const int e_last = w.first + w.count - 1;
+        // Pipeline: while entry e is evaluated, entry e+1's j-cluster record
+        // and entry e+2's header and mask byte are in flight.  The indices are
+        // clamped, so every iteration issues the same loads and no atomic is
+        // pending inside the loop.
         int2 jc0 = entry_jc(eb, w.first);
-        int lb0 = entry_bits(eb, w.first, lane);
-        int2 jc1 = jc0;
-        int lb1 = 0;
-        if (w.first + 1 < e_end) {
-            jc1 = entry_jc(eb, w.first + 1);
-            lb1 = entry_bits(eb, w.first + 1, lane);
-        }
+        int lb0 = entry_bits(eb, w.first, bl);
+        int2 jc1 = entry_jc(eb, min(w.first + 1, e_last));
+        int lb1 = entry_bits(eb, min(w.first + 1, e_last), bl);
         int J = __builtin_amdgcn_readfirstlane(jc0.x);
         double4 xj = load_slot(xq, J * NB_CL + b);
-        int tj = ty[J * NB_CL + b];
-        for (int e = w.first; e < e_end; e++) {
-            int2 jc2 = jc1;
-            int lb2 = 0;
-            if (e + 2 < e_end) {
-                jc2 = entry_jc(eb, e + 2);
-                lb2 = entry_bits(eb, e + 2, lane);
-            }
-            double4 xjn = xj;
-            int tjn = tj;
-            int Jn = J;
-            if (e + 1 < e_end) {
-                Jn = __builtin_amdgcn_readfirstlane(jc1.x);
-                xjn = load_slot(xq, Jn * NB_CL + b);
-                tjn = ty[Jn * NB_CL + b];
-            }
+        int tj = 2 * ty[J * NB_CL + b];
+        for (int e = w.first; e <= e_last; e++) {
+            const int Jn = __builtin_amdgcn_readfirstlane(jc1.x);
+            const double4 xjn = load_slot(xq, Jn * NB_CL + b);
+            const int tjn = 2 * ty[Jn * NB_CL + b];
+            const int2 jc2 = entry_jc(eb, min(e + 2, e_last));
+            const int lb2 = entry_bits(eb, min(e + 2, e_last), bl);
             const int cb = __builtin_amdgcn_readfirstlane(jc0.y);
             const int lb = lb0;
             float gx = 0.0f, gy = 0.0f, gz = 0.0f;
@@ -300,22 +306,15 @@ __global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ 
                 if (cb & (1 << c)) {
                     const bool inl = (lb >> c) & 1;
                     const double4 xi = sI[c * NB_CL + a];
-                    const int tij = sT[c * NB_CL + a] + tj;
-                    double A, B;
-                    if (LDS_LJ) {
-                        const double2 ab = sLJ[tij];
-                        A = ab.x;
-                        B = ab.y;
-                    } else {
-                        A = ljA[tij];
-                        B = ljB[tij];
-                    }
+                    const int row = sT[c * NB_CL + a] + tj;
                     const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
                     const bool act = inl && (r2 <= F.r2Off);
                     if (__any(act)) {
+                        const bool sw = r2 > F.r2On;
+                        const double4 C = ROWS_LDS ? sR[row + sw] : load_slot(rows, row + sw);
                         double fs, x1, x2;
-                        pair_tiled(F, r2, xi.w * xj.w, A, B, act, fs, x1, x2);
+                        pair_rows(F, r2, xi.w * xj.w, C, sw, act, ljA, ljB, row >> 1, fs, x1, x2);
                         const float f = (float)fs;
                         const float px = f * (float)dx, py = f * (float)dy, pz = f * (float)dz;
                         fx[c] += px; fy[c] += py; fz[c] += pz;
@@ -325,9 +324,12 @@ __global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ 
                     }
                 }
             }
-            const float gj = jforce_reduce_scatter(a, gx, gy, gz);
-            if (a == 0 || a == 2 || a == 4)
-                fx_atomic_add(facc, n_pad, J * NB_CL + b, a >> 1, (double)gj);
+            gx = sum8(gx);
+            gy = sum8(gy);
+            gz = sum8(gz);
+            const int el = e - w.first;
+            if (a < 3) sJF[el * 3 * NB_CL + 3 * b + a] = a == 0 ? gx : (a == 1 ? gy : gz);
+            if (lane == 0) sJ[el] = J;
             jc0 = jc1;
             lb0 = lb1;
             jc1 = jc2;
@@ -336,26 +338,31 @@ __global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ 
             xj = xjn;
             tj = tjn;
         }
-        // i-forces: transpose through LDS so that lane 8a+b sums atom a of
+        // i-forces: transpose through LDS so that lane 8b+a sums atom a of
         // i-cluster c = b over the eight lanes that held its partials.
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < NB_SCC; c++) {
-            sh.sF[(3 * c + 0) * NB_SC + lane] = fx[c];
-            sh.sF[(3 * c + 1) * NB_SC + lane] = fy[c];
-            sh.sF[(3 * c + 2) * NB_SC + lane] = fz[c];
+            sF[(3 * c + 0) * NB_SC + lane] = fx[c];
+            sF[(3 * c + 1) * NB_SC + lane] = fy[c];
+            sF[(3 * c + 2) * NB_SC + lane] = fz[c];
         }
         __syncthreads();
+        // Partials of (c, a) sit at lanes 8b' + a, b' = 0..7 (stride 8).
         float ig[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const float4 *row = (const float4 *)&sh.sF[(3 * b + k) * NB_SC + NB_CL * a];
-            const float4 u = row[0], v = row[1];
-            ig[k] = ((u.x + u.y) + (u.z + u.w)) + ((v.x + v.y) + (v.z + v.w));
+            const float *fr = &sF[(3 * b + k) * NB_SC + a];
+            ig[k] = ((fr[0] + fr[8]) + (fr[16] + fr[24])) + ((fr[32] + fr[40]) + (fr[48] + fr[56]));
         }
         const int si = (I0 + b) * NB_CL + a;
 #pragma unroll
         for (int k = 0; k < 3; k++) fx_atomic_add(facc, n_pad, si, k, (double)ig[k]);
+        // j-forces of the whole item.
+        for (int k = lane; k < 3 * NB_CL * w.count; k += 64) {
+            const int el = k / (3 * NB_CL), r = k - el * 3 * NB_CL;
+            fx_atomic_add(facc, n_pad, sJ[el] * NB_CL + r / 3, r % 3, (double)sJF[k]);
+        }
         eel = wave_sum_f64(eel);
         elj = wave_sum_f64(elj);
         if (lane < 2) {
@@ -534,13 +541,16 @@ void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, in
 }
 
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
-                        int nblocks, int max_ntypes, hipStream_t s)
+                        int nblocks, int max_ntypes, int max_count, hipStream_t s)
 {
     if (nblocks <= 0) return;
-    if (max_ntypes <= NB_LDS_TYPES)
-        hipLaunchKernelGGL(k_force_tiled<true>, dim3(nblocks), dim3(64), 0, s, d_desc, items, q, F);
+    const bool rows_lds = tiled_lds_bytes(max_ntypes, true, max_count) <= NB_LDS_ROW_LIMIT;
+    const int head = (int)tiled_lds_head(max_ntypes, rows_lds);
+    const size_t bytes = tiled_lds_bytes(max_ntypes, rows_lds, max_count);
+    if (rows_lds)
+        hipLaunchKernelGGL(k_force_tiled<true>, dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head);
     else
-        hipLaunchKernelGGL(k_force_tiled<false>, dim3(nblocks), dim3(64), 0, s, d_desc, items, q, F);
+        hipLaunchKernelGGL(k_force_tiled<false>, dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head);
 }
 
 void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,

@@ -279,7 +279,7 @@ struct nbabfs_state {
     std::vector<long long> p14_keys;   // canonical 1-4 set
 
     // Constant device data.
-    DBuf<double> charge, ljA, ljB, ljA14, ljB14;
+    DBuf<double> charge, ljA, ljB, ljA14, ljB14, ljrow;
     DBuf<int> atype, xptr, xidx, p14;
     DBuf<unsigned char> freef;
     bool has_fixed = false;
@@ -698,6 +698,7 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.charge = S->charge.p;
     D.ljA = S->ljA.p;
     D.ljB = S->ljB.p;
+    D.ljrow = S->ljrow.p;
     D.ljA14 = S->ljA14.p;
     D.ljB14 = S->ljB14.p;
     D.entries = S->entries.p;
@@ -752,7 +753,8 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
     if (!plain) {
         if (S->nitems > 0) {
             QueueParams q{S->opt.wavesPerCU > 0 ? S->heads.p : nullptr, S->qstarts.p};
-            launch_force_tiled(S->d_desc.p, S->items.p, q, S->F, n_persistent_blocks(S->opt, S->nitems), S->ntypes, st);
+            launch_force_tiled(S->d_desc.p, S->items.p, q, S->F, n_persistent_blocks(S->opt, S->nitems), S->ntypes,
+                               S->opt.entriesPerItem, st);
         }
     } else {
         launch_force_plain(S->d_desc.p, 0, S->n, S->F, S->rowptr.p, S->cols.p, st);
@@ -780,6 +782,9 @@ extern "C" int nbabfs_state_create(const nbabfs_options *opt, int natoms, const 
     nbabfs_default_options(&o);
     if (opt) o = *opt;
     if (o.entriesPerItem <= 0) o.entriesPerItem = 16;
+    if (o.entriesPerItem > NBABFS_MAX_ENTRIES_PER_ITEM)
+        return fail(NBABFS_STATUS_VALUE_ERROR, "entriesPerItem %d exceeds %d", o.entriesPerItem,
+                    NBABFS_MAX_ENTRIES_PER_ITEM);
     if (o.wavesPerCU < 0) o.wavesPerCU = 0;
     // NBModelABFS.SetOptions cutoff checks (pMolecule.NBModelABFS.pyx:169-176).
     if (o.dampingCutoff < 0.0 || o.innerCutoff < o.dampingCutoff || o.outerCutoff < o.innerCutoff ||
@@ -831,6 +836,25 @@ extern "C" int nbabfs_state_create(const nbabfs_options *opt, int natoms, const 
             A14[(size_t)i * ntypes + j] = tableA14 ? tableA14[t] : tableA[t];
             B14[(size_t)i * ntypes + j] = tableB14 ? tableB14[t] : tableB[t];
         }
+    // Polynomial rows of the tiled path: E = C12 s^12 + C6 s^6 + C3 s^3 + C0
+    // for the plain region (r <= innerCutoff) and the force-switched region
+    // (PairwiseInteraction.h:72-119 with the squares expanded).
+    std::vector<double> R(8 * nt2);
+    {
+        const AbfsFactors &F = S->F;
+        for (size_t k = 0; k < nt2; k++) {
+            const double a = A[k], b = B[k], ak = a * F.aK12, bk = b * F.bK6;
+            double *p = &R[8 * k];
+            p[0] = a;
+            p[1] = -b;
+            p[2] = 0.0;
+            p[3] = b * F.bShift6 - a * F.aShift12;
+            p[4] = ak;
+            p[5] = -(2.0 * ak * F.aF6 + bk);
+            p[6] = 2.0 * bk * F.bF3;
+            p[7] = ak * F.aF6 * F.aF6 - bk * F.bF3 * F.bF3;
+        }
+    }
     // Exclusions: symmetric CSR, ascending rows, no duplicates or self pairs
     // (SelfPairList_MakeConnections, pCore-1.9.0/extensions/csource/PairList.c:458-526).
     std::vector<long long> ek;
@@ -895,6 +919,7 @@ extern "C" int nbabfs_state_create(const nbabfs_options *opt, int natoms, const 
         (rc = S->atype.upload(t.data(), t.size(), st)) != NBABFS_STATUS_CONTINUE ||
         (rc = S->ljA.upload(A.data(), nt2, st)) != NBABFS_STATUS_CONTINUE ||
         (rc = S->ljB.upload(B.data(), nt2, st)) != NBABFS_STATUS_CONTINUE ||
+        (rc = S->ljrow.upload(R.data(), R.size(), st)) != NBABFS_STATUS_CONTINUE ||
         (rc = S->ljA14.upload(A14.data(), nt2, st)) != NBABFS_STATUS_CONTINUE ||
         (rc = S->ljB14.upload(B14.data(), nt2, st)) != NBABFS_STATUS_CONTINUE ||
         (rc = S->xptr.upload(xptr.data(), xptr.size(), st)) != NBABFS_STATUS_CONTINUE ||
@@ -1231,9 +1256,13 @@ static int batch_enqueue(nbabfs_batch *B)
     if (B->mixed) {
         if (B->nitems > 0) {
             QueueParams q{s0->opt.wavesPerCU > 0 ? B->heads.p : nullptr, B->qstarts.p};
-            int maxnt = 0;
-            for (auto *s : B->st) maxnt = std::max(maxnt, s->ntypes);
-            launch_force_tiled(B->d_desc.p, B->items.p, q, s0->F, n_persistent_blocks(s0->opt, B->nitems), maxnt, st);
+            int maxnt = 0, maxcount = 0;
+            for (auto *s : B->st) {
+                maxnt = std::max(maxnt, s->ntypes);
+                maxcount = std::max(maxcount, s->opt.entriesPerItem);
+            }
+            launch_force_tiled(B->d_desc.p, B->items.p, q, s0->F, n_persistent_blocks(s0->opt, B->nitems), maxnt,
+                               maxcount, st);
         }
     } else {
         for (int k = 0; k < ns; k++)

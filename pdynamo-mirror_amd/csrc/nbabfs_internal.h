@@ -65,6 +65,7 @@ struct SysDesc {
     const double *charge;       // n
     const double *ljA, *ljB;    // ntypes*ntypes FP64 (main list)
     const double *ljA14, *ljB14;// ntypes*ntypes FP64 (1-4 list)
+    const double *ljrow;        // 2*ntypes*ntypes rows of 4 FP64: [(ti*ntypes+tj)*2 + switched]
     const Entry *entries;
     const int *p14;             // 2*n14 (i, j)
     long long *facc;            // 6*n_pad

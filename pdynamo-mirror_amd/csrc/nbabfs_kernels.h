@@ -73,7 +73,7 @@ struct PrepareCheck {
 void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, int *heads, const PrepareCheck &chk,
                     hipStream_t s);
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
-                        int nblocks, int max_ntypes, hipStream_t s);
+                        int nblocks, int max_ntypes, int max_count, hipStream_t s);
 void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
                         const int *cols, hipStream_t s);
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s);

@@ -193,8 +193,8 @@ class NBModelABFS:
             self.precision = p
         if "entriesPerItem" in keywordArguments:
             v = int(keywordArguments.pop("entriesPerItem"))
-            if v <= 0:
-                raise ValueError("entriesPerItem must be positive.")
+            if v <= 0 or v > 128:
+                raise ValueError("entriesPerItem must be in [1, 128].")
             self.entriesPerItem = v
         if "wavesPerCU" in keywordArguments:
             v = int(keywordArguments.pop("wavesPerCU"))
