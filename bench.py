@@ -211,16 +211,21 @@ def main():
         step()
     rebuilds[0] = 0
     _native.synchronize()
-    st.set_timing(True)
-    _native.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     _native.synchronize()
     t1 = time.perf_counter()
+    ms = (t1 - t0) / args.steps * 1e3
+    timed_rebuilds = rebuilds[0]
+    # Pair-kernel duration from HIP events on the state's stream, in a pass of
+    # its own so that the events do not add to the step time above.
+    st.set_timing(True)
+    for _ in range(min(args.steps, 20)):
+        step()
+    _native.synchronize()
     stats = st.stats()
     st.set_timing(False)
-    ms = (t1 - t0) / args.steps * 1e3
     e, present = st.energies()
     keys = st.export_pairs(0)
     x_last = xs[(counter[0] - 1) % len(xs)].numpy(st)
@@ -245,7 +250,7 @@ def main():
                    "list_cutoff": 13.5, "cutoffs": [0.5, 8.0, 12.0],
                    "energies_kJ_mol": [float(v) for v in e], "work_items": int(stats["workItems"]),
                    "cluster_entries": int(stats["entries"]), "moves_sigma_A": args.moves,
-                   "rebuilds_in_timed_steps": int(rebuilds[0])},
+                   "rebuilds_in_timed_steps": int(timed_rebuilds)},
     }
     if not args.no_roofline:
         flops, n_list2, n_plain, n_switch = algorithmic_flops(x_last, keys, mm.natoms)

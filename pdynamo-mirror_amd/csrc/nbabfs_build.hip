@@ -458,6 +458,58 @@ __global__ __launch_bounds__(64) void k_csr_fill(const Entry *__restrict__ entri
     }
 }
 
+// Ascending sort of each CSR row, so that the plain path sums every row in the
+// same order whatever order the fill's atomics took.  A row's entries are
+// distinct atom indices.  Bitonic network in its "flip" form, which sorts any
+// length: positions at or beyond the row end act as +infinity and never move.
+#define NB_ROW_LDS 4096
+
+template <class P>
+__device__ void bitonic_row(P v, int len)
+{
+    int m = 1;
+    while (m < len) m <<= 1;
+    for (int k = 2; k <= m; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < m / 2; t += blockDim.x) {
+                const int blk = t / j, off = t % j;
+                int lo, hi;
+                if (j == (k >> 1)) {        // flip comparator of the merge's first stage
+                    const int base = (t / (k >> 1)) * k, o = t % (k >> 1);
+                    lo = base + o;
+                    hi = base + k - 1 - o;
+                } else {
+                    lo = blk * 2 * j + off;
+                    hi = lo + j;
+                }
+                if (hi < len && v[lo] > v[hi]) {
+                    const int x = v[lo];
+                    v[lo] = v[hi];
+                    v[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_csr_sort_rows(const int *__restrict__ rowptr, int n, int *__restrict__ cols)
+{
+    __shared__ int sv[NB_ROW_LDS];
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int r0 = rowptr[i], len = rowptr[i + 1] - r0;
+    if (len <= 1) return;
+    if (len <= NB_ROW_LDS) {
+        for (int k = threadIdx.x; k < len; k += 256) sv[k] = cols[r0 + k];
+        __syncthreads();
+        bitonic_row(sv, len);
+        for (int k = threadIdx.x; k < len; k += 256) cols[r0 + k] = sv[k];
+    } else {
+        bitonic_row((volatile int *)(cols + r0), len);
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Host-side launch wrappers.
 // ------------------------------------------------------------------------------------------------
@@ -569,5 +621,10 @@ void launch_csr_degree(const Entry *entries, int nent, const int *slot_atom, int
 void launch_csr_fill(const Entry *entries, int nent, const int *slot_atom, int *cursor, int *cols, hipStream_t s)
 {
     if (nent > 0) hipLaunchKernelGGL(k_csr_fill, dim3(nent), dim3(64), 0, s, entries, nent, slot_atom, cursor, cols);
+}
+
+void launch_csr_sort_rows(const int *rowptr, int n, int *cols, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_csr_sort_rows, dim3(n), dim3(256), 0, s, rowptr, n, cols);
 }
 

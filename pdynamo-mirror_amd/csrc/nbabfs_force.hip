@@ -150,7 +150,10 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
             D.styp[s] = t;
         }
     }
-    if (__any(up) && (threadIdx.x & 63) == 0) *(volatile int *)chk.flag = chk.epoch;
+    if (__any(up) && (threadIdx.x & 63) == 0) {
+        *(volatile int *)chk.flag = chk.epoch;
+        *(volatile int *)chk.dflag = chk.epoch;
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -219,8 +222,9 @@ static inline size_t tiled_lds_bytes(int ntypes, bool rows_lds, int max_count)
 template <bool ROWS_LDS>
 __global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ descs,
                                                     const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F,
-                                                    int lds_head)
+                                                    int lds_head, ForceGuard guard)
 {
+    if (guard.flag && *(const volatile int *)guard.flag == guard.epoch) return;
     // Lane l = 8b + a: a is the i-atom within an i-cluster, b the j-atom of
     // the entry's j-cluster.  The entry masks are stored per l' = 8a + b.
     const int lane = threadIdx.x;
@@ -394,9 +398,11 @@ __device__ __forceinline__ void block_sum2(double &x, double &y)
 }
 
 __global__ __launch_bounds__(256) void k_force_plain(const SysDesc *__restrict__ descs, int sys, AbfsFactors F,
-                                                     const int *__restrict__ rowptr, const int *__restrict__ cols)
+                                                     const int *__restrict__ rowptr, const int *__restrict__ cols,
+                                                     ForceGuard guard)
 {
 #pragma clang fp contract(off)
+    if (guard.flag && *(const volatile int *)guard.flag == guard.epoch) return;
     const SysDesc &D = descs[sys];
     const int i = blockIdx.x * 256 + threadIdx.x;
     double e1 = 0.0, e2 = 0.0;
@@ -541,22 +547,22 @@ void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, in
 }
 
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
-                        int nblocks, int max_ntypes, int max_count, hipStream_t s)
+                        int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s)
 {
     if (nblocks <= 0) return;
     const bool rows_lds = tiled_lds_bytes(max_ntypes, true, max_count) <= NB_LDS_ROW_LIMIT;
     const int head = (int)tiled_lds_head(max_ntypes, rows_lds);
     const size_t bytes = tiled_lds_bytes(max_ntypes, rows_lds, max_count);
     if (rows_lds)
-        hipLaunchKernelGGL(k_force_tiled<true>, dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head);
+        hipLaunchKernelGGL(k_force_tiled<true>, dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
     else
-        hipLaunchKernelGGL(k_force_tiled<false>, dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head);
+        hipLaunchKernelGGL(k_force_tiled<false>, dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
 }
 
 void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
-                        const int *cols, hipStream_t s)
+                        const int *cols, ForceGuard g, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_force_plain, dim3(max(nblk(n, 256), 1)), dim3(256), 0, s, d_desc, sys, F, rowptr, cols);
+    hipLaunchKernelGGL(k_force_plain, dim3(max(nblk(n, 256), 1)), dim3(256), 0, s, d_desc, sys, F, rowptr, cols, g);
 }
 
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s)

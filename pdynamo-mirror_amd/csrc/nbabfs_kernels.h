@@ -51,6 +51,7 @@ void launch_count_pairs(const Entry *entries, int nent, unsigned long long *tota
 void launch_export_pairs(const Entry *entries, int nent, const int *slot_atom, long long n,
                          unsigned long long *cursor, long long cap, long long *keys, hipStream_t s);
 void launch_csr_degree(const Entry *entries, int nent, const int *slot_atom, int *deg, hipStream_t s);
+void launch_csr_sort_rows(const int *rowptr, int n, int *cols, hipStream_t s);
 void launch_csr_fill(const Entry *entries, int nent, const int *slot_atom, int *cursor, int *cols, hipStream_t s);
 
 // Per-call kernels (nbabfs_force.hip).  Every launcher takes the batch's
@@ -67,14 +68,22 @@ struct PrepareCheck {
     const double *ref;                 // coordinates of the last list build (null: no test)
     const unsigned char *free_flags;   // null: every atom is free
     double bufsq;                      // (half the list buffer)^2
-    int *flag;                         // receives epoch when some free atom moved too far
+    int *flag;                         // host-mapped: receives epoch when some free atom moved too far
+    int *dflag;                        // device copy of the same flag (read by ForceGuard)
+    int epoch;
+};
+
+// A force launch queued before the update decision is known: every
+// workgroup returns at once when *flag == epoch (the list must be rebuilt).
+struct ForceGuard {
+    const int *flag;
     int epoch;
 };
 void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, int *heads, const PrepareCheck &chk,
                     hipStream_t s);
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
-                        int nblocks, int max_ntypes, int max_count, hipStream_t s);
+                        int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s);
 void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
-                        const int *cols, hipStream_t s);
+                        const int *cols, ForceGuard g, hipStream_t s);
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s);
 void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s);

@@ -171,3 +171,64 @@ def test_batch_matches_single():
             assert np.array_equal(e[k], er)
             assert np.array_equal(gs[k].numpy(), gr)
     batch.close()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_call_orders_match(precision):
+    """Update and energy calls in other orders give the same bits as update-then-energy.
+
+    The update queues the pair kernel for its coordinates before it knows whether
+    the list must be rebuilt; the cases below make that early work stale or
+    unused: an energy call on another coordinate array, two updates in a row, an
+    energy call with no update, and a move that forces a rebuild.
+    """
+    from pdynamo_amd import _native, synthetic
+    from pdynamo_amd.NBModelABFS import NBModelABFS
+    mm = synthetic.random_small_system(2500, 55.0, 21)
+    opts = NBModelABFS(precision=precision).NativeOptions()
+
+    def state():
+        return _native.NativeState(opts, mm.charges, mm.ljtypes, mm.ntypes, mm.tableA, mm.tableB,
+                                   exclusions=mm.exclusions)
+
+    x0 = mm.coordinates3
+    x1 = x0 + np.random.default_rng(4).normal(0.0, 0.05, x0.shape)      # within the half skin
+    x2 = x0 + np.random.default_rng(5).normal(0.0, 1.0, x0.shape)       # forces a rebuild
+
+    # References: the plain update-then-energy sequence through the same list
+    # history (list built at x0, reused at x1, rebuilt at x2), on a state of its own.
+    rs = state()
+    g_ref = _native.DeviceArray(x0.shape)
+    refs = {}
+    for key, x in (("x0", x0), ("x1", x1), ("x2", x2)):
+        d = _native.DeviceArray.from_numpy(x, rs)
+        rs.update(d.ptr)
+        g_ref.zero(rs)
+        rs.energy_async(d.ptr, g_ref.ptr)
+        refs[key] = (rs.energies()[0], g_ref.numpy(rs))
+        d.free()
+
+    st = state()
+    d0 = _native.DeviceArray.from_numpy(x0, st)
+    d1 = _native.DeviceArray.from_numpy(x1, st)
+    d1b = _native.DeviceArray.from_numpy(x1, st)
+    d2 = _native.DeviceArray.from_numpy(x2, st)
+    g = _native.DeviceArray(x0.shape)
+
+    def energy(d, key):
+        g.zero(st)
+        st.energy_async(d.ptr, g.ptr)
+        e, _ = st.energies()
+        assert np.array_equal(e, refs[key][0]), key
+        assert np.array_equal(g.numpy(st), refs[key][1]), key
+
+    assert st.update(d0.ptr)
+    energy(d0, "x0")
+    assert not st.update(d1.ptr)            # speculative pass for d1 ...
+    energy(d1b, "x1")                       # ... energy asked on another array
+    assert not st.update(d1.ptr)
+    assert not st.update(d1.ptr)            # two updates, one energy
+    energy(d1, "x1")
+    energy(d1, "x1")                        # energy with no update
+    assert st.update(d2.ptr)                # guarded pass discarded, list rebuilt
+    energy(d2, "x2")
