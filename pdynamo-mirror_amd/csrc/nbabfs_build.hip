@@ -228,40 +228,6 @@ __global__ __launch_bounds__(256) void k_cluster_geom(const double *__restrict__
     centre[c] = make_float4(ce[0], ce[1], ce[2], 0.0f);
 }
 
-// ------------------------------------------------------------------------------------------------
-// The reference pair predicate for atoms i, j (PairListGenerator.c:103-143,
-// 1020-1150): not excluded, OR selection (free atoms), and either the grid
-// rule (both on grid; stencil flag 2 = whole cell pair within the cutoff,
-// 1 = FP64 distance test, 0 = out of range) or, for the direct method, the
-// FP64 distance test alone.
-// ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ bool ref_pair_rule(int i, int j, const double *__restrict__ xyz, const int4 *__restrict__ cellv,
-                                              const SearchParams &P)
-{
-#pragma clang fp contract(off)
-    if (P.free_flags && !(P.free_flags[i] | P.free_flags[j])) return false;
-    int flag = 1;
-    if (P.use_grid) {
-        int4 ci = cellv[i], cj = cellv[j];
-        if (!(ci.w & cj.w)) return false;
-        int dx = cj.x - ci.x, dy = cj.y - ci.y, dz = cj.z - ci.z;
-        if (dx < -P.u || dx > P.u || dy < -P.u || dy > P.u || dz < -P.u || dz > P.u) return false;
-        int w = 2 * P.u + 1;
-        flag = P.stencil[((dx + P.u) * w + (dy + P.u)) * w + (dz + P.u)];
-        if (flag == 0) return false;
-    }
-    if (flag == 1) {
-        double xij = xyz[3 * (size_t)i] - xyz[3 * (size_t)j];
-        double yij = xyz[3 * (size_t)i + 1] - xyz[3 * (size_t)j + 1];
-        double zij = xyz[3 * (size_t)i + 2] - xyz[3 * (size_t)j + 2];
-        double r2 = ((xij * xij) + (yij * yij)) + (zij * zij);
-        if (!(r2 <= P.cutoff2)) return false;
-    }
-    for (int x = P.xptr[i]; x < P.xptr[i + 1]; x++)
-        if (P.xidx[x] == j) return false;
-    return true;
-}
-
 __device__ __forceinline__ float box_gap2(float4 alo, float4 ahi, float4 blo, float4 bhi)
 {
     float gx = fmaxf(0.0f, fmaxf(alo.x - bhi.x, blo.x - ahi.x));
@@ -271,35 +237,127 @@ __device__ __forceinline__ float box_gap2(float4 alo, float4 ahi, float4 blo, fl
 }
 
 // ------------------------------------------------------------------------------------------------
-// Cluster-pair search: one wavefront per super-cluster.  Candidate j-clusters
-// come from the x-y columns within reach, in a fixed order, so the entries of
-// each super-cluster are produced deterministically.
+// Cluster-pair search: one workgroup of NB_SEARCH_WAVES wavefronts per
+// super-cluster.  The super-cluster's atoms (FP64 position, grid cell, free
+// flag, exclusion list) and the grid stencil are staged in LDS.  The candidate
+// j-clusters of all x-y columns within reach are scanned as one flattened
+// range and appended to an LDS list in scan order; the waves then share out
+// the candidates round-robin and evaluate each with the reference pair rule
+// (the j-atom's data in registers, fetched one candidate ahead), writing the
+// result into the candidate's own scratch slot.  k_compact keeps the slots
+// with any listed pair, in candidate order, so the entries of each
+// super-cluster come out in a deterministic order.
+//
+// The reference pair predicate for atoms i, j (PairListGenerator.c:103-143,
+// 1020-1150): not excluded, OR selection (free atoms), and either the grid
+// rule (both on grid; stencil flag 2 = whole cell pair within the cutoff,
+// 1 = FP64 distance test, 0 = out of range) or, for the direct method, the
+// FP64 distance test alone.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_search(SearchParams P, const double *__restrict__ xyz,
-                                               const int4 *__restrict__ cellv, const int *__restrict__ slot_atom,
-                                               const float4 *__restrict__ blo, const float4 *__restrict__ bhi,
-                                               const int2 *__restrict__ colcl, Entry *__restrict__ scratch,
-                                               int *__restrict__ counts, int *__restrict__ overflow)
+#define NB_SEARCH_WAVES 8
+#define NB_SEARCH_THREADS (64 * NB_SEARCH_WAVES)
+#define NB_XPOOL 1024        // exclusion indices of one super-cluster held in LDS
+#define NB_STENCIL_LDS 4096  // stencil bytes held in LDS (u <= 7)
+#define NB_CAND 1024         // candidate list: flushed once it holds more than NB_CAND - NB_SEARCH_THREADS
+#define NB_MAXCOL 256        // x-y columns handled per flattened scan
+
+struct JAtom {
+    double x, y, z;
+    int4 cell;
+    int atom, free_;
+};
+
+__device__ __forceinline__ JAtom load_jatom(int J, int b, const double *__restrict__ xyz,
+                                            const int4 *__restrict__ cellv, const int *__restrict__ slot_atom,
+                                            const unsigned char *__restrict__ free_flags)
+{
+    JAtom r;
+    r.atom = slot_atom[J * NB_CL + b];
+    r.x = r.y = r.z = 0.0;
+    r.cell = make_int4(0, 0, 0, 0);
+    r.free_ = 1;
+    if (r.atom >= 0) {
+        r.x = xyz[3 * (size_t)r.atom];
+        r.y = xyz[3 * (size_t)r.atom + 1];
+        r.z = xyz[3 * (size_t)r.atom + 2];
+        r.cell = cellv[r.atom];
+        if (free_flags) r.free_ = free_flags[r.atom];
+    }
+    return r;
+}
+
+__global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, const double *__restrict__ xyz,
+                                                              const int4 *__restrict__ cellv,
+                                                              const int *__restrict__ slot_atom,
+                                                              const float4 *__restrict__ blo,
+                                                              const float4 *__restrict__ bhi,
+                                                              const int2 *__restrict__ colcl,
+                                                              Entry *__restrict__ scratch, int *__restrict__ ncand,
+                                                              int *__restrict__ counts, int *__restrict__ overflow)
 {
     const int sc = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int a = lane >> 3, b = lane & 7;
     const int I0 = sc * NB_SCC;
     __shared__ float4 sIlo[NB_SCC], sIhi[NB_SCC];
-    __shared__ int sIatom[NB_SC];
-    if (lane < NB_SCC) {
-        sIlo[lane] = blo[I0 + lane];
-        sIhi[lane] = bhi[I0 + lane];
+    __shared__ double sIx[NB_SC], sIy[NB_SC], sIz[NB_SC];
+    __shared__ int4 sIcell[NB_SC];
+    __shared__ int sIatom[NB_SC], sIfree[NB_SC], sXoff[NB_SC], sXcnt[NB_SC];
+    __shared__ int sXpool[NB_XPOOL];
+    __shared__ unsigned char sSten[NB_STENCIL_LDS];
+    __shared__ int sCand[NB_CAND];
+    __shared__ unsigned char sCm[NB_CAND];
+    __shared__ int sColStart[NB_MAXCOL], sColPre[NB_MAXCOL + 1];
+    __shared__ int sWcnt[NB_SEARCH_WAVES];
+
+    // Stage the super-cluster (wave 0) and the stencil (all waves).
+    if (w == 0) {
+        if (lane < NB_SCC) {
+            sIlo[lane] = blo[I0 + lane];
+            sIhi[lane] = bhi[I0 + lane];
+        }
+        const int ia = slot_atom[I0 * NB_CL + lane];
+        int xs = 0, xc = 0;
+        sIatom[lane] = ia;
+        if (ia >= 0) {
+            sIx[lane] = xyz[3 * (size_t)ia];
+            sIy[lane] = xyz[3 * (size_t)ia + 1];
+            sIz[lane] = xyz[3 * (size_t)ia + 2];
+            sIcell[lane] = cellv[ia];
+            sIfree[lane] = P.free_flags ? P.free_flags[ia] : 1;
+            xs = P.xptr[ia];
+            xc = P.xptr[ia + 1] - xs;
+        }
+        int pre = xc;   // inclusive scan of exclusion counts over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(pre, o, 64);
+            if (lane >= o) pre += v;
+        }
+        const int off = pre - xc;
+        if (off + xc <= NB_XPOOL) {
+            for (int k = 0; k < xc; k++) sXpool[off + k] = P.xidx[xs + k];
+            sXoff[lane] = off;
+        } else {
+            sXoff[lane] = -1 - xs;      // list read from global memory
+        }
+        sXcnt[lane] = xc;
     }
-    sIatom[lane] = slot_atom[I0 * NB_CL + lane];
+    const int wst = 2 * P.u + 1;
+    const int nst = P.use_grid ? wst * wst * wst : 0;
+    const bool sten_lds = nst <= NB_STENCIL_LDS;
+    if (sten_lds)
+        for (int k = tid; k < nst; k += NB_SEARCH_THREADS) sSten[k] = P.stencil[k];
     __syncthreads();
+
     float4 lo = sIlo[0], hi = sIhi[0];
     for (int c = 1; c < NB_SCC; c++) {
         lo.x = fminf(lo.x, sIlo[c].x); lo.y = fminf(lo.y, sIlo[c].y); lo.z = fminf(lo.z, sIlo[c].z);
         hi.x = fmaxf(hi.x, sIhi[c].x); hi.y = fmaxf(hi.y, sIhi[c].y); hi.z = fmaxf(hi.z, sIhi[c].z);
     }
-    int cnt = 0;
-    if (lo.x <= hi.x) {   // super-cluster holds at least one atom
+    int nvalid = 0;      // entries written by this wave
+    int base = 0;        // candidates handed out before the current list
+    if (lo.x <= hi.x) {  // super-cluster holds at least one atom (uniform over the block)
         const float reach = P.reach;
         const float reach2 = reach * reach;
         int cx0 = (int)floorf((lo.x - reach) / P.cw), cx1 = (int)floorf((hi.x + reach) / P.cw);
@@ -307,77 +365,181 @@ __global__ __launch_bounds__(64) void k_search(SearchParams P, const double *__r
         cx0 = max(cx0, 0); cy0 = max(cy0, 0);
         cx1 = min(cx1, P.ncx - 1); cy1 = min(cy1, P.ncy - 1);
         const int Jmax = I0 + NB_SCC - 1;
-        for (int cx = cx0; cx <= cx1; cx++) {
-            for (int cy = cy0; cy <= cy1; cy++) {
-                int2 r = colcl[cx * P.ncy + cy];
-                int e = min(r.y, Jmax + 1);
-                for (int base = r.x; base < e; base += 64) {
-                    int J = base + lane;
-                    int cm = 0;
-                    if (J < e) {
-                        float4 jl = blo[J], jh = bhi[J];
-                        if (box_gap2(lo, hi, jl, jh) < reach2) {
-                            for (int c = 0; c < NB_SCC; c++)
-                                if (J <= I0 + c && box_gap2(sIlo[c], sIhi[c], jl, jh) < reach2) cm |= 1 << c;
+        const int nyc = cy1 - cy0 + 1;
+        const int ncol = (cx1 >= cx0 && cy1 >= cy0) ? (cx1 - cx0 + 1) * nyc : 0;
+
+        // Evaluate the listed candidates sCand[0..npend): this wave takes
+        // k = w, w + NB_SEARCH_WAVES, ...
+        auto evaluate = [&](int npend) {
+            JAtom jn;
+            if (w < npend) jn = load_jatom(sCand[w], b, xyz, cellv, slot_atom, P.free_flags);
+            for (int k = w; k < npend; k += NB_SEARCH_WAVES) {
+                const JAtom jd = jn;
+                const int Jc = sCand[k];
+                const int cmc = sCm[k];
+                if (k + NB_SEARCH_WAVES < npend)
+                    jn = load_jatom(sCand[k + NB_SEARCH_WAVES], b, xyz, cellv, slot_atom, P.free_flags);
+                int lbits = 0, cbits = 0;
+                for (int c = 0; c < NB_SCC; c++) {
+                    if (!(cmc & (1 << c))) continue;
+                    const int li = c * NB_CL + a;
+                    const int i = sIatom[li];
+                    bool ok = (i >= 0) && (jd.atom >= 0) && ((Jc < I0 + c) || (b < a));
+                    if (ok && P.free_flags) ok = (sIfree[li] | jd.free_) != 0;
+                    int flag = 1;
+                    if (ok && P.use_grid) {
+                        const int4 ci = sIcell[li];
+                        ok = (ci.w & jd.cell.w) != 0;
+                        const int dx = jd.cell.x - ci.x, dy = jd.cell.y - ci.y, dz = jd.cell.z - ci.z;
+                        ok = ok && dx >= -P.u && dx <= P.u && dy >= -P.u && dy <= P.u && dz >= -P.u && dz <= P.u;
+                        if (ok) {
+                            const int si = ((dx + P.u) * wst + (dy + P.u)) * wst + (dz + P.u);
+                            flag = sten_lds ? sSten[si] : P.stencil[si];
+                            ok = flag != 0;
                         }
                     }
-                    unsigned long long cand = __ballot(cm != 0);
-                    while (cand) {
-                        int l = __ffsll((long long)cand) - 1;
-                        cand &= cand - 1;
-                        int Jc = __shfl(J, l, 64);
-                        int cmc = __shfl(cm, l, 64);
-                        int j = slot_atom[Jc * NB_CL + b];
-                        int lbits = 0, cbits = 0;
-                        for (int c = 0; c < NB_SCC; c++) {
-                            bool ok = false;
-                            if (cmc & (1 << c)) {
-                                int i = sIatom[c * NB_CL + a];
-                                ok = (i >= 0) && (j >= 0) && ((Jc < I0 + c) || (b < a)) && ref_pair_rule(i, j, xyz, cellv, P);
-                            }
-                            if (ok) lbits |= 1 << c;
-                            if (__ballot(ok)) cbits |= 1 << c;
+                    if (ok && flag == 1) {
+#pragma clang fp contract(off)
+                        const double xij = sIx[li] - jd.x, yij = sIy[li] - jd.y, zij = sIz[li] - jd.z;
+                        const double r2 = ((xij * xij) + (yij * yij)) + (zij * zij);
+                        ok = r2 <= P.cutoff2;
+                    }
+                    if (ok) {
+                        const int xo = sXoff[li], xn = sXcnt[li];
+                        if (xo >= 0) {
+                            for (int x = 0; x < xn; x++)
+                                if (sXpool[xo + x] == jd.atom) ok = false;
+                        } else {
+                            for (int x = 0; x < xn; x++)
+                                if (P.xidx[-1 - xo + x] == jd.atom) ok = false;
                         }
-                        if (cbits) {
-                            if (cnt < P.cap) {
-                                Entry *en = &scratch[(size_t)sc * P.cap + cnt];
-                                en->bits[lane] = (unsigned char)lbits;
-                                if (lane == 0) {
-                                    en->J = Jc;
-                                    en->sc = sc;
-                                    en->cbits = cbits;
-                                    en->pad = 0;
-                                }
-                            }
-                            cnt++;
-                        }
+                    }
+                    if (ok) lbits |= 1 << c;
+                    if (__ballot(ok)) cbits |= 1 << c;
+                }
+                const int slot = base + k;
+                if (slot < P.cap) {
+                    Entry *en = &scratch[(size_t)sc * P.cap + slot];
+                    en->bits[lane] = (unsigned char)lbits;
+                    if (lane == 0) {
+                        en->J = Jc;
+                        en->sc = sc;
+                        en->cbits = cbits;
+                        en->pad = 0;
                     }
                 }
+                if (cbits) nvalid++;
             }
+        };
+
+        int npend = 0;
+        for (int c0 = 0; c0 < ncol; c0 += NB_MAXCOL) {
+            const int nc = min(NB_MAXCOL, ncol - c0);
+            // Column ranges [start, min(end, Jmax+1)) and their prefix sums (wave 0).
+            if (w == 0) {
+                int carry = 0;
+                for (int cb0 = 0; cb0 < nc; cb0 += 64) {
+                    const int k = cb0 + lane;
+                    int len = 0;
+                    if (k < nc) {
+                        const int col = c0 + k;
+                        const int2 r = colcl[(cx0 + col / nyc) * P.ncy + cy0 + col % nyc];
+                        len = max(0, min(r.y, Jmax + 1) - r.x);
+                        sColStart[k] = r.x;
+                    }
+                    int incl = len;
+#pragma unroll
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const int v = __shfl_up(incl, o, 64);
+                        if (lane >= o) incl += v;
+                    }
+                    if (k < nc) sColPre[k] = carry + incl - len;
+                    carry += __shfl(incl, 63, 64);
+                }
+                if (lane == 0) sColPre[nc] = carry;
+            }
+            __syncthreads();
+            const int total = sColPre[nc];
+            for (int f0 = 0; f0 < total; f0 += NB_SEARCH_THREADS) {
+                const int f = f0 + tid;
+                int cm = 0, J = 0;
+                if (f < total) {
+                    int l = 0, h = nc;   // last column with sColPre[col] <= f
+                    while (h - l > 1) {
+                        const int m = (l + h) >> 1;
+                        if (sColPre[m] <= f) l = m; else h = m;
+                    }
+                    J = sColStart[l] + (f - sColPre[l]);
+                    const float4 jl = blo[J], jh = bhi[J];
+                    if (box_gap2(lo, hi, jl, jh) < reach2) {
+                        for (int c = 0; c < NB_SCC; c++)
+                            if (J <= I0 + c && box_gap2(sIlo[c], sIhi[c], jl, jh) < reach2) cm |= 1 << c;
+                    }
+                }
+                const unsigned long long m = __ballot(cm != 0);
+                if (lane == 0) sWcnt[w] = __popcll(m);
+                __syncthreads();
+                int off = npend, tot = 0;
+                for (int v = 0; v < NB_SEARCH_WAVES; v++) {
+                    const int cv = sWcnt[v];
+                    if (v < w) off += cv;
+                    tot += cv;
+                }
+                const int pos = off + __popcll(m & ((1ull << lane) - 1ull));
+                if (cm) {
+                    sCand[pos] = J;
+                    sCm[pos] = (unsigned char)cm;
+                }
+                npend += tot;
+                __syncthreads();
+                if (npend > NB_CAND - NB_SEARCH_THREADS) {
+                    evaluate(npend);
+                    base += npend;
+                    npend = 0;
+                    __syncthreads();
+                }
+            }
+            __syncthreads();
         }
+        evaluate(npend);
+        base += npend;
     }
-    if (lane == 0) {
-        counts[sc] = cnt;
-        if (cnt > P.cap) atomicMax(overflow, cnt);
+    if (lane == 0) sWcnt[w] = nvalid;
+    __syncthreads();
+    if (tid == 0) {
+        int tot = 0;
+        for (int v = 0; v < NB_SEARCH_WAVES; v++) tot += sWcnt[v];
+        counts[sc] = tot;
+        ncand[sc] = base;
+        if (base > P.cap) atomicMax(overflow, base);
     }
 }
 
-// Compact per-super-cluster entries and cut them into work items.
-__global__ __launch_bounds__(64) void k_compact(const Entry *__restrict__ scratch, int cap, const int *__restrict__ counts,
-                                                const int *__restrict__ eoff, const int *__restrict__ ioff, int per_item,
-                                                int sys, Entry *__restrict__ entries, WorkItem *__restrict__ items)
+// Compact per-super-cluster entries (candidate slots with any listed pair, in
+// candidate order) and cut them into work items.
+__global__ __launch_bounds__(64) void k_compact(const Entry *__restrict__ scratch, int cap, const int *__restrict__ ncand,
+                                                const int *__restrict__ counts, const int *__restrict__ eoff,
+                                                const int *__restrict__ ioff, int per_item, int sys,
+                                                Entry *__restrict__ entries, WorkItem *__restrict__ items)
 {
-    int sc = blockIdx.x;
-    int cnt = counts[sc], e0 = eoff[sc], i0 = ioff[sc];
-    for (int k = threadIdx.x; k < cnt; k += 64) entries[e0 + k] = scratch[(size_t)sc * cap + k];
-    int nit = (cnt + per_item - 1) / per_item;
-    for (int k = threadIdx.x; k < nit; k += 64) {
-        WorkItem w;
-        w.sys = sys;
-        w.sc = sc;
-        w.first = e0 + k * per_item;
-        w.count = min(per_item, cnt - k * per_item);
-        items[i0 + k] = w;
+    const int sc = blockIdx.x, lane = threadIdx.x;
+    const int nc = min(ncand[sc], cap), cnt = counts[sc], e0 = eoff[sc], i0 = ioff[sc];
+    int pos = 0;
+    for (int k0 = 0; k0 < nc; k0 += 64) {
+        const int k = k0 + lane;
+        const bool valid = k < nc && scratch[(size_t)sc * cap + k].cbits != 0;
+        const unsigned long long m = __ballot(valid);
+        if (valid) entries[e0 + pos + __popcll(m & ((1ull << lane) - 1ull))] = scratch[(size_t)sc * cap + k];
+        pos += __popcll(m);
+    }
+    const int nit = (cnt + per_item - 1) / per_item;
+    for (int k = lane; k < nit; k += 64) {
+        WorkItem wi;
+        wi.sys = sys;
+        wi.sc = sc;
+        wi.first = e0 + k * per_item;
+        wi.count = min(per_item, cnt - k * per_item);
+        items[i0 + k] = wi;
     }
 }
 
@@ -579,12 +741,12 @@ void launch_cluster_geom(const double *xyz, const int *slot_atom, int ncl, const
 }
 
 void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4 *cellv, const int *slot_atom,
-                   const float4 *blo, const float4 *bhi, const int2 *colcl, Entry *scratch, int *counts,
+                   const float4 *blo, const float4 *bhi, const int2 *colcl, Entry *scratch, int *ncand, int *counts,
                    int *overflow, hipStream_t s)
 {
     if (nsc > 0)
-        hipLaunchKernelGGL(k_search, dim3(nsc), dim3(64), 0, s, P, xyz, cellv, slot_atom, blo, bhi, colcl, scratch,
-                           counts, overflow);
+        hipLaunchKernelGGL(k_search, dim3(nsc), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, cellv, slot_atom, blo, bhi,
+                           colcl, scratch, ncand, counts, overflow);
 }
 
 void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, hipStream_t s)
@@ -592,12 +754,12 @@ void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, 
     hipLaunchKernelGGL(k_items_per_sc, dim3(nblk(nsc, 256)), dim3(256), 0, s, counts, nsc, per_item, nitems);
 }
 
-void launch_compact(const Entry *scratch, int cap, const int *counts, const int *eoff, const int *ioff, int nsc,
-                    int per_item, int sys, Entry *entries, WorkItem *items, hipStream_t s)
+void launch_compact(const Entry *scratch, int cap, const int *ncand, const int *counts, const int *eoff,
+                    const int *ioff, int nsc, int per_item, int sys, Entry *entries, WorkItem *items, hipStream_t s)
 {
     if (nsc > 0)
-        hipLaunchKernelGGL(k_compact, dim3(nsc), dim3(64), 0, s, scratch, cap, counts, eoff, ioff, per_item, sys,
-                           entries, items);
+        hipLaunchKernelGGL(k_compact, dim3(nsc), dim3(64), 0, s, scratch, cap, ncand, counts, eoff, ioff, per_item,
+                           sys, entries, items);
 }
 
 void launch_count_pairs(const Entry *entries, int nent, unsigned long long *total, hipStream_t s)

@@ -327,7 +327,7 @@ struct nbabfs_state {
     hipEvent_t prep_done = nullptr;
 
     // Build temporaries.
-    DBuf<int> key, count, off, cursor, sorted, padded, colstart, scantmp, counts, eoff, nits, ioff, flag;
+    DBuf<int> key, count, off, cursor, sorted, padded, colstart, scantmp, counts, ncand, eoff, nits, ioff, flag;
     DBuf<int2> colcl;
     DBuf<double> bbpart, bbox;
     DBuf<unsigned long long> u64;
@@ -554,13 +554,14 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     P.ncx = g.ncx;
     P.ncy = g.ncy;
     CHK(S->counts.ensure(S->nsc));
+    CHK(S->ncand.ensure(S->nsc));
     for (int attempt = 0; attempt < 4; attempt++) {
         P.cap = S->cap;
         CHK(S->scratch.ensure((size_t)S->nsc * S->cap));
         CHK(S->flag.ensure(1));
         launch_fill_int(S->flag.p, 1, 0, st);
         launch_search(P, S->nsc, d_xyz, S->cellv.p, S->slot_atom.p, S->blo.p, S->bhi.p, S->colcl.p, S->scratch.p,
-                      S->counts.p, S->flag.p, st);
+                      S->ncand.p, S->counts.p, S->flag.p, st);
         HIPCHK(hipMemcpyAsync(S->h_int.p, S->flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         const int need = S->h_int.p[0];
@@ -586,8 +587,8 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     S->nitems = S->h_int.p[2] + S->h_int.p[3];
     CHK(S->entries.ensure(S->nent));
     CHK(S->items.ensure(S->nitems));
-    launch_compact(S->scratch.p, S->cap, S->counts.p, S->eoff.p, S->ioff.p, S->nsc, per_item, 0, S->entries.p,
-                   S->items.p, st);
+    launch_compact(S->scratch.p, S->cap, S->ncand.p, S->counts.p, S->eoff.p, S->ioff.p, S->nsc, per_item, 0,
+                   S->entries.p, S->items.p, st);
     CHK(S->u64.ensure(2));
     HIPCHK(hipMemsetAsync(S->u64.p, 0, 2 * sizeof(unsigned long long), st));
     launch_count_pairs(S->entries.p, S->nent, S->u64.p, st);

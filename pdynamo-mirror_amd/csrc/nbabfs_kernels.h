@@ -26,7 +26,7 @@ struct SearchParams {
     float reach;                          // cull distance for FP32 boxes
     float cw;
     int ncx, ncy;
-    int cap;                              // entries per super-cluster
+    int cap;                              // candidate slots per super-cluster
 };
 
 // List construction (nbabfs_build.hip).
@@ -42,11 +42,11 @@ void launch_slots(const int *sorted, const int *key, int n, int nzb, const int *
 void launch_cluster_geom(const double *xyz, const int *slot_atom, int ncl, const double *origin, float4 *blo,
                          float4 *bhi, float4 *centre, hipStream_t s);
 void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4 *cellv, const int *slot_atom,
-                   const float4 *blo, const float4 *bhi, const int2 *colcl, Entry *scratch, int *counts,
+                   const float4 *blo, const float4 *bhi, const int2 *colcl, Entry *scratch, int *ncand, int *counts,
                    int *overflow, hipStream_t s);
 void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, hipStream_t s);
-void launch_compact(const Entry *scratch, int cap, const int *counts, const int *eoff, const int *ioff, int nsc,
-                    int per_item, int sys, Entry *entries, WorkItem *items, hipStream_t s);
+void launch_compact(const Entry *scratch, int cap, const int *ncand, const int *counts, const int *eoff,
+                    const int *ioff, int nsc, int per_item, int sys, Entry *entries, WorkItem *items, hipStream_t s);
 void launch_count_pairs(const Entry *entries, int nent, unsigned long long *total, hipStream_t s);
 void launch_export_pairs(const Entry *entries, int nent, const int *slot_atom, long long n,
                          unsigned long long *cursor, long long cap, long long *keys, hipStream_t s);
