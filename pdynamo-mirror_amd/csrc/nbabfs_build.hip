@@ -41,14 +41,22 @@ __global__ __launch_bounds__(256) void k_bbox_partial(const double *__restrict__
     if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(64) void k_bbox_final(const double *__restrict__ part, int nblocks, double *__restrict__ out)
+__global__ __launch_bounds__(256) void k_bbox_final(const double *__restrict__ part, int nblocks, double *__restrict__ out)
 {
-    if (threadIdx.x < 6) {
-        int d = threadIdx.x;
-        double v = part[d];
-        for (int b = 1; b < nblocks; b++) v = (d < 3) ? fmin(v, part[6 * b + d]) : fmax(v, part[6 * b + d]);
-        out[d] = v;
+    __shared__ double s[6][256];
+    double v[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int b = threadIdx.x; b < nblocks; b += 256)
+        for (int d = 0; d < 6; d++) v[d] = (d < 3) ? fmin(v[d], part[6 * b + d]) : fmax(v[d], part[6 * b + d]);
+    for (int d = 0; d < 6; d++) s[d][threadIdx.x] = v[d];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int d = 0; d < 6; d++)
+                s[d][threadIdx.x] = (d < 3) ? fmin(s[d][threadIdx.x], s[d][threadIdx.x + o])
+                                            : fmax(s[d][threadIdx.x], s[d][threadIdx.x + o]);
+        __syncthreads();
     }
+    if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -302,7 +310,7 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
     __shared__ float4 sIlo[NB_SCC], sIhi[NB_SCC];
     __shared__ double sIx[NB_SC], sIy[NB_SC], sIz[NB_SC];
     __shared__ int4 sIcell[NB_SC];
-    __shared__ int sIatom[NB_SC], sIfree[NB_SC], sXoff[NB_SC], sXcnt[NB_SC];
+    __shared__ int sIatom[NB_SC], sIfree[NB_SC], sXoff[NB_SC], sXcnt[NB_SC], sXlo[NB_SC], sXhi[NB_SC];
     __shared__ int sXpool[NB_XPOOL];
     __shared__ unsigned char sSten[NB_STENCIL_LDS];
     __shared__ int sCand[NB_CAND];
@@ -342,6 +350,9 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
             sXoff[lane] = -1 - xs;      // list read from global memory
         }
         sXcnt[lane] = xc;
+        // Rows are ascending: j outside [first, last] is not excluded.
+        sXlo[lane] = xc > 0 ? P.xidx[xs] : 0;
+        sXhi[lane] = xc > 0 ? P.xidx[xs + xc - 1] : -1;
     }
     const int wst = 2 * P.u + 1;
     const int nst = P.use_grid ? wst * wst * wst : 0;
@@ -404,7 +415,7 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
                         const double r2 = ((xij * xij) + (yij * yij)) + (zij * zij);
                         ok = r2 <= P.cutoff2;
                     }
-                    if (ok) {
+                    if (ok && jd.atom >= sXlo[li] && jd.atom <= sXhi[li]) {
                         const int xo = sXoff[li], xn = sXcnt[li];
                         if (xo >= 0) {
                             for (int x = 0; x < xn; x++)
@@ -516,22 +527,32 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
 }
 
 // Compact per-super-cluster entries (candidate slots with any listed pair, in
-// candidate order) and cut them into work items.
+// candidate order), count their pairs into *npairs, and cut them into work
+// items.
 __global__ __launch_bounds__(64) void k_compact(const Entry *__restrict__ scratch, int cap, const int *__restrict__ ncand,
                                                 const int *__restrict__ counts, const int *__restrict__ eoff,
                                                 const int *__restrict__ ioff, int per_item, int sys,
-                                                Entry *__restrict__ entries, WorkItem *__restrict__ items)
+                                                Entry *__restrict__ entries, WorkItem *__restrict__ items,
+                                                unsigned long long *__restrict__ npairs)
 {
     const int sc = blockIdx.x, lane = threadIdx.x;
     const int nc = min(ncand[sc], cap), cnt = counts[sc], e0 = eoff[sc], i0 = ioff[sc];
     int pos = 0;
+    unsigned long long pc = 0;
     for (int k0 = 0; k0 < nc; k0 += 64) {
         const int k = k0 + lane;
         const bool valid = k < nc && scratch[(size_t)sc * cap + k].cbits != 0;
         const unsigned long long m = __ballot(valid);
-        if (valid) entries[e0 + pos + __popcll(m & ((1ull << lane) - 1ull))] = scratch[(size_t)sc * cap + k];
+        if (valid) {
+            const Entry en = scratch[(size_t)sc * cap + k];
+            entries[e0 + pos + __popcll(m & ((1ull << lane) - 1ull))] = en;
+            const unsigned long long *wds = (const unsigned long long *)en.bits;
+            for (int q = 0; q < NB_SC / 8; q++) pc += __popcll(wds[q]);
+        }
         pos += __popcll(m);
     }
+    for (int o = 32; o >= 1; o >>= 1) pc += __shfl_xor(pc, o, 64);
+    if (lane == 0 && pc) atomicAdd(npairs, pc);
     const int nit = (cnt + per_item - 1) / per_item;
     for (int k = lane; k < nit; k += 64) {
         WorkItem wi;
@@ -548,19 +569,6 @@ __global__ __launch_bounds__(256) void k_items_per_sc(const int *__restrict__ co
 {
     int s = blockIdx.x * 256 + threadIdx.x;
     if (s < nsc) nitems[s] = (counts[s] + per_item - 1) / per_item;
-}
-
-// Pair count of the list (popcount of every mask).
-__global__ __launch_bounds__(256) void k_count_pairs(const Entry *__restrict__ entries, int nent,
-                                                     unsigned long long *__restrict__ total)
-{
-    unsigned long long s = 0;
-    for (int e = blockIdx.x * 256 + threadIdx.x; e < nent; e += gridDim.x * 256) {
-        const unsigned long long *w = (const unsigned long long *)entries[e].bits;
-        for (int k = 0; k < NB_SC / 8; k++) s += __popcll(w[k]);
-    }
-    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(total, s);
 }
 
 // Canonical pair keys max*n+min (for parity against the reference set).
@@ -680,7 +688,7 @@ static inline int nblk(size_t m, int b) { return (int)((m + b - 1) / b); }
 void launch_bbox(const double *xyz, int n, double *part, int nparts, double *out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_bbox_partial, dim3(nparts), dim3(256), 0, s, xyz, n, part);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, s, part, nparts, out);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, part, nparts, out);
 }
 
 void launch_bin(const double *xyz, int n, const BuildGrid &g, int4 *cellv, int *key, int *count, hipStream_t s)
@@ -755,17 +763,12 @@ void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, 
 }
 
 void launch_compact(const Entry *scratch, int cap, const int *ncand, const int *counts, const int *eoff,
-                    const int *ioff, int nsc, int per_item, int sys, Entry *entries, WorkItem *items, hipStream_t s)
+                    const int *ioff, int nsc, int per_item, int sys, Entry *entries, WorkItem *items,
+                    unsigned long long *npairs, hipStream_t s)
 {
     if (nsc > 0)
         hipLaunchKernelGGL(k_compact, dim3(nsc), dim3(64), 0, s, scratch, cap, ncand, counts, eoff, ioff, per_item,
-                           sys, entries, items);
-}
-
-void launch_count_pairs(const Entry *entries, int nent, unsigned long long *total, hipStream_t s)
-{
-    if (nent > 0)
-        hipLaunchKernelGGL(k_count_pairs, dim3(min(nblk(nent, 256), 1024)), dim3(256), 0, s, entries, nent, total);
+                           sys, entries, items, npairs);
 }
 
 void launch_export_pairs(const Entry *entries, int nent, const int *slot_atom, long long n,

@@ -244,6 +244,20 @@ __global__ void k_remap_items(const WorkItem *__restrict__ src, int count, int s
     }
 }
 
+// Totals of a list build: entries, work items, and the search's capacity
+// need (0 when every super-cluster fitted), written to mapped host memory.
+__global__ void k_list_totals(const int *__restrict__ eoff, const int *__restrict__ counts, const int *__restrict__ ioff,
+                              const int *__restrict__ nits, int nsc, const int *__restrict__ need,
+                              int *__restrict__ out)
+{
+    if (threadIdx.x == 0) {
+        volatile int *o = out;
+        o[0] = eoff[nsc - 1] + counts[nsc - 1];
+        o[1] = ioff[nsc - 1] + nits[nsc - 1];
+        o[2] = *need;
+    }
+}
+
 struct CheckDesc {
     const double *xyz, *ref;
     const unsigned char *free_flags;
@@ -321,7 +335,7 @@ struct nbabfs_state {
     // Coordinates for which the last update already queued the prepare pass
     // and the pair kernel (the energy call on the same array only finalizes).
     const double *forced_xyz = nullptr;
-    MappedInts upflag;
+    MappedInts upflag, tail;
     DBuf<int> dflag;
     int epoch = 0;
     hipEvent_t prep_done = nullptr;
@@ -555,6 +569,13 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     P.ncy = g.ncy;
     CHK(S->counts.ensure(S->nsc));
     CHK(S->ncand.ensure(S->nsc));
+    // Search, then the entry and item offsets; the totals and the capacity
+    // check reach the host through mapped memory in one synchronisation.
+    const int per_item = std::max(1, o.entriesPerItem > 0 ? o.entriesPerItem : 16);
+    CHK(S->eoff.ensure(S->nsc + 1));
+    CHK(S->nits.ensure(S->nsc));
+    CHK(S->ioff.ensure(S->nsc + 1));
+    CHK(S->tail.ensure(4));
     for (int attempt = 0; attempt < 4; attempt++) {
         P.cap = S->cap;
         CHK(S->scratch.ensure((size_t)S->nsc * S->cap));
@@ -562,36 +583,26 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
         launch_fill_int(S->flag.p, 1, 0, st);
         launch_search(P, S->nsc, d_xyz, S->cellv.p, S->slot_atom.p, S->blo.p, S->bhi.p, S->colcl.p, S->scratch.p,
                       S->ncand.p, S->counts.p, S->flag.p, st);
-        HIPCHK(hipMemcpyAsync(S->h_int.p, S->flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        launch_scan(S->counts.p, S->eoff.p, S->nsc, S->scantmp.p, st);
+        launch_items_per_sc(S->counts.p, S->nsc, per_item, S->nits.p, st);
+        launch_scan(S->nits.p, S->ioff.p, S->nsc, S->scantmp.p, st);
+        hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, S->eoff.p, S->counts.p, S->ioff.p, S->nits.p,
+                           S->nsc, S->flag.p, S->tail.d);
+        HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
-        const int need = S->h_int.p[0];
+        const int need = ((volatile int *)S->tail.h)[2];
         if (need <= 0) break;
         if (attempt == 3) return fail(NBABFS_STATUS_LOGIC_ERROR, "pair search capacity did not converge");
         S->cap = need + need / 8 + 64;
     }
-
-    // Offsets, compaction into work items.
-    const int per_item = std::max(1, o.entriesPerItem > 0 ? o.entriesPerItem : 16);
-    CHK(S->eoff.ensure(S->nsc + 1));
-    CHK(S->nits.ensure(S->nsc));
-    CHK(S->ioff.ensure(S->nsc + 1));
-    launch_scan(S->counts.p, S->eoff.p, S->nsc, S->scantmp.p, st);
-    launch_items_per_sc(S->counts.p, S->nsc, per_item, S->nits.p, st);
-    launch_scan(S->nits.p, S->ioff.p, S->nsc, S->scantmp.p, st);
-    HIPCHK(hipMemcpyAsync(S->h_int.p, S->eoff.p + (S->nsc - 1), sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(S->h_int.p + 1, S->counts.p + (S->nsc - 1), sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(S->h_int.p + 2, S->ioff.p + (S->nsc - 1), sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(S->h_int.p + 3, S->nits.p + (S->nsc - 1), sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    S->nent = S->h_int.p[0] + S->h_int.p[1];
-    S->nitems = S->h_int.p[2] + S->h_int.p[3];
+    S->nent = ((volatile int *)S->tail.h)[0];
+    S->nitems = ((volatile int *)S->tail.h)[1];
     CHK(S->entries.ensure(S->nent));
     CHK(S->items.ensure(S->nitems));
-    launch_compact(S->scratch.p, S->cap, S->ncand.p, S->counts.p, S->eoff.p, S->ioff.p, S->nsc, per_item, 0,
-                   S->entries.p, S->items.p, st);
     CHK(S->u64.ensure(2));
     HIPCHK(hipMemsetAsync(S->u64.p, 0, 2 * sizeof(unsigned long long), st));
-    launch_count_pairs(S->entries.p, S->nent, S->u64.p, st);
+    launch_compact(S->scratch.p, S->cap, S->ncand.p, S->counts.p, S->eoff.p, S->ioff.p, S->nsc, per_item, 0,
+                   S->entries.p, S->items.p, S->u64.p, st);
     unsigned long long np = 0;
     HIPCHK(hipMemcpyAsync(&np, S->u64.p, sizeof(np), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));

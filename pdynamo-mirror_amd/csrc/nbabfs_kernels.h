@@ -46,8 +46,8 @@ void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4
                    int *overflow, hipStream_t s);
 void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, hipStream_t s);
 void launch_compact(const Entry *scratch, int cap, const int *ncand, const int *counts, const int *eoff,
-                    const int *ioff, int nsc, int per_item, int sys, Entry *entries, WorkItem *items, hipStream_t s);
-void launch_count_pairs(const Entry *entries, int nent, unsigned long long *total, hipStream_t s);
+                    const int *ioff, int nsc, int per_item, int sys, Entry *entries, WorkItem *items,
+                    unsigned long long *npairs, hipStream_t s);
 void launch_export_pairs(const Entry *entries, int nent, const int *slot_atom, long long n,
                          unsigned long long *cursor, long long cap, long long *keys, hipStream_t s);
 void launch_csr_degree(const Entry *entries, int nent, const int *slot_atom, int *deg, hipStream_t s);
