@@ -41,6 +41,36 @@ def algorithmic_flops(xyz, keys, n):
     return 8.0 * n_list + 37.0 * n_plain + 63.0 * n_switch, n_list, n_plain, n_switch
 
 
+def copy_calibration(mib):
+    """Device-to-device copies of a known size (the runtime's copy kernel), so that a
+    profiler's FETCH_SIZE / WRITE_SIZE of the pair kernel can be scaled by what the same
+    counters read for a copy of exactly mib MiB read and mib MiB written."""
+    import ctypes
+    from pdynamo_amd import _native
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib * (1 << 20) // 8
+    src = _native.DeviceArray((n,))
+    dst = _native.DeviceArray((n,))
+    src.zero()
+    for _ in range(3):
+        rc = hip.hipMemcpy(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), ctypes.c_size_t(n * 8), 3)
+        if rc != 0:
+            raise RuntimeError("hipMemcpy device-to-device failed: %d" % rc)
+    _native.synchronize()
+    src.free()
+    dst.free()
+
+
+def traffic_from_summary(path):
+    """HBM bytes per launch recorded in a rocprofv3 traffic summary under profiles/."""
+    if not path or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("TRAFFIC_BYTES_PER_LAUNCH:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def make_workload(name):
     from pdynamo_amd import synthetic
     if name == "solute30000":
@@ -165,6 +195,11 @@ def main():
                     help="Gaussian displacement (A) applied before every call, cumulative; rebuilds follow the "
                          "half-skin rule (the 200,001-atom protocol uses 0.05)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device to device three times (counter calibration)")
+    ap.add_argument("--traffic-summary", default=None,
+                    help="rocprofv3 traffic summary giving roofline.traffic (default: the committed one for the "
+                         "default workload)")
     ap.add_argument("--no-roofline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -257,8 +292,12 @@ def main():
         kms = stats["forceKernelMs"]
         achieved = flops / (kms * 1e-3) / 1e12 if kms > 0 else None
         peak = PEAK_FP32_TFLOPS if args.precision == "mixed" else PEAK_FP64_TFLOPS
+        tpath = args.traffic_summary
+        if tpath is None and args.workload == "solute30000" and args.precision == "mixed":
+            tpath = os.path.join(ROOT, "profiles", "traffic_k_force_tiled_solute30000.md")
         result["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                              "frac": (achieved / peak) if achieved else None, "traffic": None,
+                              "frac": (achieved / peak) if achieved else None,
+                              "traffic": traffic_from_summary(tpath),
                               "kernel": "k_force_tiled" if args.precision == "mixed" else "k_force_plain",
                               "kernel_ms": kms, "launches": int(stats["forceKernelLaunches"]),
                               "flops_per_launch": flops, "pairs_d_le_8": n_plain, "pairs_8_12": n_switch}
@@ -267,6 +306,8 @@ def main():
         if os.environ.get("OMP_NUM_THREADS", "").isdigit():
             threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
         result["cpu_baseline"] = cpu_baseline(mm, args.cpu_calls, threads)
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
     print(json.dumps(result))
 
 

@@ -116,7 +116,11 @@ int nbabfs_energy_host(nbabfs_state *state, const double *xyz, double *grad_inou
                        int *present, int *rebuilt);
 
 /* Device-resident variants.  d_xyz / d_grad are device pointers (n*3 FP64);
- * d_grad may be NULL.  nbabfs_update: NBModelABFS_Update only.
+ * d_grad may be NULL.  nbabfs_update: NBModelABFS_Update; when the list is
+ * kept it has already queued the pair evaluation for d_xyz on the state's
+ * stream, and an nbabfs_mmmm_energy_async on the same array only finishes it
+ * (so the array must not change between the two calls; any other array, or a
+ * second energy call, is evaluated from scratch).
  * nbabfs_mmmm_energy_async: enqueue the energy/gradient on the state's stream;
  * the four energies land in state-owned device memory readable with
  * nbabfs_energies (which synchronises). */
