@@ -195,6 +195,8 @@ def main():
                     help="Gaussian displacement (A) applied before every call, cumulative; rebuilds follow the "
                          "half-skin rule (the 200,001-atom protocol uses 0.05)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the library's work-item autotuner (entriesPerItem / wavesPerCU) before timing")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy this many MiB device to device three times (counter calibration)")
     ap.add_argument("--traffic-summary", default=None,
@@ -231,6 +233,10 @@ def main():
     else:
         xs = [_native.DeviceArray.from_numpy(mm.coordinates3, st)]
     _native.synchronize()
+    tuned = None
+    if not args.no_autotune and args.entries_per_item == 0 and args.waves_per_cu < 0:
+        epi, wpc, kms = st.autotune(xs[0].ptr)
+        tuned = {"entriesPerItem": epi, "wavesPerCU": wpc, "kernel_ms": kms}
     counter = [0]
     rebuilds = [0]
 
@@ -285,7 +291,7 @@ def main():
                    "list_cutoff": 13.5, "cutoffs": [0.5, 8.0, 12.0],
                    "energies_kJ_mol": [float(v) for v in e], "work_items": int(stats["workItems"]),
                    "cluster_entries": int(stats["entries"]), "moves_sigma_A": args.moves,
-                   "rebuilds_in_timed_steps": int(timed_rebuilds)},
+                   "rebuilds_in_timed_steps": int(timed_rebuilds), "autotuned": tuned},
     }
     if not args.no_roofline:
         flops, n_list2, n_plain, n_switch = algorithmic_flops(x_last, keys, mm.natoms)

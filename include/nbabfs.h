@@ -128,6 +128,14 @@ int nbabfs_update(nbabfs_state *state, const double *d_xyz, int *rebuilt);
 int nbabfs_mmmm_energy_async(nbabfs_state *state, const double *d_xyz, double *d_grad);
 int nbabfs_energies(nbabfs_state *state, double *energies, int *present);
 
+/* Work-item autotuner (no counterpart in the reference, which has no work
+ * items): updates the list for d_xyz (as nbabfs_update), then times the pair
+ * kernel for each candidate (entriesPerItem, wavesPerCU) on that list and
+ * keeps the fastest in the state's options.  The pair list is not changed;
+ * only its cut into work items.  Outputs may be NULL. */
+int nbabfs_autotune(nbabfs_state *state, const double *d_xyz, int *best_entries_per_item, int *best_waves_per_cu,
+                    double *best_kernel_ms);
+
 /* Statistics (NBModelABFSState.pyx:68-97) and kernel timing with HIP events on
  * the launch stream (enable != 0 turns it on). */
 int nbabfs_stats_get(nbabfs_state *state, nbabfs_stats *out);

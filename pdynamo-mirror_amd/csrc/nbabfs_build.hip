@@ -564,6 +564,24 @@ __global__ __launch_bounds__(64) void k_compact(const Entry *__restrict__ scratc
     }
 }
 
+// Work items of an already compacted list for another entries-per-item cut.
+__global__ __launch_bounds__(256) void k_cut_items(const int *__restrict__ counts, const int *__restrict__ eoff,
+                                                   const int *__restrict__ ioff, int nsc, int per_item, int sys,
+                                                   WorkItem *__restrict__ items)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= nsc) return;
+    const int cnt = counts[s], e0 = eoff[s], i0 = ioff[s];
+    for (int k = 0; k * per_item < cnt; k++) {
+        WorkItem wi;
+        wi.sys = sys;
+        wi.sc = s;
+        wi.first = e0 + k * per_item;
+        wi.count = min(per_item, cnt - k * per_item);
+        items[i0 + k] = wi;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_items_per_sc(const int *__restrict__ counts, int nsc, int per_item,
                                                       int *__restrict__ nitems)
 {
@@ -755,6 +773,14 @@ void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4
     if (nsc > 0)
         hipLaunchKernelGGL(k_search, dim3(nsc), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, cellv, slot_atom, blo, bhi,
                            colcl, scratch, ncand, counts, overflow);
+}
+
+void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int nsc, int per_item, int sys,
+                      WorkItem *items, hipStream_t s)
+{
+    if (nsc > 0)
+        hipLaunchKernelGGL(k_cut_items, dim3(nblk(nsc, 256)), dim3(256), 0, s, counts, eoff, ioff, nsc, per_item, sys,
+                           items);
 }
 
 void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, hipStream_t s)

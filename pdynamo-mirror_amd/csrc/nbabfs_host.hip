@@ -814,6 +814,99 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
     return NBABFS_STATUS_CONTINUE;
 }
 
+// Re-cut the compacted entries of the current list into work items of
+// per_item entries (the entries and their order are unchanged).
+static int recut_items(nbabfs_state *S, int per_item)
+{
+    hipStream_t st = S->stream;
+    if (S->nsc <= 0) return NBABFS_STATUS_CONTINUE;
+    launch_items_per_sc(S->counts.p, S->nsc, per_item, S->nits.p, st);
+    launch_scan(S->nits.p, S->ioff.p, S->nsc, S->scantmp.p, st);
+    hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, S->eoff.p, S->counts.p, S->ioff.p, S->nits.p, S->nsc,
+                       S->flag.p, S->tail.d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    S->nitems = ((volatile int *)S->tail.h)[1];
+    CHK(S->items.ensure(std::max(S->nitems, 1)));
+    launch_cut_items(S->counts.p, S->eoff.p, S->ioff.p, S->nsc, per_item, 0, S->items.p, st);
+    std::vector<int> qs(9);
+    for (int q = 0; q <= 8; q++) qs[q] = (int)((long long)S->nitems * q / 8);
+    CHK(S->qstarts.upload(qs.data(), 9, st));
+    HIPCHK(hipGetLastError());
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// Time the pair kernel on the current list for each (entriesPerItem,
+// wavesPerCU) candidate and keep the fastest.
+static int autotune(nbabfs_state *S, const double *d_xyz, int *best_epi, int *best_wpc, double *best_ms)
+{
+    static const int cand[][2] = {{8, 0}, {12, 0}, {16, 0}, {24, 0}, {32, 0}, {48, 0}, {8, 12}, {16, 12}, {32, 12}};
+    const int reps = 5;
+    hipStream_t st = S->stream;
+    CHK(update(S, d_xyz, nullptr));
+    S->forced_xyz = nullptr;
+    if (S->opt.precision == NBABFS_PLAIN_FP64 || S->n == 0) {
+        if (best_epi) *best_epi = S->opt.entriesPerItem;
+        if (best_wpc) *best_wpc = S->opt.wavesPerCU;
+        if (best_ms) *best_ms = 0.0;
+        return NBABFS_STATUS_CONTINUE;
+    }
+    CHK(ensure_call_buffers(S));
+    CHK(ensure_desc(S, d_xyz, S->last_grad));
+    PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, 0};
+    launch_prepare(S->d_desc.p, 1, S->n_pad, 1, S->heads.p, none, st);
+    hipEvent_t ev[2 * reps];
+    for (int k = 0; k < 2 * reps; k++) HIPCHK(hipEventCreate(&ev[k]));
+    const nbabfs_options keep = S->opt;
+    const bool was_timing = S->timing;
+    S->timing = false;
+    double best = 1e300;
+    int bi = -1;
+    int rc = NBABFS_STATUS_CONTINUE;
+    for (size_t c = 0; c < sizeof(cand) / sizeof(cand[0]) && rc == NBABFS_STATUS_CONTINUE; c++) {
+        S->opt.entriesPerItem = cand[c][0];
+        S->opt.wavesPerCU = cand[c][1];
+        rc = recut_items(S, cand[c][0]);
+        // Each launch is timed alone; persistent queues are reset before it.
+        for (int r = -1; r < reps && rc == NBABFS_STATUS_CONTINUE; r++) {
+            if (S->opt.wavesPerCU > 0) launch_fill_int(S->heads.p, 8 * 32, 0, st);
+            if (r >= 0 && hipEventRecord(ev[2 * r], st) != hipSuccess) rc = fail(NBABFS_STATUS_LOGIC_ERROR, "event record failed");
+            if (rc == NBABFS_STATUS_CONTINUE) rc = launch_force(S, ForceGuard{nullptr, 0});
+            if (rc == NBABFS_STATUS_CONTINUE && r >= 0 && hipEventRecord(ev[2 * r + 1], st) != hipSuccess)
+                rc = fail(NBABFS_STATUS_LOGIC_ERROR, "event record failed");
+        }
+        if (rc != NBABFS_STATUS_CONTINUE) break;
+        if (hipEventSynchronize(ev[2 * reps - 1]) != hipSuccess) {
+            rc = fail(NBABFS_STATUS_LOGIC_ERROR, "event synchronisation failed");
+            break;
+        }
+        double tot = 0.0;
+        for (int r = 0; r < reps; r++) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]) != hipSuccess) rc = fail(NBABFS_STATUS_LOGIC_ERROR, "event timing failed");
+            tot += ms;
+        }
+        if (rc == NBABFS_STATUS_CONTINUE && tot / reps < best) {
+            best = tot / reps;
+            bi = (int)c;
+        }
+    }
+    for (int k = 0; k < 2 * reps; k++) (void)hipEventDestroy(ev[k]);
+    S->timing = was_timing;
+    S->opt = keep;
+    if (rc == NBABFS_STATUS_CONTINUE && bi >= 0) {
+        S->opt.entriesPerItem = cand[bi][0];
+        S->opt.wavesPerCU = cand[bi][1];
+    }
+    // The accumulators hold the timing launches: the next energy call prepares again.
+    CHK(recut_items(S, S->opt.entriesPerItem));
+    if (rc != NBABFS_STATUS_CONTINUE) return rc;
+    if (best_epi) *best_epi = S->opt.entriesPerItem;
+    if (best_wpc) *best_wpc = S->opt.wavesPerCU;
+    if (best_ms) *best_ms = best;
+    return NBABFS_STATUS_CONTINUE;
+}
+
 // ------------------------------------------------------------------------------------------------
 // C ABI.
 // ------------------------------------------------------------------------------------------------
@@ -1159,6 +1252,13 @@ extern "C" int nbabfs_memcpy_dtoh(nbabfs_state *S, void *dst, const void *src, s
         HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     }
     return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_autotune(nbabfs_state *S, const double *d_xyz, int *best_entries_per_item, int *best_waves_per_cu,
+                               double *best_kernel_ms)
+{
+    if (!S || !d_xyz) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
+    return autotune(S, d_xyz, best_entries_per_item, best_waves_per_cu, best_kernel_ms);
 }
 
 extern "C" int nbabfs_zero_async(nbabfs_state *S, void *dst, size_t bytes)

@@ -44,6 +44,8 @@ void launch_cluster_geom(const double *xyz, const int *slot_atom, int ncl, const
 void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4 *cellv, const int *slot_atom,
                    const float4 *blo, const float4 *bhi, const int2 *colcl, Entry *scratch, int *ncand, int *counts,
                    int *overflow, hipStream_t s);
+void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int nsc, int per_item, int sys,
+                      WorkItem *items, hipStream_t s);
 void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, hipStream_t s);
 void launch_compact(const Entry *scratch, int cap, const int *ncand, const int *counts, const int *eoff,
                     const int *ioff, int nsc, int per_item, int sys, Entry *entries, WorkItem *items,

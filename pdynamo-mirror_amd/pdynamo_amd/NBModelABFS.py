@@ -303,6 +303,32 @@ class NBModelABFS:
             configuration.nbState.GetEnergies(energies)
         return energies
 
+    def Autotune(self, system, log=logFile):
+        """Choose entriesPerItem and wavesPerCU by timing the pair kernel on the system's list.
+
+        An extension of this implementation (the reference has no work items): the
+        system's state keeps the fastest cut, and the model's options are set to it
+        for states created later.  The pair list itself is unchanged."""
+        configuration = system.configuration
+        if not hasattr(configuration, "nbState"):
+            system.Energy(log=None, doGradients=False)
+        native = configuration.nbState.native
+        x = _native.DeviceArray.from_numpy(np.ascontiguousarray(configuration.coordinates3, dtype=np.float64), native)
+        try:
+            epi, wpc, ms = native.autotune(x.ptr)
+        finally:
+            _native.synchronize()
+            x.free()
+        self.entriesPerItem, self.wavesPerCU = epi, wpc
+        if LogFileActive(log):
+            s = log.GetSummary()
+            s.Start("ABFS Work-Item Autotuner")
+            s.Entry("Entries per Item", "{:d}".format(epi))
+            s.Entry("Waves per CU", "{:d}".format(wpc))
+            s.Entry("Pair Kernel (ms)", "{:.4f}".format(ms))
+            s.Stop()
+        return {"entriesPerItem": epi, "wavesPerCU": wpc, "kernelMs": ms}
+
     def QCMMGradients(self, configuration):
         pass
 

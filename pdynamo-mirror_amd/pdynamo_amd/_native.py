@@ -21,7 +21,7 @@ EXPORTED = ["nbabfs_default_options", "nbabfs_init", "nbabfs_last_error", "nbabf
             "nbabfs_update", "nbabfs_mmmm_energy_async", "nbabfs_energies", "nbabfs_stats_get",
             "nbabfs_set_timing", "nbabfs_export_pairs", "nbabfs_batch_create", "nbabfs_batch_destroy",
             "nbabfs_batch_energy", "nbabfs_device_alloc", "nbabfs_device_free", "nbabfs_memcpy_htod",
-            "nbabfs_memcpy_dtoh", "nbabfs_zero_async", "nbabfs_synchronize"]
+            "nbabfs_memcpy_dtoh", "nbabfs_zero_async", "nbabfs_synchronize", "nbabfs_autotune"]
 
 
 class CLibraryError(Exception):
@@ -87,6 +87,7 @@ def load():
     L.nbabfs_memcpy_dtoh.argtypes = [vp, vp, vp, C.c_size_t]
     L.nbabfs_zero_async.argtypes = [vp, vp, C.c_size_t]
     L.nbabfs_synchronize.argtypes = [vp]
+    L.nbabfs_autotune.argtypes = [vp, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     _lib = L
     return L
 
@@ -211,6 +212,14 @@ class NativeState:
 
     def set_stream(self, stream_handle):
         check(load().nbabfs_set_stream(self.h, None if not stream_handle else C.c_void_p(stream_handle)))
+
+    def autotune(self, d_xyz):
+        """Time the pair kernel per work-item cut on the list for d_xyz; keep the fastest.
+
+        Returns (entriesPerItem, wavesPerCU, kernel ms) of the chosen cut."""
+        epi, wpc, ms = C.c_int(), C.c_int(), C.c_double()
+        check(load().nbabfs_autotune(self.h, C.c_void_p(d_xyz), C.byref(epi), C.byref(wpc), C.byref(ms)))
+        return epi.value, wpc.value, ms.value
 
     def stats(self):
         s = Stats()

@@ -232,3 +232,22 @@ def test_call_orders_match(precision):
     energy(d1, "x1")                        # energy with no update
     assert st.update(d2.ptr)                # guarded pass discarded, list rebuilt
     energy(d2, "x2")
+
+
+def test_autotune_keeps_the_list_and_parity(oracle_mod):
+    """The work-item autotuner changes only how the list is cut: same pairs, same tolerances."""
+    from pdynamo_amd import NBModelABFS, System, synthetic
+    mm = synthetic.random_small_system(3000, 60.0, 31)
+    model = NBModelABFS()
+    s = System.FromSynthetic(mm)
+    s.DefineNBModel(model)
+    _, _, _ = _run(s)
+    pairs0 = s.configuration.nbState.Pairs()
+    choice = model.Autotune(s, log=None)
+    assert choice["entriesPerItem"] in (8, 12, 16, 24, 32, 48) and choice["wavesPerCU"] in (0, 12)
+    assert choice["kernelMs"] > 0.0
+    _, terms, grad = _run(s)
+    assert np.array_equal(s.configuration.nbState.Pairs(), pairs0)
+    o = oracle_mod.OracleABFS(mm)
+    rterms, rgrad, _ = o.energy(mm.coordinates3)
+    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)

@@ -119,6 +119,20 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
+def test_every_entry_point_with_arguments_has_a_prototype():
+    """ctypes passes undeclared pointer arguments as C int: each exported function
+    that takes arguments must have argtypes set (a 64-bit handle would be cut)."""
+    from pdynamo_amd import _native
+    lib = _native.load()
+    text = open(os.path.join(ROOT, "include", "nbabfs.h")).read()
+    for name in _native.EXPORTED:
+        m = re.search(r"\b" + name + r"\s*\(([^)]*)\)", text)
+        assert m, name
+        args = m.group(1).strip()
+        if args and args != "void":
+            assert getattr(lib, name).argtypes is not None, name
+
+
 def test_options_struct_matches_header():
     from pdynamo_amd import _native
     text = open(os.path.join(ROOT, "include", "nbabfs.h")).read()
