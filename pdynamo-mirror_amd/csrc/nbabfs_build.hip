@@ -564,6 +564,43 @@ __global__ __launch_bounds__(64) void k_compact(const Entry *__restrict__ scratc
     }
 }
 
+// Longest-first order of the work items (by the number of i-cluster tiles of
+// their entries), so that the items still running when the GPU drains are the
+// short ones.  Results do not depend on the order: every sum is fixed point.
+__global__ __launch_bounds__(256) void k_item_cost(const WorkItem *__restrict__ items, int nitems,
+                                                   const Entry *__restrict__ entries, int maxcost,
+                                                   int *__restrict__ cost, int *__restrict__ hist)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nitems) return;
+    const WorkItem w = items[k];
+    int c = 0;
+    for (int e = w.first; e < w.first + w.count; e++) c += __popc(entries[e].cbits & 0xff);
+    c = min(c, maxcost);
+    cost[k] = c;
+    atomicAdd(&hist[maxcost - c], 1);
+}
+
+// Rank r in the longest-first order goes to position r, or, for the
+// persistent queues (interleave != 0), to queue r % 8 at place r / 8, so every
+// per-XCD queue is itself longest-first and the queues are balanced.
+__device__ __forceinline__ int queue_start(int nitems, int q)
+{
+    int s = 0;
+    for (int v = 0; v < q; v++) s += (nitems - v + 7) / 8;
+    return s;
+}
+
+__global__ __launch_bounds__(256) void k_item_scatter(const WorkItem *__restrict__ items, int nitems,
+                                                      const int *__restrict__ cost, int maxcost, int interleave,
+                                                      int *__restrict__ cursor, WorkItem *__restrict__ out)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nitems) return;
+    const int r = atomicAdd(&cursor[maxcost - cost[k]], 1);
+    out[interleave ? queue_start(nitems, r & 7) + (r >> 3) : r] = items[k];
+}
+
 // Work items of an already compacted list for another entries-per-item cut.
 __global__ __launch_bounds__(256) void k_cut_items(const int *__restrict__ counts, const int *__restrict__ eoff,
                                                    const int *__restrict__ ioff, int nsc, int per_item, int sys,
@@ -781,6 +818,18 @@ void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int n
     if (nsc > 0)
         hipLaunchKernelGGL(k_cut_items, dim3(nblk(nsc, 256)), dim3(256), 0, s, counts, eoff, ioff, nsc, per_item, sys,
                            items);
+}
+
+void launch_order_items(const WorkItem *items, int nitems, const Entry *entries, int maxcost, int interleave, int *cost,
+                        int *hist, int *cursor, int *scantmp, WorkItem *out, hipStream_t s)
+{
+    if (nitems <= 0) return;
+    launch_fill_int(hist, (size_t)maxcost + 1, 0, s);
+    hipLaunchKernelGGL(k_item_cost, dim3(nblk(nitems, 256)), dim3(256), 0, s, items, nitems, entries, maxcost, cost,
+                       hist);
+    launch_scan(hist, cursor, maxcost + 1, scantmp, s);
+    hipLaunchKernelGGL(k_item_scatter, dim3(nblk(nitems, 256)), dim3(256), 0, s, items, nitems, cost, maxcost,
+                       interleave, cursor, out);
 }
 
 void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, hipStream_t s)

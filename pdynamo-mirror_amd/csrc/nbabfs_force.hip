@@ -515,16 +515,42 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
         el *= 0.5;
         lj *= 0.5;
     } else {
-        double ex = 0.0, ey = 0.0;
-        if (threadIdx.x < NB_ESLOTS) {
+        // Exact: each slot's low word is split at 2^48 (its upper part carried
+        // into the high word), the words are summed as integers over the 64
+        // slots, the total is brought to its unique form (low word in
+        // [0, 2^48)) and converted once, so the result does not depend on which
+        // items shared a slot.
+        __shared__ double sE[2];
+        if (threadIdx.x < 64) {
             const int t = threadIdx.x;
-            ex = (double)*eacc_word(D.eacc, 0, 1, t) * NB_FX_HI_UNIT + (double)*eacc_word(D.eacc, 0, 0, t) / NB_FX_LO_SCALE;
-            ey = (double)*eacc_word(D.eacc, 1, 1, t) * NB_FX_HI_UNIT + (double)*eacc_word(D.eacc, 1, 0, t) / NB_FX_LO_SCALE;
+            long long h0 = 0, l0 = 0, h1 = 0, l1 = 0;
+            if (t < NB_ESLOTS) {
+                const long long a0 = *eacc_word(D.eacc, 0, 0, t), a1 = *eacc_word(D.eacc, 1, 0, t);
+                h0 = *eacc_word(D.eacc, 0, 1, t) + (a0 >> 48);
+                l0 = a0 & ((1ll << 48) - 1);
+                h1 = *eacc_word(D.eacc, 1, 1, t) + (a1 >> 48);
+                l1 = a1 & ((1ll << 48) - 1);
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                h0 += __shfl_xor(h0, o, 64);
+                l0 += __shfl_xor(l0, o, 64);
+                h1 += __shfl_xor(h1, o, 64);
+                l1 += __shfl_xor(l1, o, 64);
+            }
+            if (t == 0) {
+                // Canonical words of the total (0 <= low < 2^48), then one rounding.
+                h0 += l0 >> 48;
+                l0 &= (1ll << 48) - 1;
+                h1 += l1 >> 48;
+                l1 &= (1ll << 48) - 1;
+                sE[0] = (double)h0 * NB_FX_HI_UNIT + (double)l0 / NB_FX_LO_SCALE;
+                sE[1] = (double)h1 * NB_FX_HI_UNIT + (double)l1 / NB_FX_LO_SCALE;
+            }
         }
-        block_sum2<256>(ex, ey);
-        el = ex;
-        lj = ey;
         __syncthreads();
+        el = sE[0];
+        lj = sE[1];
     }
     sum_range2(D.epart14, D.n14_blocks, el14, lj14);
     if (threadIdx.x == 0) {

@@ -76,6 +76,12 @@ struct DBuf {
         p = nullptr;
         cap = 0;
     }
+    void swap(DBuf &o)
+    {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+        g_alloc_version++;
+    }
     ~DBuf() { release(); }
 };
 
@@ -316,7 +322,8 @@ struct nbabfs_state {
     DBuf<int4> cellv;
     DBuf<float4> blo, bhi, centre;
     DBuf<Entry> entries, scratch;
-    DBuf<WorkItem> items;
+    DBuf<WorkItem> items, items2;
+    DBuf<int> icost, ihist, icur;
     DBuf<int> rowptr, cols;
 
     // Per call.
@@ -448,6 +455,32 @@ static void make_stencil(double h, double cutoff, int &u, std::vector<unsigned c
                 }
                 if (minR2 < c2) flags[((size_t)(a + u) * w + (b + u)) * w + (c + u)] = (maxR2 <= c2) ? 2 : 1;
             }
+}
+
+// Work items longest-first (and, for the persistent queues, interleaved over
+// the eight per-XCD queues), then the queue bounds.
+static int order_items(nbabfs_state *S)
+{
+    hipStream_t st = S->stream;
+    const int n = S->nitems;
+    const int interleave = S->opt.wavesPerCU > 0 ? 1 : 0;
+    std::vector<int> qs(9);
+    if (n > 0) {
+        const int maxcost = NB_SCC * std::max(1, S->opt.entriesPerItem);
+        CHK(S->icost.ensure(n));
+        CHK(S->ihist.ensure(maxcost + 1));
+        CHK(S->icur.ensure(maxcost + 1));
+        CHK(S->items2.ensure(n));
+        CHK(S->scantmp.ensure(scan_tmp_size(maxcost + 1)));
+        launch_order_items(S->items.p, n, S->entries.p, maxcost, interleave, S->icost.p, S->ihist.p, S->icur.p,
+                           S->scantmp.p, S->items2.p, st);
+        HIPCHK(hipGetLastError());
+        S->items.swap(S->items2);
+    }
+    qs[0] = 0;
+    for (int q = 0; q < 8; q++) qs[q + 1] = interleave ? qs[q] + (n - q + 7) / 8 * (n > q) : (int)((long long)n * (q + 1) / 8);
+    CHK(S->qstarts.upload(qs.data(), 9, st));
+    return NBABFS_STATUS_CONTINUE;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -614,9 +647,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     CHK(S->facc.ensure(6 * (size_t)S->n_pad));
     CHK(S->eacc.ensure(NB_EACC_WORDS));
     CHK(S->energies.ensure(4));
-    std::vector<int> qs(9);
-    for (int q = 0; q <= 8; q++) qs[q] = (int)((long long)S->nitems * q / 8);
-    CHK(S->qstarts.upload(qs.data(), 9, st));
+    CHK(order_items(S));
     CHK(S->heads.ensure(8 * 32));
 
     // Plain path: full CSR rows (both directions).
@@ -829,11 +860,8 @@ static int recut_items(nbabfs_state *S, int per_item)
     S->nitems = ((volatile int *)S->tail.h)[1];
     CHK(S->items.ensure(std::max(S->nitems, 1)));
     launch_cut_items(S->counts.p, S->eoff.p, S->ioff.p, S->nsc, per_item, 0, S->items.p, st);
-    std::vector<int> qs(9);
-    for (int q = 0; q <= 8; q++) qs[q] = (int)((long long)S->nitems * q / 8);
-    CHK(S->qstarts.upload(qs.data(), 9, st));
     HIPCHK(hipGetLastError());
-    return NBABFS_STATUS_CONTINUE;
+    return order_items(S);
 }
 
 // Time the pair kernel on the current list for each (entriesPerItem,

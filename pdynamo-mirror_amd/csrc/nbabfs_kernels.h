@@ -46,6 +46,8 @@ void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4
                    int *overflow, hipStream_t s);
 void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int nsc, int per_item, int sys,
                       WorkItem *items, hipStream_t s);
+void launch_order_items(const WorkItem *items, int nitems, const Entry *entries, int maxcost, int interleave, int *cost,
+                        int *hist, int *cursor, int *scantmp, WorkItem *out, hipStream_t s);
 void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, hipStream_t s);
 void launch_compact(const Entry *scratch, int cap, const int *ncand, const int *counts, const int *eoff,
                     const int *ioff, int nsc, int per_item, int sys, Entry *entries, WorkItem *items,
