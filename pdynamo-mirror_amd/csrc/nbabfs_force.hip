@@ -150,7 +150,25 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
             D.styp[s] = t;
         }
     }
-    if (__any(up) && (threadIdx.x & 63) == 0) {
+    if (chk.counter) {
+        // One 64-bit atomic per block carries both its completion (high word)
+        // and whether any of its atoms moved (low word), so the last block knows
+        // the answer without further fences; it alone publishes the flag and the
+        // completion marker to the host (one system-scope fence).
+        const int moved = __syncthreads_or(up);
+        if (threadIdx.x == 0) {
+            if (moved) *(volatile int *)chk.dflag = chk.epoch;
+            const unsigned long long inc = (1ull << 32) + (moved ? 1ull : 0ull);
+            const unsigned long long old = atomicAdd(chk.counter, inc);
+            const unsigned long long total = (unsigned long long)gridDim.x * gridDim.y;
+            if ((old >> 32) + 1 == total) {
+                *(volatile unsigned long long *)chk.counter = 0ull;
+                if (((old + inc) & 0xffffffffull) != 0) *(volatile int *)chk.flag = chk.epoch;
+                __threadfence_system();
+                *(volatile int *)chk.done = chk.epoch;
+            }
+        }
+    } else if (chk.ref && __any(up) && (threadIdx.x & 63) == 0) {
         *(volatile int *)chk.flag = chk.epoch;
         *(volatile int *)chk.dflag = chk.epoch;
     }
@@ -219,8 +237,10 @@ static inline size_t tiled_lds_bytes(int ntypes, bool rows_lds, int max_count)
     return tiled_lds_head(ntypes, rows_lds) + (size_t)max_count * (3 * NB_CL * 4 + 4);
 }
 
+// Four waves per SIMD: the register allocator fits the kernel in 128 VGPRs
+// without spilling (137 unconstrained, three waves), measured 2-3% faster.
 template <bool ROWS_LDS>
-__global__ __launch_bounds__(64) void k_force_tiled(const SysDesc *__restrict__ descs,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_force_tiled(const SysDesc *__restrict__ descs,
                                                     const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F,
                                                     int lds_head, ForceGuard guard)
 {

@@ -342,10 +342,10 @@ struct nbabfs_state {
     // Coordinates for which the last update already queued the prepare pass
     // and the pair kernel (the energy call on the same array only finalizes).
     const double *forced_xyz = nullptr;
-    MappedInts upflag, tail;
+    MappedInts upflag, tail, prepdone;
     DBuf<int> dflag;
+    DBuf<unsigned long long> prepcount;
     int epoch = 0;
-    hipEvent_t prep_done = nullptr;
 
     // Build temporaries.
     DBuf<int> key, count, off, cursor, sorted, padded, colstart, scantmp, counts, ncand, eoff, nits, ioff, flag;
@@ -365,7 +365,6 @@ struct nbabfs_state {
     ~nbabfs_state()
     {
         for (auto e : ev) (void)hipEventDestroy(e);
-        if (prep_done) (void)hipEventDestroy(prep_done);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -714,24 +713,35 @@ static int update(nbabfs_state *S, const double *d_xyz, int *rebuilt)
         CHK(S->upflag.ensure(1));
         if (!S->dflag.p) {
             CHK(S->dflag.ensure(1));
+            CHK(S->prepcount.ensure(1));
+            CHK(S->prepdone.ensure(1));
             launch_fill_int(S->dflag.p, 1, 0, st);
+            launch_fill_int((int *)S->prepcount.p, 2, 0, st);
         }
-        if (!S->prep_done) HIPCHK(hipEventCreateWithFlags(&S->prep_done, hipEventDisableTiming));
         CHK(ensure_call_buffers(S));
         CHK(ensure_desc(S, d_xyz, S->last_grad));
         S->epoch = next_epoch(S->epoch);
         const bool plain = S->opt.precision == NBABFS_PLAIN_FP64;
         PrepareCheck chk{d_xyz, S->xyz_ref.p, S->has_fixed ? S->freef.p : nullptr, buf * buf, S->upflag.d,
-                         S->dflag.p, S->epoch};
+                         S->dflag.p, S->epoch, S->prepcount.p, S->prepdone.d};
         launch_prepare(S->d_desc.p, 1, S->n_pad, plain ? 0 : 1, S->heads.p, chk, st);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(S->prep_done, st));
         // The descriptor's coordinates must be these for the speculative pass.
         if (S->last_xyz == d_xyz && S->n > 0) {
             CHK(launch_force(S, ForceGuard{S->dflag.p, S->epoch}));
             S->forced_xyz = d_xyz;
         }
-        HIPCHK(hipEventSynchronize(S->prep_done));
+        // Wait for the test through the mapped completion marker (no event in
+        // the stream between the two kernels); the stream is checked as well,
+        // so a failed launch cannot leave the host waiting.
+        for (long spin = 0; *(volatile int *)S->prepdone.h != S->epoch; spin++) {
+            if ((spin & 1023) == 1023) {
+                const hipError_t q = hipStreamQuery(st);
+                if (q != hipSuccess && q != hipErrorNotReady) return fail(NBABFS_STATUS_LOGIC_ERROR, "%s", hipGetErrorString(q));
+                if (q == hipSuccess && *(volatile int *)S->prepdone.h != S->epoch)
+                    return fail(NBABFS_STATUS_LOGIC_ERROR, "update test finished without its completion marker");
+            }
+        }
         doUpdate = *(volatile int *)S->upflag.h == S->epoch;
         if (doUpdate && S->forced_xyz) {
             // The guarded launch returns at once; keep it out of the timing.
@@ -834,7 +844,7 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
         return NBABFS_STATUS_CONTINUE;
     }
     if (S->forced_xyz != d_xyz) {
-        PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, 0};
+        PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, 0, nullptr, nullptr};
         launch_prepare(S->d_desc.p, 1, S->n_pad, plain ? 0 : 1, S->heads.p, none, st);
         CHK(launch_force(S, ForceGuard{nullptr, 0}));
     }
@@ -881,7 +891,7 @@ static int autotune(nbabfs_state *S, const double *d_xyz, int *best_epi, int *be
     }
     CHK(ensure_call_buffers(S));
     CHK(ensure_desc(S, d_xyz, S->last_grad));
-    PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, 0};
+    PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, 0, nullptr, nullptr};
     launch_prepare(S->d_desc.p, 1, S->n_pad, 1, S->heads.p, none, st);
     hipEvent_t ev[2 * reps];
     for (int k = 0; k < 2 * reps; k++) HIPCHK(hipEventCreate(&ev[k]));
@@ -1427,7 +1437,7 @@ static int batch_enqueue(nbabfs_batch *B)
     hipStream_t st = B->stream;
     const int ns = (int)B->st.size();
     nbabfs_state *s0 = B->st[0];
-    PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, 0};
+    PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, 0, nullptr, nullptr};
     launch_prepare(B->d_desc.p, ns, B->max_npad, B->mixed ? 1 : 0, B->heads.p, none, st);
     if (B->mixed) {
         if (B->nitems > 0) {

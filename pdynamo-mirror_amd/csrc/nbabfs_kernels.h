@@ -75,6 +75,8 @@ struct PrepareCheck {
     int *flag;                         // host-mapped: receives epoch when some free atom moved too far
     int *dflag;                        // device copy of the same flag (read by ForceGuard)
     int epoch;
+    unsigned long long *counter;       // device: blocks finished << 32 | blocks with a moved atom (reset by the last)
+    int *done;                         // host-mapped: receives epoch once every block has finished
 };
 
 // A force launch queued before the update decision is known: every
