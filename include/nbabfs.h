@@ -128,6 +128,14 @@ int nbabfs_update(nbabfs_state *state, const double *d_xyz, int *rebuilt);
 int nbabfs_mmmm_energy_async(nbabfs_state *state, const double *d_xyz, double *d_grad);
 int nbabfs_energies(nbabfs_state *state, double *energies, int *present);
 
+/* Virial of the last energy call's gradient contribution, row-major 3x3:
+ * W[3a+b] = sum_i (x_i - o)_a * dE/dx_i,b over the atoms (main list, 1-4
+ * list and, for the plain path, its rows), o being the state's cluster-grid
+ * origin; for the non-periodic pair model this equals sum over pairs of
+ * r_ij,a * g_ij,b.  No counterpart in the reference, which computes no virial
+ * for non-periodic systems; synchronises the state's stream. */
+int nbabfs_virial(nbabfs_state *state, double *virial);
+
 /* Work-item autotuner (no counterpart in the reference, which has no work
  * items): updates the list for d_xyz (as nbabfs_update), then times the pair
  * kernel for each candidate (entriesPerItem, wavesPerCU) on that list and

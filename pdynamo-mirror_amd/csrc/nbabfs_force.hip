@@ -91,7 +91,7 @@ __device__ __forceinline__ void pair_rows(const AbfsFactors &F, double r2, doubl
     const double inner = fma(fma(4.0 * C.x, s6, 2.0 * C.y), s3, C.z);
     double d2 = (-1.5 * s2) * s3 * inner;
     const bool dmp = r2 < F.r2Damp;
-    if (__any(act && dmp)) {
+    if (__builtin_amdgcn_ballot_w64(act && dmp) != 0ull) {
         if (dmp) {
             const double A = ljA[tij], B = ljB[tij];
             e1 = qq * fma(-F.qAlpha, r2, F.qF0);
@@ -121,6 +121,8 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
     if (heads && blockIdx.y == 0 && blockIdx.x == 0 && threadIdx.x < 8) heads[32 * threadIdx.x] = 0;
     if (blockIdx.x == 0 && D.eacc)
         for (int k = threadIdx.x; k < NB_EACC_WORDS; k += 256) D.eacc[k] = 0;
+    if (blockIdx.x == 0 && D.vacc)
+        for (int k = threadIdx.x; k < NB_VACC_WORDS; k += 256) D.vacc[k] = 0;
     bool up = false;
     if (s < D.n_pad) {
         for (int k = 0; k < 6; k++) D.facc[(size_t)k * D.n_pad + s] = 0;
@@ -297,7 +299,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (ROWS_LDS)
             for (int k = lane; k < 2 * nt * nt; k += 64) sR[k] = load_slot(rows, k);
         sI[lane] = load_slot(xq, I0 * NB_CL + lane);
-        sT[lane] = 2 * nt * ty[I0 * NB_CL + lane];
+        sT[lane] = 64 * nt * ty[I0 * NB_CL + lane];   // byte offset of the type's first row pair
         __syncthreads();
 
         float fx[NB_SCC], fy[NB_SCC], fz[NB_SCC];
@@ -315,11 +317,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         int lb1 = entry_bits(eb, min(w.first + 1, e_last), bl);
         int J = __builtin_amdgcn_readfirstlane(jc0.x);
         double4 xj = load_slot(xq, J * NB_CL + b);
-        int tj = 2 * ty[J * NB_CL + b];
+        int tj = 64 * ty[J * NB_CL + b];
         for (int e = w.first; e <= e_last; e++) {
             const int Jn = __builtin_amdgcn_readfirstlane(jc1.x);
             const double4 xjn = load_slot(xq, Jn * NB_CL + b);
-            const int tjn = 2 * ty[Jn * NB_CL + b];
+            const int tjn = 64 * ty[Jn * NB_CL + b];
             const int2 jc2 = entry_jc(eb, min(e + 2, e_last));
             const int lb2 = entry_bits(eb, min(e + 2, e_last), bl);
             const int cb = __builtin_amdgcn_readfirstlane(jc0.y);
@@ -328,17 +330,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
             for (int c = 0; c < NB_SCC; c++) {
                 if (cb & (1 << c)) {
-                    const bool inl = (lb >> c) & 1;
+                    const bool inl = (lb & (1 << c)) != 0;
                     const double4 xi = sI[c * NB_CL + a];
-                    const int row = sT[c * NB_CL + a] + tj;
+                    const int roff = sT[c * NB_CL + a] + tj;   // byte offset of the pair's plain row
                     const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
                     const bool act = inl && (r2 <= F.r2Off);
-                    if (__any(act)) {
+                    if (__builtin_amdgcn_ballot_w64(act) != 0ull) {
                         const bool sw = r2 > F.r2On;
-                        const double4 C = ROWS_LDS ? sR[row + sw] : load_slot(rows, row + sw);
+                        const int off = roff + (sw ? 32 : 0);
+                        const double4 C = ROWS_LDS ? *(const double4 *)((const unsigned char *)sR + off)
+                                                   : load_slot(rows, off >> 5);
                         double fs, x1, x2;
-                        pair_rows(F, r2, xi.w * xj.w, C, sw, act, ljA, ljB, row >> 1, fs, x1, x2);
+                        pair_rows(F, r2, xi.w * xj.w, C, sw, act, ljA, ljB, roff >> 6, fs, x1, x2);
                         const float f = (float)fs;
                         const float px = f * (float)dx, py = f * (float)dy, pz = f * (float)dz;
                         fx[c] += px; fy[c] += py; fz[c] += pz;
@@ -520,12 +524,25 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
 {
     const SysDesc &D = descs[blockIdx.y];
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < D.n && D.grad != nullptr) {
+    double g[3] = {0.0, 0.0, 0.0}, x[3] = {0.0, 0.0, 0.0};
+    if (i < D.n) {
         const int s = D.atom_slot[i];
         for (int k = 0; k < 3; k++) {
-            double g = fx_value(D.facc, D.n_pad, s, k);
-            if (plain) g += D.gplain[3 * (size_t)i + k];
-            D.grad[3 * (size_t)i + k] += g;
+            g[k] = fx_value(D.facc, D.n_pad, s, k);
+            if (plain) g[k] += D.gplain[3 * (size_t)i + k];
+            x[k] = D.xyz[3 * (size_t)i + k] - D.origin[k];
+            if (D.grad != nullptr) D.grad[3 * (size_t)i + k] += g[k];
+        }
+    }
+    // Virial of this call's gradient, W_ab = sum_i x_i,a g_i,b: wave sums in
+    // FP64, added into fixed point (slot = wave index modulo 64).
+    if (D.vacc) {
+        const int slot = ((blockIdx.x * 4 + (threadIdx.x >> 6)) & (NB_ESLOTS - 1));
+#pragma unroll
+        for (int ab = 0; ab < 9; ab++) {
+            const double w = wave_sum_f64(x[ab / 3] * g[ab % 3]);
+            if ((threadIdx.x & 63) == ab && w != 0.0)
+                fx_add(&D.vacc[(2 * ab) * NB_ESLOTS + slot], &D.vacc[(2 * ab + 1) * NB_ESLOTS + slot], w);
         }
     }
     if (blockIdx.x != 0) return;

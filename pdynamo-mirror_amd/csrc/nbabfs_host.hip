@@ -331,7 +331,7 @@ struct nbabfs_state {
     DBuf<int> styp;
     DBuf<long long> facc;
     DBuf<double> epart14, eplain, gplain, energies;
-    DBuf<long long> eacc;
+    DBuf<long long> eacc, vacc;
     DBuf<SysDesc> d_desc;
     DBuf<int> heads, qstarts;
     PinBuf<SysDesc> h_desc;
@@ -645,6 +645,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     CHK(S->styp.ensure(S->n_pad));
     CHK(S->facc.ensure(6 * (size_t)S->n_pad));
     CHK(S->eacc.ensure(NB_EACC_WORDS));
+    CHK(S->vacc.ensure(NB_VACC_WORDS));
     CHK(S->energies.ensure(4));
     CHK(order_items(S));
     CHK(S->heads.ensure(8 * 32));
@@ -691,6 +692,7 @@ static int ensure_call_buffers(nbabfs_state *S)
 {
     CHK(S->energies.ensure(4));
     CHK(S->eacc.ensure(NB_EACC_WORDS));
+    CHK(S->vacc.ensure(NB_VACC_WORDS));
     CHK(S->epart14.ensure(2 * (size_t)std::max(nblk(S->n14, 256), 1)));
     return NBABFS_STATUS_CONTINUE;
 }
@@ -784,6 +786,7 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.facc = S->facc.p;
     D.gplain = S->gplain.p;
     D.eacc = S->eacc.p;
+    D.vacc = S->vacc.p;
     D.epart14 = S->epart14.p;
     D.eplain = S->eplain.p;
     D.energies = S->energies.p;
@@ -1288,6 +1291,31 @@ extern "C" int nbabfs_memcpy_dtoh(nbabfs_state *S, void *dst, const void *src, s
         HIPCHK(hipStreamSynchronize(S->stream));
     } else {
         HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    }
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_virial(nbabfs_state *S, double *virial)
+{
+    if (!S || !virial) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
+    for (int k = 0; k < 9; k++) virial[k] = 0.0;
+    if (!S->vacc.p || S->n == 0) return NBABFS_STATUS_CONTINUE;
+    std::vector<long long> w(NB_VACC_WORDS);
+    HIPCHK(hipMemcpyAsync(w.data(), S->vacc.p, NB_VACC_WORDS * sizeof(long long), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    // Integer sums over the slots in canonical form (low word in [0, 2^48)),
+    // then one conversion: independent of which waves shared a slot.
+    const long long mask = (1ll << 48) - 1;
+    for (int ab = 0; ab < 9; ab++) {
+        long long h = 0, l = 0;
+        for (int q = 0; q < NB_ESLOTS; q++) {
+            const long long lo = w[(2 * ab) * NB_ESLOTS + q], hi = w[(2 * ab + 1) * NB_ESLOTS + q];
+            h += hi + (lo >> 48);
+            l += lo & mask;
+        }
+        h += l >> 48;
+        l &= mask;
+        virial[ab] = (double)h * NB_FX_HI_UNIT + (double)l / NB_FX_LO_SCALE;
     }
     return NBABFS_STATUS_CONTINUE;
 }

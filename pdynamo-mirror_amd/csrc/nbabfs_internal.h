@@ -15,6 +15,8 @@
 //   facc     six int64 arrays of n_pad (x,y,z low words, x,y,z high words):
 //            deterministic fixed-point gradient accumulators.
 //   eacc     fixed-point Coulomb and LJ energy accumulators, 64 slots each.
+//   vacc     fixed-point virial W_ab = sum_i (x_i - origin)_a g_i,b of the
+//            call's gradient, 9 components x 64 slots.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -71,6 +73,7 @@ struct SysDesc {
     long long *facc;            // 6*n_pad
     double *gplain;             // n*3 plain-path gradients (or null)
     long long *eacc;            // NB_EACC_WORDS fixed-point energy accumulators (tiled path)
+    long long *vacc;            // NB_VACC_WORDS fixed-point virial accumulators
     double *epart14;            // 2 per 1-4 block
     double *eplain;             // 2 per plain block
     double *energies;           // 4 outputs (device)
@@ -141,6 +144,8 @@ __device__ __forceinline__ double fx_value(const long long *facc, int n_pad, int
 // 1 = LJ of the main list; a work item adds into slot (item % NB_ESLOTS).
 #define NB_ESLOTS 64
 #define NB_EACC_WORDS (2 * 2 * NB_ESLOTS)
+// Virial accumulators: [component ab = 3a+b][lo word, hi word][slot].
+#define NB_VACC_WORDS (9 * 2 * NB_ESLOTS)
 
 __device__ __forceinline__ long long *eacc_word(long long *eacc, int kind, int hi, int slot)
 {

@@ -132,6 +132,10 @@ class NBModelABFSState:
                 s.Entry("Grid Updating", "{!r}".format(bool(st["useGridSearch"])))
             s.Stop()
 
+    def Virial(self):
+        """3x3 virial of the last call's MM/MM gradient (sum_i x_i (x) g_i; an extension)."""
+        return self.native.virial()
+
     def Pairs(self, which=0):
         """The list as sorted canonical keys max(i,j)*n + min(i,j)."""
         return self.native.export_pairs(which)

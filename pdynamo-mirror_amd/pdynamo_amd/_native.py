@@ -21,7 +21,8 @@ EXPORTED = ["nbabfs_default_options", "nbabfs_init", "nbabfs_last_error", "nbabf
             "nbabfs_update", "nbabfs_mmmm_energy_async", "nbabfs_energies", "nbabfs_stats_get",
             "nbabfs_set_timing", "nbabfs_export_pairs", "nbabfs_batch_create", "nbabfs_batch_destroy",
             "nbabfs_batch_energy", "nbabfs_device_alloc", "nbabfs_device_free", "nbabfs_memcpy_htod",
-            "nbabfs_memcpy_dtoh", "nbabfs_zero_async", "nbabfs_synchronize", "nbabfs_autotune"]
+            "nbabfs_memcpy_dtoh", "nbabfs_zero_async", "nbabfs_synchronize", "nbabfs_autotune",
+            "nbabfs_virial"]
 
 
 class CLibraryError(Exception):
@@ -88,6 +89,7 @@ def load():
     L.nbabfs_zero_async.argtypes = [vp, vp, C.c_size_t]
     L.nbabfs_synchronize.argtypes = [vp]
     L.nbabfs_autotune.argtypes = [vp, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.nbabfs_virial.argtypes = [vp, vp]
     _lib = L
     return L
 
@@ -212,6 +214,12 @@ class NativeState:
 
     def set_stream(self, stream_handle):
         check(load().nbabfs_set_stream(self.h, None if not stream_handle else C.c_void_p(stream_handle)))
+
+    def virial(self):
+        """3x3 virial W_ab = sum_i x_i,a g_i,b of the last energy call's gradient."""
+        w = np.zeros(9)
+        check(load().nbabfs_virial(self.h, _ptr(w)))
+        return w.reshape(3, 3)
 
     def autotune(self, d_xyz):
         """Time the pair kernel per work-item cut on the list for d_xyz; keep the fastest.

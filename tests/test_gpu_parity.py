@@ -251,3 +251,29 @@ def test_autotune_keeps_the_list_and_parity(oracle_mod):
     o = oracle_mod.OracleABFS(mm)
     rterms, rgrad, _ = o.energy(mm.coordinates3)
     _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)
+
+
+@pytest.mark.parametrize("precision,tol", [("mixed", 2e-5), ("fp64", 1e-7)])
+def test_virial_matches_strain_derivative(oracle_mod, precision, tol):
+    """W_ab = sum_i x_i,a g_i,b against central differences of the FP64 oracle energy
+    under homogeneous strains x -> (I + eps S) x:  dE/deps = tr(S W).  Also the atom-sum
+    identity with the returned gradients (W is taken about the state's grid origin,
+    which moves it by o (x) sum_i g_i ~ 0)."""
+    from pdynamo_amd import synthetic
+    mm = synthetic.config0()
+    s = _system(mm, precision=precision)
+    _, terms, grad = _run(s)
+    W = s.configuration.nbState.Virial()
+    x = mm.coordinates3
+    Wsum = x.T @ grad
+    assert np.allclose(W, Wsum, rtol=0.0, atol=tol * np.abs(Wsum).max())
+    rng = np.random.default_rng(12)
+    eps = 1e-5
+    for S in [np.eye(3), rng.normal(size=(3, 3)), rng.normal(size=(3, 3))]:
+        def energy(e):
+            o = oracle_mod.OracleABFS(mm)
+            t, _, _ = o.energy(x @ (np.eye(3) + e * S).T, gradients=False)
+            return sum(v for _, v in t)
+        fd = (energy(eps) - energy(-eps)) / (2.0 * eps)
+        an = float(np.trace(S @ W))
+        assert abs(an - fd) <= tol * max(abs(fd), np.abs(W).max()), (S, an, fd)
