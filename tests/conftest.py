@@ -18,3 +18,41 @@ def oracle_mod():
     import oracle
     oracle.lib()
     return oracle
+
+
+class DHFRFixture:
+    """The 23,558-atom DHFR non-bonding inputs (tests/golden/dhfr_nb.npz, written by make_dhfr.py from the
+    reference's benchmark data) and the reference's published first-call values."""
+
+    def __init__(self, path):
+        from pdynamo_amd.synthetic import MMSystem
+        with np_load(path) as z:
+            self.mm = MMSystem("dhfr", z["coordinates3"], z["charges"], z["ljtypes"], int(z["ntypes"]), z["tableA"],
+                               z["tableB"], z["exclusions"], z["interactions14"])
+            self.tableA14, self.tableB14 = z["tableA14"], z["tableB14"]
+            self.scale14 = float(z["scale14"])
+            self.published_terms = dict(zip(("MM/MM Elect.", "MM/MM LJ", "MM/MM 1-4 Elect.", "MM/MM 1-4 LJ"),
+                                            (float(v) for v in z["published_terms"])))
+            self.published_pairs, self.published_pairs14 = (int(v) for v in z["published_pairs"])
+
+    def system(self, **model_options):
+        from pdynamo_amd import MMAtomContainer, NBModelABFS, System
+        from pdynamo_amd.Parameters import LJParameterContainer_FromTableCoefficients
+        m = self.mm
+        lj = LJParameterContainer_FromTableCoefficients(m.ntypes, m.tableA, m.tableB)
+        lj14 = LJParameterContainer_FromTableCoefficients(m.ntypes, self.tableA14, self.tableB14)
+        s = System.FromMMParameters(m.coordinates3, MMAtomContainer(m.charges, m.ljtypes), lj,
+                                    exclusions=m.exclusions, interactions14=m.interactions14, ljParameters14=lj14,
+                                    electrostaticScale14=self.scale14, label="DHFR")
+        s.DefineNBModel(NBModelABFS(**model_options))
+        return s
+
+
+def np_load(path):
+    import numpy as np
+    return np.load(path, allow_pickle=False)
+
+
+@pytest.fixture(scope="session")
+def dhfr():
+    return DHFRFixture(os.path.join(ROOT, "tests", "golden", "dhfr_nb.npz"))

@@ -1,0 +1,55 @@
+"""Write the DHFR non-bonding fixture from the reference's benchmark data.
+
+Reads benchmarks/data/dhfr/{dhfr.psfx, par_all22_prot.prm, dhfr.xyz} of the
+reference tree with pdynamo_amd.CHARMM (the CHARMM PSF/parameter/XYZ rules
+of pBabel) and stores the inputs of the MM/MM non-bonding model as
+dhfr_nb.npz next to this script: coordinates, charges, LJ types, the packed
+normal and 1-4 LJ tables (AMBER form), the exclusions (1-2, 1-3 and every 1-4
+candidate) and the 1-4 pairs.
+
+The expected values are the reference's own first-call output for this
+system, printed identically in every benchmarks/log/systemBenchmarks_*.log
+(systemBenchmarks_Serial_1ps.log:392-393,400-403): MM/MM pairs 8,885,288,
+MM/MM 1-4 pairs 6,556, and the four non-image terms in kJ/mol.  They are
+stored in the fixture as well.
+
+Usage (in a container holding the reference tree):
+    python tests/golden/make_dhfr.py [/path/to/reference]
+"""
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "pdynamo-mirror_amd"))
+
+from pdynamo_amd import CHARMM  # noqa: E402
+
+# benchmarks/log/systemBenchmarks_Serial_1ps.log:392-393,400-403 (first energy call).
+PUBLISHED = {"pairs": 8885288, "pairs14": 6556,
+             "MM/MM Elect.": -388870.0641, "MM/MM LJ": 26802.2738,
+             "MM/MM 1-4 Elect.": 17884.0130, "MM/MM 1-4 LJ": 1421.1985}
+
+
+def main(reference="/root/reference"):
+    d = os.path.join(reference, "benchmarks", "data", "dhfr")
+    nb = CHARMM.CHARMMPSFFile_ToNBParameters(os.path.join(d, "dhfr.psfx"), [os.path.join(d, "par_all22_prot.prm")])
+    xyz = CHARMM.XYZFile_ToCoordinates3(os.path.join(d, "dhfr.xyz"))
+    a, lj, lj14 = nb["atoms"], nb["ljParameters"], nb["ljParameters14"]
+    np.savez_compressed(os.path.join(HERE, "dhfr_nb.npz"), coordinates3=xyz, charges=a.charges, ljtypes=a.ljtypes,
+                        ntypes=lj.ntypes, tableA=lj.tableA, tableB=lj.tableB, tableA14=lj14.tableA,
+                        tableB14=lj14.tableB, exclusions=nb["exclusions"].pairs,
+                        interactions14=nb["interactions14"].pairs, scale14=nb["electrostaticScale14"],
+                        atomTypes=np.array(nb["atomTypes"]),
+                        published_terms=np.array([PUBLISHED["MM/MM Elect."], PUBLISHED["MM/MM LJ"],
+                                                  PUBLISHED["MM/MM 1-4 Elect."], PUBLISHED["MM/MM 1-4 LJ"]]),
+                        published_pairs=np.array([PUBLISHED["pairs"], PUBLISHED["pairs14"]]))
+    print("dhfr", a.size, "atoms", lj.ntypes, "types", len(nb["exclusions"]), "exclusions",
+          len(nb["interactions14"]), "1-4 pairs")
+
+
+if __name__ == "__main__":
+    main(*sys.argv[1:])

@@ -277,3 +277,22 @@ def test_virial_matches_strain_derivative(oracle_mod, precision, tol):
         fd = (energy(eps) - energy(-eps)) / (2.0 * eps)
         an = float(np.trace(S @ W))
         assert abs(an - fd) <= tol * max(abs(fd), np.abs(W).max()), (S, an, fd)
+
+
+@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("fp64", None, 1e-12)])
+def test_dhfr(oracle_mod, dhfr, precision, etol, gtol):
+    """The reference's DHFR benchmark system (CHARMM22, 23,558 atoms, 35 LJ types so the LJ rows are read from
+    global memory): the published pair counts exactly; the published four terms to their 4 printed decimals
+    (plain FP64) or to 1e-6 of the total (mixed); gradients against the oracle."""
+    s = dhfr.system(precision=precision)
+    _, terms, grad = _run(s)
+    st = s.configuration.nbState.Statistics()
+    assert st["npairs"] == dhfr.published_pairs and st["npairs14"] == dhfr.published_pairs14
+    etot = sum(dhfr.published_terms.values())
+    for k, v in dhfr.published_terms.items():
+        tol = 5.01e-5 if etol is None else etol * abs(etot)
+        assert abs(terms[k] - v) <= tol, (k, terms[k], v)
+    o = oracle_mod.OracleABFS(dhfr.mm, tableA14=dhfr.tableA14, tableB14=dhfr.tableB14, scale14=dhfr.scale14,
+                              nthreads=8)
+    _, rgrad, _ = o.energy(dhfr.mm.coordinates3)
+    assert np.linalg.norm(grad - rgrad) <= gtol * np.linalg.norm(rgrad)

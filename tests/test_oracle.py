@@ -44,7 +44,11 @@ class _Sys:
 # ------------------------------------------------------------------------------------------------
 # Golden vectors from the reference C library.
 # ------------------------------------------------------------------------------------------------
-@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "*.npz"))))
+# Fixtures written by make_golden.py (the DHFR inputs of make_dhfr.py are checked in test_dhfr_published_terms).
+REFERENCE_FIXTURES = sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.npz")) if not p.endswith("dhfr_nb.npz"))
+
+
+@pytest.mark.parametrize("path", REFERENCE_FIXTURES)
 def test_oracle_matches_reference_fixtures(oracle_mod, path):
     fx = dict(np.load(path, allow_pickle=False))
     o = fx["options"]
@@ -268,3 +272,19 @@ def test_lj_tables_closed_form(oracle_mod):
     ti = np.zeros(9, np.int32)
     oracle_mod.lib().orc_lj_make_table(3, np.array(eps), np.array(sig), 1, 1, ta, tb, ti)
     assert np.allclose(ta, opls.tableA, rtol=1e-14) and np.allclose(tb, opls.tableB, rtol=1e-14)
+
+
+def test_dhfr_published_terms(oracle_mod, dhfr):
+    """The oracle on the reference's DHFR benchmark system (CHARMM22, 23,558 atoms) reproduces the reference's
+    published first-call pair counts and the four non-image energy terms to the printed 4 decimals
+    (benchmarks/log/systemBenchmarks_Serial_1ps.log:392-393,400-403)."""
+    o = oracle_mod.OracleABFS(dhfr.mm, tableA14=dhfr.tableA14, tableB14=dhfr.tableB14, scale14=dhfr.scale14,
+                              nthreads=8)
+    terms, grad, _ = o.energy(dhfr.mm.coordinates3)
+    assert o.records.npairs == dhfr.published_pairs
+    assert o.rec14.npairs == dhfr.published_pairs14
+    terms = dict(terms)
+    assert set(terms) == set(dhfr.published_terms)
+    for k, v in dhfr.published_terms.items():
+        assert abs(terms[k] - v) <= 5.01e-5, (k, terms[k], v)
+    assert np.abs(grad.sum(0)).max() < 1e-8 * np.abs(grad).max()
