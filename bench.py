@@ -71,14 +71,28 @@ def traffic_from_summary(path):
     return None
 
 
+def load_dhfr():
+    """The reference's DHFR benchmark system (CHARMM22, 23,558 atoms), non-bonding inputs from the committed
+    fixture tests/golden/dhfr_nb.npz, with its 1-4 LJ tables."""
+    from pdynamo_amd.synthetic import MMSystem
+    with np.load(os.path.join(ROOT, "tests", "golden", "dhfr_nb.npz"), allow_pickle=False) as z:
+        mm = MMSystem("dhfr", z["coordinates3"], z["charges"], z["ljtypes"], int(z["ntypes"]), z["tableA"],
+                      z["tableB"], z["exclusions"], z["interactions14"])
+        return mm, {"tableA14": z["tableA14"], "tableB14": z["tableB14"]}
+
+
 def make_workload(name):
+    """(system, description, extra state arguments)."""
     from pdynamo_amd import synthetic
     if name == "solute30000":
-        return synthetic.config1(), "C1 solute-in-water 30,000 atoms (9,667 waters + 999 solute, 67 A cube)"
+        return synthetic.config1(), "C1 solute-in-water 30,000 atoms (9,667 waters + 999 solute, 67 A cube)", {}
     if name == "water200001":
-        return synthetic.config2(), "C2 water box 200,001 atoms (66,667 waters, 126 A cube)"
+        return synthetic.config2(), "C2 water box 200,001 atoms (66,667 waters, 126 A cube)", {}
     if name == "water648":
-        return synthetic.config0(), "C0 water 648 atoms (216 waters, 18.6 A cube)"
+        return synthetic.config0(), "C0 water 648 atoms (216 waters, 18.6 A cube)", {}
+    if name == "dhfr":
+        mm, extra = load_dhfr()
+        return mm, "DHFR CHARMM22 23,558 atoms (reference benchmark system, primary + 1-4 lists, no images)", extra
     raise SystemExit("unknown workload " + name)
 
 
@@ -157,15 +171,15 @@ def run_batch(args, systems, wdesc):
     print(json.dumps(result))
 
 
-def cpu_baseline(mm, calls, threads):
+def cpu_baseline(mm, calls, threads, extra=None):
     """The reference pDynamo C (oracle/_ref, -O3 -fopenmp) or, if absent, the
     FP64 C restatement, timed for `calls` E+grad calls after the first build."""
     import oracle as O
     kind = "reference" if O.ref_available() else "port"
     if kind == "reference":
-        eng = O.ReferenceABFS(mm, nthreads=threads)
+        eng = O.ReferenceABFS(mm, nthreads=threads, **(extra or {}))
     else:
-        eng = O.OracleABFS(mm, nthreads=threads)
+        eng = O.OracleABFS(mm, nthreads=threads, **(extra or {}))
     eng.energy(mm.coordinates3)
     t0 = time.perf_counter()
     for _ in range(calls):
@@ -216,7 +230,7 @@ def main():
     if systems is not None:
         run_batch(args, systems, bdesc)
         return
-    mm, wdesc = make_workload(args.workload)
+    mm, wdesc, extra = make_workload(args.workload)
     opts = {"precision": args.precision}
     if args.entries_per_item:
         opts["entriesPerItem"] = args.entries_per_item
@@ -224,7 +238,7 @@ def main():
         opts["wavesPerCU"] = args.waves_per_cu
     model = NBModelABFS(**opts)
     st = _native.NativeState(model.NativeOptions(), mm.charges, mm.ljtypes, mm.ntypes, mm.tableA, mm.tableB,
-                             exclusions=mm.exclusions, interactions14=mm.interactions14)
+                             exclusions=mm.exclusions, interactions14=mm.interactions14, **extra)
     g = _native.DeviceArray(mm.coordinates3.shape)
     if args.moves > 0.0:
         from pdynamo_amd import synthetic
@@ -311,7 +325,7 @@ def main():
         threads = len(os.sched_getaffinity(0))
         if os.environ.get("OMP_NUM_THREADS", "").isdigit():
             threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
-        result["cpu_baseline"] = cpu_baseline(mm, args.cpu_calls, threads)
+        result["cpu_baseline"] = cpu_baseline(mm, args.cpu_calls, threads, extra)
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
     print(json.dumps(result))
