@@ -414,3 +414,517 @@ void orc_lj_make_table(int ntypes, const double *eps, const double *sigma, int g
             n++;
         }
 }
+
+/* ==========================================================================
+ * Crystal symmetry: the MM/MM image list and image energy.
+ * 3x3 matrices are row-major double[9]; every product keeps the reference's
+ * operand order (pCx/csource/Matrix33.c), so that, compiled without FMA
+ * contraction, the matrices and image coordinates match the reference bit for
+ * bit.
+ * ========================================================================== */
+
+/* Matrix33_Determinant (pCx/csource/Matrix33.c:96-111). */
+static double m33_det(const double *a)
+{
+    return a[0] * (a[4] * a[8] - a[7] * a[5]) - a[1] * (a[3] * a[8] - a[6] * a[5]) +
+           a[2] * (a[3] * a[7] - a[6] * a[4]);
+}
+
+/* Matrix33_Invert (pCx/csource/Matrix33.c:165-195): cofactors, then a scale
+ * by 1/det. */
+static void m33_invert(const double *a, double *r)
+{
+    double det = m33_det(a), s;
+    r[0] = (a[4] * a[8] - a[5] * a[7]);
+    r[1] = (a[2] * a[7] - a[8] * a[1]);
+    r[2] = (a[1] * a[5] - a[4] * a[2]);
+    r[3] = (a[5] * a[6] - a[3] * a[8]);
+    r[4] = (a[0] * a[8] - a[2] * a[6]);
+    r[5] = (a[2] * a[3] - a[0] * a[5]);
+    r[6] = (a[3] * a[7] - a[4] * a[6]);
+    r[7] = (a[1] * a[6] - a[0] * a[7]);
+    r[8] = (a[0] * a[4] - a[3] * a[1]);
+    s = 1.0 / det;
+    for (int k = 0; k < 9; k++) r[k] *= s;
+}
+
+/* Matrix33_PreMultiplyBy (:330-356): self = other * self. */
+static void m33_premul(double *self, const double *o)
+{
+    double s[9], r[9];
+    memcpy(s, self, sizeof(s));
+    for (int i = 0; i < 3; i++) {
+        double o0 = o[3 * i], o1 = o[3 * i + 1], o2 = o[3 * i + 2];
+        r[3 * i] = s[0] * o0 + s[3] * o1 + s[6] * o2;
+        r[3 * i + 1] = s[1] * o0 + s[4] * o1 + s[7] * o2;
+        r[3 * i + 2] = s[2] * o0 + s[5] * o1 + s[8] * o2;
+    }
+    memcpy(self, r, sizeof(r));
+}
+
+/* Matrix33_PostMultiplyBy (:298-325): self = self * other. */
+static void m33_postmul(double *self, const double *o)
+{
+    double s[9], r[9];
+    memcpy(s, self, sizeof(s));
+    for (int i = 0; i < 3; i++) {
+        double o0 = o[i], o1 = o[3 + i], o2 = o[6 + i];
+        r[i] = s[0] * o0 + s[1] * o1 + s[2] * o2;
+        r[3 + i] = s[3] * o0 + s[4] * o1 + s[5] * o2;
+        r[6 + i] = s[6] * o0 + s[7] * o1 + s[8] * o2;
+    }
+    memcpy(self, r, sizeof(r));
+}
+
+/* Matrix33_ApplyToVector3 (:17-36). */
+static void m33_apply(const double *m, double *v)
+{
+    double x = v[0], y = v[1], z = v[2];
+    v[0] = x * m[0] + y * m[1] + z * m[2];
+    v[1] = x * m[3] + y * m[4] + z * m[5];
+    v[2] = x * m[6] + y * m[7] + z * m[8];
+}
+
+/* SymmetryParameters_MakeM (pMx/csource/SymmetryParameters.c:392-425): the
+ * columns of M are the lattice vectors in the standard orientation. */
+void orc_symmetry_make_m(const double *sp, double *M, double *Minv)
+{
+    const double dr = M_PI / 180.0e+00;
+    double alpha = sp[3] * dr, beta = sp[4] * dr, gamma = sp[5] * dr;
+    double ca = cos(alpha), cb = cos(beta), cg = cos(gamma), sg = sin(gamma);
+    for (int k = 0; k < 9; k++) M[k] = 0.0;
+    M[0] = sp[0];
+    M[1] = sp[1] * cg;
+    M[4] = sp[1] * sg;
+    M[2] = sp[2] * cb;
+    M[5] = sp[2] * (ca - cb * cg) / sg;
+    M[8] = sp[2] * sqrt(1.0e+00 - ca * ca - cb * cb - cg * cg + 2.0e+00 * ca * cb * cg) / sg;
+    m33_invert(M, Minv);
+}
+
+/* Transformation3_Orthogonalize (pCx/csource/Transformation3.c:91-99):
+ * R' = M R M^-1, T' = M t. */
+void orc_orthogonalize(const double *M, const double *Minv, const double *rot, const double *tr, double *Ro, double *To)
+{
+    memcpy(Ro, rot, 9 * sizeof(double));
+    m33_premul(Ro, M);
+    m33_postmul(Ro, Minv);
+    To[0] = tr[0]; To[1] = tr[1]; To[2] = tr[2];
+    m33_apply(M, To);
+}
+
+/* Matrix33_IsIdentity / Matrix33_IsEqual with their 1e-6 tolerance
+ * (pCx/csource/Matrix33.c:200-253); Transformation3_IsIdentity (Transformation3.c:79-86). */
+static int m33_is_equal(const double *a, const double *b)
+{
+    for (int k = 0; k < 9; k++) if (fabs(a[k] - b[k]) > 1.0e-6) return 0;
+    return 1;
+}
+
+/* Transformation3Container_FindIdentity and _FindInverses (rotations only;
+ * pCx/csource/Transformation3Container.c:52-66,107-141).  Returns the identity's
+ * index (-1 if none); inverses[ntrans]. */
+int orc_find_inverses(int ntrans, const double *rot, const double *tr, int *inverses)
+{
+    int identity = -1;
+    static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int t = 0; t < ntrans; t++) {
+        if (m33_is_equal(rot + 9 * t, I) && fabs(tr[3 * t]) <= 1.0e-6 && fabs(tr[3 * t + 1]) <= 1.0e-6 &&
+            fabs(tr[3 * t + 2]) <= 1.0e-6) { identity = t; break; }
+    }
+    for (int t = 0; t < ntrans; t++) inverses[t] = -1;
+    for (int i = 0; i < ntrans; i++) {
+        if (inverses[i] >= 0) continue;
+        double m[9];
+        m33_invert(rot + 9 * i, m);
+        for (int j = 0; j <= i; j++)
+            if (inverses[j] < 0 && m33_is_equal(m, rot + 9 * j)) { inverses[i] = j; inverses[j] = i; break; }
+    }
+    return identity;
+}
+
+/* GetLimits of SymmetryParameters_FindBoxSearchLimits (SymmetryParameters.c:182-205). */
+static int get_limits(double bl, double bu, double il, double iu, double t, int *low, int *high)
+{
+    int n = 0;
+    while (iu >= bl) { il -= t; iu -= t; n--; }
+    while (iu < bl) { il += t; iu += t; n++; }
+    if (il <= bu) {
+        *low = n;
+        while (il <= bu) { il += t; iu += t; n++; }
+        *high = n - 1;
+        return 1;
+    }
+    return 0;
+}
+
+/* SymmetryParameters_FindBoxSearchLimits (SymmetryParameters.c:207-246). */
+static void box_search_limits(const double *M, const double *lower, const double *upper, const double *ilower,
+                              const double *iupper, int *lim)
+{
+    lim[0] = lim[2] = lim[4] = 0;
+    lim[1] = lim[3] = lim[5] = -1;
+    if (!get_limits(lower[2], upper[2], ilower[2], iupper[2], M[8], &lim[4], &lim[5])) return;
+    double d1 = lim[4] * M[5], d2 = lim[5] * M[5];
+    double bl = lower[1] - (d1 > d2 ? d1 : d2), bu = upper[1] - (d1 < d2 ? d1 : d2);
+    if (!get_limits(bl, bu, ilower[1], iupper[1], M[4], &lim[2], &lim[3])) return;
+    d1 = lim[4] * M[2]; d2 = lim[5] * M[2];
+    bl = lower[0] - (d1 > d2 ? d1 : d2); bu = upper[0] - (d1 < d2 ? d1 : d2);
+    d1 = lim[2] * M[1]; d2 = lim[3] * M[1];
+    bl -= (d1 > d2 ? d1 : d2); bu -= (d1 < d2 ? d1 : d2);
+    get_limits(bl, bu, ilower[0], iupper[0], M[0], &lim[0], &lim[1]);
+}
+
+/* Round (pCx/cinclude/Macros.h:34). */
+static int ref_round(double a) { return a >= 0 ? (int)(a + 0.5) : (int)(a - 0.5); }
+
+/* Transformation3Container_FindInverseIntegerTranslation
+ * (Transformation3Container.c:70-101). */
+static void inverse_integer_translation(const double *rot, const double *tr, int t, int tinv, int a, int b, int c,
+                                        int *inv)
+{
+    const int BIG = -999999;
+    double v[3] = {tr[3 * t] + (double)a, tr[3 * t + 1] + (double)b, tr[3 * t + 2] + (double)c};
+    m33_apply(rot + 9 * tinv, v);
+    for (int i = 0; i < 3; i++) {
+        v[i] *= -1.0e+00;
+        v[i] += -1.0e+00 * tr[3 * tinv + i];
+        int r = ref_round(v[i]);
+        inv[i] = (fabs(v[i] - (double)r) < 1.0e-4) ? r : BIG;
+    }
+}
+
+/* Coordinates3_Rotate then Coordinates3_Translate (pCx/csource/Coordinates3.c:1473-1500,1760-1800). */
+static void transform_coordinates(int n, const double *xyz, const double *R, const double *T, double *out)
+{
+    for (int i = 0; i < n; i++) {
+        double x0 = xyz[3 * i], y0 = xyz[3 * i + 1], z0 = xyz[3 * i + 2];
+        double x1 = R[0] * x0 + R[1] * y0 + R[2] * z0;
+        double y1 = R[3] * x0 + R[4] * y0 + R[5] * z0;
+        double z1 = R[6] * x0 + R[7] * y0 + R[8] * z0;
+        out[3 * i] = x1 + T[0]; out[3 * i + 1] = y1 + T[1]; out[3 * i + 2] = z1 + T[2];
+    }
+}
+
+static void translate_coordinates(int n, double *xyz, const double *d)
+{
+    for (int i = 0; i < n; i++) { xyz[3 * i] += d[0]; xyz[3 * i + 1] += d[1]; xyz[3 * i + 2] += d[2]; }
+}
+
+/* Cross pair list between xyz (i) and the image coordinates ixyz (j): every
+ * pair with the FP64 squared distance (x, y, z order) <= cutoff^2 and i on
+ * the primary grid when ongrid is given (MakeCrossPairListFromCoordinates3
+ * CellCell / Direct, pCx/csource/PairListGenerator.c:611-860, without
+ * exclusions or selections).  Records: i ascending, j ascending. */
+static long cross_pair_list(int n, const double *xyz, const double *ixyz, const int *ongrid, double cutoff,
+                            PairSink *s)
+{
+    double c2 = cutoff * cutoff;
+    int *ind = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    /* Buckets of the image points on a cutoff-sized grid bound the scan. */
+    double lo[3], ext[3];
+    enclosing_box(n, ixyz, lo, ext);
+    int nb[3];
+    for (int d = 0; d < 3; d++) nb[d] = (int)(ext[d] / cutoff) + 1;
+    long nbk = (long)nb[0] * nb[1] * nb[2];
+    long *first = (long *)calloc((size_t)nbk + 1, sizeof(long));
+    int *pts = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *bk = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    for (int j = 0; j < n; j++) {
+        long b = 0;
+        for (int d = 0; d < 3; d++) {
+            int k = (int)((ixyz[3 * j + d] - lo[d]) / cutoff);
+            if (k >= nb[d]) k = nb[d] - 1;
+            b = b * nb[d] + k;
+        }
+        bk[j] = (int)b;
+        first[b + 1]++;
+    }
+    for (long b = 0; b < nbk; b++) first[b + 1] += first[b];
+    {
+        long *cur = (long *)malloc(sizeof(long) * (size_t)nbk);
+        memcpy(cur, first, sizeof(long) * (size_t)nbk);
+        for (int j = 0; j < n; j++) pts[cur[bk[j]]++] = j;   /* ascending j within a bucket */
+        free(cur);
+    }
+    for (int i = 0; i < n; i++) {
+        if (ongrid && !ongrid[i]) continue;
+        int m = 0, k0[3], k1[3];
+        for (int d = 0; d < 3; d++) {
+            double f = (xyz[3 * i + d] - lo[d]) / cutoff;
+            k0[d] = (int)floor(f) - 1;
+            k1[d] = (int)floor(f) + 1;
+            if (k0[d] < 0) k0[d] = 0;
+            if (k1[d] > nb[d] - 1) k1[d] = nb[d] - 1;
+        }
+        for (int a = k0[0]; a <= k1[0]; a++)
+            for (int b = k0[1]; b <= k1[1]; b++)
+                for (int c = k0[2]; c <= k1[2]; c++) {
+                    long q = ((long)a * nb[1] + b) * nb[2] + c;
+                    for (long p = first[q]; p < first[q + 1]; p++) {
+                        int j = pts[p];
+                        double dx = xyz[3 * i] - ixyz[3 * j], dy = xyz[3 * i + 1] - ixyz[3 * j + 1],
+                               dz = xyz[3 * i + 2] - ixyz[3 * j + 2];
+                        if ((dx * dx) + (dy * dy) + (dz * dz) <= c2) ind[m++] = j;
+                    }
+                }
+        /* ascending j */
+        for (int x = 1; x < m; x++) {
+            int v = ind[x], y = x - 1;
+            while (y >= 0 && ind[y] > v) { ind[y + 1] = ind[y]; y--; }
+            ind[y + 1] = v;
+        }
+        save_record(s, i, m, ind);
+    }
+    free(ind); free(first); free(pts); free(bk);
+    return s->npairs;
+}
+
+/* GenerateImageLists for the MM/MM list of a pure MM system
+ * (pMx/csource/NBModelABFS.c:753-1045): transformations t (fractional
+ * rotation rot[9t..], translation tr[3t..]) with their inverses, box search
+ * limits, inverse skipping and scales, the box-overlap test and one cross
+ * pair list per image.  The image coordinates are generated as the reference
+ * does: transform once per t, then translate by +d and back by -d for every
+ * (a, b, c) that is not skipped.
+ *
+ * Outputs per image (at most cap_img, in reference order, images with no
+ * pairs dropped as ImageList_CreateImage does, pMx/csource/ImageList.c:31-62):
+ * abct[4k..] = a, b, c, t; scale[k]; records appended to the sink with
+ * img_rec[k] = first record of image k (img_rec[nimg] = total records).
+ * ongrid (n) may be NULL (direct method).  Returns the number of images, or
+ * -1 on a capacity overflow; sizes[3] = images, records, pairs needed. */
+long orc_image_lists(int n, const double *xyz, const int *ongrid, int ntrans, const double *rot, const double *tr,
+                     const double *sp, double cutoff, int checkForInverses, int expandFactor,
+                     long cap_img, int *abct, double *scale, long *img_rec,
+                     long cap_rec, long cap_pairs, int *rec_i, long *rec_ptr, int *js, long *sizes)
+{
+    double M[9], Minv[9], lower[3], upper[3], ext[3];
+    int *inverses = (int *)malloc(sizeof(int) * (size_t)(ntrans > 0 ? ntrans : 1));
+    int identity = orc_find_inverses(ntrans, rot, tr, inverses);
+    double *ixyz = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    PairSink s = {rec_i, rec_ptr, js, cap_rec, cap_pairs, 0, 0, 0};
+    long nimg = 0;
+    if (cap_rec > 0 && rec_ptr) rec_ptr[0] = 0;
+    orc_symmetry_make_m(sp, M, Minv);
+    enclosing_box(n, xyz, lower, ext);
+    for (int d = 0; d < 3; d++) { upper[d] = ext[d] + lower[d]; lower[d] += -cutoff; upper[d] += cutoff; }
+    for (int t = 0; t < ntrans && n > 0; t++) {
+        int tinv = checkForInverses ? inverses[t] : -1, qtskip = 0;
+        double defscale = 0.5;
+        if (tinv >= 0) {
+            if (t < tinv) continue;          /* pure MM: skip, the inverse comes later */
+            defscale = 1.0;
+        }
+        double Ro[9], To[3], ilower[3], iupper[3], iext[3];
+        orc_orthogonalize(M, Minv, rot + 9 * t, tr + 3 * t, Ro, To);
+        transform_coordinates(n, xyz, Ro, To, ixyz);
+        enclosing_box(n, ixyz, ilower, iext);
+        for (int d = 0; d < 3; d++) iupper[d] = iext[d] + ilower[d];
+        int lim[6];
+        box_search_limits(M, lower, upper, ilower, iupper, lim);
+        if (expandFactor > 0) {
+            lim[0] -= expandFactor; lim[1] += expandFactor;
+            lim[2] -= expandFactor; lim[3] += expandFactor;
+            lim[4] -= expandFactor; lim[5] += expandFactor;
+        }
+        for (int a = lim[0]; a <= lim[1]; a++)
+            for (int b = lim[2]; b <= lim[3]; b++)
+                for (int c = lim[4]; c <= lim[5]; c++) {
+                    if (a == 0 && b == 0 && c == 0 && t == identity) continue;
+                    double sc = defscale;
+                    if (tinv >= 0 && tinv == t) {
+                        int inv[3];
+                        inverse_integer_translation(rot, tr, t, tinv, a, b, c, inv);
+                        if (inv[0] >= lim[0] && inv[0] <= lim[1] && inv[1] >= lim[2] && inv[1] <= lim[3] &&
+                            inv[2] >= lim[4] && inv[2] <= lim[5]) {
+                            if (a == inv[0] && b == inv[1] && c == inv[2]) sc = 0.5;
+                            else {
+                                sc = 1.0;
+                                if (a < inv[0] || (a == inv[0] && b < inv[1]) || (a == inv[0] && b == inv[1] && c < inv[2]))
+                                    continue;
+                            }
+                        } else sc = 1.0;
+                    }
+                    (void)qtskip;
+                    /* SymmetryParameters_Displacement (SymmetryParameters.c:166-178). */
+                    double dsp[3], na = (double)a, nb = (double)b, nc = (double)c;
+                    for (int i = 0; i < 3; i++) dsp[i] = na * M[3 * i] + nb * M[3 * i + 1] + nc * M[3 * i + 2];
+                    translate_coordinates(n, ixyz, dsp);
+                    for (int d = 0; d < 3; d++) { ilower[d] += dsp[d]; iupper[d] += dsp[d]; }
+                    if (ilower[0] <= upper[0] && ilower[1] <= upper[1] && ilower[2] <= upper[2] &&
+                        iupper[0] >= lower[0] && iupper[1] >= lower[1] && iupper[2] >= lower[2]) {
+                        long r0 = s.nrec, p0 = s.npairs;
+                        cross_pair_list(n, xyz, ixyz, ongrid, cutoff, &s);
+                        if (s.npairs > p0) {
+                            if (nimg < cap_img) {
+                                abct[4 * nimg] = a; abct[4 * nimg + 1] = b; abct[4 * nimg + 2] = c;
+                                abct[4 * nimg + 3] = t;
+                                scale[nimg] = sc;
+                                img_rec[nimg] = r0;
+                            }
+                            nimg++;
+                        }
+                    }
+                    for (int d = 0; d < 3; d++) dsp[d] *= -1.0e+00;
+                    translate_coordinates(n, ixyz, dsp);
+                    for (int d = 0; d < 3; d++) { ilower[d] += dsp[d]; iupper[d] += dsp[d]; }
+                }
+    }
+    if (nimg <= cap_img && img_rec) img_rec[nimg] = s.nrec;
+    free(inverses);
+    free(ixyz);
+    sizes[0] = nimg;
+    sizes[1] = s.nrec;
+    sizes[2] = s.npairs;
+    if (s.overflow || nimg > cap_img) return -1;
+    return nimg;
+}
+
+/* Matrix33_InverseDerivative (pCx/csource/Matrix33.c:116-160): d(M^-1)/dM_ij. */
+static void m33_inverse_derivative(const double *m, int i, int j, double *o)
+{
+    double ddet = 0.0;
+    double m00 = m[0], m01 = m[1], m02 = m[2], m10 = m[3], m11 = m[4], m12 = m[5], m20 = m[6], m21 = m[7], m22 = m[8];
+    m33_invert(m, o);
+    switch (3 * i + j) {
+    case 0: ddet = -m12 * m21 + m11 * m22; break;
+    case 1: ddet = m12 * m20 - m10 * m22; break;
+    case 2: ddet = -m11 * m20 + m10 * m21; break;
+    case 3: ddet = m02 * m21 - m01 * m22; break;
+    case 4: ddet = -m02 * m20 + m00 * m22; break;
+    case 5: ddet = m01 * m20 - m00 * m21; break;
+    case 6: ddet = -m02 * m11 + m01 * m12; break;
+    case 7: ddet = m02 * m10 - m00 * m12; break;
+    case 8: ddet = -m01 * m10 + m00 * m11; break;
+    }
+    for (int k = 0; k < 9; k++) o[k] *= -ddet;
+    switch (3 * i + j) {
+    case 0: o[4] += m22; o[5] -= m12; o[7] -= m21; o[8] += m11; break;
+    case 1: o[1] -= m22; o[2] += m12; o[7] += m20; o[8] -= m10; break;
+    case 2: o[1] += m21; o[2] -= m11; o[4] -= m20; o[5] += m10; break;
+    case 3: o[3] -= m22; o[5] += m02; o[6] += m21; o[8] -= m01; break;
+    case 4: o[0] += m22; o[2] -= m02; o[6] -= m20; o[8] += m00; break;
+    case 5: o[0] -= m21; o[2] += m01; o[3] += m20; o[5] -= m00; break;
+    case 6: o[3] += m12; o[4] -= m02; o[6] -= m11; o[7] += m01; break;
+    case 7: o[0] -= m12; o[1] += m02; o[6] += m10; o[7] -= m00; break;
+    case 8: o[0] += m11; o[1] -= m01; o[3] -= m10; o[4] += m00; break;
+    }
+    double det = m33_det(m), s = 1.0 / det;
+    for (int k = 0; k < 9; k++) o[k] *= s;
+}
+
+/* SymmetryParameterGradients_ImageDerivatives (pMx/csource/
+ * SymmetryParameterGradients.c:142-216): dE/dM of one image from its
+ * unrotated image gradients; xrot/xtr is the fractional transformation with
+ * the integer translation added. */
+static void image_derivatives(const double *M, const double *Minv, const double *xrot, const double *xtr, int n,
+                              const double *xyz, const double *ig, double *dEdM)
+{
+    double ms[9], si[9];
+    memcpy(ms, xrot, sizeof(ms)); m33_premul(ms, M);
+    memcpy(si, xrot, sizeof(si)); m33_postmul(si, Minv);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            double di[9];
+            m33_inverse_derivative(M, a, b, di);
+            m33_premul(di, ms);
+            di[3 * a] += si[3 * b]; di[3 * a + 1] += si[3 * b + 1]; di[3 * a + 2] += si[3 * b + 2];
+            double t = xtr[b], sum = 0.0;
+            for (int i = 0; i < n; i++) {
+                double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+                double dx = di[0] * x + di[1] * y + di[2] * z;
+                double dy = di[3] * x + di[4] * y + di[5] * z;
+                double dz = di[6] * x + di[7] * y + di[8] * z;
+                if (a == 0) dx += t; else if (a == 1) dy += t; else dz += t;
+                sum += dx * ig[3 * i] + dy * ig[3 * i + 1] + dz * ig[3 * i + 2];
+            }
+            dEdM[3 * a + b] += sum;
+        }
+}
+
+/* MMMMImageEnergy (pMx/csource/NBModelABFS.c:1161-1290), serially over the
+ * images: for image k the fractional transformation gets (a, b, c) added,
+ * is orthogonalized with the current M, the image coordinates are generated,
+ * PairwiseInteractionABFS_MMMMEnergy runs over the image's records with
+ * electrostatic and LJ scales times the image scale (i gradients into grad,
+ * j gradients into the image gradient), then the lattice derivative, and the
+ * image gradient rotated by R'^T is added to grad.  grad and dEdM (row-major,
+ * incremented) may be NULL. */
+void orc_image_energy(const double *f, int n, const double *xyz, const double *charge, const int *ljtype, int ntypes,
+                      const double *tableA, const double *tableB, const int *tableindex, double electrostaticScale,
+                      const double *rot, const double *tr, const double *sp, long nimg, const int *abct,
+                      const double *scale, const long *img_rec, const int *rec_i, const long *rec_ptr, const int *js,
+                      double *eel, double *elj, double *grad, double *dEdM)
+{
+    double M[9], Minv[9];
+    double *ixyz = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    double *ig = grad ? (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1)) : NULL;
+    double eQQ = 0.0, eLJ = 0.0;
+    orc_symmetry_make_m(sp, M, Minv);
+    for (long k = 0; k < nimg; k++) {
+        int t = abct[4 * k + 3];
+        double xtr[3] = {tr[3 * t] + (double)abct[4 * k], tr[3 * t + 1] + (double)abct[4 * k + 1],
+                         tr[3 * t + 2] + (double)abct[4 * k + 2]};
+        double Ro[9], To[3];
+        orc_orthogonalize(M, Minv, rot + 9 * t, xtr, Ro, To);
+        transform_coordinates(n, xyz, Ro, To, ixyz);
+        if (ig) memset(ig, 0, sizeof(double) * 3 * (size_t)n);
+        double escale = electrostaticScale * scale[k] * orc_coulomb_factor(), ljs = scale[k];
+        for (long r = img_rec[k]; r < img_rec[k + 1]; r++) {
+            int i = rec_i[r];
+            double qi = escale * charge[i];
+            int ti = ntypes * ljtype[i];
+            double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+            for (long q = rec_ptr[r]; q < rec_ptr[r + 1]; q++) {
+                int j = js[q];
+                double xij = xi - ixyz[3 * j], yij = yi - ixyz[3 * j + 1], zij = zi - ixyz[3 * j + 2];
+                double r2 = (xij * xij + yij * yij + zij * zij);
+                double qij = qi * charge[j];
+                int tij = tableindex[ti + ljtype[j]];
+                double e1, e2, dF;
+                if (!orc_pair_terms(f, r2, qij, tableA[tij] * ljs, tableB[tij] * ljs, &e1, &e2, &dF)) continue;
+                eQQ += e1;
+                eLJ += e2;
+                if (grad) {
+                    xij *= (2.0 * dF); yij *= (2.0 * dF); zij *= (2.0 * dF);
+                    grad[3 * i] += xij; grad[3 * i + 1] += yij; grad[3 * i + 2] += zij;
+                    ig[3 * j] -= xij; ig[3 * j + 1] -= yij; ig[3 * j + 2] -= zij;
+                }
+            }
+        }
+        if (grad) {
+            if (dEdM) image_derivatives(M, Minv, rot + 9 * t, xtr, n, xyz, ig, dEdM);
+            /* Coordinates3_Rotate by R'^T, then add. */
+            for (int i = 0; i < n; i++) {
+                double x0 = ig[3 * i], y0 = ig[3 * i + 1], z0 = ig[3 * i + 2];
+                grad[3 * i] += Ro[0] * x0 + Ro[3] * y0 + Ro[6] * z0;
+                grad[3 * i + 1] += Ro[1] * x0 + Ro[4] * y0 + Ro[7] * z0;
+                grad[3 * i + 2] += Ro[2] * x0 + Ro[5] * y0 + Ro[8] * z0;
+            }
+        }
+    }
+    free(ixyz);
+    free(ig);
+    *eel = eQQ;
+    *elj = eLJ;
+}
+
+/* SymmetryParameterGradients_CrystalDerivatives (SymmetryParameterGradients.c:50-82):
+ * out = dE/da, dE/db, dE/dc, dE/dalpha, dE/dbeta, dE/dgamma (angles in degrees). */
+void orc_crystal_derivatives(const double *sp, const double *dEdM, double *out)
+{
+    const double dr = M_PI / 180.0e+00;
+    double alpha = sp[3] * dr, beta = sp[4] * dr, gamma = sp[5] * dr;
+    double ca = cos(alpha), cb = cos(beta), cg = cos(gamma), sg = sin(gamma);
+    double f12 = (ca - cb * cg);
+    double f22 = sqrt(1.0e+00 - ca * ca - cb * cb - cg * cg + 2.0e+00 * ca * cb * cg);
+    const double *d = dEdM;
+    out[0] = d[0];
+    out[1] = cg * d[1] + sg * d[4];
+    out[2] = cb * d[2] + f12 * d[5] / sg + f22 * d[8] / sg;
+    out[3] = -(d[5] - f12 * d[8] / f22) * (sp[2] * sin(alpha) * dr) / sg;
+    out[4] = -(sg * d[2] - cg * d[5] + (ca * cg - cb) * d[8] / f22) * (sp[2] * sin(beta) * dr) / sg;
+    out[5] = (sp[1] * (-sg * d[1] + cg * d[4]) +
+              sp[2] * ((cb - f12 * cg / (sg * sg)) * d[5] + ((cg - ca * cb) / f22 - f22 * cg / (sg * sg)) * d[8])) * dr;
+}

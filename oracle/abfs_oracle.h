@@ -83,6 +83,26 @@ void orc_mmmm_energy(const double *f, int n, const double *xyz, const double *ch
 void orc_lj_make_table(int ntypes, const double *eps, const double *sigma, int geometric,
                        int four_eps, double *tableA, double *tableB, int *tableindex);
 
+/* ---- Crystal symmetry and the MM/MM image list (pMx/csource/NBModelABFS.c:
+ * 625-684,753-1045,1161-1290; pMx/csource/SymmetryParameters.c;
+ * pMx/csource/SymmetryParameterGradients.c; pCx/csource/Transformation3*.c).
+ * sp = a, b, c, alpha, beta, gamma (degrees); rot 9 per transformation
+ * (fractional, row-major), tr 3 per transformation; matrices row-major. */
+void orc_symmetry_make_m(const double *sp, double *M, double *Minv);
+void orc_orthogonalize(const double *M, const double *Minv, const double *rot, const double *tr, double *Ro,
+                       double *To);
+int orc_find_inverses(int ntrans, const double *rot, const double *tr, int *inverses);
+long orc_image_lists(int n, const double *xyz, const int *ongrid, int ntrans, const double *rot, const double *tr,
+                     const double *sp, double cutoff, int checkForInverses, int expandFactor,
+                     long cap_img, int *abct, double *scale, long *img_rec,
+                     long cap_rec, long cap_pairs, int *rec_i, long *rec_ptr, int *js, long *sizes);
+void orc_image_energy(const double *f, int n, const double *xyz, const double *charge, const int *ljtype, int ntypes,
+                      const double *tableA, const double *tableB, const int *tableindex, double electrostaticScale,
+                      const double *rot, const double *tr, const double *sp, long nimg, const int *abct,
+                      const double *scale, const long *img_rec, const int *rec_i, const long *rec_ptr, const int *js,
+                      double *eel, double *elj, double *grad, double *dEdM);
+void orc_crystal_derivatives(const double *sp, const double *dEdM, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,17 @@ def lib():
                                       C.c_long, _ip, _lp, _ip, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.c_void_p]
         L.orc_lj_make_table.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _ip]
+        L.orc_symmetry_make_m.argtypes = [_dp, _dp, _dp]
+        L.orc_orthogonalize.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_find_inverses.argtypes = [C.c_int, _dp, _dp, _ip]
+        L.orc_image_lists.restype = C.c_long
+        L.orc_image_lists.argtypes = [C.c_int, _dp, C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int,
+                                      C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, _lp]
+        L.orc_image_energy.argtypes = [_dp, C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp, _ip, C.c_double, _dp, _dp, _dp,
+                                       C.c_long, _ip, _dp, _lp, _ip, _lp, _ip, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
+        L.orc_crystal_derivatives.argtypes = [_dp, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -225,6 +236,131 @@ def records_from_pairs(n, pairs):
     return PairRecords(ri.astype(np.int32), ptr, lo.astype(np.int32), n)
 
 
+# ----------------------------------------------------------------------------------------------------------------
+# Crystal symmetry: image lists and image energy.
+# ----------------------------------------------------------------------------------------------------------------
+def symmetry_make_m(sp):
+    """M (columns = lattice vectors) and M^-1 (SymmetryParameters_MakeM)."""
+    M, Mi = np.zeros(9), np.zeros(9)
+    lib().orc_symmetry_make_m(np.asarray(sp, np.float64), M, Mi)
+    return M.reshape(3, 3), Mi.reshape(3, 3)
+
+
+def orthogonalize(sp, rot, tr):
+    """Real-space R' = M R M^-1 and T' = M t of a fractional transformation."""
+    M, Mi = symmetry_make_m(sp)
+    Ro, To = np.zeros(9), np.zeros(3)
+    lib().orc_orthogonalize(M.reshape(-1).copy(), Mi.reshape(-1).copy(), np.ascontiguousarray(rot, np.float64).reshape(-1),
+                            np.ascontiguousarray(tr, np.float64), Ro, To)
+    return Ro.reshape(3, 3), To
+
+
+def find_inverses(rot, tr):
+    """(identity index, inverses) of a transformation set (rotations only)."""
+    rot = np.ascontiguousarray(rot, np.float64).reshape(-1)
+    tr = np.ascontiguousarray(tr, np.float64).reshape(-1)
+    inv = np.zeros(tr.shape[0] // 3, np.int32)
+    ident = lib().orc_find_inverses(inv.shape[0], rot, tr, inv)
+    return ident, inv
+
+
+def crystal_derivatives(sp, dEdM):
+    """dE/d(a, b, c, alpha, beta, gamma) from dE/dM (SymmetryParameterGradients_CrystalDerivatives)."""
+    out = np.zeros(6)
+    lib().orc_crystal_derivatives(np.asarray(sp, np.float64), np.ascontiguousarray(dEdM, np.float64).reshape(-1), out)
+    return out
+
+
+class ImageLists:
+    """MM/MM image list: abct (k, 4) = a, b, c, t; scale (k); records of every image back to back,
+    image k owning records img_rec[k]..img_rec[k+1]."""
+
+    def __init__(self, n, abct, scale, img_rec, records):
+        self.n, self.abct, self.scale, self.img_rec, self.records = n, abct, scale, img_rec, records
+
+    @property
+    def nimages(self):
+        return self.abct.shape[0]
+
+    @property
+    def npairs(self):
+        return self.records.npairs
+
+    def image_pairs(self, k):
+        r = self.records
+        r0, r1 = self.img_rec[k], self.img_rec[k + 1]
+        cnt = np.diff(r.rec_ptr[r0:r1 + 1])
+        i = np.repeat(r.rec_i[r0:r1], cnt)
+        j = r.js[r.rec_ptr[r0]:r.rec_ptr[r1]]
+        return i.astype(np.int64), j.astype(np.int64)
+
+    def image_keys(self, k):
+        """Sorted i * n + j keys of image k (i primary, j image)."""
+        i, j = self.image_pairs(k)
+        return np.sort(i * self.n + j)
+
+
+def image_lists(xyz, transformations, sp, cutoff=13.5, use_grid=True, cell_size=6.75, check_for_inverses=True,
+                expand_factor=0):
+    """GenerateImageLists (MM/MM) on the restatement."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = xyz.shape[0]
+    rot = np.ascontiguousarray(transformations[0], np.float64).reshape(-1)
+    tr = np.ascontiguousarray(transformations[1], np.float64).reshape(-1)
+    ongrid = None
+    if use_grid:
+        bins = np.zeros(3, np.int32)
+        lower = np.zeros(3)
+        cell = np.zeros(max(n, 1), np.int32)
+        lib().orc_make_grid(n, xyz, cell_size, bins, lower, cell)
+        ongrid = np.ascontiguousarray((cell[:n] >= 0).astype(np.int32))
+    sp = np.asarray(sp, np.float64)
+    sizes = np.zeros(3, np.int64)
+    L = lib()
+    args = [n, xyz, None if ongrid is None else ongrid.ctypes.data, tr.shape[0] // 3, rot, tr, sp, cutoff,
+            1 if check_for_inverses else 0, int(expand_factor)]
+    L.orc_image_lists(*args, 0, None, None, None, 0, 0, None, None, None, sizes)
+    ni, nr, npr = (int(v) for v in sizes)
+    abct = np.zeros(4 * max(ni, 1), np.int32)
+    scale = np.zeros(max(ni, 1))
+    img_rec = np.zeros(ni + 1, np.int64)
+    rec_i = np.zeros(max(nr, 1), np.int32)
+    rec_ptr = np.zeros(nr + 1, np.int64)
+    js = np.zeros(max(npr, 1), np.int32)
+    got = L.orc_image_lists(*args, ni, abct.ctypes.data, scale.ctypes.data, img_rec.ctypes.data, nr, npr,
+                            rec_i.ctypes.data, rec_ptr.ctypes.data, js.ctypes.data, sizes)
+    assert got == ni
+    return ImageLists(n, abct[:4 * ni].reshape(-1, 4), scale[:ni], img_rec,
+                      PairRecords(rec_i[:nr], rec_ptr, js[:npr], n))
+
+
+def image_energy(f, xyz, charges, ljtypes, ntypes, tableA, tableB, transformations, sp, images, electrostatic_scale=1.0,
+                 grad=None, dEdM=None):
+    """MMMMImageEnergy on the restatement; grad (n,3) and dEdM (3,3) are incremented when given."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = xyz.shape[0]
+    tidx = full_table(ntypes, tableA, tableB)
+    rot = np.ascontiguousarray(transformations[0], np.float64).reshape(-1)
+    tr = np.ascontiguousarray(transformations[1], np.float64).reshape(-1)
+    eel, elj = C.c_double(), C.c_double()
+    if grad is not None:
+        assert grad.dtype == np.float64 and grad.flags.c_contiguous and grad.shape == (n, 3)
+    dm = None
+    if dEdM is not None:
+        dm = np.ascontiguousarray(dEdM, np.float64).reshape(9).copy()
+    r = images.records
+    lib().orc_image_energy(f, n, xyz, np.ascontiguousarray(charges, np.float64), np.ascontiguousarray(ljtypes, np.int32),
+                           ntypes, np.ascontiguousarray(tableA, np.float64), np.ascontiguousarray(tableB, np.float64),
+                           tidx, electrostatic_scale, rot, tr, np.asarray(sp, np.float64), images.nimages,
+                           np.ascontiguousarray(images.abct, np.int32).reshape(-1), np.ascontiguousarray(images.scale),
+                           np.ascontiguousarray(images.img_rec, np.int64), r.rec_i, r.rec_ptr, r.js,
+                           C.byref(eel), C.byref(elj), None if grad is None else grad.ctypes.data,
+                           None if dm is None else dm.ctypes.data)
+    if dEdM is not None:
+        dEdM[...] = dm.reshape(dEdM.shape)
+    return eel.value, elj.value
+
+
 class OracleABFS:
     """NBModelABFS state machine on the FP64 restatement.
 
@@ -237,8 +373,14 @@ class OracleABFS:
 
     def __init__(self, system, damping=0.5, inner=8.0, outer=12.0, list_cutoff=13.5, dielectric=1.0,
                  scale14=1.0, cell_factor=0.5, min_extent=2, min_points=500, tableA14=None, tableB14=None,
-                 fixed=None, nthreads=0):
+                 fixed=None, nthreads=0, transformations=None, check_for_inverses=True, image_expand_factor=0):
         self.s = system
+        self.transformations = transformations
+        self.check_for_inverses = check_for_inverses
+        self.image_expand_factor = image_expand_factor
+        self.images = None
+        self.ref_sp = None
+        self.max_displacement = 0.0
         self.f = factors(damping, inner, outer)
         self.outer, self.list_cutoff = outer, list_cutoff
         self.dielectric, self.scale14 = dielectric, scale14
@@ -257,8 +399,9 @@ class OracleABFS:
 
     def update(self, xyz):
         rebuild = self.ref is None
+        self.max_displacement = 0.0
         if not rebuild:
-            rebuild, _ = check_for_update(xyz, self.ref, self.list_cutoff, self.outer, self.fixed)
+            rebuild, self.max_displacement = check_for_update(xyz, self.ref, self.list_cutoff, self.outer, self.fixed)
         if rebuild:
             if self.use_grid is None:
                 self.use_grid = determine_method(xyz, self.list_cutoff, self.cell_factor, self.min_extent,
@@ -271,11 +414,55 @@ class OracleABFS:
             self.updates += 1
         return rebuild
 
-    def energy(self, xyz, gradients=True):
-        """Returns (terms, gradient, rebuilt) with pDynamo's labels."""
+    def check_for_image_update(self, sp):
+        """CheckForImageUpdate (pMolecule-1.9.0/extensions/csource/NBModelABFS.c:635-684): some image's lattice
+        translation moved by more than listCutoff - outerCutoff - the largest atom displacement."""
+        if self.images is None or self.images.nimages == 0:
+            return False
+        buffac = self.list_cutoff - self.outer - self.max_displacement
+        M, _ = symmetry_make_m(sp)
+        Mr, _ = symmetry_make_m(self.ref_sp)
+        dM = M - Mr
+        tr = np.asarray(self.transformations[1], np.float64).reshape(-1, 3)
+        for a, b, c, t in self.images.abct:
+            d = tr[t] + np.array([a, b, c], np.float64)
+            if np.linalg.norm(dM @ d) > buffac:
+                return True
+        return False
+
+    def energy_sym(self, xyz, sp, gradients=True):
+        """A call with crystal parameters sp = (a, b, c, alpha, beta, gamma): the update of NBModelABFS_Update
+        (NBModelABFS.c:508-623, image lists rebuilt with the main list or when CheckForImageUpdate holds), then
+        the image energy before the main and 1-4 lists (NBModelABFS.c:228-301).
+        Returns (terms, gradient, dEdM (3x3), rebuilt)."""
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
         self.calls += 1
         rebuilt = self.update(xyz)
+        if rebuilt or self.check_for_image_update(sp):
+            self.images = image_lists(xyz, self.transformations, sp, self.list_cutoff, bool(self.use_grid),
+                                      self.cell_factor * self.list_cutoff, self.check_for_inverses,
+                                      self.image_expand_factor)
+            self.ref_sp = np.array(sp, np.float64)
+            rebuilt = True
+        s = self.s
+        g = np.zeros((s.natoms, 3)) if gradients else None
+        dm = np.zeros((3, 3)) if gradients else None
+        iterms = []
+        if self.images.nimages > 0:
+            e1, e2 = image_energy(self.f, xyz, s.charges, s.ljtypes, s.ntypes, s.tableA, s.tableB, self.transformations,
+                                  sp, self.images, 1.0 / self.dielectric, g, dm)
+            iterms = [("MM/MM Image Elect.", e1), ("MM/MM Image LJ", e2)]
+        self.calls -= 1
+        terms, g2, _ = self.energy(xyz, gradients, _rebuilt=rebuilt)
+        if gradients:
+            g += g2
+        return terms + iterms, g, dm, rebuilt
+
+    def energy(self, xyz, gradients=True, _rebuilt=None):
+        """Returns (terms, gradient, rebuilt) with pDynamo's labels."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.calls += 1
+        rebuilt = self.update(xyz) if _rebuilt is None else _rebuilt
         s = self.s
         g = np.zeros((s.natoms, 3)) if gradients else None
         terms = []
@@ -314,6 +501,13 @@ def ref_lib():
         L.ref_pairs.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_long]
         L.ref_use_grid.argtypes = [C.c_void_p]
         L.ref_destroy.argtypes = [C.c_void_p]
+        L.ref_create_sym.restype = C.c_void_p
+        L.ref_create_sym.argtypes = L.ref_create.argtypes + [C.c_int, _dp, _dp, C.c_int, C.c_int]
+        L.ref_energy_sym.argtypes = [C.c_void_p, _dp, _dp, C.c_void_p, _dp, _dp, _ip]
+        L.ref_images.restype = C.c_long
+        L.ref_images.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]
+        L.ref_image_pairs.restype = C.c_long
+        L.ref_image_pairs.argtypes = [C.c_void_p, C.c_long, C.c_void_p, C.c_void_p, C.c_long]
         _REF = L
     return _REF
 
@@ -323,7 +517,8 @@ class ReferenceABFS:
 
     def __init__(self, system, damping=0.5, inner=8.0, outer=12.0, list_cutoff=13.5, dielectric=1.0,
                  scale14=1.0, cell_factor=0.5, min_extent=2, min_points=500, tableA14=None, tableB14=None,
-                 nthreads=1):
+                 nthreads=1, transformations=None, check_for_inverses=True, image_expand_factor=0):
+        """transformations: None, or (rotations (k,3,3), translations (k,3)) in fractional form."""
         L = ref_lib()
         s = system
         self.n = s.natoms
@@ -335,12 +530,61 @@ class ReferenceABFS:
         self._keep += [tA14, tB14]
         opts = np.array([damping, inner, outer, list_cutoff, dielectric, scale14])
         gen = np.array([cell_factor, min_extent, min_points], dtype=np.float64)
-        self.h = L.ref_create(self.n, np.ascontiguousarray(s.charges, np.float64),
-                              np.ascontiguousarray(s.ljtypes, np.int32), s.ntypes,
-                              np.ascontiguousarray(s.tableA, np.float64), np.ascontiguousarray(s.tableB, np.float64),
-                              None if tA14 is None else tA14.ctypes.data, None if tB14 is None else tB14.ctypes.data,
-                              ex.shape[0] // 2, ex.ctypes.data if ex.size else None,
-                              p14.shape[0] // 2, p14.ctypes.data if p14.size else None, opts, gen, nthreads)
+        args = [self.n, np.ascontiguousarray(s.charges, np.float64), np.ascontiguousarray(s.ljtypes, np.int32),
+                s.ntypes, np.ascontiguousarray(s.tableA, np.float64), np.ascontiguousarray(s.tableB, np.float64),
+                None if tA14 is None else tA14.ctypes.data, None if tB14 is None else tB14.ctypes.data,
+                ex.shape[0] // 2, ex.ctypes.data if ex.size else None,
+                p14.shape[0] // 2, p14.ctypes.data if p14.size else None, opts, gen, nthreads]
+        self.symmetric = transformations is not None
+        if self.symmetric:
+            rot = np.ascontiguousarray(transformations[0], np.float64).reshape(-1)
+            tra = np.ascontiguousarray(transformations[1], np.float64).reshape(-1)
+            self._keep += [rot, tra]
+            self.h = L.ref_create_sym(*args, tra.shape[0] // 3, rot, tra, 1 if check_for_inverses else 0,
+                                      int(image_expand_factor))
+        else:
+            self.h = L.ref_create(*args)
+
+    def energy_sym(self, xyz, symmetry_parameters, gradients=True):
+        """Returns (terms, gradient, dEdM (3x3), rebuilt) for a call with crystal parameters
+        (a, b, c, alpha, beta, gamma)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        e = np.zeros(6)
+        present = np.zeros(3, np.int32)
+        dm = np.zeros(9)
+        g = np.zeros((self.n, 3)) if gradients else None
+        up = ref_lib().ref_energy_sym(self.h, xyz, np.asarray(symmetry_parameters, np.float64),
+                                      None if g is None else g.ctypes.data, e, dm, present)
+        if up < 0:
+            raise RuntimeError("reference library error")
+        terms = []
+        if present[0]:
+            terms += [("MM/MM Elect.", e[0]), ("MM/MM LJ", e[1])]
+        if present[1]:
+            terms += [("MM/MM 1-4 Elect.", e[2]), ("MM/MM 1-4 LJ", e[3])]
+        if present[2]:
+            terms += [("MM/MM Image Elect.", e[4]), ("MM/MM Image LJ", e[5])]
+        return terms, g, dm.reshape(3, 3), bool(up)
+
+    def images(self):
+        """[(a, b, c, t, scale, npairs)] of the MM/MM image list, in list order."""
+        L = ref_lib()
+        m = L.ref_images(self.h, None, None, None, 0)
+        abct = np.zeros(4 * max(m, 1), np.int32)
+        sc = np.zeros(max(m, 1))
+        npr = np.zeros(max(m, 1), np.int64)
+        L.ref_images(self.h, abct.ctypes.data, sc.ctypes.data, npr.ctypes.data, m)
+        return [(int(abct[4 * k]), int(abct[4 * k + 1]), int(abct[4 * k + 2]), int(abct[4 * k + 3]), float(sc[k]),
+                 int(npr[k])) for k in range(m)]
+
+    def image_pairs(self, k):
+        """(i, j) pairs of image k: i in the primary image, j in image k."""
+        L = ref_lib()
+        m = L.ref_image_pairs(self.h, k, None, None, 0)
+        oi = np.zeros(max(m, 1), np.int32)
+        oj = np.zeros(max(m, 1), np.int32)
+        L.ref_image_pairs(self.h, k, oi.ctypes.data, oj.ctypes.data, m)
+        return oi[:m], oj[:m]
 
     def energy(self, xyz, gradients=True):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)

@@ -26,6 +26,9 @@
 #include "PairList.h"
 #include "PairListGenerator.h"
 #include "PairwiseInteraction.h"
+#include "SymmetryParameterGradients.h"
+#include "SymmetryParameters.h"
+#include "Transformation3Container.h"
 
 typedef struct {
     int n;
@@ -37,6 +40,9 @@ typedef struct {
     PairList *excl, *i14;
     NBModelABFSState *state;
     Coordinates3 *crd, *grd;
+    Transformation3Container *trans;     /* NULL: no symmetry */
+    SymmetryParameters *sp;
+    SymmetryParameterGradients *spg;
 } RefHandle;
 
 static PairList *make_self_list(int npairs, const int *pairs)
@@ -60,10 +66,29 @@ static LJParameterContainer *make_lj(int ntypes, const double *tA, const double 
 /* opts: damping, inner, outer, list, dielectric, scale14.
  * gen: cutoffCellSizeFactor, minimumCellExtent, minimumPoints (the ABFS
  * default generator of pMolecule.NBModelABFS.pyx:59-71 otherwise). */
-void *ref_create(int n, const double *charges, const int *ljtypes, int ntypes,
-                 const double *tA, const double *tB, const double *tA14, const double *tB14,
-                 int nexcl, const int *excl, int n14, const int *p14,
-                 const double *opts, const double *gen, int nthreads)
+static Transformation3Container *make_transformations(int ntrans, const double *rot, const double *trans)
+{
+    Transformation3Container *tc = Transformation3Container_Allocate(ntrans);
+    tc->QOWNER = True;
+    for (int t = 0; t < ntrans; t++) {
+        Transformation3 *x = Transformation3_Allocate();
+        for (int r = 0; r < 3; r++) {
+            for (int c = 0; c < 3; c++) Matrix33_Item(x->rotation, r, c) = rot[9 * t + 3 * r + c];
+            Vector3_Item(x->translation, r) = trans[3 * t + r];
+        }
+        tc->items[t] = x;
+    }
+    /* As pCore.Transformation3Container.FillCObject (pyx:93-104). */
+    Transformation3Container_FindIdentity(tc);
+    Transformation3Container_FindInverses(tc);
+    return tc;
+}
+
+static void *create_common(int n, const double *charges, const int *ljtypes, int ntypes,
+                           const double *tA, const double *tB, const double *tA14, const double *tB14,
+                           int nexcl, const int *excl, int n14, const int *p14,
+                           const double *opts, const double *gen, int nthreads,
+                           Transformation3Container *tc, int checkForInverses, int expandFactor)
 {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -102,11 +127,39 @@ void *ref_create(int n, const double *charges, const int *ljtypes, int ntypes,
     h->lj14 = make_lj(ntypes, tA14 ? tA14 : tA, tB14 ? tB14 : tB);
     h->excl = make_self_list(nexcl, excl);
     h->i14 = make_self_list(n14, p14);
+    h->trans = tc;
+    if (tc) {
+        h->model->checkForInverses = checkForInverses ? True : False;
+        h->model->imageExpandFactor = expandFactor;
+        h->sp = SymmetryParameters_Allocate();
+        h->spg = SymmetryParameterGradients_Allocate();
+    }
     h->state = NBModelABFSState_SetUp(h->atoms, NULL, NULL, h->excl, h->i14, h->lj, h->lj14,
-                                      NULL, NULL, NULL, NULL, QCMMLinkAtomCoupling_MM);
+                                      NULL, NULL, NULL, tc, QCMMLinkAtomCoupling_MM);
     h->crd = Coordinates3_Allocate(n);
     h->grd = Coordinates3_Allocate(n);
     return h;
+}
+
+void *ref_create(int n, const double *charges, const int *ljtypes, int ntypes,
+                 const double *tA, const double *tB, const double *tA14, const double *tB14,
+                 int nexcl, const int *excl, int n14, const int *p14,
+                 const double *opts, const double *gen, int nthreads)
+{
+    return create_common(n, charges, ljtypes, ntypes, tA, tB, tA14, tB14, nexcl, excl, n14, p14, opts, gen,
+                         nthreads, NULL, 1, 0);
+}
+
+/* With crystal symmetry: ntrans fractional transformations (rot 9 per item,
+ * row-major; trans 3 per item), as System.DefineSymmetry's transformations. */
+void *ref_create_sym(int n, const double *charges, const int *ljtypes, int ntypes,
+                     const double *tA, const double *tB, const double *tA14, const double *tB14,
+                     int nexcl, const int *excl, int n14, const int *p14,
+                     const double *opts, const double *gen, int nthreads,
+                     int ntrans, const double *rot, const double *trans, int checkForInverses, int expandFactor)
+{
+    return create_common(n, charges, ljtypes, ntypes, tA, tB, tA14, tB14, nexcl, excl, n14, p14, opts, gen,
+                         nthreads, make_transformations(ntrans, rot, trans), checkForInverses, expandFactor);
 }
 
 /* One System.Energy-style call.  energies[4] = MM/MM Elect., LJ, 1-4 Elect.,
@@ -138,6 +191,90 @@ int ref_energy(void *vh, const double *coords, double *grads, double *energies, 
         }
     }
     return up == True ? 1 : 0;
+}
+
+/* As ref_energy with symmetry parameters sp6 = a, b, c, alpha, beta, gamma
+ * (System.Energy creates fresh SymmetryParameterGradients per call,
+ * pMolecule-1.9.0/pMolecule/System.py:284-285).  energies[6] adds MM/MM Image
+ * Elect. and LJ; present[3] adds the image list; dEdM[9] row-major. */
+int ref_energy_sym(void *vh, const double *coords, const double *sp6, double *grads, double *energies, double *dEdM,
+                   int *present)
+{
+    RefHandle *h = (RefHandle *)vh;
+    Status status = Status_Continue;
+    if (!h->trans) return -1;
+    for (int i = 0; i < h->n; i++) {
+        Coordinates3_SetRow(h->crd, i, coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]);
+        if (grads) Coordinates3_SetRow(h->grd, i, 0.0, 0.0, 0.0);
+    }
+    SymmetryParameters_SetCrystalParameters(h->sp, sp6[0], sp6[1], sp6[2], sp6[3], sp6[4], sp6[5]);
+    Matrix33_Set(h->spg->dEdM, 0.0);
+    NBModelABFSState_Initialize(h->state, h->crd, h->sp, grads ? h->grd : NULL, grads ? h->spg : NULL);
+    Boolean up = NBModelABFS_Update(h->model, h->gen, h->state, &status);
+    if (status != Status_Continue) return -1;
+    NBModelABFS_MMMMEnergy(h->model, h->pw, h->state);
+    energies[0] = h->state->emmel;
+    energies[1] = h->state->emmlj;
+    energies[2] = h->state->emmel14;
+    energies[3] = h->state->emmlj14;
+    energies[4] = h->state->eimmmel;
+    energies[5] = h->state->eimmmlj;
+    present[0] = (h->state->nbmmmm != NULL);
+    present[1] = (h->state->nbmmmm14 != NULL);
+    present[2] = (h->state->inbmmmm != NULL);
+    if (grads) {
+        for (int i = 0; i < h->n; i++) {
+            double x, y, z;
+            Coordinates3_GetRow(h->grd, i, x, y, z);
+            grads[3 * i] = x; grads[3 * i + 1] = y; grads[3 * i + 2] = z;
+        }
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) dEdM[3 * r + c] = Matrix33_Item(h->spg->dEdM, r, c);
+    }
+    return up == True ? 1 : 0;
+}
+
+/* Images of the MM/MM image list in list order: abct[4k..4k+3] = a, b, c and
+ * the transformation's index; scale[k]; npairs[k].  Returns the image count
+ * (writes at most cap). */
+long ref_images(void *vh, int *abct, double *scale, long *npairs, long cap)
+{
+    RefHandle *h = (RefHandle *)vh;
+    ImageList *il = h->state->inbmmmm;
+    if (il == NULL) return 0;
+    long k = 0;
+    Image *im;
+    List_Iterate_Initialize(il->images);
+    while ((im = ImageList_Iterate(il)) != NULL) {
+        if (k < cap) {
+            int t = -1;
+            for (int q = 0; q < h->trans->nitems; q++)
+                if (h->trans->items[q] == im->transformation3) t = q;
+            abct[4 * k] = im->a; abct[4 * k + 1] = im->b; abct[4 * k + 2] = im->c; abct[4 * k + 3] = t;
+            scale[k] = im->scale;
+            npairs[k] = PairList_Length(im->pairlist);
+        }
+        k++;
+    }
+    return k;
+}
+
+static long dump_list(PairList *pl, int *oi, int *oj, long cap);
+
+/* Pairs (i in the primary image, j in image k) of image k of the list. */
+long ref_image_pairs(void *vh, long image, int *oi, int *oj, long cap)
+{
+    RefHandle *h = (RefHandle *)vh;
+    ImageList *il = h->state->inbmmmm;
+    if (il == NULL) return -1;
+    long k = 0;
+    Image *im;
+    List_Iterate_Initialize(il->images);
+    while ((im = ImageList_Iterate(il)) != NULL) {
+        if (k == image) return dump_list(im->pairlist, oi, oj, cap);
+        k++;
+    }
+    return -1;
 }
 
 static long dump_list(PairList *pl, int *oi, int *oj, long cap)
@@ -179,5 +316,8 @@ void ref_destroy(void *vh)
     NBModelABFS_Deallocate(&h->model);
     Coordinates3_Deallocate(&h->crd);
     Coordinates3_Deallocate(&h->grd);
+    Transformation3Container_Deallocate(&h->trans);
+    SymmetryParameters_Deallocate(&h->sp);
+    SymmetryParameterGradients_Deallocate(&h->spg);
     free(h);
 }

@@ -252,3 +252,71 @@ def random_small_system(natoms, box, seed, ntypes=3, exclusion_fraction=0.3):
     else:
         excl = np.zeros((0, 2), np.int32)
     return MMSystem("random%d" % natoms, xyz, q, t, ntypes, a, b, excl)
+
+
+# Space groups as operation strings; the sets are those of
+# pMolecule-1.9.0/tests/CrystalMMEnergies.py:31-40 plus P-1.
+SPACE_GROUPS = {
+    "P1": ["(x,y,z)"],
+    "P-1": ["(x,y,z)", "(-x,-y,-z)"],
+    "C2": ["(x,y,z)", "(-x,y,-z)"],
+    "P21c": ["(x,y,z)", "(-x,y+1/2,-z+1/2)", "(-x,-y,-z)", "(x,-y+1/2,z+1/2)"],
+    "P212121": ["(x,y,z)", "(-x+1/2,-y,z+1/2)", "(-x,y+1/2,-z+1/2)", "(x+1/2,-y+1/2,-z)"],
+    "P61": ["(x,y,z)", "(-y,x-y,z+1/3)", "(-x+y,-x,z+2/3)", "(-x,-y,z+1/2)", "(y,-x+y,z+5/6)", "(x-y,x,z+1/6)"],
+    "I4": ["(x,y,z)", "(-x,-y,z)", "(-y,x,z)", "(y,-x,z)", "(x+1/2,y+1/2,z+1/2)", "(-x+1/2,-y+1/2,z+1/2)",
+           "(-y+1/2,x+1/2,z+1/2)", "(y+1/2,-x+1/2,z+1/2)"],
+    "R3": ["(x,y,z)", "(z,x,y)", "(y,z,x)"],
+}
+
+
+def molecular_crystal(group, cell, nwater, seed, min_dist=2.7, name=None):
+    """Asymmetric unit of ``nwater`` rigid waters in a unit cell with the given space group.
+
+    cell = (a, b, c, alpha, beta, gamma).  Oxygens are drawn at random
+    fractional positions and kept when at least ``min_dist`` from every
+    symmetry and lattice copy of the oxygens already kept (and of themselves).
+    Returns (MMSystem, (rotations (k,3,3), translations (k,3)), cell).
+    """
+    from .Symmetry import SymmetryParameters, Transformation3_FromSymmetryOperationString
+    rng = np.random.default_rng(seed)
+    ops = [Transformation3_FromSymmetryOperationString(s) for s in SPACE_GROUPS[group]]
+    rot = np.array([o.rotation for o in ops])
+    tra = np.array([o.translation for o in ops])
+    sp = SymmetryParameters(*cell)
+    M = sp.M
+    r2 = (-2, -1, 0, 1, 2)
+    shifts = np.array([[i, j, k] for i in r2 for j in r2 for k in r2], np.float64)
+
+    def copies(f):
+        fr = np.einsum("tij,j->ti", rot, f) + tra                     # (t, 3)
+        fr = fr[:, None, :] + shifts[None, :, :]                      # (t, 125, 3)
+        return fr.reshape(-1, 3) @ M.T
+
+    kept, kept_copies = [], np.zeros((0, 3))
+    tries = 0
+    while len(kept) < nwater and tries < 20000:
+        tries += 1
+        f = rng.uniform(0.0, 1.0, 3)
+        x0 = M @ f
+        own = copies(f)
+        own = own[np.linalg.norm(own - x0, axis=1) > 1.0e-9]
+        pool = np.concatenate([kept_copies, own]) if kept_copies.size else own
+        if pool.size and np.min(np.linalg.norm(pool - x0, axis=1)) < min_dist:
+            continue
+        kept.append(x0)
+        kept_copies = np.concatenate([kept_copies, copies(f)])
+    if len(kept) < nwater:
+        raise ValueError("could not place %d waters in the cell" % nwater)
+    xyz = _water_sites(rng, np.array(kept))
+    n, q, t, excl = _water_params(nwater)
+    a, b = opls_table([EPS_O_KCAL * KCAL_TO_KJ, 0.0], [SIGMA_O, 0.0])
+    sysm = MMSystem(name or ("crystal_%s_%d" % (group, nwater)), xyz, q, t, 2, a, b, excl.astype(np.int32))
+    return sysm, (rot, tra), tuple(float(v) for v in cell)
+
+
+def periodic_water_box(nwater, box, seed, jitter=0.3, name=None):
+    """Waters on a jittered lattice filling a cubic box, for P1 periodic images (cell = box)."""
+    s = water_lattice(nwater, box, seed, jitter, name or "water_box%d" % (3 * nwater))
+    rot = np.eye(3)[None]
+    tra = np.zeros((1, 3))
+    return s, (rot, tra), (float(box), float(box), float(box), 90.0, 90.0, 90.0)

@@ -1,6 +1,7 @@
 import os
 import sys
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -34,8 +35,13 @@ class DHFRFixture:
             self.published_terms = dict(zip(("MM/MM Elect.", "MM/MM LJ", "MM/MM 1-4 Elect.", "MM/MM 1-4 LJ"),
                                             (float(v) for v in z["published_terms"])))
             self.published_pairs, self.published_pairs14 = (int(v) for v in z["published_pairs"])
+            self.published_image_terms = dict(zip(("MM/MM Image Elect.", "MM/MM Image LJ"),
+                                                  (float(v) for v in z["published_image_terms"])))
+            self.published_images, self.published_image_pairs = (int(v) for v in z["published_images"])
+            self.cell = tuple(float(v) for v in z["cell"])
+            self.transformations = (np.eye(3)[None], np.zeros((1, 3)))
 
-    def system(self, **model_options):
+    def system(self, periodic=False, **model_options):
         from pdynamo_amd import MMAtomContainer, NBModelABFS, System
         from pdynamo_amd.Parameters import LJParameterContainer_FromTableCoefficients
         m = self.mm
@@ -44,6 +50,9 @@ class DHFRFixture:
         s = System.FromMMParameters(m.coordinates3, MMAtomContainer(m.charges, m.ljtypes), lj,
                                     exclusions=m.exclusions, interactions14=m.interactions14, ljParameters14=lj14,
                                     electrostaticScale14=self.scale14, label="DHFR")
+        if periodic:
+            from pdynamo_amd.Symmetry import CrystalClassCubic
+            s.DefineSymmetry(crystalClass=CrystalClassCubic(), a=self.cell[0])
         s.DefineNBModel(NBModelABFS(**model_options))
         return s
 

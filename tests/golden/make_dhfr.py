@@ -9,8 +9,10 @@ candidate) and the 1-4 pairs.
 
 The expected values are the reference's own first-call output for this
 system, printed identically in every benchmarks/log/systemBenchmarks_*.log
-(systemBenchmarks_Serial_1ps.log:392-393,400-403): MM/MM pairs 8,885,288,
-MM/MM 1-4 pairs 6,556, and the four non-image terms in kJ/mol.  They are
+(systemBenchmarks_Serial_1ps.log:392-393,400-404): MM/MM pairs 8,885,288,
+MM/MM 1-4 pairs 6,556, the four non-image terms in kJ/mol, and, with the
+cubic cell of benchmarks/data/dhfr/systemData.yaml (a = 62.23 A, P1), 13
+MM/MM images with 2,942,151 image pairs and the two image terms.  They are
 stored in the fixture as well.
 
 Usage (in a container holding the reference tree):
@@ -29,9 +31,12 @@ sys.path.insert(0, os.path.join(ROOT, "pdynamo-mirror_amd"))
 from pdynamo_amd import CHARMM  # noqa: E402
 
 # benchmarks/log/systemBenchmarks_Serial_1ps.log:392-393,400-403 (first energy call).
-PUBLISHED = {"pairs": 8885288, "pairs14": 6556,
+PUBLISHED = {"pairs": 8885288, "pairs14": 6556, "images": 13, "image_pairs": 2942151,
              "MM/MM Elect.": -388870.0641, "MM/MM LJ": 26802.2738,
-             "MM/MM 1-4 Elect.": 17884.0130, "MM/MM 1-4 LJ": 1421.1985}
+             "MM/MM 1-4 Elect.": 17884.0130, "MM/MM 1-4 LJ": 1421.1985,
+             "MM/MM Image Elect.": -62084.6864, "MM/MM Image LJ": 4280.7766}
+# benchmarks/data/dhfr/systemData.yaml: Crystal Class Cubic, a = 62.23 (identity transformation only).
+CELL = (62.23, 62.23, 62.23, 90.0, 90.0, 90.0)
 
 
 def main(reference="/root/reference"):
@@ -46,7 +51,11 @@ def main(reference="/root/reference"):
                         atomTypes=np.array(nb["atomTypes"]),
                         published_terms=np.array([PUBLISHED["MM/MM Elect."], PUBLISHED["MM/MM LJ"],
                                                   PUBLISHED["MM/MM 1-4 Elect."], PUBLISHED["MM/MM 1-4 LJ"]]),
-                        published_pairs=np.array([PUBLISHED["pairs"], PUBLISHED["pairs14"]]))
+                        published_pairs=np.array([PUBLISHED["pairs"], PUBLISHED["pairs14"]]),
+                        published_image_terms=np.array([PUBLISHED["MM/MM Image Elect."],
+                                                        PUBLISHED["MM/MM Image LJ"]]),
+                        published_images=np.array([PUBLISHED["images"], PUBLISHED["image_pairs"]]),
+                        cell=np.array(CELL))
     print("dhfr", a.size, "atoms", lj.ntypes, "types", len(nb["exclusions"]), "exclusions",
           len(nb["interactions14"]), "1-4 pairs")
 
