@@ -79,6 +79,11 @@ typedef struct {
     long long workItems;
     double    forceKernelMs;       /* mean duration of the pair kernel (timing on) */
     long long forceKernelLaunches;
+    int       images;              /* MM/MM image list: images with pairs */
+    int       reserved2;
+    long long imagePairs;          /* pairs of the image list */
+    double    numberOfMMMMImageImages;  /* summed over updates (StatisticsAccumulate) */
+    double    numberOfMMMMImagePairs;
 } nbabfs_stats;
 
 /* Fill *opt with the reference defaults. */
@@ -143,6 +148,42 @@ int nbabfs_virial(nbabfs_state *state, double *virial);
  * only its cut into work items.  Outputs may be NULL. */
 int nbabfs_autotune(nbabfs_state *state, const double *d_xyz, int *best_entries_per_item, int *best_waves_per_cu,
                     double *best_kernel_ms);
+
+/* ---- Crystal symmetry: the MM/MM image list (pMx/csource/NBModelABFS.c:
+ * 625-684 CheckForImageUpdate, 753-1045 GenerateImageLists, 1161-1290
+ * MMMMImageEnergy). */
+
+/* The transformations of NBModelABFSState_SetUp (pMx/cinclude/NBModelABFSState.h:
+ * 145-156, argument Transformation3Container): ntransformations fractional
+ * rotations (9 each, row-major) and translations (3 each); their identity and
+ * rotational inverses are found as Transformation3Container_FindIdentity /
+ * _FindInverses do (pCx/csource/Transformation3Container.c:52-141).
+ * checkForInverses and imageExpandFactor are the NBModelABFS options
+ * (pMx/cinclude/NBModelABFS.h:22-33).  Call before the first update; the image
+ * terms are evaluated by the NBABFS_MIXED path. */
+int nbabfs_set_transformations(nbabfs_state *state, int ntransformations, const double *rotations,
+                               const double *translations, int checkForInverses, int imageExpandFactor);
+
+/* The symmetry parameters of NBModelABFSState_Initialize (pMx/cinclude/
+ * NBModelABFSState.h:138-142): crystal = a, b, c (A), alpha, beta, gamma
+ * (degrees), from which M is formed as SymmetryParameters_MakeM does.  Applies
+ * to the following updates and energy calls. */
+int nbabfs_set_symmetry_parameters(nbabfs_state *state, const double *crystal);
+
+/* The last energy call's image terms: energies[2] = MM/MM Image Elect. and LJ,
+ * *present = the image list is non-empty, and dE/dM (row-major 3x3,
+ * INCREMENTED, may be NULL) as MMMMImageEnergy adds it into
+ * SymmetryParameterGradients.dEdM.  Synchronises the state's stream. */
+int nbabfs_image_terms(nbabfs_state *state, double *energies, double *dEdM, int *present);
+
+/* The image list (ImageList of inbmmmm, pMx/cinclude/ImageList.h): per image
+ * abct[4k..] = a, b, c, transformation index, scale[k], npairs[k] (outputs may
+ * be NULL, at most cap written).  Returns the number of images. */
+int nbabfs_images(nbabfs_state *state, int cap, int *abct, double *scale, long long *npairs);
+
+/* Pairs of image k as sorted keys i*n + j (i in the primary cell, j in the
+ * image).  Returns the pair count (keys may be NULL to query it). */
+long long nbabfs_export_image_pairs(nbabfs_state *state, int image, long long *keys, long long cap);
 
 /* Statistics (NBModelABFSState.pyx:68-97) and kernel timing with HIP events on
  * the launch stream (enable != 0 turns it on). */

@@ -1,9 +1,12 @@
 // nbabfs_build.hip -- pair-list construction on gfx950.
 //
 // Replaces PairListGenerator_SelfPairListFromCoordinates3 (CellCell and
-// Direct, pCore-1.9.0/extensions/csource/PairListGenerator.c:946-1150) and
+// Direct, pCore-1.9.0/extensions/csource/PairListGenerator.c:946-1150),
 // RegularGridOccupancy_Fill (pCore-1.9.0/extensions/csource/
-// RegularGridOccupancy.c:77-147).  The pair SET is the reference's exactly:
+// RegularGridOccupancy.c:77-147) and, for the periodic image lists,
+// PairListGenerator_CrossPairListFromDoubleCoordinates3 (PairListGenerator.c:
+// 414-450,611-860) as GenerateImageLists calls it (pMolecule-1.9.0/extensions/
+// csource/NBModelABFS.c:753-1045).  The pair SET is the reference's exactly:
 // every candidate that survives an FP32 bounding-box cull is re-tested with
 // the reference predicate in FP64 (same cell grid, same stencil, same
 // x,y,z-ordered sum of squares, no FMA contraction).  Only the storage
@@ -275,6 +278,7 @@ struct JAtom {
     int atom, free_;
 };
 
+template <bool CROSS>
 __device__ __forceinline__ JAtom load_jatom(int J, int b, const double *__restrict__ xyz,
                                             const int4 *__restrict__ cellv, const int *__restrict__ slot_atom,
                                             const unsigned char *__restrict__ free_flags)
@@ -288,12 +292,21 @@ __device__ __forceinline__ JAtom load_jatom(int J, int b, const double *__restri
         r.x = xyz[3 * (size_t)r.atom];
         r.y = xyz[3 * (size_t)r.atom + 1];
         r.z = xyz[3 * (size_t)r.atom + 2];
-        r.cell = cellv[r.atom];
+        if (!CROSS) r.cell = cellv[r.atom];
         if (free_flags) r.free_ = free_flags[r.atom];
     }
     return r;
 }
 
+//
+// CROSS (image lists, blockIdx.y = image): the j-clusters are the image's
+// copies of all clusters, scanned as one flat range after a test of the
+// super-cluster against the image's bounding box; the predicate is the cross
+// list's (pCore-1.9.0/extensions/csource/PairListGenerator.c:28-85,703-794):
+// no exclusions, every (i, j) pair including an atom with its own image, the
+// OR selection, i on the reference grid (grid method), and the FP64 distance
+// test against the image coordinates GenerateImageLists produced.
+template <bool CROSS>
 __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, const double *__restrict__ xyz,
                                                               const int4 *__restrict__ cellv,
                                                               const int *__restrict__ slot_atom,
@@ -304,6 +317,13 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
                                                               int *__restrict__ counts, int *__restrict__ overflow)
 {
     const int sc = blockIdx.x;
+    const int img = CROSS ? (int)blockIdx.y : -1;
+    const int blk = CROSS ? img * (int)gridDim.x + sc : sc;        // scratch / count slot of this block
+    const double *__restrict__ jxyz = CROSS ? P.ixyz + (size_t)img * 3 * P.n : xyz;
+    const float4 *__restrict__ jblo = CROSS ? P.iblo + (size_t)img * P.ncl : blo;
+    const float4 *__restrict__ jbhi = CROSS ? P.ibhi + (size_t)img * P.ncl : bhi;
+    const size_t sbase = CROSS ? (size_t)P.soff[blk] : (size_t)blk * P.cap;
+    const int scap = CROSS ? P.scap[blk] : P.cap;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
     const int a = lane >> 3, b = lane & 7;
     const int I0 = sc * NB_SCC;
@@ -331,10 +351,13 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
             sIx[lane] = xyz[3 * (size_t)ia];
             sIy[lane] = xyz[3 * (size_t)ia + 1];
             sIz[lane] = xyz[3 * (size_t)ia + 2];
-            sIcell[lane] = cellv[ia];
+            if (CROSS) sIcell[lane] = make_int4(0, 0, 0, P.ongrid ? (int)P.ongrid[ia] : 1);
+            else sIcell[lane] = cellv[ia];
             sIfree[lane] = P.free_flags ? P.free_flags[ia] : 1;
-            xs = P.xptr[ia];
-            xc = P.xptr[ia + 1] - xs;
+            if (!CROSS) {
+                xs = P.xptr[ia];
+                xc = P.xptr[ia + 1] - xs;
+            }
         }
         int pre = xc;   // inclusive scan of exclusion counts over the wave
 #pragma unroll
@@ -355,7 +378,7 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
         sXhi[lane] = xc > 0 ? P.xidx[xs + xc - 1] : -1;
     }
     const int wst = 2 * P.u + 1;
-    const int nst = P.use_grid ? wst * wst * wst : 0;
+    const int nst = (P.use_grid && !CROSS) ? wst * wst * wst : 0;
     const bool sten_lds = nst <= NB_STENCIL_LDS;
     if (sten_lds)
         for (int k = tid; k < nst; k += NB_SEARCH_THREADS) sSten[k] = P.stencil[k];
@@ -371,34 +394,45 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
     if (lo.x <= hi.x) {  // super-cluster holds at least one atom (uniform over the block)
         const float reach = P.reach;
         const float reach2 = reach * reach;
-        int cx0 = (int)floorf((lo.x - reach) / P.cw), cx1 = (int)floorf((hi.x + reach) / P.cw);
-        int cy0 = (int)floorf((lo.y - reach) / P.cw), cy1 = (int)floorf((hi.y + reach) / P.cw);
-        cx0 = max(cx0, 0); cy0 = max(cy0, 0);
-        cx1 = min(cx1, P.ncx - 1); cy1 = min(cy1, P.ncy - 1);
+        int cx0 = 0, cy0 = 0, nyc = 1, ncol = 0;
         const int Jmax = I0 + NB_SCC - 1;
-        const int nyc = cy1 - cy0 + 1;
-        const int ncol = (cx1 >= cx0 && cy1 >= cy0) ? (cx1 - cx0 + 1) * nyc : 0;
+        if (!CROSS) {
+            cx0 = (int)floorf((lo.x - reach) / P.cw);
+            cy0 = (int)floorf((lo.y - reach) / P.cw);
+            int cx1 = (int)floorf((hi.x + reach) / P.cw), cy1 = (int)floorf((hi.y + reach) / P.cw);
+            cx0 = max(cx0, 0); cy0 = max(cy0, 0);
+            cx1 = min(cx1, P.ncx - 1); cy1 = min(cy1, P.ncy - 1);
+            nyc = cy1 - cy0 + 1;
+            ncol = (cx1 >= cx0 && cy1 >= cy0) ? (cx1 - cx0 + 1) * nyc : 0;
+        } else {
+            // One flat range over the image's clusters, if its box is in reach.
+            const float4 ilo = make_float4(P.ibox[6 * img], P.ibox[6 * img + 1], P.ibox[6 * img + 2], 0.0f);
+            const float4 ihi = make_float4(P.ibox[6 * img + 3], P.ibox[6 * img + 4], P.ibox[6 * img + 5], 0.0f);
+            ncol = box_gap2(lo, hi, ilo, ihi) < reach2 ? 1 : 0;
+        }
 
         // Evaluate the listed candidates sCand[0..npend): this wave takes
         // k = w, w + NB_SEARCH_WAVES, ...
         auto evaluate = [&](int npend) {
             JAtom jn;
-            if (w < npend) jn = load_jatom(sCand[w], b, xyz, cellv, slot_atom, P.free_flags);
+            if (w < npend) jn = load_jatom<CROSS>(sCand[w], b, jxyz, cellv, slot_atom, P.free_flags);
             for (int k = w; k < npend; k += NB_SEARCH_WAVES) {
                 const JAtom jd = jn;
                 const int Jc = sCand[k];
                 const int cmc = sCm[k];
                 if (k + NB_SEARCH_WAVES < npend)
-                    jn = load_jatom(sCand[k + NB_SEARCH_WAVES], b, xyz, cellv, slot_atom, P.free_flags);
+                    jn = load_jatom<CROSS>(sCand[k + NB_SEARCH_WAVES], b, jxyz, cellv, slot_atom, P.free_flags);
                 int lbits = 0, cbits = 0;
                 for (int c = 0; c < NB_SCC; c++) {
                     if (!(cmc & (1 << c))) continue;
                     const int li = c * NB_CL + a;
                     const int i = sIatom[li];
-                    bool ok = (i >= 0) && (jd.atom >= 0) && ((Jc < I0 + c) || (b < a));
+                    bool ok = (i >= 0) && (jd.atom >= 0) && (CROSS || (Jc < I0 + c) || (b < a));
                     if (ok && P.free_flags) ok = (sIfree[li] | jd.free_) != 0;
                     int flag = 1;
-                    if (ok && P.use_grid) {
+                    if (CROSS) {
+                        if (ok && P.use_grid) ok = sIcell[li].w != 0;
+                    } else if (ok && P.use_grid) {
                         const int4 ci = sIcell[li];
                         ok = (ci.w & jd.cell.w) != 0;
                         const int dx = jd.cell.x - ci.x, dy = jd.cell.y - ci.y, dz = jd.cell.z - ci.z;
@@ -415,7 +449,7 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
                         const double r2 = ((xij * xij) + (yij * yij)) + (zij * zij);
                         ok = r2 <= P.cutoff2;
                     }
-                    if (ok && jd.atom >= sXlo[li] && jd.atom <= sXhi[li]) {
+                    if (!CROSS && ok && jd.atom >= sXlo[li] && jd.atom <= sXhi[li]) {
                         const int xo = sXoff[li], xn = sXcnt[li];
                         if (xo >= 0) {
                             for (int x = 0; x < xn; x++)
@@ -429,14 +463,14 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
                     if (__ballot(ok)) cbits |= 1 << c;
                 }
                 const int slot = base + k;
-                if (slot < P.cap) {
-                    Entry *en = &scratch[(size_t)sc * P.cap + slot];
+                if (slot < scap) {
+                    Entry *en = &scratch[sbase + slot];
                     en->bits[lane] = (unsigned char)lbits;
                     if (lane == 0) {
                         en->J = Jc;
                         en->sc = sc;
                         en->cbits = cbits;
-                        en->pad = 0;
+                        en->img = img;
                     }
                 }
                 if (cbits) nvalid++;
@@ -447,7 +481,7 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
         for (int c0 = 0; c0 < ncol; c0 += NB_MAXCOL) {
             const int nc = min(NB_MAXCOL, ncol - c0);
             // Column ranges [start, min(end, Jmax+1)) and their prefix sums (wave 0).
-            if (w == 0) {
+            if (!CROSS && w == 0) {
                 int carry = 0;
                 for (int cb0 = 0; cb0 < nc; cb0 += 64) {
                     const int k = cb0 + lane;
@@ -470,21 +504,25 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
                 if (lane == 0) sColPre[nc] = carry;
             }
             __syncthreads();
-            const int total = sColPre[nc];
+            const int total = CROSS ? P.ncl : sColPre[nc];
             for (int f0 = 0; f0 < total; f0 += NB_SEARCH_THREADS) {
                 const int f = f0 + tid;
                 int cm = 0, J = 0;
                 if (f < total) {
-                    int l = 0, h = nc;   // last column with sColPre[col] <= f
-                    while (h - l > 1) {
-                        const int m = (l + h) >> 1;
-                        if (sColPre[m] <= f) l = m; else h = m;
+                    if (CROSS) {
+                        J = f;
+                    } else {
+                        int l = 0, h = nc;   // last column with sColPre[col] <= f
+                        while (h - l > 1) {
+                            const int m = (l + h) >> 1;
+                            if (sColPre[m] <= f) l = m; else h = m;
+                        }
+                        J = sColStart[l] + (f - sColPre[l]);
                     }
-                    J = sColStart[l] + (f - sColPre[l]);
-                    const float4 jl = blo[J], jh = bhi[J];
+                    const float4 jl = jblo[J], jh = jbhi[J];
                     if (box_gap2(lo, hi, jl, jh) < reach2) {
                         for (int c = 0; c < NB_SCC; c++)
-                            if (J <= I0 + c && box_gap2(sIlo[c], sIhi[c], jl, jh) < reach2) cm |= 1 << c;
+                            if ((CROSS || J <= I0 + c) && box_gap2(sIlo[c], sIhi[c], jl, jh) < reach2) cm |= 1 << c;
                     }
                 }
                 const unsigned long long m = __ballot(cm != 0);
@@ -520,31 +558,34 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
     if (tid == 0) {
         int tot = 0;
         for (int v = 0; v < NB_SEARCH_WAVES; v++) tot += sWcnt[v];
-        counts[sc] = tot;
-        ncand[sc] = base;
-        if (base > P.cap) atomicMax(overflow, base);
+        counts[blk] = tot;
+        ncand[blk] = base;
+        if (base > scap) atomicMax(overflow, base);
     }
 }
 
 // Compact per-super-cluster entries (candidate slots with any listed pair, in
-// candidate order), count their pairs into *npairs, and cut them into work
-// items.
-__global__ __launch_bounds__(64) void k_compact(const Entry *__restrict__ scratch, int cap, const int *__restrict__ ncand,
-                                                const int *__restrict__ counts, const int *__restrict__ eoff,
-                                                const int *__restrict__ ioff, int per_item, int sys,
-                                                Entry *__restrict__ entries, WorkItem *__restrict__ items,
-                                                unsigned long long *__restrict__ npairs)
+// candidate order), count their pairs into *npairs (or per block), and cut
+// them into work items.  Block b compacts search block b, whose
+// super-cluster is b % nsc.
+__global__ __launch_bounds__(64) void k_compact(const Entry *__restrict__ scratch, int cap, const int *__restrict__ soff,
+                                                const int *__restrict__ ncand, const int *__restrict__ counts,
+                                                const int *__restrict__ eoff, const int *__restrict__ ioff, int nsc,
+                                                int per_item, int sys, Entry *__restrict__ entries,
+                                                WorkItem *__restrict__ items, unsigned long long *__restrict__ npairs,
+                                                unsigned long long *__restrict__ npairs_block)
 {
     const int sc = blockIdx.x, lane = threadIdx.x;
-    const int nc = min(ncand[sc], cap), cnt = counts[sc], e0 = eoff[sc], i0 = ioff[sc];
+    const int nc = soff ? ncand[sc] : min(ncand[sc], cap), cnt = counts[sc], e0 = eoff[sc], i0 = ioff[sc];
+    const Entry *__restrict__ src = scratch + (soff ? (size_t)soff[sc] : (size_t)sc * cap);
     int pos = 0;
     unsigned long long pc = 0;
     for (int k0 = 0; k0 < nc; k0 += 64) {
         const int k = k0 + lane;
-        const bool valid = k < nc && scratch[(size_t)sc * cap + k].cbits != 0;
+        const bool valid = k < nc && src[k].cbits != 0;
         const unsigned long long m = __ballot(valid);
         if (valid) {
-            const Entry en = scratch[(size_t)sc * cap + k];
+            const Entry en = src[k];
             entries[e0 + pos + __popcll(m & ((1ull << lane) - 1ull))] = en;
             const unsigned long long *wds = (const unsigned long long *)en.bits;
             for (int q = 0; q < NB_SC / 8; q++) pc += __popcll(wds[q]);
@@ -552,12 +593,16 @@ __global__ __launch_bounds__(64) void k_compact(const Entry *__restrict__ scratc
         pos += __popcll(m);
     }
     for (int o = 32; o >= 1; o >>= 1) pc += __shfl_xor(pc, o, 64);
-    if (lane == 0 && pc) atomicAdd(npairs, pc);
+    if (npairs_block) {
+        if (lane == 0) npairs_block[sc] = pc;
+    } else if (lane == 0 && pc) {
+        atomicAdd(npairs, pc);
+    }
     const int nit = (cnt + per_item - 1) / per_item;
     for (int k = lane; k < nit; k += 64) {
         WorkItem wi;
         wi.sys = sys;
-        wi.sc = sc;
+        wi.sc = sc % nsc;
         wi.first = e0 + k * per_item;
         wi.count = min(per_item, cnt - k * per_item);
         items[i0 + k] = wi;
@@ -603,16 +648,16 @@ __global__ __launch_bounds__(256) void k_item_scatter(const WorkItem *__restrict
 
 // Work items of an already compacted list for another entries-per-item cut.
 __global__ __launch_bounds__(256) void k_cut_items(const int *__restrict__ counts, const int *__restrict__ eoff,
-                                                   const int *__restrict__ ioff, int nsc, int per_item, int sys,
-                                                   WorkItem *__restrict__ items)
+                                                   const int *__restrict__ ioff, int nblocks, int nsc, int per_item,
+                                                   int sys, WorkItem *__restrict__ items)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
-    if (s >= nsc) return;
+    if (s >= nblocks) return;
     const int cnt = counts[s], e0 = eoff[s], i0 = ioff[s];
     for (int k = 0; k * per_item < cnt; k++) {
         WorkItem wi;
         wi.sys = sys;
-        wi.sc = s;
+        wi.sc = s % nsc;
         wi.first = e0 + k * per_item;
         wi.count = min(per_item, cnt - k * per_item);
         items[i0 + k] = wi;
@@ -645,6 +690,131 @@ __global__ __launch_bounds__(64) void k_export_pairs(const Entry *__restrict__ e
         unsigned long long pos = atomicAdd(cursor, 1ull);
         if ((long long)pos < cap) keys[pos] = hi * n + lo;
     }
+}
+
+// Candidate bound of each (image, super-cluster) block of the cross search:
+// the image clusters whose box is within reach of the super-cluster's box
+// (the first cull of k_search), zero when the image's box is out of reach.
+__global__ __launch_bounds__(256) void k_image_bounds(SearchParams P, const float4 *__restrict__ blo,
+                                                      const float4 *__restrict__ bhi, int *__restrict__ bound)
+{
+    __shared__ int sCnt[4];
+    const int sc = blockIdx.x, img = blockIdx.y, I0 = sc * NB_SCC;
+    float4 lo = blo[I0], hi = bhi[I0];
+    for (int c = 1; c < NB_SCC; c++) {
+        const float4 l = blo[I0 + c], h = bhi[I0 + c];
+        lo.x = fminf(lo.x, l.x); lo.y = fminf(lo.y, l.y); lo.z = fminf(lo.z, l.z);
+        hi.x = fmaxf(hi.x, h.x); hi.y = fmaxf(hi.y, h.y); hi.z = fmaxf(hi.z, h.z);
+    }
+    const float reach2 = P.reach * P.reach;
+    const float4 ilo = make_float4(P.ibox[6 * img], P.ibox[6 * img + 1], P.ibox[6 * img + 2], 0.0f);
+    const float4 ihi = make_float4(P.ibox[6 * img + 3], P.ibox[6 * img + 4], P.ibox[6 * img + 5], 0.0f);
+    int cnt = 0;
+    if (lo.x <= hi.x && box_gap2(lo, hi, ilo, ihi) < reach2) {
+        const float4 *__restrict__ jl = P.iblo + (size_t)img * P.ncl;
+        const float4 *__restrict__ jh = P.ibhi + (size_t)img * P.ncl;
+        for (int J = threadIdx.x; J < P.ncl; J += 256)
+            if (box_gap2(lo, hi, jl[J], jh[J]) < reach2) cnt++;
+    }
+    for (int o = 32; o >= 1; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if ((threadIdx.x & 63) == 0) sCnt[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) bound[img * gridDim.x + sc] = sCnt[0] + sCnt[1] + sCnt[2] + sCnt[3];
+}
+
+__global__ __launch_bounds__(256) void k_add_int(int *__restrict__ p, int m, int v)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < m) p[i] += v;
+}
+
+// Pairs (i, j) of one image as keys i*n + j (i in the primary cell, j in the image).
+__global__ __launch_bounds__(64) void k_export_image_pairs(const Entry *__restrict__ entries, int nent,
+                                                           const int *__restrict__ slot_atom, long long n, int img,
+                                                           unsigned long long *__restrict__ cursor, long long cap,
+                                                           long long *__restrict__ keys)
+{
+    int e = blockIdx.x;
+    if (e >= nent) return;
+    const Entry &en = entries[e];
+    if (en.img != img) return;
+    int lane = threadIdx.x, a = lane >> 3, b = lane & 7;
+    int j = slot_atom[en.J * NB_CL + b];
+    const int lb = en.bits[lane];
+    for (int c = 0; c < NB_SCC; c++) {
+        if (!((lb >> c) & 1)) continue;
+        int i = slot_atom[(en.sc * NB_SCC + c) * NB_CL + a];
+        unsigned long long pos = atomicAdd(cursor, 1ull);
+        if ((long long)pos < cap) keys[pos] = (long long)i * n + j;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Image coordinates (GenerateImageLists, pMolecule-1.9.0/extensions/csource/
+// NBModelABFS.c:841-1000), in the reference's operation order so that the
+// image lists are searched with the reference's coordinates bit for bit:
+// Coordinates3_Rotate + _Translate with the orthogonalized transformation
+// (pCore-1.9.0/extensions/csource/Coordinates3.c:1473-1500,1760-1800), then per
+// (a, b, c) of the transformation's sequence a translation by +d and one by -d.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_image_transform(const double *__restrict__ xyz, int n,
+                                                         const double *__restrict__ rt, double *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 256 + threadIdx.x, t = blockIdx.y;
+    if (i >= n) return;
+    const double *R = rt + 12 * t, *T = R + 9;
+    const double x0 = xyz[3 * (size_t)i], y0 = xyz[3 * (size_t)i + 1], z0 = xyz[3 * (size_t)i + 2];
+    const double x1 = R[0] * x0 + R[1] * y0 + R[2] * z0;
+    const double y1 = R[3] * x0 + R[4] * y0 + R[5] * z0;
+    const double z1 = R[6] * x0 + R[7] * y0 + R[8] * z0;
+    double *o = out + ((size_t)t * n + i) * 3;
+    o[0] = x1 + T[0];
+    o[1] = y1 + T[1];
+    o[2] = z1 + T[2];
+}
+
+__global__ __launch_bounds__(256) void k_image_coords(const double *__restrict__ tcoords, int n,
+                                                      const int *__restrict__ step_off,
+                                                      const ImageStep *__restrict__ steps, double *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 256 + threadIdx.x, t = blockIdx.y;
+    if (i >= n) return;
+    const double *c = tcoords + ((size_t)t * n + i) * 3;
+    double x = c[0], y = c[1], z = c[2];
+    for (int k = step_off[t]; k < step_off[t + 1]; k++) {
+        const ImageStep q = steps[k];
+        x += q.d[0];
+        y += q.d[1];
+        z += q.d[2];
+        if (q.out >= 0) {
+            double *o = out + ((size_t)q.out * n + i) * 3;
+            o[0] = x;
+            o[1] = y;
+            o[2] = z;
+        }
+        x += -q.d[0];
+        y += -q.d[1];
+        z += -q.d[2];
+    }
+}
+
+// Whether each atom lies on the reference grid built from the current
+// coordinates (the cross list's i loop runs over grid cells only).
+__global__ __launch_bounds__(256) void k_ongrid(const double *__restrict__ xyz, int n, BuildGrid g,
+                                                unsigned char *__restrict__ flags)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    int on = 1;
+    for (int d = 0; d < 3; d++) {
+        const double fr = (xyz[3 * (size_t)i + d] - g.lower[d]) / g.h;
+        const int k = (int)floor(fr);
+        if (k < 0 || k >= g.bins[d]) on = 0;
+    }
+    flags[i] = (unsigned char)on;
 }
 
 // Full (both directions) CSR neighbour rows for the plain FP64 path.
@@ -808,16 +978,52 @@ void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4
                    int *overflow, hipStream_t s)
 {
     if (nsc > 0)
-        hipLaunchKernelGGL(k_search, dim3(nsc), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, cellv, slot_atom, blo, bhi,
-                           colcl, scratch, ncand, counts, overflow);
+        hipLaunchKernelGGL(k_search<false>, dim3(nsc), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, cellv, slot_atom, blo,
+                           bhi, colcl, scratch, ncand, counts, overflow);
 }
 
-void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int nsc, int per_item, int sys,
-                      WorkItem *items, hipStream_t s)
+void launch_search_images(const SearchParams &P, int nsc, int nimg, const double *xyz, const int4 *cellv,
+                          const int *slot_atom, const float4 *blo, const float4 *bhi, Entry *scratch, int *ncand,
+                          int *counts, int *overflow, hipStream_t s)
 {
-    if (nsc > 0)
-        hipLaunchKernelGGL(k_cut_items, dim3(nblk(nsc, 256)), dim3(256), 0, s, counts, eoff, ioff, nsc, per_item, sys,
-                           items);
+    if (nsc > 0 && nimg > 0)
+        hipLaunchKernelGGL(k_search<true>, dim3(nsc, nimg), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, cellv,
+                           slot_atom, blo, bhi, (const int2 *)nullptr, scratch, ncand, counts, overflow);
+}
+
+void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int nblocks, int nsc, int per_item,
+                      int sys, WorkItem *items, hipStream_t s)
+{
+    if (nblocks > 0)
+        hipLaunchKernelGGL(k_cut_items, dim3(nblk(nblocks, 256)), dim3(256), 0, s, counts, eoff, ioff, nblocks, nsc,
+                           per_item, sys, items);
+}
+
+void launch_image_transform(const double *xyz, int n, const double *rt, int ntrans, double *out, hipStream_t s)
+{
+    if (n > 0 && ntrans > 0)
+        hipLaunchKernelGGL(k_image_transform, dim3(nblk(n, 256), ntrans), dim3(256), 0, s, xyz, n, rt, out);
+}
+
+void launch_image_coords(const double *tcoords, int n, int ntrans, const int *step_off, const ImageStep *steps,
+                         double *out, hipStream_t s)
+{
+    if (n > 0 && ntrans > 0)
+        hipLaunchKernelGGL(k_image_coords, dim3(nblk(n, 256), ntrans), dim3(256), 0, s, tcoords, n, step_off, steps,
+                           out);
+}
+
+void launch_ongrid(const double *xyz, int n, const BuildGrid &g, unsigned char *flags, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_ongrid, dim3(nblk(n, 256)), dim3(256), 0, s, xyz, n, g, flags);
+}
+
+void launch_export_image_pairs(const Entry *entries, int nent, const int *slot_atom, long long n, int img,
+                               unsigned long long *cursor, long long cap, long long *keys, hipStream_t s)
+{
+    if (nent > 0)
+        hipLaunchKernelGGL(k_export_image_pairs, dim3(nent), dim3(64), 0, s, entries, nent, slot_atom, n, img, cursor,
+                           cap, keys);
 }
 
 void launch_order_items(const WorkItem *items, int nitems, const Entry *entries, int maxcost, int interleave, int *cost,
@@ -837,13 +1043,25 @@ void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, 
     hipLaunchKernelGGL(k_items_per_sc, dim3(nblk(nsc, 256)), dim3(256), 0, s, counts, nsc, per_item, nitems);
 }
 
-void launch_compact(const Entry *scratch, int cap, const int *ncand, const int *counts, const int *eoff,
-                    const int *ioff, int nsc, int per_item, int sys, Entry *entries, WorkItem *items,
-                    unsigned long long *npairs, hipStream_t s)
+void launch_compact(const Entry *scratch, int cap, const int *soff, const int *ncand, const int *counts,
+                    const int *eoff, const int *ioff, int nblocks, int nsc, int per_item, int sys, Entry *entries,
+                    WorkItem *items, unsigned long long *npairs, unsigned long long *npairs_block, hipStream_t s)
 {
-    if (nsc > 0)
-        hipLaunchKernelGGL(k_compact, dim3(nsc), dim3(64), 0, s, scratch, cap, ncand, counts, eoff, ioff, per_item,
-                           sys, entries, items, npairs);
+    if (nblocks > 0)
+        hipLaunchKernelGGL(k_compact, dim3(nblocks), dim3(64), 0, s, scratch, cap, soff, ncand, counts, eoff, ioff,
+                           nsc, per_item, sys, entries, items, npairs, npairs_block);
+}
+
+void launch_image_bounds(const SearchParams &P, int nsc, int nimg, const float4 *blo, const float4 *bhi, int *bound,
+                         hipStream_t s)
+{
+    if (nsc > 0 && nimg > 0)
+        hipLaunchKernelGGL(k_image_bounds, dim3(nsc, nimg), dim3(256), 0, s, P, blo, bhi, bound);
+}
+
+void launch_add_int(int *p, int m, int v, hipStream_t s)
+{
+    if (m > 0 && v != 0) hipLaunchKernelGGL(k_add_int, dim3(nblk(m, 256)), dim3(256), 0, s, p, m, v);
 }
 
 void launch_export_pairs(const Entry *entries, int nent, const int *slot_atom, long long n,

@@ -12,6 +12,12 @@
 // lanes that share b and added once per j-cluster.  All gradient sums land in
 // 64-bit fixed point, so the result does not depend on the order in which
 // items finish.
+//
+// Periodic images (MMMMImageEnergy, pMolecule-1.9.0/extensions/csource/
+// NBModelABFS.c:1161-1290): an image item's j-clusters are transformed to the
+// image on the fly (x' = R x + T, FP64), its energies and forces carry the
+// image scale, its j-gradients are rotated back by R^T before they are added,
+// and it sums the image-side gradients and their moments for dE/dM.
 #include "nbabfs_internal.h"
 #include "nbabfs_kernels.h"
 
@@ -123,6 +129,8 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
         for (int k = threadIdx.x; k < NB_EACC_WORDS; k += 256) D.eacc[k] = 0;
     if (blockIdx.x == 0 && D.vacc)
         for (int k = threadIdx.x; k < NB_VACC_WORDS; k += 256) D.vacc[k] = 0;
+    if (blockIdx.x == 0 && D.gacc)
+        for (int k = threadIdx.x; k < NB_GACC_WORDS_PER_IMAGE * D.nimg; k += 256) D.gacc[k] = 0;
     bool up = false;
     if (s < D.n_pad) {
         for (int k = 0; k < 6; k++) D.facc[(size_t)k * D.n_pad + s] = 0;
@@ -174,6 +182,14 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
         *(volatile int *)chk.flag = chk.epoch;
         *(volatile int *)chk.dflag = chk.epoch;
     }
+}
+
+// Cluster-frame image position x' = R x + T of a slot record (charge kept).
+__device__ __forceinline__ double4 image_slot(double4 v, const double *R, const double *T)
+{
+    return make_double4(fma(R[0], v.x, fma(R[1], v.y, fma(R[2], v.z, T[0]))),
+                        fma(R[3], v.x, fma(R[4], v.y, fma(R[5], v.z, T[1]))),
+                        fma(R[6], v.x, fma(R[7], v.y, fma(R[8], v.z, T[2]))), v.w);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -294,6 +310,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const int I0 = w.sc * NB_SCC;
         float *sJF = (float *)(lds + lds_head);           // [entry][3 * NB_CL]
         int *sJ = (int *)(sJF + 3 * NB_CL * w.count);      // [entry]
+        // The item's image (-1: main list); all of its entries share it.
+        const int img = __builtin_amdgcn_readfirstlane(((const GAS int *)(eb + (size_t)w.first * sizeof(Entry)))[3]);
+        double R[9], T[3], isc = 1.0;
+        if (img >= 0) {
+            const ImageXf *X = &D->imgxf[img];
+#pragma unroll
+            for (int k = 0; k < 9; k++) R[k] = X->R[k];
+#pragma unroll
+            for (int k = 0; k < 3; k++) T[k] = X->T[k];
+            isc = X->scale;
+        }
 
         __syncthreads();
         if (ROWS_LDS)
@@ -317,10 +344,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         int lb1 = entry_bits(eb, min(w.first + 1, e_last), bl);
         int J = __builtin_amdgcn_readfirstlane(jc0.x);
         double4 xj = load_slot(xq, J * NB_CL + b);
+        if (img >= 0) xj = image_slot(xj, R, T);
         int tj = 64 * ty[J * NB_CL + b];
         for (int e = w.first; e <= e_last; e++) {
             const int Jn = __builtin_amdgcn_readfirstlane(jc1.x);
-            const double4 xjn = load_slot(xq, Jn * NB_CL + b);
+            double4 xjn = load_slot(xq, Jn * NB_CL + b);
+            if (img >= 0) xjn = image_slot(xjn, R, T);
             const int tjn = 64 * ty[Jn * NB_CL + b];
             const int2 jc2 = entry_jc(eb, min(e + 2, e_last));
             const int lb2 = entry_bits(eb, min(e + 2, e_last), bl);
@@ -385,17 +414,49 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         const int si = (I0 + b) * NB_CL + a;
 #pragma unroll
-        for (int k = 0; k < 3; k++) fx_atomic_add(facc, n_pad, si, k, (double)ig[k]);
-        // j-forces of the whole item.
-        for (int k = lane; k < 3 * NB_CL * w.count; k += 64) {
-            const int el = k / (3 * NB_CL), r = k - el * 3 * NB_CL;
-            fx_atomic_add(facc, n_pad, sJ[el] * NB_CL + r / 3, r % 3, (double)sJF[k]);
+        for (int k = 0; k < 3; k++) fx_atomic_add(facc, n_pad, si, k, isc * (double)ig[k]);
+        if (img < 0) {
+            // j-forces of the whole item.
+            for (int k = lane; k < 3 * NB_CL * w.count; k += 64) {
+                const int el = k / (3 * NB_CL), r = k - el * 3 * NB_CL;
+                fx_atomic_add(facc, n_pad, sJ[el] * NB_CL + r / 3, r % 3, (double)sJF[k]);
+            }
+        } else {
+            // Image side: g'_j (scaled) is rotated back, g_j += R^T g'_j, and the
+            // sums h = sum g'_j and G_ab = sum g'_j,a (x_j - origin)_b are kept
+            // for the lattice derivatives (SymmetryParameterGradients_ImageDerivatives,
+            // pMolecule-1.9.0/extensions/csource/SymmetryParameterGradients.c:142-216).
+            double G[12];
+#pragma unroll
+            for (int k = 0; k < 12; k++) G[k] = 0.0;
+            for (int k = lane; k < NB_CL * w.count; k += 64) {
+                const int el = k >> 3, jb = k & 7;
+                const int sj = sJ[el] * NB_CL + jb;
+                const float *f = &sJF[el * 3 * NB_CL + 3 * jb];
+                const double gx = isc * (double)f[0], gy = isc * (double)f[1], gz = isc * (double)f[2];
+                fx_atomic_add(facc, n_pad, sj, 0, fma(R[0], gx, fma(R[3], gy, R[6] * gz)));
+                fx_atomic_add(facc, n_pad, sj, 1, fma(R[1], gx, fma(R[4], gy, R[7] * gz)));
+                fx_atomic_add(facc, n_pad, sj, 2, fma(R[2], gx, fma(R[5], gy, R[8] * gz)));
+                const double4 xr = load_slot(xq, sj);
+                G[0] += gx; G[1] += gy; G[2] += gz;
+                G[3] = fma(gx, xr.x, G[3]); G[4] = fma(gx, xr.y, G[4]); G[5] = fma(gx, xr.z, G[5]);
+                G[6] = fma(gy, xr.x, G[6]); G[7] = fma(gy, xr.y, G[7]); G[8] = fma(gy, xr.z, G[8]);
+                G[9] = fma(gz, xr.x, G[9]); G[10] = fma(gz, xr.y, G[10]); G[11] = fma(gz, xr.z, G[11]);
+            }
+            long long *ga = D->gacc + (size_t)img * NB_GACC_WORDS_PER_IMAGE;
+#pragma unroll
+            for (int k = 0; k < 12; k++) {
+                const double v = wave_sum_f64(G[k]);
+                if (lane == k) fx_add(&ga[2 * k], &ga[2 * k + 1], v);
+            }
         }
         eel = wave_sum_f64(eel);
         elj = wave_sum_f64(elj);
         if (lane < 2) {
             const int slot = it & (NB_ESLOTS - 1);
-            fx_add(eacc_word(D->eacc, lane, 0, slot), eacc_word(D->eacc, lane, 1, slot), lane == 0 ? eel : elj);
+            const int kind = lane + (img >= 0 ? 2 : 0);
+            fx_add(eacc_word(D->eacc, kind, 0, slot), eacc_word(D->eacc, kind, 1, slot),
+                   isc * (lane == 0 ? eel : elj));
         }
     }
 }
@@ -546,7 +607,7 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
         }
     }
     if (blockIdx.x != 0) return;
-    double el, lj, el14, lj14;
+    double el, lj, el14, lj14, iel = 0.0, ilj = 0.0;
     if (plain) {
         sum_range2(D.eplain, D.n_plain_blocks, el, lj);
         el *= 0.5;
@@ -557,37 +618,32 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
         // slots, the total is brought to its unique form (low word in
         // [0, 2^48)) and converted once, so the result does not depend on which
         // items shared a slot.
-        __shared__ double sE[2];
-        if (threadIdx.x < 64) {
-            const int t = threadIdx.x;
-            long long h0 = 0, l0 = 0, h1 = 0, l1 = 0;
+        __shared__ double sE[NB_EKINDS];
+        if (threadIdx.x < 64 * NB_EKINDS) {
+            const int t = threadIdx.x & 63, kind = threadIdx.x >> 6;
+            long long h = 0, l = 0;
             if (t < NB_ESLOTS) {
-                const long long a0 = *eacc_word(D.eacc, 0, 0, t), a1 = *eacc_word(D.eacc, 1, 0, t);
-                h0 = *eacc_word(D.eacc, 0, 1, t) + (a0 >> 48);
-                l0 = a0 & ((1ll << 48) - 1);
-                h1 = *eacc_word(D.eacc, 1, 1, t) + (a1 >> 48);
-                l1 = a1 & ((1ll << 48) - 1);
+                const long long a0 = *eacc_word(D.eacc, kind, 0, t);
+                h = *eacc_word(D.eacc, kind, 1, t) + (a0 >> 48);
+                l = a0 & ((1ll << 48) - 1);
             }
 #pragma unroll
             for (int o = 32; o >= 1; o >>= 1) {
-                h0 += __shfl_xor(h0, o, 64);
-                l0 += __shfl_xor(l0, o, 64);
-                h1 += __shfl_xor(h1, o, 64);
-                l1 += __shfl_xor(l1, o, 64);
+                h += __shfl_xor(h, o, 64);
+                l += __shfl_xor(l, o, 64);
             }
             if (t == 0) {
                 // Canonical words of the total (0 <= low < 2^48), then one rounding.
-                h0 += l0 >> 48;
-                l0 &= (1ll << 48) - 1;
-                h1 += l1 >> 48;
-                l1 &= (1ll << 48) - 1;
-                sE[0] = (double)h0 * NB_FX_HI_UNIT + (double)l0 / NB_FX_LO_SCALE;
-                sE[1] = (double)h1 * NB_FX_HI_UNIT + (double)l1 / NB_FX_LO_SCALE;
+                h += l >> 48;
+                l &= (1ll << 48) - 1;
+                sE[kind] = (double)h * NB_FX_HI_UNIT + (double)l / NB_FX_LO_SCALE;
             }
         }
         __syncthreads();
         el = sE[0];
         lj = sE[1];
+        iel = sE[2];
+        ilj = sE[3];
     }
     sum_range2(D.epart14, D.n14_blocks, el14, lj14);
     if (threadIdx.x == 0) {
@@ -595,6 +651,8 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
         D.energies[1] = lj;
         D.energies[2] = el14;
         D.energies[3] = lj14;
+        D.energies[4] = iel;
+        D.energies[5] = ilj;
     }
 }
 

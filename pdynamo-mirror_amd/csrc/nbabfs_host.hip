@@ -286,6 +286,25 @@ __global__ __launch_bounds__(256) void k_check_batch(const CheckDesc *__restrict
     if (__any(up) && threadIdx.x % 64 == 0) *(volatile int *)&flags[blockIdx.y] = epoch;
 }
 
+// Largest displacement of a free atom since the last list build (CheckForUpdate's maximumDisplacement).
+__global__ __launch_bounds__(256) void k_max_displacement(const double *__restrict__ xyz, const double *__restrict__ ref,
+                                                          const unsigned char *__restrict__ free_flags, int n,
+                                                          unsigned long long *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double r2 = 0.0;
+    if (i < n && (!free_flags || free_flags[i])) {
+        const double dx = xyz[3 * (size_t)i] - ref[3 * (size_t)i];
+        const double dy = xyz[3 * (size_t)i + 1] - ref[3 * (size_t)i + 1];
+        const double dz = xyz[3 * (size_t)i + 2] - ref[3 * (size_t)i + 2];
+        r2 = dx * dx + dy * dy + dz * dz;
+    }
+    for (int o = 32; o >= 1; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+    // Non-negative doubles order as their bit patterns.
+    if ((threadIdx.x & 63) == 0 && r2 > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(r2));
+}
+
 // ------------------------------------------------------------------------------------------------
 // The state.
 // ------------------------------------------------------------------------------------------------
@@ -354,6 +373,44 @@ struct nbabfs_state {
     DBuf<unsigned long long> u64;
     PinBuf<double> h_bbox;
     PinBuf<int> h_int;
+
+    // Crystal symmetry: the transformations given at set-up (fractional, 9 + 3
+    // per item, with the identity and rotational inverses) and the crystal
+    // parameters of the current call (NBModelABFSState_SetUp's transformations
+    // and NBModelABFSState_Initialize's symmetryParameters).
+    bool has_sym = false, sp_set = false;
+    int ntrans = 0, identity = -1, checkInv = 1, expandFactor = 0;
+    std::vector<double> trot, ttra;
+    std::vector<int> tinv;
+    double sp[6] = {0, 0, 0, 0, 0, 0}, M[9], Minv[9];
+    double Mref[9];                 // M of the last image-list build (referenceSymmetryParameters)
+    double Mxf[9];                  // M of the uploaded image transformations
+    bool xf_valid = false;
+    // Image slots: the (transformation, a, b, c) searched at the last build in
+    // the reference's order; the image list proper is the slots with pairs.
+    struct ImageSlot {
+        int a, b, c, t;
+        double scale;
+        long long npairs;
+    };
+    std::vector<ImageSlot> slots;
+    int nimages = 0, nent_main = 0, nitems_main = 0;
+    long long img_npairs = 0;
+    double sumImages = 0.0, sumImagePairs = 0.0;
+    DBuf<double> itc, ixyz, rtbuf, ibb;
+    DBuf<float4> iblo, ibhi, icen;
+    DBuf<float> ibox;
+    DBuf<unsigned char> ongrid;
+    DBuf<int> istep_off, ibound, isoff, icounts, incand, ieoff, inits, iioff;
+    DBuf<ImageStep> isteps;
+    DBuf<Entry> iscratch, entries_tmp;
+    DBuf<unsigned long long> inpairs;
+    DBuf<ImageXf> imgxf;
+    DBuf<long long> gacc;
+    PinBuf<double> h_ibb;
+    PinBuf<ImageXf> h_imgxf;
+    PinBuf<long long> h_gacc;
+    PinBuf<unsigned long long> h_inpairs;
 
     // Kernel timing.
     bool timing = false;
@@ -454,6 +511,206 @@ static void make_stencil(double h, double cutoff, int &u, std::vector<unsigned c
                 }
                 if (minR2 < c2) flags[((size_t)(a + u) * w + (b + u)) * w + (c + u)] = (maxR2 <= c2) ? 2 : 1;
             }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Crystal symmetry (host FP64 in the reference's operation order): 3x3
+// matrices row-major.  pCore-1.9.0/extensions/csource/Matrix33.c,
+// Transformation3.c, Transformation3Container.c; pMolecule-1.9.0/extensions/
+// csource/SymmetryParameters.c.
+// ------------------------------------------------------------------------------------------------
+static double m33_det(const double *a)
+{
+#pragma clang fp contract(off)
+    return a[0] * (a[4] * a[8] - a[7] * a[5]) - a[1] * (a[3] * a[8] - a[6] * a[5]) +
+           a[2] * (a[3] * a[7] - a[6] * a[4]);
+}
+
+// Matrix33_Invert: cofactors, then a scale by 1/det.
+static void m33_invert(const double *a, double *r)
+{
+#pragma clang fp contract(off)
+    const double det = m33_det(a);
+    r[0] = (a[4] * a[8] - a[5] * a[7]);
+    r[1] = (a[2] * a[7] - a[8] * a[1]);
+    r[2] = (a[1] * a[5] - a[4] * a[2]);
+    r[3] = (a[5] * a[6] - a[3] * a[8]);
+    r[4] = (a[0] * a[8] - a[2] * a[6]);
+    r[5] = (a[2] * a[3] - a[0] * a[5]);
+    r[6] = (a[3] * a[7] - a[4] * a[6]);
+    r[7] = (a[1] * a[6] - a[0] * a[7]);
+    r[8] = (a[0] * a[4] - a[3] * a[1]);
+    const double sc = 1.0 / det;
+    for (int k = 0; k < 9; k++) r[k] *= sc;
+}
+
+// Matrix33_PreMultiplyBy: self = o * self.
+static void m33_premul(double *self, const double *o)
+{
+#pragma clang fp contract(off)
+    double s[9], r[9];
+    std::memcpy(s, self, sizeof(s));
+    for (int i = 0; i < 3; i++) {
+        const double o0 = o[3 * i], o1 = o[3 * i + 1], o2 = o[3 * i + 2];
+        r[3 * i] = s[0] * o0 + s[3] * o1 + s[6] * o2;
+        r[3 * i + 1] = s[1] * o0 + s[4] * o1 + s[7] * o2;
+        r[3 * i + 2] = s[2] * o0 + s[5] * o1 + s[8] * o2;
+    }
+    std::memcpy(self, r, sizeof(r));
+}
+
+// Matrix33_PostMultiplyBy: self = self * o.
+static void m33_postmul(double *self, const double *o)
+{
+#pragma clang fp contract(off)
+    double s[9], r[9];
+    std::memcpy(s, self, sizeof(s));
+    for (int i = 0; i < 3; i++) {
+        const double o0 = o[i], o1 = o[3 + i], o2 = o[6 + i];
+        r[i] = s[0] * o0 + s[1] * o1 + s[2] * o2;
+        r[3 + i] = s[3] * o0 + s[4] * o1 + s[5] * o2;
+        r[6 + i] = s[6] * o0 + s[7] * o1 + s[8] * o2;
+    }
+    std::memcpy(self, r, sizeof(r));
+}
+
+// Matrix33_ApplyToVector3.
+static void m33_apply(const double *m, double *v)
+{
+#pragma clang fp contract(off)
+    const double x = v[0], y = v[1], z = v[2];
+    v[0] = x * m[0] + y * m[1] + z * m[2];
+    v[1] = x * m[3] + y * m[4] + z * m[5];
+    v[2] = x * m[6] + y * m[7] + z * m[8];
+}
+
+static bool m33_equal_tol(const double *a, const double *b)
+{
+    for (int k = 0; k < 9; k++)
+        if (std::fabs(a[k] - b[k]) > 1.0e-6) return false;
+    return true;
+}
+
+// SymmetryParameters_MakeM: the columns of M are the lattice vectors.
+static void symmetry_make_m(const double *sp, double *M, double *Minv)
+{
+#pragma clang fp contract(off)
+    const double dr = M_PI / 180.0e+00;
+    const double al = sp[3] * dr, be = sp[4] * dr, ga = sp[5] * dr;
+    const double ca = std::cos(al), cb = std::cos(be), cg = std::cos(ga), sg = std::sin(ga);
+    for (int k = 0; k < 9; k++) M[k] = 0.0;
+    M[0] = sp[0];
+    M[1] = sp[1] * cg;
+    M[4] = sp[1] * sg;
+    M[2] = sp[2] * cb;
+    M[5] = sp[2] * (ca - cb * cg) / sg;
+    M[8] = sp[2] * std::sqrt(1.0e+00 - ca * ca - cb * cb - cg * cg + 2.0e+00 * ca * cb * cg) / sg;
+    m33_invert(M, Minv);
+}
+
+// Transformation3_Orthogonalize: R' = M R M^-1, T' = M t.
+static void orthogonalize(const double *M, const double *Minv, const double *rot, const double *tr, double *Ro,
+                          double *To)
+{
+    std::memcpy(Ro, rot, 9 * sizeof(double));
+    m33_premul(Ro, M);
+    m33_postmul(Ro, Minv);
+    To[0] = tr[0];
+    To[1] = tr[1];
+    To[2] = tr[2];
+    m33_apply(M, To);
+}
+
+// GetLimits of SymmetryParameters_FindBoxSearchLimits.
+static bool get_limits(double bl, double bu, double il, double iu, double t, int *low, int *high)
+{
+#pragma clang fp contract(off)
+    int n = 0;
+    while (iu >= bl) { il -= t; iu -= t; n--; }
+    while (iu < bl) { il += t; iu += t; n++; }
+    if (il <= bu) {
+        *low = n;
+        while (il <= bu) { il += t; iu += t; n++; }
+        *high = n - 1;
+        return true;
+    }
+    return false;
+}
+
+// SymmetryParameters_FindBoxSearchLimits: lim = alow, ahigh, blow, bhigh, clow, chigh.
+static void box_search_limits(const double *M, const double *lower, const double *upper, const double *ilower,
+                              const double *iupper, int *lim)
+{
+#pragma clang fp contract(off)
+    lim[0] = lim[2] = lim[4] = 0;
+    lim[1] = lim[3] = lim[5] = -1;
+    if (!(M[8] > 0.0 && M[4] > 0.0 && M[0] > 0.0)) return;
+    if (!get_limits(lower[2], upper[2], ilower[2], iupper[2], M[8], &lim[4], &lim[5])) return;
+    double d1 = lim[4] * M[5], d2 = lim[5] * M[5];
+    double bl = lower[1] - std::max(d1, d2), bu = upper[1] - std::min(d1, d2);
+    if (!get_limits(bl, bu, ilower[1], iupper[1], M[4], &lim[2], &lim[3])) return;
+    d1 = lim[4] * M[2];
+    d2 = lim[5] * M[2];
+    bl = lower[0] - std::max(d1, d2);
+    bu = upper[0] - std::min(d1, d2);
+    d1 = lim[2] * M[1];
+    d2 = lim[3] * M[1];
+    bl -= std::max(d1, d2);
+    bu -= std::min(d1, d2);
+    get_limits(bl, bu, ilower[0], iupper[0], M[0], &lim[0], &lim[1]);
+}
+
+// Round (pCore-1.9.0/extensions/cinclude/Macros.h:34).
+static int ref_round(double a) { return a >= 0 ? (int)(a + 0.5) : (int)(a - 0.5); }
+
+// Transformation3Container_FindInverseIntegerTranslation.
+static void inverse_integer_translation(const double *rot, const double *tr, int t, int tinv, int a, int b, int c,
+                                        int *inv)
+{
+#pragma clang fp contract(off)
+    double v[3] = {tr[3 * t] + (double)a, tr[3 * t + 1] + (double)b, tr[3 * t + 2] + (double)c};
+    m33_apply(rot + 9 * tinv, v);
+    for (int i = 0; i < 3; i++) {
+        v[i] *= -1.0e+00;
+        v[i] += -1.0e+00 * tr[3 * tinv + i];
+        const int r = ref_round(v[i]);
+        inv[i] = (std::fabs(v[i] - (double)r) < 1.0e-4) ? r : -999999;
+    }
+}
+
+// Matrix33_InverseDerivative: d(M^-1)/dM_ij.
+static void inverse_derivative(const double *m, int i, int j, double *o)
+{
+#pragma clang fp contract(off)
+    const double m00 = m[0], m01 = m[1], m02 = m[2], m10 = m[3], m11 = m[4], m12 = m[5], m20 = m[6], m21 = m[7],
+                 m22 = m[8];
+    double ddet = 0.0;
+    m33_invert(m, o);
+    switch (3 * i + j) {
+    case 0: ddet = -m12 * m21 + m11 * m22; break;
+    case 1: ddet = m12 * m20 - m10 * m22; break;
+    case 2: ddet = -m11 * m20 + m10 * m21; break;
+    case 3: ddet = m02 * m21 - m01 * m22; break;
+    case 4: ddet = -m02 * m20 + m00 * m22; break;
+    case 5: ddet = m01 * m20 - m00 * m21; break;
+    case 6: ddet = -m02 * m11 + m01 * m12; break;
+    case 7: ddet = m02 * m10 - m00 * m12; break;
+    default: ddet = -m01 * m10 + m00 * m11; break;
+    }
+    for (int k = 0; k < 9; k++) o[k] *= -ddet;
+    switch (3 * i + j) {
+    case 0: o[4] += m22; o[5] -= m12; o[7] -= m21; o[8] += m11; break;
+    case 1: o[1] -= m22; o[2] += m12; o[7] += m20; o[8] -= m10; break;
+    case 2: o[1] += m21; o[2] -= m11; o[4] -= m20; o[5] += m10; break;
+    case 3: o[3] -= m22; o[5] += m02; o[6] += m21; o[8] -= m01; break;
+    case 4: o[0] += m22; o[2] -= m02; o[6] -= m20; o[8] += m00; break;
+    case 5: o[0] -= m21; o[2] += m01; o[3] += m20; o[5] -= m00; break;
+    case 6: o[3] += m12; o[4] -= m02; o[6] -= m11; o[7] += m01; break;
+    case 7: o[0] -= m12; o[1] += m02; o[6] += m10; o[7] -= m00; break;
+    default: o[0] += m11; o[1] -= m01; o[3] -= m10; o[4] += m00; break;
+    }
+    const double sc = 1.0 / m33_det(m);
+    for (int k = 0; k < 9; k++) o[k] *= sc;
 }
 
 // Work items longest-first (and, for the persistent queues, interleaved over
@@ -629,12 +886,17 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     }
     S->nent = ((volatile int *)S->tail.h)[0];
     S->nitems = ((volatile int *)S->tail.h)[1];
+    S->nent_main = S->nent;
+    S->nitems_main = S->nitems;
+    S->slots.clear();
+    S->nimages = 0;
+    S->img_npairs = 0;
     CHK(S->entries.ensure(S->nent));
     CHK(S->items.ensure(S->nitems));
     CHK(S->u64.ensure(2));
     HIPCHK(hipMemsetAsync(S->u64.p, 0, 2 * sizeof(unsigned long long), st));
-    launch_compact(S->scratch.p, S->cap, S->ncand.p, S->counts.p, S->eoff.p, S->ioff.p, S->nsc, per_item, 0,
-                   S->entries.p, S->items.p, S->u64.p, st);
+    launch_compact(S->scratch.p, S->cap, nullptr, S->ncand.p, S->counts.p, S->eoff.p, S->ioff.p, S->nsc, S->nsc,
+                   per_item, 0, S->entries.p, S->items.p, S->u64.p, nullptr, st);
     unsigned long long np = 0;
     HIPCHK(hipMemcpyAsync(&np, S->u64.p, sizeof(np), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -646,7 +908,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     CHK(S->facc.ensure(6 * (size_t)S->n_pad));
     CHK(S->eacc.ensure(NB_EACC_WORDS));
     CHK(S->vacc.ensure(NB_VACC_WORDS));
-    CHK(S->energies.ensure(4));
+    CHK(S->energies.ensure(NB_NENERGIES));
     CHK(order_items(S));
     CHK(S->heads.ensure(8 * 32));
 
@@ -657,10 +919,10 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
         CHK(S->gplain.ensure(3 * (size_t)n));
         CHK(S->eplain.ensure(2 * (size_t)nblk(n, 256)));
         launch_fill_int(S->cursor.p, n + 1, 0, st);
-        launch_csr_degree(S->entries.p, S->nent, S->slot_atom.p, S->cursor.p, st);
+        launch_csr_degree(S->entries.p, S->nent_main, S->slot_atom.p, S->cursor.p, st);
         launch_scan(S->cursor.p, S->rowptr.p, n + 1, S->scantmp.p, st);
         HIPCHK(hipMemcpyAsync(S->cursor.p, S->rowptr.p, (n + 1) * sizeof(int), hipMemcpyDeviceToDevice, st));
-        launch_csr_fill(S->entries.p, S->nent, S->slot_atom.p, S->cursor.p, S->cols.p, st);
+        launch_csr_fill(S->entries.p, S->nent_main, S->slot_atom.p, S->cursor.p, S->cols.p, st);
         launch_csr_sort_rows(S->rowptr.p, n, S->cols.p, st);
         S->csrValid = true;
     }
@@ -690,7 +952,7 @@ static int ensure_desc(nbabfs_state *S, const double *d_xyz, double *d_grad)
 
 static int ensure_call_buffers(nbabfs_state *S)
 {
-    CHK(S->energies.ensure(4));
+    CHK(S->energies.ensure(NB_NENERGIES));
     CHK(S->eacc.ensure(NB_EACC_WORDS));
     CHK(S->vacc.ensure(NB_VACC_WORDS));
     CHK(S->epart14.ensure(2 * (size_t)std::max(nblk(S->n14, 256), 1)));
@@ -705,11 +967,23 @@ static int launch_force(nbabfs_state *S, ForceGuard g);
 // before the host waits for the test: the GPU does not idle while the host
 // decides.  If the list must be rebuilt, the guarded launch does nothing and
 // the energy call prepares and evaluates again on the new list.
+static int upload_image_xf(nbabfs_state *S);
+static int build_images(nbabfs_state *S, const double *d_xyz);
+static bool check_for_image_update(const nbabfs_state *S, double maxdisp);
+
 static int update(nbabfs_state *S, const double *d_xyz, int *rebuilt)
 {
     hipStream_t st = S->stream;
     bool doUpdate = S->isNew || !S->have_ref || !S->listValid;
+    bool doImages = false;
     S->forced_xyz = nullptr;
+    if (S->has_sym && !S->sp_set)
+        return fail(NBABFS_STATUS_INVALID_ARGUMENT, "crystal symmetry without symmetry parameters");
+    // With images and a changed cell, CheckForImageUpdate needs the largest
+    // displacement before anything is launched: no speculative pair kernel.
+    const bool cellCheck = S->has_sym && S->nimages > 0 && std::memcmp(S->M, S->Mref, sizeof(S->M)) != 0;
+    if (!doUpdate && S->has_sym && !S->xf_valid) CHK(upload_image_xf(S));
+    if (!doUpdate && S->has_sym && std::memcmp(S->M, S->Mxf, sizeof(S->M)) != 0) CHK(upload_image_xf(S));
     if (!doUpdate) {
         const double buf = 0.5 * (S->opt.listCutoff - S->opt.outerCutoff);
         CHK(S->upflag.ensure(1));
@@ -729,7 +1003,7 @@ static int update(nbabfs_state *S, const double *d_xyz, int *rebuilt)
         launch_prepare(S->d_desc.p, 1, S->n_pad, plain ? 0 : 1, S->heads.p, chk, st);
         HIPCHK(hipGetLastError());
         // The descriptor's coordinates must be these for the speculative pass.
-        if (S->last_xyz == d_xyz && S->n > 0) {
+        if (S->last_xyz == d_xyz && S->n > 0 && !cellCheck) {
             CHK(launch_force(S, ForceGuard{S->dflag.p, S->epoch}));
             S->forced_xyz = d_xyz;
         }
@@ -750,17 +1024,35 @@ static int update(nbabfs_state *S, const double *d_xyz, int *rebuilt)
             if (S->timing && S->ev_used >= 2) S->ev_used -= 2;
             S->forced_xyz = nullptr;
         }
+        if (!doUpdate && cellCheck) {
+            CHK(S->u64.ensure(2));
+            launch_fill_int((int *)S->u64.p, 2, 0, st);
+            if (S->n > 0)
+                hipLaunchKernelGGL(k_max_displacement, dim3(nblk(S->n, 256)), dim3(256), 0, st, d_xyz, S->xyz_ref.p,
+                                   S->has_fixed ? S->freef.p : nullptr, S->n, S->u64.p);
+            unsigned long long bits = 0;
+            HIPCHK(hipMemcpyAsync(&bits, S->u64.p, sizeof(bits), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            double r2;
+            std::memcpy(&r2, &bits, sizeof(r2));
+            doImages = check_for_image_update(S, std::sqrt(r2));
+        }
     }
     if (doUpdate) {
         CHK(build_list(S, d_xyz));
         CHK(S->xyz_ref.ensure(3 * (size_t)std::max(S->n, 1)));
         if (S->n) HIPCHK(hipMemcpyAsync(S->xyz_ref.p, d_xyz, 3 * (size_t)S->n * sizeof(double), hipMemcpyDeviceToDevice, st));
         S->have_ref = true;
+    }
+    if (S->has_sym && (doUpdate || doImages)) CHK(build_images(S, d_xyz));
+    if (doUpdate || doImages) {
         S->updates++;
         S->sumPairs += (double)S->npairs;
+        S->sumImages += (double)S->nimages;
+        S->sumImagePairs += (double)S->img_npairs;
     }
     S->isNew = false;
-    if (rebuilt) *rebuilt = doUpdate ? 1 : 0;
+    if (rebuilt) *rebuilt = (doUpdate || doImages) ? 1 : 0;
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -801,6 +1093,9 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.n14 = S->n14;
     D.n_plain_blocks = nblk(S->n, 256);
     D.n14_blocks = nblk(S->n14, 256);
+    D.nimg = (int)S->slots.size();
+    D.imgxf = D.nimg ? S->imgxf.p : nullptr;
+    D.gacc = D.nimg ? S->gacc.p : nullptr;
 }
 
 // wavesPerCU > 0: persistent grid of that many waves per CU; otherwise one
@@ -842,7 +1137,7 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
     // finalize pass, which reads the descriptor after the pair kernel.
     CHK(ensure_desc(S, d_xyz, d_grad));
     if (S->n == 0) {
-        launch_fill_int((int *)S->energies.p, 8, 0, st);
+        launch_fill_int((int *)S->energies.p, 12, 0, st);
         HIPCHK(hipGetLastError());
         return NBABFS_STATUS_CONTINUE;
     }
@@ -858,24 +1153,335 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
     return NBABFS_STATUS_CONTINUE;
 }
 
-// Re-cut the compacted entries of the current list into work items of
-// per_item entries (the entries and their order are unchanged).
+// Cut the compacted entries of the current lists (main, then images) into
+// work items of per_item entries (the entries and their order are unchanged)
+// and order them longest-first.
 static int recut_items(nbabfs_state *S, int per_item)
 {
     hipStream_t st = S->stream;
     if (S->nsc <= 0) return NBABFS_STATUS_CONTINUE;
+    const int nb = (int)S->slots.size() * S->nsc;
     launch_items_per_sc(S->counts.p, S->nsc, per_item, S->nits.p, st);
     launch_scan(S->nits.p, S->ioff.p, S->nsc, S->scantmp.p, st);
     hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, S->eoff.p, S->counts.p, S->ioff.p, S->nits.p, S->nsc,
                        S->flag.p, S->tail.d);
+    if (nb > 0) {
+        launch_items_per_sc(S->icounts.p, nb, per_item, S->inits.p, st);
+        launch_scan(S->inits.p, S->iioff.p, nb, S->scantmp.p, st);
+        hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, S->ieoff.p, S->icounts.p, S->iioff.p, S->inits.p,
+                           nb, S->flag.p, S->tail.d + 4);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
-    S->nitems = ((volatile int *)S->tail.h)[1];
+    S->nitems_main = ((volatile int *)S->tail.h)[1];
+    const int nimg_items = nb > 0 ? ((volatile int *)S->tail.h)[5] : 0;
+    S->nitems = S->nitems_main + nimg_items;
     CHK(S->items.ensure(std::max(S->nitems, 1)));
-    launch_cut_items(S->counts.p, S->eoff.p, S->ioff.p, S->nsc, per_item, 0, S->items.p, st);
+    launch_cut_items(S->counts.p, S->eoff.p, S->ioff.p, S->nsc, S->nsc, per_item, 0, S->items.p, st);
+    if (nb > 0) {
+        launch_add_int(S->iioff.p, nb, S->nitems_main, st);
+        launch_cut_items(S->icounts.p, S->ieoff.p, S->iioff.p, nb, S->nsc, per_item, 0, S->items.p, st);
+    }
     HIPCHK(hipGetLastError());
     return order_items(S);
 }
+
+// ------------------------------------------------------------------------------------------------
+// Image lists (GenerateImageLists, pMolecule-1.9.0/extensions/csource/NBModelABFS.c:753-1045, the
+// MM/MM list of a pure MM system): the host walks the transformations and
+// (a, b, c) ranges as the reference does (inverse skipping and scales, box
+// search limits, the box-overlap test, the +d/-d stepping of the image
+// coordinates); the GPU generates the stepped image coordinates and runs one
+// cross search per image slot.  Slots whose image has no pair are dropped from
+// the reported image list (ImageList_CreateImage, ImageList.c:31-62).
+// ------------------------------------------------------------------------------------------------
+// Grow an entry buffer to hold `need` entries, keeping its first `keep`.
+static int ensure_entries_keep(nbabfs_state *S, size_t need, size_t keep)
+{
+    if (S->entries.p && need <= S->entries.cap) return NBABFS_STATUS_CONTINUE;
+    CHK(S->entries_tmp.ensure(need));
+    if (keep) HIPCHK(hipMemcpyAsync(S->entries_tmp.p, S->entries.p, keep * sizeof(Entry), hipMemcpyDeviceToDevice, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    S->entries.swap(S->entries_tmp);
+    S->entries_tmp.release();
+    return NBABFS_STATUS_CONTINUE;
+}
+
+static int build_images(nbabfs_state *S, const double *d_xyz)
+{
+    hipStream_t st = S->stream;
+    const int n = S->n;
+    const nbabfs_options &o = S->opt;
+    const double cutoff = o.listCutoff;
+    S->slots.clear();
+    S->nimages = 0;
+    S->img_npairs = 0;
+    S->nent = S->nent_main;
+    if (S->has_sym && n > 0 && S->nsc > 0 && cutoff > 0.0) {
+        // Box of the current coordinates (Coordinates3_EnclosingOrthorhombicBox; upper = extent + lower),
+        // widened by the cutoff.
+        const int nparts = std::min(nblk(n, 256), 1024);
+        CHK(S->bbpart.ensure(6 * (size_t)nparts));
+        CHK(S->bbox.ensure(6));
+        CHK(S->h_bbox.ensure(6));
+        launch_bbox(d_xyz, n, S->bbpart.p, nparts, S->bbox.p, st);
+        HIPCHK(hipMemcpyAsync(S->h_bbox.p, S->bbox.p, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+        // Transformations searched: with inverses, t is skipped when its inverse comes later.
+        std::vector<int> tl;
+        std::vector<double> rt;
+        for (int t = 0; t < S->ntrans; t++) {
+            const int ti = S->checkInv ? S->tinv[t] : -1;
+            if (ti >= 0 && t < ti) continue;
+            double Ro[9], To[3];
+            orthogonalize(S->M, S->Minv, &S->trot[9 * t], &S->ttra[3 * t], Ro, To);
+            tl.push_back(t);
+            rt.insert(rt.end(), Ro, Ro + 9);
+            rt.insert(rt.end(), To, To + 3);
+        }
+        const int nt = (int)tl.size();
+        CHK(S->rtbuf.upload(rt.data(), rt.size(), st));
+        CHK(S->itc.ensure((size_t)nt * 3 * n));
+        launch_image_transform(d_xyz, n, S->rtbuf.p, nt, S->itc.p, st);
+        CHK(S->ibb.ensure(6 * (size_t)nt));
+        CHK(S->h_ibb.ensure(6 * (size_t)nt));
+        for (int k = 0; k < nt; k++) launch_bbox(S->itc.p + (size_t)k * 3 * n, n, S->bbpart.p, nparts, S->ibb.p + 6 * k, st);
+        HIPCHK(hipMemcpyAsync(S->h_ibb.p, S->ibb.p, 6 * (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        double lower[3], upper[3];
+        {
+#pragma clang fp contract(off)
+            for (int d = 0; d < 3; d++) {
+                lower[d] = S->h_bbox.p[d];
+                upper[d] = (S->h_bbox.p[3 + d] - S->h_bbox.p[d]) + lower[d];
+                lower[d] += -cutoff;
+                upper[d] += cutoff;
+            }
+        }
+        // Walk the (a, b, c) ranges.
+        std::vector<ImageStep> steps;
+        std::vector<int> step_off(nt + 1, 0);
+        std::vector<float> boxes;
+        for (int k = 0; k < nt; k++) {
+#pragma clang fp contract(off)
+            const int t = tl[k];
+            const int ti = S->checkInv ? S->tinv[t] : -1;
+            const double defscale = ti >= 0 ? 1.0 : 0.5;
+            double ilower[3], iupper[3];
+            for (int d = 0; d < 3; d++) {
+                ilower[d] = S->h_ibb.p[6 * k + d];
+                iupper[d] = (S->h_ibb.p[6 * k + 3 + d] - S->h_ibb.p[6 * k + d]) + ilower[d];
+            }
+            int lim[6];
+            box_search_limits(S->M, lower, upper, ilower, iupper, lim);
+            if (S->expandFactor > 0)
+                for (int q = 0; q < 6; q++) lim[q] += (q & 1) ? S->expandFactor : -S->expandFactor;
+            for (int a = lim[0]; a <= lim[1]; a++)
+                for (int b = lim[2]; b <= lim[3]; b++)
+                    for (int c = lim[4]; c <= lim[5]; c++) {
+                        if (a == 0 && b == 0 && c == 0 && t == S->identity) continue;
+                        double scale = defscale;
+                        if (ti >= 0 && ti == t) {
+                            int inv[3];
+                            inverse_integer_translation(S->trot.data(), S->ttra.data(), t, ti, a, b, c, inv);
+                            if (inv[0] >= lim[0] && inv[0] <= lim[1] && inv[1] >= lim[2] && inv[1] <= lim[3] &&
+                                inv[2] >= lim[4] && inv[2] <= lim[5]) {
+                                if (a == inv[0] && b == inv[1] && c == inv[2]) {
+                                    scale = 0.5;
+                                } else {
+                                    scale = 1.0;
+                                    if (a < inv[0] || (a == inv[0] && b < inv[1]) ||
+                                        (a == inv[0] && b == inv[1] && c < inv[2]))
+                                        continue;
+                                }
+                            } else {
+                                scale = 1.0;
+                            }
+                        }
+                        // SymmetryParameters_Displacement.
+                        ImageStep q;
+                        const double na = (double)a, nb = (double)b, nc = (double)c;
+                        for (int i = 0; i < 3; i++) q.d[i] = na * S->M[3 * i] + nb * S->M[3 * i + 1] + nc * S->M[3 * i + 2];
+                        q.out = -1;
+                        q.pad = 0;
+                        for (int d = 0; d < 3; d++) { ilower[d] += q.d[d]; iupper[d] += q.d[d]; }
+                        if (ilower[0] <= upper[0] && ilower[1] <= upper[1] && ilower[2] <= upper[2] &&
+                            iupper[0] >= lower[0] && iupper[1] >= lower[1] && iupper[2] >= lower[2]) {
+                            q.out = (int)S->slots.size();
+                            S->slots.push_back({a, b, c, t, scale, 0});
+                            // Image box in the cluster frame, padded outwards for FP32.
+                            for (int d = 0; d < 3; d++) boxes.push_back((float)(ilower[d] - S->grid.corg[d] - 1.0e-2));
+                            for (int d = 0; d < 3; d++) boxes.push_back((float)(iupper[d] - S->grid.corg[d] + 1.0e-2));
+                        }
+                        steps.push_back(q);
+                        for (int d = 0; d < 3; d++) {
+                            const double md = -q.d[d];
+                            ilower[d] += md;
+                            iupper[d] += md;
+                        }
+                    }
+            step_off[k + 1] = (int)steps.size();
+        }
+        const int ns = (int)S->slots.size();
+        if (ns > 0) {
+            const int nsc = S->nsc, ncl = S->ncl, nb = ns * nsc;
+            CHK(S->isteps.upload(steps.data(), steps.size(), st));
+            CHK(S->istep_off.upload(step_off.data(), step_off.size(), st));
+            CHK(S->ixyz.ensure((size_t)ns * 3 * n));
+            launch_image_coords(S->itc.p, n, nt, S->istep_off.p, S->isteps.p, S->ixyz.p, st);
+            CHK(S->iblo.ensure((size_t)ns * ncl));
+            CHK(S->ibhi.ensure((size_t)ns * ncl));
+            CHK(S->icen.ensure(ncl));
+            for (int k = 0; k < ns; k++)
+                launch_cluster_geom(S->ixyz.p + (size_t)k * 3 * n, S->slot_atom.p, ncl, S->grid.corg,
+                                    S->iblo.p + (size_t)k * ncl, S->ibhi.p + (size_t)k * ncl, S->icen.p, st);
+            CHK(S->ibox.upload(boxes.data(), boxes.size(), st));
+            // The primary clusters' boxes for the current coordinates: atoms may
+            // have moved since the main list (and its boxes) was built.
+            launch_cluster_geom(d_xyz, S->slot_atom.p, ncl, S->grid.corg, S->blo.p, S->bhi.p, S->centre.p, st);
+            // i atoms on the grid made from the current coordinates (NBModelABFSState_GridInitialize).
+            const unsigned char *og = nullptr;
+            if (S->useGrid) {
+                BuildGrid g2;
+                std::memset(&g2, 0, sizeof(g2));
+                reference_grid(S->h_bbox.p, o.cutoffCellSizeFactor * o.listCutoff, g2);
+                CHK(S->ongrid.ensure(n));
+                launch_ongrid(d_xyz, n, g2, S->ongrid.p, st);
+                og = S->ongrid.p;
+            }
+            SearchParams P;
+            std::memset(&P, 0, sizeof(P));
+            P.cutoff2 = cutoff * cutoff;
+            P.use_grid = S->useGrid ? 1 : 0;
+            P.free_flags = S->has_fixed ? S->freef.p : nullptr;
+            P.reach = (float)(cutoff + 0.01);
+            P.ixyz = S->ixyz.p;
+            P.iblo = S->iblo.p;
+            P.ibhi = S->ibhi.p;
+            P.ibox = S->ibox.p;
+            P.ongrid = og;
+            P.n = n;
+            P.ncl = ncl;
+            // Scratch: each (image, super-cluster) block gets its candidate bound.
+            CHK(S->ibound.ensure(nb));
+            CHK(S->isoff.ensure(nb + 1));
+            CHK(S->scantmp.ensure(scan_tmp_size(nb + 1) + 1024));
+            launch_image_bounds(P, nsc, ns, S->blo.p, S->bhi.p, S->ibound.p, st);
+            launch_scan(S->ibound.p, S->isoff.p, nb, S->scantmp.p, st);
+            CHK(S->tail.ensure(8));
+            CHK(S->flag.ensure(1));
+            launch_fill_int(S->flag.p, 1, 0, st);
+            CHK(S->icounts.ensure(nb));
+            CHK(S->incand.ensure(nb));
+            CHK(S->ieoff.ensure(nb + 1));
+            CHK(S->inits.ensure(nb));
+            CHK(S->iioff.ensure(nb + 1));
+            hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, S->isoff.p, S->ibound.p, S->isoff.p,
+                               S->ibound.p, nb, S->flag.p, S->tail.d);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(st));
+            const int total_cand = ((volatile int *)S->tail.h)[0];
+            CHK(S->iscratch.ensure(std::max(total_cand, 1)));
+            P.soff = S->isoff.p;
+            P.scap = S->ibound.p;
+            launch_search_images(P, nsc, ns, d_xyz, S->cellv.p, S->slot_atom.p, S->blo.p, S->bhi.p, S->iscratch.p,
+                                 S->incand.p, S->icounts.p, S->flag.p, st);
+            launch_scan(S->icounts.p, S->ieoff.p, nb, S->scantmp.p, st);
+            const int per_item = std::max(1, o.entriesPerItem > 0 ? o.entriesPerItem : 16);
+            launch_items_per_sc(S->icounts.p, nb, per_item, S->inits.p, st);
+            launch_scan(S->inits.p, S->iioff.p, nb, S->scantmp.p, st);
+            hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, S->ieoff.p, S->icounts.p, S->iioff.p,
+                               S->inits.p, nb, S->flag.p, S->tail.d);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(st));
+            if (((volatile int *)S->tail.h)[2] > 0)
+                return fail(NBABFS_STATUS_LOGIC_ERROR, "image search exceeded its candidate bound");
+            const int nent_img = ((volatile int *)S->tail.h)[0];
+            const int nit_img = ((volatile int *)S->tail.h)[1];
+            launch_add_int(S->ieoff.p, nb, S->nent_main, st);
+            launch_add_int(S->iioff.p, nb, S->nitems_main, st);
+            CHK(ensure_entries_keep(S, (size_t)S->nent_main + nent_img, (size_t)S->nent_main));
+            CHK(S->items.ensure((size_t)S->nitems_main + nit_img + 1));
+            CHK(S->inpairs.ensure(nb));
+            CHK(S->h_inpairs.ensure(nb));
+            launch_compact(S->iscratch.p, 0, S->isoff.p, S->incand.p, S->icounts.p, S->ieoff.p, S->iioff.p, nb, nsc,
+                           per_item, 0, S->entries.p, S->items.p, nullptr, S->inpairs.p, st);
+            HIPCHK(hipMemcpyAsync(S->h_inpairs.p, S->inpairs.p, nb * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(st));
+            for (int k = 0; k < ns; k++) {
+                long long np = 0;
+                for (int q = 0; q < nsc; q++) np += (long long)S->h_inpairs.p[(size_t)k * nsc + q];
+                S->slots[k].npairs = np;
+                if (np > 0) S->nimages++;
+                S->img_npairs += np;
+            }
+            S->nent = S->nent_main + nent_img;
+        }
+    }
+    std::memcpy(S->Mref, S->M, sizeof(S->Mref));
+    S->xf_valid = false;
+    CHK(recut_items(S, std::max(1, o.entriesPerItem > 0 ? o.entriesPerItem : 16)));
+    CHK(upload_image_xf(S));
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// Energy-pass transformations of the image slots for the current M
+// (MMMMImageEnergy: the fractional transformation plus (a, b, c),
+// orthogonalized), expressed in the cluster frame, and the dE/dM accumulators.
+static int upload_image_xf(nbabfs_state *S)
+{
+    const int ns = (int)S->slots.size();
+    if (ns == 0) {
+        S->xf_valid = true;
+        std::memcpy(S->Mxf, S->M, sizeof(S->Mxf));
+        return NBABFS_STATUS_CONTINUE;
+    }
+    CHK(S->h_imgxf.ensure(ns));
+    CHK(S->imgxf.ensure(ns));
+    CHK(S->gacc.ensure((size_t)NB_GACC_WORDS_PER_IMAGE * ns));
+    HIPCHK(hipStreamSynchronize(S->stream));   // the staging copy may still be read by an earlier upload
+    const double *og = S->grid.corg;
+    for (int k = 0; k < ns; k++) {
+        const auto &sl = S->slots[k];
+        const double xtr[3] = {S->ttra[3 * sl.t] + (double)sl.a, S->ttra[3 * sl.t + 1] + (double)sl.b,
+                               S->ttra[3 * sl.t + 2] + (double)sl.c};
+        double Ro[9], To[3];
+        orthogonalize(S->M, S->Minv, &S->trot[9 * sl.t], xtr, Ro, To);
+        ImageXf &X = S->h_imgxf.p[k];
+        std::memset(&X, 0, sizeof(X));
+        for (int q = 0; q < 9; q++) X.R[q] = Ro[q];
+        for (int r = 0; r < 3; r++) X.T[r] = (Ro[3 * r] * og[0] + Ro[3 * r + 1] * og[1] + Ro[3 * r + 2] * og[2]) + To[r] - og[r];
+        X.scale = sl.scale;
+    }
+    HIPCHK(hipMemcpyAsync(S->imgxf.p, S->h_imgxf.p, ns * sizeof(ImageXf), hipMemcpyHostToDevice, S->stream));
+    std::memcpy(S->Mxf, S->M, sizeof(S->Mxf));
+    S->xf_valid = true;
+    g_alloc_version++;   // descriptors pick up imgxf / gacc
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// CheckForImageUpdate (NBModelABFS.c:635-684): some image's lattice
+// translation (transformation translation plus (a, b, c), through M - M_ref)
+// moved by more than listCutoff - outerCutoff - the largest atom displacement.
+static bool check_for_image_update(const nbabfs_state *S, double maxdisp)
+{
+#pragma clang fp contract(off)
+    if (S->nimages <= 0) return false;
+    const double buffac = S->opt.listCutoff - S->opt.outerCutoff - maxdisp;
+    double dM[9];
+    for (int k = 0; k < 9; k++) dM[k] = S->M[k] + -1.0e+00 * S->Mref[k];
+    for (const auto &sl : S->slots) {
+        if (sl.npairs <= 0) continue;
+        double v[3] = {S->ttra[3 * sl.t] + (double)sl.a, S->ttra[3 * sl.t + 1] + (double)sl.b,
+                       S->ttra[3 * sl.t + 2] + (double)sl.c};
+        m33_apply(dM, v);
+        const double di = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (di > buffac) return true;
+    }
+    return false;
+}
+
 
 // Time the pair kernel on the current list for each (entriesPerItem,
 // wavesPerCU) candidate and keep the fastest.
@@ -1152,7 +1758,7 @@ extern "C" int nbabfs_energies(nbabfs_state *S, double *energies, int *present)
 {
     if (!S || !energies) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
     CHK(S->h_energies.ensure(4));
-    CHK(S->energies.ensure(4));
+    CHK(S->energies.ensure(NB_NENERGIES));
     HIPCHK(hipMemcpyAsync(S->h_energies.p, S->energies.p, 4 * sizeof(double), hipMemcpyDeviceToHost, S->stream));
     HIPCHK(hipStreamSynchronize(S->stream));
     std::memcpy(energies, S->h_energies.p, 4 * sizeof(double));
@@ -1218,7 +1824,185 @@ extern "C" int nbabfs_stats_get(nbabfs_state *S, nbabfs_stats *out)
     out->workItems = S->nitems;
     out->forceKernelMs = S->force_launches ? S->force_ms / (double)S->force_launches : 0.0;
     out->forceKernelLaunches = S->force_launches;
+    out->images = S->nimages;
+    out->imagePairs = S->img_npairs;
+    out->numberOfMMMMImageImages = S->sumImages;
+    out->numberOfMMMMImagePairs = S->sumImagePairs;
     return NBABFS_STATUS_CONTINUE;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Crystal symmetry.
+// ------------------------------------------------------------------------------------------------
+extern "C" int nbabfs_set_transformations(nbabfs_state *S, int ntransformations, const double *rotations,
+                                          const double *translations, int checkForInverses, int imageExpandFactor)
+{
+    if (!S) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state is NULL");
+    if (!S->isNew) return fail(NBABFS_STATUS_LOGIC_ERROR, "transformations must be set before the first update");
+    if (ntransformations < 0 || (ntransformations > 0 && (!rotations || !translations)) || imageExpandFactor < 0)
+        return fail(NBABFS_STATUS_INVALID_ARGUMENT, "invalid transformations");
+    if (ntransformations > 0 && S->opt.precision == NBABFS_PLAIN_FP64)
+        return fail(NBABFS_STATUS_INVALID_ARGUMENT, "periodic images are evaluated by the mixed-precision path only");
+    S->has_sym = ntransformations > 0;
+    S->ntrans = ntransformations;
+    S->trot.assign(rotations, rotations + 9 * (size_t)ntransformations);
+    S->ttra.assign(translations, translations + 3 * (size_t)ntransformations);
+    S->checkInv = checkForInverses ? 1 : 0;
+    S->expandFactor = imageExpandFactor;
+    // Transformation3Container_FindIdentity / _FindInverses (rotations only, 1e-6 tolerance).
+    static const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    S->identity = -1;
+    for (int t = 0; t < S->ntrans; t++) {
+        const double *tr = &S->ttra[3 * t];
+        if (m33_equal_tol(&S->trot[9 * t], I3) && std::fabs(tr[0]) <= 1.0e-6 && std::fabs(tr[1]) <= 1.0e-6 &&
+            std::fabs(tr[2]) <= 1.0e-6) {
+            S->identity = t;
+            break;
+        }
+    }
+    S->tinv.assign(S->ntrans, -1);
+    for (int i = 0; i < S->ntrans; i++) {
+        if (S->tinv[i] >= 0) continue;
+        double m[9];
+        m33_invert(&S->trot[9 * i], m);
+        for (int j = 0; j <= i; j++)
+            if (S->tinv[j] < 0 && m33_equal_tol(m, &S->trot[9 * j])) {
+                S->tinv[i] = j;
+                S->tinv[j] = i;
+                break;
+            }
+    }
+    S->sp_set = false;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_set_symmetry_parameters(nbabfs_state *S, const double *crystal)
+{
+    if (!S || !crystal) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
+    if (!S->has_sym) return fail(NBABFS_STATUS_LOGIC_ERROR, "the state has no transformations");
+    if (!(crystal[0] > 0.0 && crystal[1] > 0.0 && crystal[2] > 0.0) || !(crystal[3] > 0.0 && crystal[3] < 180.0) ||
+        !(crystal[4] > 0.0 && crystal[4] < 180.0) || !(crystal[5] > 0.0 && crystal[5] < 180.0))
+        return fail(NBABFS_STATUS_INVALID_ARGUMENT, "invalid crystal parameters");
+    for (int k = 0; k < 6; k++) S->sp[k] = crystal[k];
+    symmetry_make_m(S->sp, S->M, S->Minv);
+    if (!(std::isfinite(m33_det(S->M)) && m33_det(S->M) > 0.0))
+        return fail(NBABFS_STATUS_INVALID_ARGUMENT, "crystal parameters give a singular cell");
+    S->sp_set = true;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// Images of the current image list (slots with pairs, reference order).
+extern "C" int nbabfs_images(nbabfs_state *S, int cap, int *abct, double *scale, long long *npairs)
+{
+    if (!S) return -fail(NBABFS_STATUS_INVALID_ARGUMENT, "state is NULL");
+    int k = 0;
+    for (const auto &sl : S->slots) {
+        if (sl.npairs <= 0) continue;
+        if (k < cap) {
+            if (abct) {
+                abct[4 * k] = sl.a;
+                abct[4 * k + 1] = sl.b;
+                abct[4 * k + 2] = sl.c;
+                abct[4 * k + 3] = sl.t;
+            }
+            if (scale) scale[k] = sl.scale;
+            if (npairs) npairs[k] = sl.npairs;
+        }
+        k++;
+    }
+    return k;
+}
+
+// Image terms of the last energy call: the two energies and dE/dM (row-major,
+// incremented) assembled from the per-slot sums h = sum g'_j and
+// G_kl = sum g'_j,k x_j,l as SymmetryParameterGradients_ImageDerivatives does
+// (pMolecule-1.9.0/extensions/csource/SymmetryParameterGradients.c:142-216):
+// dE/dM_ab += sum_kl D_ab[k][l] G_kl + (t + n)_b h_a, with
+// D_ab = M S d(M^-1)/dM_ab + E_a row_b(S M^-1).
+extern "C" int nbabfs_image_terms(nbabfs_state *S, double *energies, double *dEdM, int *present)
+{
+    if (!S) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state is NULL");
+    hipStream_t st = S->stream;
+    const int ns = (int)S->slots.size();
+    CHK(S->h_energies.ensure(6));
+    CHK(S->energies.ensure(NB_NENERGIES));
+    HIPCHK(hipMemcpyAsync(S->h_energies.p, S->energies.p, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (dEdM && ns > 0) {
+        CHK(S->h_gacc.ensure((size_t)NB_GACC_WORDS_PER_IMAGE * ns));
+        HIPCHK(hipMemcpyAsync(S->h_gacc.p, S->gacc.p, (size_t)NB_GACC_WORDS_PER_IMAGE * ns * sizeof(long long),
+                              hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    if (energies) {
+        energies[0] = S->h_energies.p[4];
+        energies[1] = S->h_energies.p[5];
+    }
+    if (present) *present = S->nimages > 0;
+    if (!dEdM || ns == 0) return NBABFS_STATUS_CONTINUE;
+    const double *o = S->grid.corg;
+    for (int k = 0; k < ns; k++) {
+        const auto &sl = S->slots[k];
+        if (sl.npairs <= 0) continue;
+        double g[12];
+        for (int q = 0; q < 12; q++) {
+            const long long lo = S->h_gacc.p[(size_t)NB_GACC_WORDS_PER_IMAGE * k + 2 * q];
+            const long long hi = S->h_gacc.p[(size_t)NB_GACC_WORDS_PER_IMAGE * k + 2 * q + 1];
+            g[q] = (double)hi * NB_FX_HI_UNIT + (double)lo * (1.0 / NB_FX_LO_SCALE);
+        }
+        const double *h = g;
+        double G[9];
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) G[3 * a + b] = g[3 + 3 * a + b] + h[a] * o[b];
+        const double *rot = &S->trot[9 * sl.t];
+        const double xtr[3] = {S->ttra[3 * sl.t] + (double)sl.a, S->ttra[3 * sl.t + 1] + (double)sl.b,
+                               S->ttra[3 * sl.t + 2] + (double)sl.c};
+        double ms[9], si[9];
+        std::memcpy(ms, rot, sizeof(ms));
+        m33_premul(ms, S->M);
+        std::memcpy(si, rot, sizeof(si));
+        m33_postmul(si, S->Minv);
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                double di[9];
+                inverse_derivative(S->M, a, b, di);
+                m33_premul(di, ms);
+                for (int q = 0; q < 3; q++) di[3 * a + q] += si[3 * b + q];
+                double sum = 0.0;
+                for (int q = 0; q < 9; q++) sum += di[q] * G[q];
+                sum += xtr[b] * h[a];
+                dEdM[3 * a + b] += sum;
+            }
+    }
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// Pairs of image k of the image list as sorted keys i*n + j.
+extern "C" long long nbabfs_export_image_pairs(nbabfs_state *S, int image, long long *keys, long long cap)
+{
+    if (!S) return -fail(NBABFS_STATUS_INVALID_ARGUMENT, "state is NULL");
+    int slot = -1, k = 0;
+    for (int q = 0; q < (int)S->slots.size(); q++) {
+        if (S->slots[q].npairs <= 0) continue;
+        if (k == image) slot = q;
+        k++;
+    }
+    if (slot < 0) return -fail(NBABFS_STATUS_INVALID_ARGUMENT, "no image %d", image);
+    const long long m = S->slots[slot].npairs;
+    if (!keys) return m;
+    hipStream_t st = S->stream;
+    DBuf<long long> dk;
+    if (dk.ensure(std::max<long long>(m, 1)) != NBABFS_STATUS_CONTINUE) return -NBABFS_STATUS_OUT_OF_MEMORY;
+    if (S->u64.ensure(2) != NBABFS_STATUS_CONTINUE) return -NBABFS_STATUS_OUT_OF_MEMORY;
+    if (hipMemsetAsync(S->u64.p, 0, sizeof(unsigned long long), st) != hipSuccess) return -NBABFS_STATUS_LOGIC_ERROR;
+    launch_export_image_pairs(S->entries.p + S->nent_main, S->nent - S->nent_main, S->slot_atom.p, S->n, slot, S->u64.p,
+                              m, dk.p, st);
+    std::vector<long long> hk((size_t)m);
+    if (hipMemcpyAsync(hk.data(), dk.p, m * sizeof(long long), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return -fail(NBABFS_STATUS_LOGIC_ERROR, "image pair export failed");
+    std::sort(hk.begin(), hk.end());
+    for (long long q = 0; q < std::min(m, cap); q++) keys[q] = hk[q];
+    return m;
 }
 
 extern "C" long long nbabfs_export_pairs(nbabfs_state *S, int which, long long *keys, long long cap)
@@ -1237,7 +2021,7 @@ extern "C" long long nbabfs_export_pairs(nbabfs_state *S, int which, long long *
     if (dk.ensure(std::max<long long>(S->npairs, 1)) != NBABFS_STATUS_CONTINUE) return -NBABFS_STATUS_OUT_OF_MEMORY;
     if (S->u64.ensure(2) != NBABFS_STATUS_CONTINUE) return -NBABFS_STATUS_OUT_OF_MEMORY;
     if (hipMemsetAsync(S->u64.p, 0, sizeof(unsigned long long), st) != hipSuccess) return -NBABFS_STATUS_LOGIC_ERROR;
-    launch_export_pairs(S->entries.p, S->nent, S->slot_atom.p, S->n, S->u64.p, S->npairs, dk.p, st);
+    launch_export_pairs(S->entries.p, S->nent_main, S->slot_atom.p, S->n, S->u64.p, S->npairs, dk.p, st);
     std::vector<long long> hk((size_t)S->npairs);
     if (hipMemcpyAsync(hk.data(), dk.p, S->npairs * sizeof(long long), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -1387,6 +2171,10 @@ extern "C" int nbabfs_batch_create(nbabfs_state *const *states, int nsystems, in
             delete B;
             return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state %d is NULL", k);
         }
+        if (states[k]->has_sym) {
+            delete B;
+            return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state %d has crystal symmetry; batches take non-periodic states", k);
+        }
         B->st.push_back(states[k]);
     }
     B->mixed = states[0]->opt.precision == NBABFS_MIXED;
@@ -1430,7 +2218,7 @@ static int batch_layout(nbabfs_batch *B, const double *const *d_xyz, double *con
     CHK(B->items.ensure(std::max(total, 1)));
     CHK(B->d_desc.ensure(ns));
     CHK(B->h_desc.ensure(ns));
-    CHK(B->energies.ensure(4 * (size_t)ns));
+    CHK(B->energies.ensure(NB_NENERGIES * (size_t)ns));
     CHK(B->heads.ensure(8 * 32));
     int first = 0;
     for (int k = 0; k < ns; k++) {
@@ -1438,7 +2226,7 @@ static int batch_layout(nbabfs_batch *B, const double *const *d_xyz, double *con
         CHK(ensure_call_buffers(s));
         SysDesc &D = B->h_desc.p[k];
         fill_desc(s, D, d_xyz[k], d_grad ? d_grad[k] : nullptr);
-        D.energies = B->energies.p + 4 * (size_t)k;
+        D.energies = B->energies.p + NB_NENERGIES * (size_t)k;
         if (s->nitems > 0)
             hipLaunchKernelGGL(k_remap_items, dim3(nblk(s->nitems, 256)), dim3(256), 0, st, s->items.p, s->nitems, k,
                                B->items.p + first);
@@ -1485,7 +2273,8 @@ static int batch_enqueue(nbabfs_batch *B)
     }
     launch_pairs14(B->d_desc.p, ns, B->max_n14b, s0->F, st);
     launch_finalize(B->d_desc.p, ns, B->max_n, B->mixed ? 0 : 1, st);
-    HIPCHK(hipMemcpyAsync(B->h_energies.p, B->energies.p, 4 * (size_t)ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(B->h_energies.p, B->energies.p, NB_NENERGIES * (size_t)ns * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -1502,7 +2291,7 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
     CHK(B->d_check.ensure(ns));
     CHK(B->h_check.ensure(ns));
     CHK(B->flags.ensure(ns));
-    CHK(B->h_energies.ensure(4 * (size_t)ns));
+    CHK(B->h_energies.ensure(NB_NENERGIES * (size_t)ns));
     int maxn = 0;
     for (int k = 0; k < ns; k++) {
         nbabfs_state *s = B->st[k];
@@ -1561,6 +2350,7 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
         CHK(batch_enqueue(B));
     }
     HIPCHK(hipStreamSynchronize(st));
-    std::memcpy(energies, B->h_energies.p, 4 * (size_t)ns * sizeof(double));
+    for (int k = 0; k < ns; k++)
+        std::memcpy(energies + 4 * (size_t)k, B->h_energies.p + NB_NENERGIES * (size_t)k, 4 * sizeof(double));
     return NBABFS_STATUS_CONTINUE;
 }

@@ -11,10 +11,15 @@
 //   entries  one per (SC, j-cluster) pair: j-cluster index plus one byte per
 //            lane l = 8a+b whose bit c says that i = atom a of i-cluster c and
 //            j = atom b of the j-cluster are on the list.  The set bits are
-//            exactly the reference pair list.
+//            exactly the reference pair list.  Entries of the periodic image
+//            lists follow the main list's; their j-cluster is taken from an
+//            image (a crystal transformation plus a lattice translation).
 //   facc     six int64 arrays of n_pad (x,y,z low words, x,y,z high words):
 //            deterministic fixed-point gradient accumulators.
-//   eacc     fixed-point Coulomb and LJ energy accumulators, 64 slots each.
+//   eacc     fixed-point Coulomb and LJ energy accumulators of the main and
+//            the image lists, 64 slots each.
+//   gacc     per image, fixed-point sums of the image-side gradients g'_j
+//            (3) and of g'_j (x) (x_j - origin) (9), for dE/dM.
 //   vacc     fixed-point virial W_ab = sum_i (x_i - origin)_a g_i,b of the
 //            call's gradient, 9 components x 64 slots.
 #pragma once
@@ -43,9 +48,22 @@ struct Entry {              // 80 bytes
     int J;                  // j-cluster (local index)
     int sc;                 // owning super-cluster
     int cbits;              // bit c set if i-cluster c has any pair with J
-    int pad;
+    int img;                // -1: main list; else the image whose copy of J is paired
     unsigned char bits[NB_SC];  // per lane l = 8a+b: bit c = pair (c,a)-(J,b) listed
 };
+
+// Real-space transformation of one image for the pair kernel, in the cluster
+// frame: x' - origin = R (x - origin) + T.  scale multiplies the image's
+// energies and forces (1/2 for an image that is its own inverse, or for every
+// image when inverses are not used).
+struct ImageXf {            // 128 bytes
+    double R[9];
+    double T[3];
+    double scale;
+    double pad[3];
+};
+// gacc words per image: 12 components (g' sum x3, g' (x) x x9) x (low, high).
+#define NB_GACC_WORDS_PER_IMAGE 24
 
 struct WorkItem {           // 16 bytes
     int sys;                // system index in the batch
@@ -73,10 +91,12 @@ struct SysDesc {
     long long *facc;            // 6*n_pad
     double *gplain;             // n*3 plain-path gradients (or null)
     long long *eacc;            // NB_EACC_WORDS fixed-point energy accumulators (tiled path)
+    const ImageXf *imgxf;       // per image transformation (null: no images)
+    long long *gacc;            // NB_GACC_WORDS_PER_IMAGE per image
     long long *vacc;            // NB_VACC_WORDS fixed-point virial accumulators
     double *epart14;            // 2 per 1-4 block
     double *eplain;             // 2 per plain block
-    double *energies;           // 4 outputs (device)
+    double *energies;           // NB_NENERGIES outputs (device)
     double origin[3];           // FP64 frame origin of the cluster coordinates
     double escale;              // Coulomb factor / dielectric
     double escale14;            // times electrostaticScale14
@@ -84,6 +104,8 @@ struct SysDesc {
     int n, n_pad, ncl, ntypes, n14;
     int item_first, item_count; // range of this system's items in the batch
     int n_plain_blocks, n14_blocks;
+    int nimg;                   // images of the image table (gacc length / NB_GACC_WORDS_PER_IMAGE)
+    int pad_;
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -140,10 +162,15 @@ __device__ __forceinline__ double fx_value(const long long *facc, int n_pad, int
            (double)facc[(size_t)comp * n_pad + slot] * (1.0 / NB_FX_LO_SCALE);
 }
 
+// Energies of one system: MM/MM Elect., LJ, 1-4 Elect., 1-4 LJ, Image Elect., Image LJ.
+#define NB_NENERGIES 6
+
 // Energy accumulators: [kind][lo word, hi word][slot], kinds 0 = Coulomb and
-// 1 = LJ of the main list; a work item adds into slot (item % NB_ESLOTS).
+// 1 = LJ of the main list, 2 and 3 of the image lists; a work item adds into
+// slot (item % NB_ESLOTS).
 #define NB_ESLOTS 64
-#define NB_EACC_WORDS (2 * 2 * NB_ESLOTS)
+#define NB_EKINDS 4
+#define NB_EACC_WORDS (2 * NB_EKINDS * NB_ESLOTS)
 // Virial accumulators: [component ab = 3a+b][lo word, hi word][slot].
 #define NB_VACC_WORDS (9 * 2 * NB_ESLOTS)
 

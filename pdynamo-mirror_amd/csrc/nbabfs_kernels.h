@@ -27,6 +27,25 @@ struct SearchParams {
     float cw;
     int ncx, ncy;
     int cap;                              // candidate slots per super-cluster
+    // Image lists (cross search; blockIdx.y = image): the image's exact
+    // coordinates (n*3 FP64 each), its cluster boxes (ncl each, cluster frame)
+    // and its bounding box (6 floats, cluster frame, padded).
+    const double *ixyz;
+    const float4 *iblo, *ibhi;
+    const float *ibox;
+    const unsigned char *ongrid;          // i atoms on the reference grid (null: direct method)
+    const int *soff;                      // scratch offset and capacity of each (image, super-cluster)
+    const int *scap;
+    int n, ncl;
+};
+
+// One step of a transformation's (a, b, c) sequence in GenerateImageLists:
+// the coordinates are displaced by d, stored as image `out` when that image is
+// searched (out >= 0), and displaced back.
+struct ImageStep {
+    double d[3];
+    int out;
+    int pad;
 };
 
 // List construction (nbabfs_build.hip).
@@ -44,14 +63,32 @@ void launch_cluster_geom(const double *xyz, const int *slot_atom, int ncl, const
 void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4 *cellv, const int *slot_atom,
                    const float4 *blo, const float4 *bhi, const int2 *colcl, Entry *scratch, int *ncand, int *counts,
                    int *overflow, hipStream_t s);
-void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int nsc, int per_item, int sys,
-                      WorkItem *items, hipStream_t s);
+void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int nblocks, int nsc, int per_item,
+                      int sys, WorkItem *items, hipStream_t s);
+// Image lists: R'x + T' per transformation (12 doubles each), the stepped image
+// coordinates, the on-grid flags of the current coordinates, the cross search.
+void launch_image_transform(const double *xyz, int n, const double *rt, int ntrans, double *out, hipStream_t s);
+void launch_image_coords(const double *tcoords, int n, int ntrans, const int *step_off, const ImageStep *steps,
+                         double *out, hipStream_t s);
+void launch_ongrid(const double *xyz, int n, const BuildGrid &g, unsigned char *flags, hipStream_t s);
+void launch_search_images(const SearchParams &P, int nsc, int nimg, const double *xyz, const int4 *cellv,
+                          const int *slot_atom, const float4 *blo, const float4 *bhi, Entry *scratch, int *ncand,
+                          int *counts, int *overflow, hipStream_t s);
+// Upper bound of each (image, super-cluster) block's candidates (its scratch capacity).
+void launch_image_bounds(const SearchParams &P, int nsc, int nimg, const float4 *blo, const float4 *bhi, int *bound,
+                         hipStream_t s);
+void launch_add_int(int *p, int m, int v, hipStream_t s);
+void launch_export_image_pairs(const Entry *entries, int nent, const int *slot_atom, long long n, int img,
+                               unsigned long long *cursor, long long cap, long long *keys, hipStream_t s);
 void launch_order_items(const WorkItem *items, int nitems, const Entry *entries, int maxcost, int interleave, int *cost,
                         int *hist, int *cursor, int *scantmp, WorkItem *out, hipStream_t s);
 void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, hipStream_t s);
-void launch_compact(const Entry *scratch, int cap, const int *ncand, const int *counts, const int *eoff,
-                    const int *ioff, int nsc, int per_item, int sys, Entry *entries, WorkItem *items,
-                    unsigned long long *npairs, hipStream_t s);
+// nblocks compaction blocks (super-cluster = block % nsc); pairs are added
+// into *npairs, or stored per block into npairs_block when that is given.
+// soff (or null: block * cap) gives each block's scratch offset.
+void launch_compact(const Entry *scratch, int cap, const int *soff, const int *ncand, const int *counts,
+                    const int *eoff, const int *ioff, int nblocks, int nsc, int per_item, int sys, Entry *entries,
+                    WorkItem *items, unsigned long long *npairs, unsigned long long *npairs_block, hipStream_t s);
 void launch_export_pairs(const Entry *entries, int nent, const int *slot_atom, long long n,
                          unsigned long long *cursor, long long cap, long long *keys, hipStream_t s);
 void launch_csr_degree(const Entry *entries, int nent, const int *slot_atom, int *deg, hipStream_t s);

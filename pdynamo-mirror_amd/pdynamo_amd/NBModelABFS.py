@@ -89,7 +89,9 @@ class NBModelABFSState:
         self.native = native
         self.natoms = mmAtoms.size
         self.emmel = self.emmlj = self.emmel14 = self.emmlj14 = 0.0
+        self.eimmmel = self.eimmmlj = 0.0
         self.present = (False, False)
+        self.imagesPresent = False
         self.lastRebuilt = False
 
     def GetEnergies(self, energies):
@@ -100,6 +102,17 @@ class NBModelABFSState:
         if self.present[1]:
             energies.append(("MM/MM 1-4 Elect.", self.emmel14))
             energies.append(("MM/MM 1-4 LJ", self.emmlj14))
+        if self.imagesPresent:
+            energies.append(("MM/MM Image Elect.", self.eimmmel))
+            energies.append(("MM/MM Image LJ", self.eimmmlj))
+
+    def Images(self):
+        """[(a, b, c, transformation, scale, pairs)] of the MM/MM image list."""
+        return self.native.images()
+
+    def ImagePairs(self, image):
+        """Pairs of one image as sorted keys i * n + j (i primary, j in the image)."""
+        return self.native.export_image_pairs(image)
 
     def Statistics(self):
         return self.native.stats()
@@ -113,6 +126,9 @@ class NBModelABFSState:
                 s.Entry("MM/MM Pairs", "{:d}".format(st["npairs"]))
             if st["npairs14"] > 0:
                 s.Entry("MM/MM 1-4 Pairs", "{:d}".format(st["npairs14"]))
+            if st["images"] > 0:
+                s.Entry("MM/MM Image Images", "{:d}".format(st["images"]))
+                s.Entry("MM/MM Image Pairs", "{:d}".format(st["imagePairs"]))
             s.Stop()
 
     def StatisticsSummary(self, log=logFile):
@@ -128,6 +144,10 @@ class NBModelABFSState:
                     s.Entry("<MM/MM Pairs>", "{:.1f}".format(st["numberOfMMMMPairs"] / n))
                 if st["npairs14"] > 0:
                     s.Entry("MM/MM 1-4 Pairs", "{:d}".format(st["npairs14"]))
+                if st["numberOfMMMMImageImages"] != 0.0:
+                    s.Entry("<MM/MM Image Images>", "{:.1f}".format(st["numberOfMMMMImageImages"] / n))
+                if st["numberOfMMMMImagePairs"] != 0.0:
+                    s.Entry("<MM/MM Image Pairs>", "{:.1f}".format(st["numberOfMMMMImagePairs"] / n))
                 s.Entry("Calls per Update", "{:.1f}".format(float(st["numberOfCalls"]) / n))
                 s.Entry("Grid Updating", "{!r}".format(bool(st["useGridSearch"])))
             s.Stop()
@@ -275,8 +295,9 @@ class NBModelABFS:
             return
         if qcAtoms is not None and getattr(qcAtoms, "size", 0) > 0:
             raise NotImplementedError("QC/MM interactions are not part of this implementation")
-        if symmetry is not None:
-            raise NotImplementedError("periodic images are not part of this implementation")
+        if self.useCentering and symmetry is not None:
+            raise NotImplementedError("useCentering is not part of this implementation")
+        transformations = None if symmetry is None else getattr(symmetry, "transformations", None)
         if not hasattr(configuration, "nbState"):
             if ljParameters is None:
                 raise CLibraryError("Unable to create NB state: LJ parameters are required.")
@@ -289,13 +310,25 @@ class NBModelABFS:
                                          ljParameters.tableA, ljParameters.tableB, lj14.tableA, lj14.tableB,
                                          None if exclusions is None else exclusions.pairs,
                                          None if interactions14 is None else interactions14.pairs, fixed)
+            if transformations is not None:
+                rot, tra = transformations.Arrays()
+                native.set_transformations(rot, tra, self.checkForInverses, self.imageExpandFactor)
             configuration.nbState = NBModelABFSState(native, mmAtoms)
         nbState = configuration.nbState
         xyz = configuration.coordinates3
         grad = getattr(configuration, "gradients3", None)
+        if transformations is not None:
+            sp = getattr(configuration, "symmetryParameters", None)
+            if sp is None:
+                raise CLibraryError("Symmetry without symmetry parameters.")
+            nbState.native.set_symmetry_parameters(sp.AsArray())
         e, present, rebuilt = nbState.native.energy_host(xyz, grad)
         nbState.emmel, nbState.emmlj, nbState.emmel14, nbState.emmlj14 = (float(x) for x in e)
         nbState.present = (bool(present[0]), bool(present[1]))
+        if transformations is not None:
+            spg = getattr(configuration, "symmetryParameterGradients", None)
+            ie, nbState.imagesPresent = nbState.native.image_terms(None if spg is None else spg.dEdM)
+            nbState.eimmmel, nbState.eimmmlj = float(ie[0]), float(ie[1])
         nbState.lastRebuilt = rebuilt
         if rebuilt:
             nbState.Summary(log=log)

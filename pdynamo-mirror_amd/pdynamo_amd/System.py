@@ -175,6 +175,35 @@ class System:
     def nbModel(self):
         return None if self.energyModel is None else self.energyModel.nbModel
 
+    @property
+    def symmetryParameters(self):
+        return getattr(self.configuration, "symmetryParameters", None)
+
+    @symmetryParameters.setter
+    def symmetryParameters(self, value):
+        self.configuration.symmetryParameters = value
+
+    def DefineSymmetry(self, a=None, alpha=None, b=None, beta=None, c=None, crystalClass=None, gamma=None,
+                       transformations=None):
+        """Define crystal symmetry (pMolecule-1.9.0/pMolecule/System.py:243-258): the crystal class and
+        transformations (the identity alone by default) plus the cell's symmetry parameters.  The model
+        state of an earlier definition is cleared."""
+        from .Symmetry import CrystalClass, Symmetry, Transformation3Container, Transformation3Container_Identity
+        if self.energyModel is not None and self.energyModel.nbModel is not None:
+            self.energyModel.nbModel.Clear(self.configuration)
+        self.symmetry = None
+        self.symmetryParameters = None
+        if crystalClass is not None:
+            if not isinstance(crystalClass, CrystalClass):
+                raise TypeError("Invalid crystal class.")
+            if transformations is None:
+                transformations = Transformation3Container_Identity()
+            elif not isinstance(transformations, Transformation3Container):
+                raise TypeError("Invalid symmetry transformations.")
+            self.symmetry = Symmetry(crystalClass, transformations)
+            self.symmetryParameters = crystalClass.CreateSymmetryParameters(a=a, b=b, c=c, alpha=alpha, beta=beta,
+                                                                           gamma=gamma)
+
     def DefineNBModel(self, nbModel):
         """Define a non-bonding model (System.py:182-197)."""
         if isinstance(nbModel, NBModelABFS):
@@ -199,6 +228,9 @@ class System:
         cfg.SetTemporaryAttribute("energyTerms", EnergyTerms())
         if doGradients:
             cfg.SetTemporaryAttribute("gradients3", np.zeros((self.natoms, 3)))
+            if self.symmetry is not None:
+                from .Symmetry import SymmetryParameterGradients
+                cfg.SetTemporaryAttribute("symmetryParameterGradients", SymmetryParameterGradients())
         em = self.energyModel
         et = cfg.energyTerms
         if em.nbModel is not None:

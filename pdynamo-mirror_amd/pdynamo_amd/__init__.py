@@ -4,7 +4,10 @@ pair list, rebuilt for AMD Instinct MI355X (gfx950).
 Drop-in names: ``System``, ``NBModelABFS``, ``NBModelABFSState``,
 ``PairListGenerator``, ``LJParameterContainer`` (+ ``_FromEpsilonSigma`` /
 ``_FromTableCoefficients``), ``MMAtomContainer``, ``SelfPairList``,
-``EnergyTerms``.  Batches of images / replicas: ``BatchEnergy``.
+``EnergyTerms``; crystal symmetry for the periodic image lists:
+``System.DefineSymmetry`` with the ``CrystalClass*`` classes,
+``Transformation3Container`` and ``Transformation3_FromSymmetryOperationString``.
+Batches of images / replicas: ``BatchEnergy``.
 """
 
 from ._native import CLibraryError
@@ -13,10 +16,18 @@ from .NBModelABFS import NBModelABFS, NBModelABFSState, PairListGenerator
 from .Parameters import (LJParameterContainer, LJParameterContainer_FromEpsilonSigma,
                          LJParameterContainer_FromTableCoefficients, MMAtomContainer, SelfPairList,
                          SelfPairList_FromIntegerPairs, UNITS_ENERGY_E2ANGSTROMS_TO_KILOJOULES_PER_MOLE)
+from .Symmetry import (CrystalClassCubic, CrystalClassHexagonal, CrystalClassMonoclinic, CrystalClassOrthorhombic,
+                       CrystalClassRhombohedral, CrystalClassTetragonal, CrystalClassTriclinic, Symmetry,
+                       SymmetryParameterGradients, SymmetryParameters, Transformation3, Transformation3Container,
+                       Transformation3Container_Identity, Transformation3_FromSymmetryOperationString)
 from .System import Configuration, EnergyTerms, System
 
 __all__ = ["CLibraryError", "LogFileActive", "TextLogFile", "logFile", "NBModelABFS", "NBModelABFSState",
            "PairListGenerator", "LJParameterContainer", "LJParameterContainer_FromEpsilonSigma",
            "LJParameterContainer_FromTableCoefficients", "MMAtomContainer", "SelfPairList",
            "SelfPairList_FromIntegerPairs", "UNITS_ENERGY_E2ANGSTROMS_TO_KILOJOULES_PER_MOLE", "Configuration",
-           "EnergyTerms", "System"]
+           "EnergyTerms", "System", "CrystalClassCubic", "CrystalClassHexagonal", "CrystalClassMonoclinic",
+           "CrystalClassOrthorhombic", "CrystalClassRhombohedral", "CrystalClassTetragonal", "CrystalClassTriclinic",
+           "Symmetry", "SymmetryParameterGradients", "SymmetryParameters", "Transformation3",
+           "Transformation3Container", "Transformation3Container_Identity",
+           "Transformation3_FromSymmetryOperationString"]

@@ -22,7 +22,8 @@ EXPORTED = ["nbabfs_default_options", "nbabfs_init", "nbabfs_last_error", "nbabf
             "nbabfs_set_timing", "nbabfs_export_pairs", "nbabfs_batch_create", "nbabfs_batch_destroy",
             "nbabfs_batch_energy", "nbabfs_device_alloc", "nbabfs_device_free", "nbabfs_memcpy_htod",
             "nbabfs_memcpy_dtoh", "nbabfs_zero_async", "nbabfs_synchronize", "nbabfs_autotune",
-            "nbabfs_virial"]
+            "nbabfs_virial", "nbabfs_set_transformations", "nbabfs_set_symmetry_parameters", "nbabfs_image_terms",
+            "nbabfs_images", "nbabfs_export_image_pairs"]
 
 
 class CLibraryError(Exception):
@@ -41,7 +42,9 @@ class Stats(C.Structure):
     _fields_ = [("numberOfCalls", C.c_longlong), ("numberOfUpdates", C.c_longlong),
                 ("numberOfMMMMPairs", C.c_double), ("npairs", C.c_longlong), ("npairs14", C.c_longlong),
                 ("useGridSearch", C.c_int), ("clusters", C.c_int), ("entries", C.c_longlong),
-                ("workItems", C.c_longlong), ("forceKernelMs", C.c_double), ("forceKernelLaunches", C.c_longlong)]
+                ("workItems", C.c_longlong), ("forceKernelMs", C.c_double), ("forceKernelLaunches", C.c_longlong),
+                ("images", C.c_int), ("reserved2", C.c_int), ("imagePairs", C.c_longlong),
+                ("numberOfMMMMImageImages", C.c_double), ("numberOfMMMMImagePairs", C.c_double)]
 
 
 _lib = None
@@ -90,6 +93,12 @@ def load():
     L.nbabfs_synchronize.argtypes = [vp]
     L.nbabfs_autotune.argtypes = [vp, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.nbabfs_virial.argtypes = [vp, vp]
+    L.nbabfs_set_transformations.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_int]
+    L.nbabfs_set_symmetry_parameters.argtypes = [vp, vp]
+    L.nbabfs_image_terms.argtypes = [vp, vp, vp, vp]
+    L.nbabfs_images.argtypes = [vp, C.c_int, vp, vp, vp]
+    L.nbabfs_export_image_pairs.argtypes = [vp, C.c_int, vp, C.c_longlong]
+    L.nbabfs_export_image_pairs.restype = C.c_longlong
     _lib = L
     return L
 
@@ -211,6 +220,58 @@ class NativeState:
         present = np.zeros(2, np.int32)
         check(load().nbabfs_energies(self.h, _ptr(e), _ptr(present)))
         return e, present
+
+    def set_transformations(self, rotations, translations, check_for_inverses=True, image_expand_factor=0):
+        """Fractional transformations (k, 3, 3) / (k, 3) of the crystal symmetry."""
+        rot = np.ascontiguousarray(rotations, dtype=np.float64).reshape(-1, 9)
+        tra = np.ascontiguousarray(translations, dtype=np.float64).reshape(-1, 3)
+        if rot.shape[0] != tra.shape[0]:
+            raise ValueError("rotations and translations differ in number")
+        self._sym = (rot, tra)
+        check(load().nbabfs_set_transformations(self.h, rot.shape[0], _ptr(rot), _ptr(tra),
+                                                1 if check_for_inverses else 0, int(image_expand_factor)))
+
+    def set_symmetry_parameters(self, crystal):
+        """Crystal parameters a, b, c, alpha, beta, gamma of the next calls."""
+        c = np.ascontiguousarray(crystal, dtype=np.float64).reshape(6)
+        check(load().nbabfs_set_symmetry_parameters(self.h, _ptr(c)))
+
+    def image_terms(self, dEdM=None):
+        """(energies (Image Elect., Image LJ), present) of the last energy call; dEdM (3x3) is incremented."""
+        e = np.zeros(2)
+        present = C.c_int()
+        dm = None
+        if dEdM is not None:
+            dm = np.ascontiguousarray(dEdM, dtype=np.float64).reshape(9).copy()
+        check(load().nbabfs_image_terms(self.h, _ptr(e), _ptr(dm), C.byref(present)))
+        if dEdM is not None:
+            dEdM[...] = dm.reshape(dEdM.shape)
+        return e, bool(present.value)
+
+    def images(self):
+        """[(a, b, c, t, scale, npairs)] of the image list, in list order."""
+        L = load()
+        m = L.nbabfs_images(self.h, 0, None, None, None)
+        if m < 0:
+            check(int(-m))
+        abct = np.zeros(4 * max(m, 1), np.int32)
+        sc = np.zeros(max(m, 1))
+        npr = np.zeros(max(m, 1), np.int64)
+        L.nbabfs_images(self.h, m, _ptr(abct), _ptr(sc), _ptr(npr))
+        return [(int(abct[4 * k]), int(abct[4 * k + 1]), int(abct[4 * k + 2]), int(abct[4 * k + 3]), float(sc[k]),
+                 int(npr[k])) for k in range(m)]
+
+    def export_image_pairs(self, image):
+        """Sorted keys i * n + j of image `image` (i primary, j image)."""
+        L = load()
+        m = L.nbabfs_export_image_pairs(self.h, int(image), None, 0)
+        if m < 0:
+            check(int(-m))
+        keys = np.zeros(max(m, 1), np.int64)
+        m2 = L.nbabfs_export_image_pairs(self.h, int(image), _ptr(keys), m)
+        if m2 < 0:
+            check(int(-m2))
+        return keys[:m]
 
     def set_stream(self, stream_handle):
         check(load().nbabfs_set_stream(self.h, None if not stream_handle else C.c_void_p(stream_handle)))
