@@ -41,6 +41,27 @@ def algorithmic_flops(xyz, keys, n):
     return 8.0 * n_list + 37.0 * n_plain + 63.0 * n_switch, n_list, n_plain, n_switch
 
 
+def image_flops(xyz, st, n, transformations, cell):
+    """algorithmic_flops over the image pairs (i primary, j image) of the state's image list."""
+    from pdynamo_amd.Symmetry import SymmetryParameters
+    sp = SymmetryParameters(*cell)
+    rot, tra = transformations
+    tot = [0.0, 0, 0, 0]
+    for q, (a, b, c, t, scale, npr) in enumerate(st.images()):
+        keys = st.export_image_pairs(q)
+        R = sp.M @ np.asarray(rot[t]).reshape(3, 3) @ sp.inverseM
+        T = sp.M @ (np.asarray(tra[t]) + np.array([a, b, c], np.float64))
+        d = xyz[keys // n] - (xyz[keys % n] @ R.T + T)
+        r2 = np.einsum("ij,ij->i", d, d)
+        n_plain = int(np.count_nonzero(r2 <= 64.0))
+        n_switch = int(np.count_nonzero((r2 > 64.0) & (r2 <= 144.0)))
+        tot[0] += 8.0 * keys.size + 37.0 * n_plain + 63.0 * n_switch
+        tot[1] += int(keys.size)
+        tot[2] += n_plain
+        tot[3] += n_switch
+    return tuple(tot)
+
+
 def copy_calibration(mib):
     """Device-to-device copies of a known size (the runtime's copy kernel), so that a
     profiler's FETCH_SIZE / WRITE_SIZE of the pair kernel can be scaled by what the same
@@ -93,6 +114,13 @@ def make_workload(name):
     if name == "dhfr":
         mm, extra = load_dhfr()
         return mm, "DHFR CHARMM22 23,558 atoms (reference benchmark system, primary + 1-4 lists, no images)", extra
+    if name == "dhfr_periodic":
+        mm, extra = load_dhfr()
+        with np.load(os.path.join(ROOT, "tests", "golden", "dhfr_nb.npz"), allow_pickle=False) as z:
+            cell = tuple(float(v) for v in z["cell"])
+        extra["symmetry"] = ((np.eye(3)[None], np.zeros((1, 3))), cell)
+        return mm, ("DHFR CHARMM22 23,558 atoms in its cubic cell a = 62.23 A (the reference's benchmark "
+                    "configuration: primary, 1-4 and 13 P1 image lists)"), extra
     raise SystemExit("unknown workload " + name)
 
 
@@ -175,21 +203,32 @@ def cpu_baseline(mm, calls, threads, extra=None):
     """The reference pDynamo C (oracle/_ref, -O3 -fopenmp) or, if absent, the
     FP64 C restatement, timed for `calls` E+grad calls after the first build."""
     import oracle as O
+    extra = dict(extra or {})
+    sym = extra.pop("symmetry", None)
     kind = "reference" if O.ref_available() else "port"
+    if sym is not None:
+        extra["transformations"] = sym[0]
     if kind == "reference":
-        eng = O.ReferenceABFS(mm, nthreads=threads, **(extra or {}))
+        eng = O.ReferenceABFS(mm, nthreads=threads, **extra)
     else:
-        eng = O.OracleABFS(mm, nthreads=threads, **(extra or {}))
-    eng.energy(mm.coordinates3)
+        eng = O.OracleABFS(mm, nthreads=threads, **extra)
+
+    def call():
+        if sym is None:
+            eng.energy(mm.coordinates3)
+        else:
+            eng.energy_sym(mm.coordinates3, sym[1])
+
+    call()
     t0 = time.perf_counter()
     for _ in range(calls):
-        eng.energy(mm.coordinates3)
+        call()
     dt = (time.perf_counter() - t0) / calls
     if kind == "reference":
-        npairs = eng.canonical(0).size
+        npairs = eng.canonical(0).size + (sum(x[5] for x in eng.images()) if sym is not None else 0)
         eng.close()
     else:
-        npairs = eng.records.npairs
+        npairs = eng.records.npairs + (eng.images.npairs if sym is not None else 0)
     return {"value": npairs / dt, "unit": "pair interactions/s", "cores": threads, "kind": kind,
             "sample": "%d E+grad calls on the same workload after the first list build (%.1f ms each, %d list pairs)"
                       % (calls, dt * 1e3, npairs)}
@@ -204,6 +243,7 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--entries-per-item", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=-1)
+    ap.add_argument("--image-entries-per-item", type=int, default=0)
     ap.add_argument("--cpu-calls", type=int, default=100)
     ap.add_argument("--moves", type=float, default=0.0,
                     help="Gaussian displacement (A) applied before every call, cumulative; rebuilds follow the "
@@ -237,8 +277,16 @@ def main():
     if args.waves_per_cu >= 0:
         opts["wavesPerCU"] = args.waves_per_cu
     model = NBModelABFS(**opts)
-    st = _native.NativeState(model.NativeOptions(), mm.charges, mm.ljtypes, mm.ntypes, mm.tableA, mm.tableB,
-                             exclusions=mm.exclusions, interactions14=mm.interactions14, **extra)
+    sym = extra.get("symmetry")
+    nopt = model.NativeOptions()
+    if args.image_entries_per_item > 0:
+        nopt.imageEntriesPerItem = args.image_entries_per_item
+    st = _native.NativeState(nopt, mm.charges, mm.ljtypes, mm.ntypes, mm.tableA, mm.tableB,
+                             exclusions=mm.exclusions, interactions14=mm.interactions14,
+                             **{k: v for k, v in extra.items() if k != "symmetry"})
+    if sym is not None:
+        st.set_transformations(sym[0][0], sym[0][1])
+        st.set_symmetry_parameters(sym[1])
     g = _native.DeviceArray(mm.coordinates3.shape)
     if args.moves > 0.0:
         from pdynamo_amd import synthetic
@@ -282,9 +330,13 @@ def main():
     stats = st.stats()
     st.set_timing(False)
     e, present = st.energies()
+    e = list(e)
+    if sym is not None:
+        ie, _ = st.image_terms()
+        e += list(ie)
     keys = st.export_pairs(0)
     x_last = xs[(counter[0] - 1) % len(xs)].numpy(st)
-    n_list = int(keys.size)
+    n_list = int(keys.size) + (int(stats["imagePairs"]) if sym is not None else 0)
     value = n_list / (ms * 1e-3)
 
     result = {
@@ -307,8 +359,14 @@ def main():
                    "cluster_entries": int(stats["entries"]), "moves_sigma_A": args.moves,
                    "rebuilds_in_timed_steps": int(timed_rebuilds), "autotuned": tuned},
     }
+    if sym is not None:
+        result["config"]["images"] = int(stats["images"])
+        result["config"]["image_pairs"] = int(stats["imagePairs"])
     if not args.no_roofline:
         flops, n_list2, n_plain, n_switch = algorithmic_flops(x_last, keys, mm.natoms)
+        if sym is not None:
+            f2, _, p2, s2 = image_flops(x_last, st, mm.natoms, sym[0], sym[1])
+            flops, n_plain, n_switch = flops + f2, n_plain + p2, n_switch + s2
         kms = stats["forceKernelMs"]
         achieved = flops / (kms * 1e-3) / 1e12 if kms > 0 else None
         peak = PEAK_FP32_TFLOPS if args.precision == "mixed" else PEAK_FP64_TFLOPS

@@ -64,7 +64,7 @@ typedef struct {
     int    entriesPerItem;         /* cluster-pair entries per work item (0 = 16, at most
                                       NBABFS_MAX_ENTRIES_PER_ITEM) */
     int    wavesPerCU;             /* persistent-kernel waves per CU; 0 = one workgroup per work item */
-    int    reserved;
+    int    imageEntriesPerItem;    /* entries per work item of the image lists (0 = 16) */
 } nbabfs_options;
 
 typedef struct {

@@ -602,6 +602,7 @@ __global__ __launch_bounds__(64) void k_compact(const Entry *__restrict__ scratc
     for (int k = lane; k < nit; k += 64) {
         WorkItem wi;
         wi.sys = sys;
+        if (soff) wi.sys |= (sc / nsc + 1) << 16;   // image lists: image of the block, plus one
         wi.sc = sc % nsc;
         wi.first = e0 + k * per_item;
         wi.count = min(per_item, cnt - k * per_item);
@@ -647,16 +648,18 @@ __global__ __launch_bounds__(256) void k_item_scatter(const WorkItem *__restrict
 }
 
 // Work items of an already compacted list for another entries-per-item cut.
+// images != 0: the blocks are (image, super-cluster) blocks of the image
+// lists, and each item's sys field carries (image + 1) << 16.
 __global__ __launch_bounds__(256) void k_cut_items(const int *__restrict__ counts, const int *__restrict__ eoff,
                                                    const int *__restrict__ ioff, int nblocks, int nsc, int per_item,
-                                                   int sys, WorkItem *__restrict__ items)
+                                                   int sys, int images, WorkItem *__restrict__ items)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= nblocks) return;
     const int cnt = counts[s], e0 = eoff[s], i0 = ioff[s];
     for (int k = 0; k * per_item < cnt; k++) {
         WorkItem wi;
-        wi.sys = sys;
+        wi.sys = images ? (sys | ((s / nsc + 1) << 16)) : sys;
         wi.sc = s % nsc;
         wi.first = e0 + k * per_item;
         wi.count = min(per_item, cnt - k * per_item);
@@ -992,11 +995,11 @@ void launch_search_images(const SearchParams &P, int nsc, int nimg, const double
 }
 
 void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int nblocks, int nsc, int per_item,
-                      int sys, WorkItem *items, hipStream_t s)
+                      int sys, WorkItem *items, hipStream_t s, bool images)
 {
     if (nblocks > 0)
         hipLaunchKernelGGL(k_cut_items, dim3(nblk(nblocks, 256)), dim3(256), 0, s, counts, eoff, ioff, nblocks, nsc,
-                           per_item, sys, items);
+                           per_item, sys, images ? 1 : 0, items);
 }
 
 void launch_image_transform(const double *xyz, int n, const double *rt, int ntrans, double *out, hipStream_t s)

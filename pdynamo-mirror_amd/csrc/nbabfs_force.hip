@@ -129,8 +129,9 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
         for (int k = threadIdx.x; k < NB_EACC_WORDS; k += 256) D.eacc[k] = 0;
     if (blockIdx.x == 0 && D.vacc)
         for (int k = threadIdx.x; k < NB_VACC_WORDS; k += 256) D.vacc[k] = 0;
-    if (blockIdx.x == 0 && D.gacc)
-        for (int k = threadIdx.x; k < NB_GACC_WORDS_PER_IMAGE * D.nimg; k += 256) D.gacc[k] = 0;
+    if (D.gacc)
+        for (int k = blockIdx.x * 256 + threadIdx.x; k < NB_GACC_WORDS_PER_IMAGE * D.nimg; k += gridDim.x * 256)
+            D.gacc[k] = 0;
     bool up = false;
     if (s < D.n_pad) {
         for (int k = 0; k < 6; k++) D.facc[(size_t)k * D.n_pad + s] = 0;
@@ -184,12 +185,23 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
     }
 }
 
-// Cluster-frame image position x' = R x + T of a slot record (charge kept).
-__device__ __forceinline__ double4 image_slot(double4 v, const double *R, const double *T)
+// A wave-uniform FP64 value moved into scalar registers.
+__device__ __forceinline__ double uniform_f64(double v)
 {
-    return make_double4(fma(R[0], v.x, fma(R[1], v.y, fma(R[2], v.z, T[0]))),
-                        fma(R[3], v.x, fma(R[4], v.y, fma(R[5], v.z, T[1]))),
-                        fma(R[6], v.x, fma(R[7], v.y, fma(R[8], v.z, T[2]))), v.w);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Cluster-frame image position x' = R x + T of a slot record (charge kept);
+// X = R (9, row-major) then T (3), read from LDS at each use (broadcast reads)
+// so that the transformation holds no registers across the entry loop.
+__device__ __forceinline__ double4 image_slot(double4 v, const volatile double *X)
+{
+    return make_double4(fma(X[0], v.x, fma(X[1], v.y, fma(X[2], v.z, X[9]))),
+                        fma(X[3], v.x, fma(X[4], v.y, fma(X[5], v.z, X[10]))),
+                        fma(X[6], v.x, fma(X[7], v.y, fma(X[8], v.z, X[11]))), v.w);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -257,8 +269,11 @@ static inline size_t tiled_lds_bytes(int ntypes, bool rows_lds, int max_count)
 
 // Four waves per SIMD: the register allocator fits the kernel in 128 VGPRs
 // without spilling (137 unconstrained, three waves), measured 2-3% faster.
-template <bool ROWS_LDS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_force_tiled(const SysDesc *__restrict__ descs,
+// IMG: the launch over the image lists' work items (the main list's items run
+// in an instance without any image code); it runs at three waves per SIMD,
+// whose register budget holds the transformation without spilling.
+template <bool ROWS_LDS, bool IMG>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4))) void k_force_tiled(const SysDesc *__restrict__ descs,
                                                     const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F,
                                                     int lds_head, ForceGuard guard)
 {
@@ -269,6 +284,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int a = lane & 7, b = lane >> 3;
     const int bl = 8 * a + b;
     extern __shared__ __align__(32) unsigned char lds[];
+    __shared__ __align__(16) double sX[12];     // image items: R then T of the item's image
     double4 *sI = (double4 *)lds;
     int *sT = (int *)(lds + NB_SC * 32);
     double4 *sR = (double4 *)(lds + NB_LDS_I_BYTES);
@@ -297,7 +313,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             it = qs + local;
         }
         const WorkItem w = items[it];
-        const SysDesc *D = &descs[w.sys];
+        const SysDesc *D = &descs[IMG ? (w.sys & 0xffff) : w.sys];
         const GAS double *__restrict__ xq = (const GAS double *)D->xyzq;
         const GAS int *__restrict__ ty = as_global(D->styp);
         const GAS double *__restrict__ ljA = as_global(D->ljA);
@@ -311,18 +327,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         float *sJF = (float *)(lds + lds_head);           // [entry][3 * NB_CL]
         int *sJ = (int *)(sJF + 3 * NB_CL * w.count);      // [entry]
         // The item's image (-1: main list); all of its entries share it.
-        const int img = __builtin_amdgcn_readfirstlane(((const GAS int *)(eb + (size_t)w.first * sizeof(Entry)))[3]);
-        double R[9], T[3], isc = 1.0;
-        if (img >= 0) {
-            const ImageXf *X = &D->imgxf[img];
-#pragma unroll
-            for (int k = 0; k < 9; k++) R[k] = X->R[k];
-#pragma unroll
-            for (int k = 0; k < 3; k++) T[k] = X->T[k];
-            isc = X->scale;
+        int img = -1;
+        double isc = 1.0;
+        const GAS double *X = nullptr;
+        if (IMG) {
+            img = __builtin_amdgcn_readfirstlane((w.sys >> 16) - 1);
+            X = (const GAS double *)&D->imgxf[img];
+            isc = uniform_f64(X[12]);
         }
 
         __syncthreads();
+        if (IMG && lane < 12) sX[lane] = X[lane];
         if (ROWS_LDS)
             for (int k = lane; k < 2 * nt * nt; k += 64) sR[k] = load_slot(rows, k);
         sI[lane] = load_slot(xq, I0 * NB_CL + lane);
@@ -344,17 +359,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         int lb1 = entry_bits(eb, min(w.first + 1, e_last), bl);
         int J = __builtin_amdgcn_readfirstlane(jc0.x);
         double4 xj = load_slot(xq, J * NB_CL + b);
-        if (img >= 0) xj = image_slot(xj, R, T);
         int tj = 64 * ty[J * NB_CL + b];
         for (int e = w.first; e <= e_last; e++) {
             const int Jn = __builtin_amdgcn_readfirstlane(jc1.x);
-            double4 xjn = load_slot(xq, Jn * NB_CL + b);
-            if (img >= 0) xjn = image_slot(xjn, R, T);
+            const double4 xjn = load_slot(xq, Jn * NB_CL + b);
             const int tjn = 64 * ty[Jn * NB_CL + b];
             const int2 jc2 = entry_jc(eb, min(e + 2, e_last));
             const int lb2 = entry_bits(eb, min(e + 2, e_last), bl);
             const int cb = __builtin_amdgcn_readfirstlane(jc0.y);
             const int lb = lb0;
+            // Image items: the j-atom's image position (its load was issued one entry earlier).
+            const double4 xjt = IMG ? image_slot(xj, sX) : xj;
             float gx = 0.0f, gy = 0.0f, gz = 0.0f;
 #pragma unroll
             for (int c = 0; c < NB_SCC; c++) {
@@ -362,7 +377,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     const bool inl = (lb & (1 << c)) != 0;
                     const double4 xi = sI[c * NB_CL + a];
                     const int roff = sT[c * NB_CL + a] + tj;   // byte offset of the pair's plain row
-                    const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    const double dx = xi.x - xjt.x, dy = xi.y - xjt.y, dz = xi.z - xjt.z;
                     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
                     const bool act = inl && (r2 <= F.r2Off);
                     if (__builtin_amdgcn_ballot_w64(act) != 0ull) {
@@ -371,7 +386,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         const double4 C = ROWS_LDS ? *(const double4 *)((const unsigned char *)sR + off)
                                                    : load_slot(rows, off >> 5);
                         double fs, x1, x2;
-                        pair_rows(F, r2, xi.w * xj.w, C, sw, act, ljA, ljB, roff >> 6, fs, x1, x2);
+                        pair_rows(F, r2, xi.w * xjt.w, C, sw, act, ljA, ljB, roff >> 6, fs, x1, x2);
                         const float f = (float)fs;
                         const float px = f * (float)dx, py = f * (float)dy, pz = f * (float)dz;
                         fx[c] += px; fy[c] += py; fz[c] += pz;
@@ -414,8 +429,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         const int si = (I0 + b) * NB_CL + a;
 #pragma unroll
-        for (int k = 0; k < 3; k++) fx_atomic_add(facc, n_pad, si, k, isc * (double)ig[k]);
-        if (img < 0) {
+        for (int k = 0; k < 3; k++) fx_atomic_add(facc, n_pad, si, k, IMG ? isc * (double)ig[k] : (double)ig[k]);
+        if (!IMG) {
             // j-forces of the whole item.
             for (int k = lane; k < 3 * NB_CL * w.count; k += 64) {
                 const int el = k / (3 * NB_CL), r = k - el * 3 * NB_CL;
@@ -426,6 +441,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             // sums h = sum g'_j and G_ab = sum g'_j,a (x_j - origin)_b are kept
             // for the lattice derivatives (SymmetryParameterGradients_ImageDerivatives,
             // pMolecule-1.9.0/extensions/csource/SymmetryParameterGradients.c:142-216).
+            // A pure translation needs only h (the moments' derivative term vanishes).
+            const bool moments = __builtin_amdgcn_readfirstlane(X[13] == 0.0 ? 1 : 0) != 0;
             double G[12];
 #pragma unroll
             for (int k = 0; k < 12; k++) G[k] = 0.0;
@@ -434,29 +451,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 const int sj = sJ[el] * NB_CL + jb;
                 const float *f = &sJF[el * 3 * NB_CL + 3 * jb];
                 const double gx = isc * (double)f[0], gy = isc * (double)f[1], gz = isc * (double)f[2];
-                fx_atomic_add(facc, n_pad, sj, 0, fma(R[0], gx, fma(R[3], gy, R[6] * gz)));
-                fx_atomic_add(facc, n_pad, sj, 1, fma(R[1], gx, fma(R[4], gy, R[7] * gz)));
-                fx_atomic_add(facc, n_pad, sj, 2, fma(R[2], gx, fma(R[5], gy, R[8] * gz)));
-                const double4 xr = load_slot(xq, sj);
+                fx_atomic_add(facc, n_pad, sj, 0, fma(sX[0], gx, fma(sX[3], gy, sX[6] * gz)));
+                fx_atomic_add(facc, n_pad, sj, 1, fma(sX[1], gx, fma(sX[4], gy, sX[7] * gz)));
+                fx_atomic_add(facc, n_pad, sj, 2, fma(sX[2], gx, fma(sX[5], gy, sX[8] * gz)));
                 G[0] += gx; G[1] += gy; G[2] += gz;
-                G[3] = fma(gx, xr.x, G[3]); G[4] = fma(gx, xr.y, G[4]); G[5] = fma(gx, xr.z, G[5]);
-                G[6] = fma(gy, xr.x, G[6]); G[7] = fma(gy, xr.y, G[7]); G[8] = fma(gy, xr.z, G[8]);
-                G[9] = fma(gz, xr.x, G[9]); G[10] = fma(gz, xr.y, G[10]); G[11] = fma(gz, xr.z, G[11]);
+                if (moments) {
+                    const double4 xr = load_slot(xq, sj);
+                    G[3] = fma(gx, xr.x, G[3]); G[4] = fma(gx, xr.y, G[4]); G[5] = fma(gx, xr.z, G[5]);
+                    G[6] = fma(gy, xr.x, G[6]); G[7] = fma(gy, xr.y, G[7]); G[8] = fma(gy, xr.z, G[8]);
+                    G[9] = fma(gz, xr.x, G[9]); G[10] = fma(gz, xr.y, G[10]); G[11] = fma(gz, xr.z, G[11]);
+                }
             }
-            long long *ga = D->gacc + (size_t)img * NB_GACC_WORDS_PER_IMAGE;
+            long long *ga = D->gacc + (size_t)img * NB_GACC_WORDS_PER_IMAGE +
+                            (size_t)(it & (NB_GSLOTS - 1)) * NB_GACC_WORDS_PER_SLOT;
+            const int ncomp = moments ? 12 : 3;
 #pragma unroll
             for (int k = 0; k < 12; k++) {
-                const double v = wave_sum_f64(G[k]);
-                if (lane == k) fx_add(&ga[2 * k], &ga[2 * k + 1], v);
+                if (k < ncomp) {
+                    const double v = wave_sum_f64(G[k]);
+                    if (lane == k) fx_add(&ga[2 * k], &ga[2 * k + 1], v);
+                }
             }
         }
         eel = wave_sum_f64(eel);
         elj = wave_sum_f64(elj);
         if (lane < 2) {
             const int slot = it & (NB_ESLOTS - 1);
-            const int kind = lane + (img >= 0 ? 2 : 0);
-            fx_add(eacc_word(D->eacc, kind, 0, slot), eacc_word(D->eacc, kind, 1, slot),
-                   isc * (lane == 0 ? eel : elj));
+            const int kind = lane + (IMG ? 2 : 0);
+            const double v = lane == 0 ? eel : elj;
+            fx_add(eacc_word(D->eacc, kind, 0, slot), eacc_word(D->eacc, kind, 1, slot), IMG ? isc * v : v);
         }
     }
 }
@@ -668,16 +691,23 @@ void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, in
 }
 
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
-                        int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s)
+                        int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images)
 {
     if (nblocks <= 0) return;
     const bool rows_lds = tiled_lds_bytes(max_ntypes, true, max_count) <= NB_LDS_ROW_LIMIT;
     const int head = (int)tiled_lds_head(max_ntypes, rows_lds);
     const size_t bytes = tiled_lds_bytes(max_ntypes, rows_lds, max_count);
-    if (rows_lds)
-        hipLaunchKernelGGL(k_force_tiled<true>, dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
-    else
-        hipLaunchKernelGGL(k_force_tiled<false>, dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
+    if (images) {
+        if (rows_lds)
+            hipLaunchKernelGGL((k_force_tiled<true, true>), dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
+        else
+            hipLaunchKernelGGL((k_force_tiled<false, true>), dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
+    } else {
+        if (rows_lds)
+            hipLaunchKernelGGL((k_force_tiled<true, false>), dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
+        else
+            hipLaunchKernelGGL((k_force_tiled<false, false>), dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
+    }
 }
 
 void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,

@@ -238,6 +238,7 @@ extern "C" void nbabfs_default_options(nbabfs_options *o)
     o->precision = NBABFS_MIXED;
     o->entriesPerItem = 16;
     o->wavesPerCU = 0;
+    o->imageEntriesPerItem = 16;
 }
 
 __global__ void k_remap_items(const WorkItem *__restrict__ src, int count, int sys, WorkItem *__restrict__ dst)
@@ -419,9 +420,17 @@ struct nbabfs_state {
     double force_ms = 0.0;
     long long force_launches = 0;
 
+    // Second stream for the image-list launch, which runs beside the main
+    // list's launch (fork and join events on the state's stream).
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
     ~nbabfs_state()
     {
         for (auto e : ev) (void)hipEventDestroy(e);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (stream2) (void)hipStreamDestroy(stream2);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -714,22 +723,26 @@ static void inverse_derivative(const double *m, int i, int j, double *o)
 }
 
 // Work items longest-first (and, for the persistent queues, interleaved over
-// the eight per-XCD queues), then the queue bounds.
+// the eight per-XCD queues), then the queue bounds.  The main list's items
+// [0, nitems_main) and the image lists' items after them are ordered apart:
+// they run in two launches.
 static int order_items(nbabfs_state *S)
 {
     hipStream_t st = S->stream;
-    const int n = S->nitems;
+    const int n = S->nitems_main, ni = S->nitems - S->nitems_main;
     const int interleave = S->opt.wavesPerCU > 0 ? 1 : 0;
     std::vector<int> qs(9);
-    if (n > 0) {
+    if (n + ni > 0) {
         const int maxcost = NB_SCC * std::max(1, S->opt.entriesPerItem);
-        CHK(S->icost.ensure(n));
+        CHK(S->icost.ensure(std::max(n, ni)));
         CHK(S->ihist.ensure(maxcost + 1));
         CHK(S->icur.ensure(maxcost + 1));
-        CHK(S->items2.ensure(n));
+        CHK(S->items2.ensure(n + ni));
         CHK(S->scantmp.ensure(scan_tmp_size(maxcost + 1)));
         launch_order_items(S->items.p, n, S->entries.p, maxcost, interleave, S->icost.p, S->ihist.p, S->icur.p,
                            S->scantmp.p, S->items2.p, st);
+        launch_order_items(S->items.p + n, ni, S->entries.p, maxcost, 0, S->icost.p, S->ihist.p, S->icur.p,
+                           S->scantmp.p, S->items2.p + n, st);
         HIPCHK(hipGetLastError());
         S->items.swap(S->items2);
     }
@@ -750,8 +763,11 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     S->listValid = false;
     S->csrValid = false;
     S->npairs = 0;
-    S->nent = 0;
-    S->nitems = 0;
+    S->nent = S->nent_main = 0;
+    S->nitems = S->nitems_main = 0;
+    S->slots.clear();
+    S->nimages = 0;
+    S->img_npairs = 0;
     if (n <= 0) {
         S->n_pad = S->ncl = S->nsc = 0;
         S->listValid = true;
@@ -1098,6 +1114,13 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.gacc = D.nimg ? S->gacc.p : nullptr;
 }
 
+// Entries per work item of the image lists (they run in a launch of their own,
+// one workgroup per item, so smaller items balance the launch).
+static int image_per_item(const nbabfs_options &o)
+{
+    return std::max(1, std::min(o.imageEntriesPerItem > 0 ? o.imageEntriesPerItem : 16, NBABFS_MAX_ENTRIES_PER_ITEM));
+}
+
 // wavesPerCU > 0: persistent grid of that many waves per CU; otherwise one
 // workgroup per work item.
 static int n_persistent_blocks(const nbabfs_options &o, int nitems)
@@ -1114,10 +1137,32 @@ static int launch_force(nbabfs_state *S, ForceGuard g)
     hipStream_t st = S->stream;
     if (S->timing) CHK(record_event(S));
     if (S->opt.precision != NBABFS_PLAIN_FP64) {
-        if (S->nitems > 0) {
+        const int nm = S->nitems_main, ni = S->nitems - S->nitems_main;
+        // The image launch runs on the second stream, overlapping the main one.
+        hipStream_t sti = st;
+        if (ni > 0 && nm > 0) {
+            if (!S->stream2) {
+                HIPCHK(hipStreamCreateWithFlags(&S->stream2, hipStreamNonBlocking));
+                HIPCHK(hipEventCreateWithFlags(&S->ev_fork, hipEventDisableTiming));
+                HIPCHK(hipEventCreateWithFlags(&S->ev_join, hipEventDisableTiming));
+            }
+            HIPCHK(hipEventRecord(S->ev_fork, st));
+            HIPCHK(hipStreamWaitEvent(S->stream2, S->ev_fork, 0));
+            sti = S->stream2;
+        }
+        if (ni > 0) {
+            QueueParams q{nullptr, S->qstarts.p};
+            launch_force_tiled(S->d_desc.p, S->items.p + nm, q, S->F, ni, S->ntypes, image_per_item(S->opt), g, sti,
+                               true);
+        }
+        if (nm > 0) {
             QueueParams q{S->opt.wavesPerCU > 0 ? S->heads.p : nullptr, S->qstarts.p};
-            launch_force_tiled(S->d_desc.p, S->items.p, q, S->F, n_persistent_blocks(S->opt, S->nitems), S->ntypes,
+            launch_force_tiled(S->d_desc.p, S->items.p, q, S->F, n_persistent_blocks(S->opt, nm), S->ntypes,
                                S->opt.entriesPerItem, g, st);
+        }
+        if (sti != st) {
+            HIPCHK(hipEventRecord(S->ev_join, sti));
+            HIPCHK(hipStreamWaitEvent(st, S->ev_join, 0));
         }
     } else {
         launch_force_plain(S->d_desc.p, 0, S->n, S->F, S->rowptr.p, S->cols.p, g, st);
@@ -1159,6 +1204,7 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
 static int recut_items(nbabfs_state *S, int per_item)
 {
     hipStream_t st = S->stream;
+    const int ipi = image_per_item(S->opt);
     if (S->nsc <= 0) return NBABFS_STATUS_CONTINUE;
     const int nb = (int)S->slots.size() * S->nsc;
     launch_items_per_sc(S->counts.p, S->nsc, per_item, S->nits.p, st);
@@ -1166,7 +1212,7 @@ static int recut_items(nbabfs_state *S, int per_item)
     hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, S->eoff.p, S->counts.p, S->ioff.p, S->nits.p, S->nsc,
                        S->flag.p, S->tail.d);
     if (nb > 0) {
-        launch_items_per_sc(S->icounts.p, nb, per_item, S->inits.p, st);
+        launch_items_per_sc(S->icounts.p, nb, ipi, S->inits.p, st);
         launch_scan(S->inits.p, S->iioff.p, nb, S->scantmp.p, st);
         hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, S->ieoff.p, S->icounts.p, S->iioff.p, S->inits.p,
                            nb, S->flag.p, S->tail.d + 4);
@@ -1180,7 +1226,7 @@ static int recut_items(nbabfs_state *S, int per_item)
     launch_cut_items(S->counts.p, S->eoff.p, S->ioff.p, S->nsc, S->nsc, per_item, 0, S->items.p, st);
     if (nb > 0) {
         launch_add_int(S->iioff.p, nb, S->nitems_main, st);
-        launch_cut_items(S->icounts.p, S->ieoff.p, S->iioff.p, nb, S->nsc, per_item, 0, S->items.p, st);
+        launch_cut_items(S->icounts.p, S->ieoff.p, S->iioff.p, nb, S->nsc, ipi, 0, S->items.p, st, true);
     }
     HIPCHK(hipGetLastError());
     return order_items(S);
@@ -1387,7 +1433,7 @@ static int build_images(nbabfs_state *S, const double *d_xyz)
             launch_search_images(P, nsc, ns, d_xyz, S->cellv.p, S->slot_atom.p, S->blo.p, S->bhi.p, S->iscratch.p,
                                  S->incand.p, S->icounts.p, S->flag.p, st);
             launch_scan(S->icounts.p, S->ieoff.p, nb, S->scantmp.p, st);
-            const int per_item = std::max(1, o.entriesPerItem > 0 ? o.entriesPerItem : 16);
+            const int per_item = image_per_item(o);
             launch_items_per_sc(S->icounts.p, nb, per_item, S->inits.p, st);
             launch_scan(S->inits.p, S->iioff.p, nb, S->scantmp.p, st);
             hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, S->ieoff.p, S->icounts.p, S->iioff.p,
@@ -1453,6 +1499,8 @@ static int upload_image_xf(nbabfs_state *S)
         for (int q = 0; q < 9; q++) X.R[q] = Ro[q];
         for (int r = 0; r < 3; r++) X.T[r] = (Ro[3 * r] * og[0] + Ro[3 * r + 1] * og[1] + Ro[3 * r + 2] * og[2]) + To[r] - og[r];
         X.scale = sl.scale;
+        static const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        X.translation = std::memcmp(&S->trot[9 * sl.t], I3, sizeof(I3)) == 0 ? 1.0 : 0.0;
     }
     HIPCHK(hipMemcpyAsync(S->imgxf.p, S->h_imgxf.p, ns * sizeof(ImageXf), hipMemcpyHostToDevice, S->stream));
     std::memcpy(S->Mxf, S->M, sizeof(S->Mxf));
@@ -1574,6 +1622,10 @@ extern "C" int nbabfs_state_create(const nbabfs_options *opt, int natoms, const 
         return fail(NBABFS_STATUS_VALUE_ERROR, "entriesPerItem %d exceeds %d", o.entriesPerItem,
                     NBABFS_MAX_ENTRIES_PER_ITEM);
     if (o.wavesPerCU < 0) o.wavesPerCU = 0;
+    if (o.imageEntriesPerItem <= 0) o.imageEntriesPerItem = 16;
+    if (o.imageEntriesPerItem > NBABFS_MAX_ENTRIES_PER_ITEM)
+        return fail(NBABFS_STATUS_VALUE_ERROR, "imageEntriesPerItem %d exceeds %d", o.imageEntriesPerItem,
+                    NBABFS_MAX_ENTRIES_PER_ITEM);
     // NBModelABFS.SetOptions cutoff checks (pMolecule.NBModelABFS.pyx:169-176).
     if (o.dampingCutoff < 0.0 || o.innerCutoff < o.dampingCutoff || o.outerCutoff < o.innerCutoff ||
         o.listCutoff < o.outerCutoff)
@@ -1730,6 +1782,7 @@ extern "C" int nbabfs_set_stream(nbabfs_state *S, void *hip_stream)
 {
     if (!S) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state is NULL");
     HIPCHK(hipStreamSynchronize(S->stream));
+    if (S->stream2) HIPCHK(hipStreamSynchronize(S->stream2));
     if (S->own_stream) (void)hipStreamDestroy(S->stream);
     S->own_stream = false;
     if (hip_stream) {
@@ -1945,8 +1998,12 @@ extern "C" int nbabfs_image_terms(nbabfs_state *S, double *energies, double *dEd
         if (sl.npairs <= 0) continue;
         double g[12];
         for (int q = 0; q < 12; q++) {
-            const long long lo = S->h_gacc.p[(size_t)NB_GACC_WORDS_PER_IMAGE * k + 2 * q];
-            const long long hi = S->h_gacc.p[(size_t)NB_GACC_WORDS_PER_IMAGE * k + 2 * q + 1];
+            long long lo = 0, hi = 0;
+            for (int sl2 = 0; sl2 < NB_GSLOTS; sl2++) {
+                const long long *w = S->h_gacc.p + (size_t)NB_GACC_WORDS_PER_IMAGE * k + (size_t)NB_GACC_WORDS_PER_SLOT * sl2;
+                lo += w[2 * q];
+                hi += w[2 * q + 1];
+            }
             g[q] = (double)hi * NB_FX_HI_UNIT + (double)lo * (1.0 / NB_FX_LO_SCALE);
         }
         const double *h = g;

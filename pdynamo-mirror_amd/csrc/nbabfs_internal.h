@@ -55,18 +55,23 @@ struct Entry {              // 80 bytes
 // Real-space transformation of one image for the pair kernel, in the cluster
 // frame: x' - origin = R (x - origin) + T.  scale multiplies the image's
 // energies and forces (1/2 for an image that is its own inverse, or for every
-// image when inverses are not used).
+// image when inverses are not used).  translation != 0 when the fractional
+// rotation is the identity: dE/dM then needs only the gradient sum.
 struct ImageXf {            // 128 bytes
     double R[9];
     double T[3];
     double scale;
-    double pad[3];
+    double translation;
+    double pad[2];
 };
-// gacc words per image: 12 components (g' sum x3, g' (x) x x9) x (low, high).
-#define NB_GACC_WORDS_PER_IMAGE 24
+// gacc words per image: NB_GSLOTS slots of 12 components (g' sum x3,
+// g' (x) x x9) x (low, high); a work item adds into slot (item % NB_GSLOTS).
+#define NB_GSLOTS 16
+#define NB_GACC_WORDS_PER_SLOT 24
+#define NB_GACC_WORDS_PER_IMAGE (NB_GSLOTS * NB_GACC_WORDS_PER_SLOT)
 
 struct WorkItem {           // 16 bytes
-    int sys;                // system index in the batch
+    int sys;                // system index in the batch; image-list items: | (image + 1) << 16
     int sc;                 // super-cluster
     int first;              // first entry
     int count;              // entries

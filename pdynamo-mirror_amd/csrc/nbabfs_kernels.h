@@ -64,7 +64,7 @@ void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4
                    const float4 *blo, const float4 *bhi, const int2 *colcl, Entry *scratch, int *ncand, int *counts,
                    int *overflow, hipStream_t s);
 void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int nblocks, int nsc, int per_item,
-                      int sys, WorkItem *items, hipStream_t s);
+                      int sys, WorkItem *items, hipStream_t s, bool images = false);
 // Image lists: R'x + T' per transformation (12 doubles each), the stepped image
 // coordinates, the on-grid flags of the current coordinates, the cross search.
 void launch_image_transform(const double *xyz, int n, const double *rt, int ntrans, double *out, hipStream_t s);
@@ -124,8 +124,9 @@ struct ForceGuard {
 };
 void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, int *heads, const PrepareCheck &chk,
                     hipStream_t s);
+// images: the launch over image-list items (one workgroup per item).
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
-                        int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s);
+                        int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images = false);
 void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
                         const int *cols, ForceGuard g, hipStream_t s);
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s);
