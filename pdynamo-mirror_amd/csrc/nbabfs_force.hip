@@ -547,6 +547,140 @@ __global__ __launch_bounds__(256) void k_force_plain(const SysDesc *__restrict__
 }
 
 // ------------------------------------------------------------------------------------------------
+// Plain FP64 image pass (MMMMImageEnergy, pMolecule-1.9.0/extensions/csource/NBModelABFS.c:1161-1290, with
+// PairwiseInteractionABFS_MMMMEnergy's expression order, PairwiseInteraction.c:292-429): one wavefront per
+// image work item, lane l = 8b + a pairing i-atom a of each i-cluster of the super-cluster with j-atom b of
+// the entry's j-cluster.  Each item's energies are stored in FP64 and summed in item order by the finalize.  Absolute coordinates; the image coordinate of j is formed as Coordinates3_Rotate +
+// _Translate form it (R' x, then + M (t + n)); qi = escale_image * q_i, qij = qi * q_j, A and B times the
+// image scale; dF from pair_fp64; i += 2 dF r_ij, j (image side) -= 2 dF r_ij.  j-gradients are summed over
+// the 8 lanes of the j-atom, rotated back (R'^T) and added per entry; i-gradients per item; all into the
+// fixed-point accumulators, with the energies and the dE/dM sums as in the tiled image instance.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ double sum_lanes_xor(double v, int m0, int m1, int m2)
+{
+    v += __shfl_xor(v, m0, 64);
+    v += __shfl_xor(v, m1, 64);
+    v += __shfl_xor(v, m2, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(64) void k_force_image_fp64(const SysDesc *__restrict__ descs,
+                                                         const WorkItem *__restrict__ items, AbfsFactors F,
+                                                         ForceGuard guard)
+{
+#pragma clang fp contract(off)
+    if (guard.flag && *(const volatile int *)guard.flag == guard.epoch) return;
+    const int it = blockIdx.x;
+    const WorkItem w = items[it];
+    const SysDesc &D = descs[w.sys & 0xffff];
+    const int img = (w.sys >> 16) - 1;
+    const ImageXf &X = D.imgxf[img];
+    const int lane = threadIdx.x, a = lane & 7, b = lane >> 3;
+    const int bl = 8 * a + b;
+    const double isc = X.scale, esc = X.escale;
+    const bool moments = X.translation == 0.0;
+    const int nt = D.ntypes;
+    double gi[NB_SCC][3];
+    int ia[NB_SCC];
+    double xi[NB_SCC][3], qi[NB_SCC];
+    int ti[NB_SCC];
+#pragma unroll
+    for (int c = 0; c < NB_SCC; c++) {
+        gi[c][0] = gi[c][1] = gi[c][2] = 0.0;
+        ia[c] = D.slot_atom[(w.sc * NB_SCC + c) * NB_CL + a];
+        const int i = ia[c] >= 0 ? ia[c] : 0;
+        xi[c][0] = D.xyz[3 * (size_t)i];
+        xi[c][1] = D.xyz[3 * (size_t)i + 1];
+        xi[c][2] = D.xyz[3 * (size_t)i + 2];
+        qi[c] = esc * D.charge[i];
+        ti[c] = nt * D.atype[i];
+    }
+    double e1 = 0.0, e2 = 0.0, G[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) G[k] = 0.0;
+    for (int e = w.first; e < w.first + w.count; e++) {
+        const Entry &en = D.entries[e];
+        const int J = en.J, cb = en.cbits, lb = en.bits[bl];
+        const int j = D.slot_atom[J * NB_CL + b];
+        const int jj = j >= 0 ? j : 0;
+        const double x0 = D.xyz[3 * (size_t)jj], y0 = D.xyz[3 * (size_t)jj + 1], z0 = D.xyz[3 * (size_t)jj + 2];
+        const double xj = (X.R[0] * x0 + X.R[1] * y0 + X.R[2] * z0) + X.Tabs[0];
+        const double yj = (X.R[3] * x0 + X.R[4] * y0 + X.R[5] * z0) + X.Tabs[1];
+        const double zj = (X.R[6] * x0 + X.R[7] * y0 + X.R[8] * z0) + X.Tabs[2];
+        const double qj = D.charge[jj];
+        const int tj = D.atype[jj];
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+#pragma unroll
+        for (int c = 0; c < NB_SCC; c++) {
+            if (!((cb >> c) & 1) || !((lb >> c) & 1) || ia[c] < 0 || j < 0) continue;
+            double xij = xi[c][0] - xj, yij = xi[c][1] - yj, zij = xi[c][2] - zj;
+            const double r2 = (xij * xij + yij * yij + zij * zij);
+            const double qij = qi[c] * qj;
+            const int tij = ti[c] + tj;
+            double x1, x2, dF;
+            if (!pair_fp64(F, r2, qij, D.ljA[tij] * isc, D.ljB[tij] * isc, x1, x2, dF)) continue;
+            e1 += x1;
+            e2 += x2;
+            xij *= (2.0 * dF);
+            yij *= (2.0 * dF);
+            zij *= (2.0 * dF);
+            gi[c][0] += xij;
+            gi[c][1] += yij;
+            gi[c][2] += zij;
+            gx -= xij;
+            gy -= yij;
+            gz -= zij;
+        }
+        // The j-atom's image-side gradient: lanes 8b .. 8b+7.
+        gx = sum_lanes_xor(gx, 1, 2, 4);
+        gy = sum_lanes_xor(gy, 1, 2, 4);
+        gz = sum_lanes_xor(gz, 1, 2, 4);
+        if (a == 0 && j >= 0) {
+            const int sj = J * NB_CL + b;
+            // Coordinates3_Rotate by R'^T.
+            fx_atomic_add(D.facc, D.n_pad, sj, 0, X.R[0] * gx + X.R[3] * gy + X.R[6] * gz);
+            fx_atomic_add(D.facc, D.n_pad, sj, 1, X.R[1] * gx + X.R[4] * gy + X.R[7] * gz);
+            fx_atomic_add(D.facc, D.n_pad, sj, 2, X.R[2] * gx + X.R[5] * gy + X.R[8] * gz);
+            G[0] += gx;
+            G[1] += gy;
+            G[2] += gz;
+            if (moments) {
+                const double rx = x0 - D.origin[0], ry = y0 - D.origin[1], rz = z0 - D.origin[2];
+                G[3] += gx * rx; G[4] += gx * ry; G[5] += gx * rz;
+                G[6] += gy * rx; G[7] += gy * ry; G[8] += gy * rz;
+                G[9] += gz * rx; G[10] += gz * ry; G[11] += gz * rz;
+            }
+        }
+    }
+    // i-gradients: lanes a, a+8, ..., a+56.
+#pragma unroll
+    for (int c = 0; c < NB_SCC; c++) {
+        const double vx = sum_lanes_xor(gi[c][0], 8, 16, 32);
+        const double vy = sum_lanes_xor(gi[c][1], 8, 16, 32);
+        const double vz = sum_lanes_xor(gi[c][2], 8, 16, 32);
+        if (b == 0 && ia[c] >= 0) {
+            const int si = (w.sc * NB_SCC + c) * NB_CL + a;
+            fx_atomic_add(D.facc, D.n_pad, si, 0, vx);
+            fx_atomic_add(D.facc, D.n_pad, si, 1, vy);
+            fx_atomic_add(D.facc, D.n_pad, si, 2, vz);
+        }
+    }
+    long long *ga = D.gacc + (size_t)img * NB_GACC_WORDS_PER_IMAGE +
+                    (size_t)(it & (NB_GSLOTS - 1)) * NB_GACC_WORDS_PER_SLOT;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const double v = wave_sum_f64(G[k]);
+        if (lane == k && (moments || k < 3)) fx_add(&ga[2 * k], &ga[2 * k + 1], v);
+    }
+    e1 = wave_sum_f64(e1);
+    e2 = wave_sum_f64(e2);
+    if (lane == 0) {
+        D.eimgpart[2 * (size_t)it] = e1;
+        D.eimgpart[2 * (size_t)it + 1] = e2;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
 // 1-4 list (NBModelABFS_MMMMEnergy's second pass, NBModelABFS.c:276-294): FP64
 // with scale14 and the 1-4 LJ table; gradients into the fixed-point arrays.
 // ------------------------------------------------------------------------------------------------
@@ -631,11 +765,7 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
     }
     if (blockIdx.x != 0) return;
     double el, lj, el14, lj14, iel = 0.0, ilj = 0.0;
-    if (plain) {
-        sum_range2(D.eplain, D.n_plain_blocks, el, lj);
-        el *= 0.5;
-        lj *= 0.5;
-    } else {
+    {
         // Exact: each slot's low word is split at 2^48 (its upper part carried
         // into the high word), the words are summed as integers over the 64
         // slots, the total is brought to its unique form (low word in
@@ -667,6 +797,12 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
         lj = sE[1];
         iel = sE[2];
         ilj = sE[3];
+    }
+    if (plain) {
+        sum_range2(D.eplain, D.n_plain_blocks, el, lj);
+        el *= 0.5;
+        lj *= 0.5;
+        if (D.eimgpart) sum_range2(D.eimgpart, D.n_img_items, iel, ilj);
     }
     sum_range2(D.epart14, D.n14_blocks, el14, lj14);
     if (threadIdx.x == 0) {
@@ -714,6 +850,12 @@ void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors
                         const int *cols, ForceGuard g, hipStream_t s)
 {
     hipLaunchKernelGGL(k_force_plain, dim3(max(nblk(n, 256), 1)), dim3(256), 0, s, d_desc, sys, F, rowptr, cols, g);
+}
+
+void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int nitems, const AbfsFactors &F,
+                             ForceGuard g, hipStream_t s)
+{
+    if (nitems > 0) hipLaunchKernelGGL(k_force_image_fp64, dim3(nitems), dim3(64), 0, s, d_desc, items, F, g);
 }
 
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s)

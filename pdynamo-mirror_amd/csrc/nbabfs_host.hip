@@ -350,7 +350,7 @@ struct nbabfs_state {
     DBuf<double4> xyzq;
     DBuf<int> styp;
     DBuf<long long> facc;
-    DBuf<double> epart14, eplain, gplain, energies;
+    DBuf<double> epart14, eplain, gplain, energies, eimgpart;
     DBuf<long long> eacc, vacc;
     DBuf<SysDesc> d_desc;
     DBuf<int> heads, qstarts;
@@ -904,6 +904,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     S->nitems = ((volatile int *)S->tail.h)[1];
     S->nent_main = S->nent;
     S->nitems_main = S->nitems;
+    S->desc_version = -1;
     S->slots.clear();
     S->nimages = 0;
     S->img_npairs = 0;
@@ -969,6 +970,7 @@ static int ensure_desc(nbabfs_state *S, const double *d_xyz, double *d_grad)
 static int ensure_call_buffers(nbabfs_state *S)
 {
     CHK(S->energies.ensure(NB_NENERGIES));
+    if (S->opt.precision == NBABFS_PLAIN_FP64) CHK(S->eimgpart.ensure(2 * (size_t)std::max(S->nitems - S->nitems_main, 1)));
     CHK(S->eacc.ensure(NB_EACC_WORDS));
     CHK(S->vacc.ensure(NB_VACC_WORDS));
     CHK(S->epart14.ensure(2 * (size_t)std::max(nblk(S->n14, 256), 1)));
@@ -1112,6 +1114,8 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.nimg = (int)S->slots.size();
     D.imgxf = D.nimg ? S->imgxf.p : nullptr;
     D.gacc = D.nimg ? S->gacc.p : nullptr;
+    D.n_img_items = S->nitems - S->nitems_main;
+    D.eimgpart = (S->opt.precision == NBABFS_PLAIN_FP64 && D.n_img_items > 0) ? S->eimgpart.p : nullptr;
 }
 
 // Entries per work item of the image lists (they run in a launch of their own,
@@ -1166,6 +1170,8 @@ static int launch_force(nbabfs_state *S, ForceGuard g)
         }
     } else {
         launch_force_plain(S->d_desc.p, 0, S->n, S->F, S->rowptr.p, S->cols.p, g, st);
+        const int nm = S->nitems_main, ni = S->nitems - S->nitems_main;
+        if (ni > 0) launch_force_image_fp64(S->d_desc.p, S->items.p + nm, ni, S->F, g, st);
     }
     if (S->timing) CHK(record_event(S));
     HIPCHK(hipGetLastError());
@@ -1229,6 +1235,7 @@ static int recut_items(nbabfs_state *S, int per_item)
         launch_cut_items(S->icounts.p, S->ieoff.p, S->iioff.p, nb, S->nsc, ipi, 0, S->items.p, st, true);
     }
     HIPCHK(hipGetLastError());
+    S->desc_version = -1;   // the descriptor carries the image item count
     return order_items(S);
 }
 
@@ -1501,6 +1508,12 @@ static int upload_image_xf(nbabfs_state *S)
         X.scale = sl.scale;
         static const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
         X.translation = std::memcmp(&S->trot[9 * sl.t], I3, sizeof(I3)) == 0 ? 1.0 : 0.0;
+        for (int r = 0; r < 3; r++) X.Tabs[r] = To[r];
+        {
+#pragma clang fp contract(off)
+            // PairwiseInteractionABFS_MMMMEnergy's eScale for an image: (1/dielectric * scale) * factor.
+            X.escale = ((1.0e+00 / S->opt.dielectric) * sl.scale) * coulomb_factor();
+        }
     }
     HIPCHK(hipMemcpyAsync(S->imgxf.p, S->h_imgxf.p, ns * sizeof(ImageXf), hipMemcpyHostToDevice, S->stream));
     std::memcpy(S->Mxf, S->M, sizeof(S->Mxf));
@@ -1894,8 +1907,6 @@ extern "C" int nbabfs_set_transformations(nbabfs_state *S, int ntransformations,
     if (!S->isNew) return fail(NBABFS_STATUS_LOGIC_ERROR, "transformations must be set before the first update");
     if (ntransformations < 0 || (ntransformations > 0 && (!rotations || !translations)) || imageExpandFactor < 0)
         return fail(NBABFS_STATUS_INVALID_ARGUMENT, "invalid transformations");
-    if (ntransformations > 0 && S->opt.precision == NBABFS_PLAIN_FP64)
-        return fail(NBABFS_STATUS_INVALID_ARGUMENT, "periodic images are evaluated by the mixed-precision path only");
     S->has_sym = ntransformations > 0;
     S->ntrans = ntransformations;
     S->trot.assign(rotations, rotations + 9 * (size_t)ntransformations);

@@ -57,11 +57,13 @@ struct Entry {              // 80 bytes
 // energies and forces (1/2 for an image that is its own inverse, or for every
 // image when inverses are not used).  translation != 0 when the fractional
 // rotation is the identity: dE/dM then needs only the gradient sum.
-struct ImageXf {            // 128 bytes
+struct ImageXf {            // 160 bytes
     double R[9];
     double T[3];
     double scale;
     double translation;
+    double Tabs[3];         // T of the plain path: x' = R x + Tabs on absolute coordinates
+    double escale;          // plain path: ((1/dielectric) * scale) * Coulomb factor
     double pad[2];
 };
 // gacc words per image: NB_GSLOTS slots of 12 components (g' sum x3,
@@ -110,7 +112,8 @@ struct SysDesc {
     int item_first, item_count; // range of this system's items in the batch
     int n_plain_blocks, n14_blocks;
     int nimg;                   // images of the image table (gacc length / NB_GACC_WORDS_PER_IMAGE)
-    int pad_;
+    int n_img_items;            // plain path: image work items (eimgpart holds 2 per item)
+    double *eimgpart;           // plain path: FP64 image energies per work item, summed in order
 };
 
 // ------------------------------------------------------------------------------------------------

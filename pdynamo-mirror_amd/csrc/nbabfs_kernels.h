@@ -129,5 +129,8 @@ void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParam
                         int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images = false);
 void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
                         const int *cols, ForceGuard g, hipStream_t s);
+// Plain FP64 image pass over the image lists' work items.
+void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int nitems, const AbfsFactors &F,
+                             ForceGuard g, hipStream_t s);
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s);
 void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s);
