@@ -84,6 +84,10 @@ typedef struct {
     long long imagePairs;          /* pairs of the image list */
     double    numberOfMMMMImageImages;  /* summed over updates (StatisticsAccumulate) */
     double    numberOfMMMMImagePairs;
+    long long qcmmElPairs;         /* current QC/MM electrostatic list (QC atom, MM site) */
+    long long qcmmLJPairs;         /* current QC/MM LJ list (pure QC atom, MM atom) */
+    double    numberOfQCMMElPairs; /* summed over updates (StatisticsAccumulate) */
+    double    numberOfQCMMLJPairs;
 } nbabfs_stats;
 
 /* Fill *opt with the reference defaults. */
@@ -184,6 +188,57 @@ int nbabfs_images(nbabfs_state *state, int cap, int *abct, double *scale, long l
 /* Pairs of image k as sorted keys i*n + j (i in the primary cell, j in the
  * image).  Returns the pair count (keys may be NULL to query it). */
 long long nbabfs_export_image_pairs(nbabfs_state *state, int image, long long *keys, long long cap);
+
+/* QC/MM systems ------------------------------------------------------------------------------
+ *
+ * QC atoms (NBModelABFSState_SetUp's QCAtomContainer argument and the model's
+ * qcmmCoupling, pMolecule-1.9.0/extensions/cinclude/NBModelABFSState.h:145-156,
+ * QCAtomContainer.h:24-60): nqc atoms in ascending atom order index[q]; a
+ * boundary (link) atom has boundary[q] != 0, its QC partner qcpartner[q], link
+ * factor linkfactor[q] (a distance, or a ratio when linkRatio != 0) and MM
+ * partners mmpartners[mmpartnerOffsets[q] .. mmpartnerOffsets[q+1]) (arrays
+ * of the other atoms are ignored; qcpartner, linkfactor and the offsets may
+ * be NULL when no atom is a boundary atom).  coupling: NBABFS_QCMM_MM, _RC or
+ * _RD (QCMMLinkAtomCouplingOptions.h).  Call once, before the first update.
+ * Pure QC atoms leave the MM/MM lists; the QC/MM lists are built with them. */
+#define NBABFS_QCMM_MM 0
+#define NBABFS_QCMM_RC 1
+#define NBABFS_QCMM_RD 2
+int nbabfs_set_qc_atoms(nbabfs_state *state, int nqc, const int *index, const int *boundary, const int *qcpartner,
+                        const double *linkfactor, int linkRatio, const int *mmpartnerOffsets, const int *mmpartners,
+                        int coupling);
+
+/* The QC/MM electrostatic interaction: the cubic spline through (x[k] = r^2,
+ * y[k]) with zero first derivatives at both ends, as
+ * CubicSpline_MakeFromReal1DArrays builds PairwiseInteractionABFS's
+ * electrostaticSpline (pCore-1.9.0/extensions/csource/CubicSpline.c:306-405,
+ * pMolecule.PairwiseInteraction.pyx:182-194), in atomic units. */
+int nbabfs_set_qcmm_spline(nbabfs_state *state, int npoints, const double *x, const double *y);
+
+/* After an energy call: QC/MM LJ and QC/MM 1-4 LJ (NBModelABFS_QCMMEnergyLJ,
+ * NBModelABFS.h:43, NBModelABFSState eqcmmlj / eqcmmlj14) and whether each list
+ * has pairs.  Their gradients are part of the energy call's. */
+int nbabfs_qcmm_terms(nbabfs_state *state, double *energies, int *present);
+
+/* NBModelABFS_QCMMPotentials (NBModelABFS.h:45) on the coordinates of the last
+ * update: potentials[nqc] (atomic units) are incremented. */
+int nbabfs_qcmm_potentials(nbabfs_state *state, double *potentials);
+
+/* NBModelABFS_QCMMGradients (NBModelABFS.h:44) for the QC charges
+ * qcCharges[nqc] on the coordinates of the last update: grad_inout (n x 3,
+ * host, kJ/mol/A) is incremented. */
+int nbabfs_qcmm_gradients_host(nbabfs_state *state, const double *qcCharges, double *grad_inout);
+
+/* The QC/MM lists of the last build as sorted keys row * 2^32 + column.
+ * which: 0 electrostatic (row = QC atom position, column = MM site: the atom
+ * for MM coupling, else the site of nbabfs_qcmm_sites), 1 LJ (atom, atom),
+ * 2 electrostatic 1-4 (QC position, atom), 3 LJ 1-4 (atom, atom).  Returns the
+ * pair count (keys may be NULL). */
+long long nbabfs_export_qcmm_pairs(nbabfs_state *state, int which, long long *keys, long long cap);
+
+/* QC atom coordinates (nqc x 3) and MM sites (coordinates, charges) of the
+ * last update's coordinates (any output may be NULL).  Returns the site count. */
+long long nbabfs_qcmm_sites(nbabfs_state *state, double *qcxyz, double *sitexyz, double *siteq, long long cap);
 
 /* Statistics (NBModelABFSState.pyx:68-97) and kernel timing with HIP events on
  * the launch stream (enable != 0 turns it on). */

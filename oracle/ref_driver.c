@@ -19,13 +19,18 @@
 #endif
 
 #include "Coordinates3.h"
+#include "CubicSpline.h"
 #include "LJParameterContainer.h"
 #include "MMAtomContainer.h"
+#include "Memory.h"
 #include "NBModelABFS.h"
 #include "NBModelABFSState.h"
 #include "PairList.h"
 #include "PairListGenerator.h"
 #include "PairwiseInteraction.h"
+#include "QCAtomContainer.h"
+#include "Real1DArray.h"
+#include "Selection.h"
 #include "SymmetryParameterGradients.h"
 #include "SymmetryParameters.h"
 #include "Transformation3Container.h"
@@ -43,7 +48,14 @@ typedef struct {
     Transformation3Container *trans;     /* NULL: no symmetry */
     SymmetryParameters *sp;
     SymmetryParameterGradients *spg;
+    /* QC/MM (NULL without QC atoms). */
+    QCAtomContainer *qc;
+    Selection *fixed;
+    Real1DArray *qcCharges, *qcmmPotentials;
+    PairwiseInteractionABFS *pwqcmm, *pwqcqc;
 } RefHandle;
+
+static long dump_list(PairList *pl, int *oi, int *oj, long cap);
 
 static PairList *make_self_list(int npairs, const int *pairs)
 {
@@ -88,7 +100,8 @@ static void *create_common(int n, const double *charges, const int *ljtypes, int
                            const double *tA, const double *tB, const double *tA14, const double *tB14,
                            int nexcl, const int *excl, int n14, const int *p14,
                            const double *opts, const double *gen, int nthreads,
-                           Transformation3Container *tc, int checkForInverses, int expandFactor)
+                           Transformation3Container *tc, int checkForInverses, int expandFactor,
+                           QCAtomContainer *qc, Selection *fixed, int coupling)
 {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -134,8 +147,21 @@ static void *create_common(int n, const double *charges, const int *ljtypes, int
         h->sp = SymmetryParameters_Allocate();
         h->spg = SymmetryParameterGradients_Allocate();
     }
-    h->state = NBModelABFSState_SetUp(h->atoms, NULL, NULL, h->excl, h->i14, h->lj, h->lj14,
-                                      NULL, NULL, NULL, tc, QCMMLinkAtomCoupling_MM);
+    h->qc = qc;
+    h->fixed = fixed;
+    if (qc) {
+        /* MMAtomContainer.DeactivateQCAtoms (pMolecule.MMAtomContainer.pyx:116-123): all QC atoms
+         * inactive, boundary atoms active again. */
+        for (int q = 0; q < qc->natoms; q++) h->atoms->data[qc->data[q].index].QACTIVE = False;
+        for (int q = 0; q < qc->natoms; q++)
+            if (qc->data[q].QBOUNDARY) h->atoms->data[qc->data[q].index].QACTIVE = True;
+        h->qcCharges = Real1DArray_Allocate(qc->natoms, NULL);
+        h->qcmmPotentials = Real1DArray_Allocate(qc->natoms, NULL);
+        h->model->qcmmCoupling = (QCMMLinkAtomCoupling)coupling;
+    }
+    h->state = NBModelABFSState_SetUp(h->atoms, qc, fixed, h->excl, h->i14, h->lj, h->lj14,
+                                      h->qcCharges, h->qcmmPotentials, NULL, tc,
+                                      qc ? (QCMMLinkAtomCoupling)coupling : QCMMLinkAtomCoupling_MM);
     h->crd = Coordinates3_Allocate(n);
     h->grd = Coordinates3_Allocate(n);
     return h;
@@ -147,7 +173,7 @@ void *ref_create(int n, const double *charges, const int *ljtypes, int ntypes,
                  const double *opts, const double *gen, int nthreads)
 {
     return create_common(n, charges, ljtypes, ntypes, tA, tB, tA14, tB14, nexcl, excl, n14, p14, opts, gen,
-                         nthreads, NULL, 1, 0);
+                         nthreads, NULL, 1, 0, NULL, NULL, 0);
 }
 
 /* With crystal symmetry: ntrans fractional transformations (rot 9 per item,
@@ -159,7 +185,170 @@ void *ref_create_sym(int n, const double *charges, const int *ljtypes, int ntype
                      int ntrans, const double *rot, const double *trans, int checkForInverses, int expandFactor)
 {
     return create_common(n, charges, ljtypes, ntypes, tA, tB, tA14, tB14, nexcl, excl, n14, p14, opts, gen,
-                         nthreads, make_transformations(ntrans, rot, trans), checkForInverses, expandFactor);
+                         nthreads, make_transformations(ntrans, rot, trans), checkForInverses, expandFactor, NULL, NULL,
+                         0);
+}
+
+/* A QC/MM system (no symmetry).  QC atoms in ascending atom order, as
+ * QCAtomContainer_FromAtomContainer (pMolecule.QCAtomContainer.pyx:430-496)
+ * makes them: qc_index[q]; boundary[q] != 0 for a link atom replacing MM atom
+ * qc_index[q], bonded to QC atom qc_partner[q], with link factor linkfactor[q]
+ * and MM partners mmp[mmp_off[q] .. mmp_off[q+1]).  coupling: 0 MM, 1 RC,
+ * 2 RD (QCMMLinkAtomCouplingOptions.h).  fixed: nfixed atom indices (may be
+ * 0).  The QC/MM electrostatic interaction gets the cubic spline through
+ * (spline_x, spline_y) with zero end derivatives, as
+ * PairwiseInteractionABFS.MakeElectrostaticSplines makes it
+ * (pMolecule.PairwiseInteraction.pyx:182-194). */
+void *ref_create_qc(int n, const double *charges, const int *ljtypes, int ntypes,
+                    const double *tA, const double *tB, const double *tA14, const double *tB14,
+                    int nexcl, const int *excl, int n14, const int *p14,
+                    const double *opts, const double *gen, int nthreads,
+                    int nqc, const int *qc_index, const int *boundary, const int *qc_partner, const double *linkfactor,
+                    int linkRatio, const int *mmp_off, const int *mmp, int coupling, int nfixed, const int *fixed,
+                    int nspline, const double *spline_x, const double *spline_y)
+{
+    QCAtomContainer *qc = QCAtomContainer_Allocate(nqc);
+    int nb = 0;
+    qc->QLINKRATIO = linkRatio ? True : False;
+    for (int q = 0; q < nqc; q++) {
+        qc->data[q].index = qc_index[q];
+        qc->data[q].atomicNumber = 1;
+        if (boundary[q]) {
+            int np = mmp_off[q + 1] - mmp_off[q];
+            nb++;
+            qc->data[q].QBOUNDARY = True;
+            qc->data[q].qcpartner = qc_partner[q];
+            qc->data[q].linkfactor = linkfactor[q];
+            if (np > 0) {
+                qc->data[q].nmmpartners = np;
+                qc->data[q].mmpartners = Memory_Allocate_Array_Integer(np);
+                for (int t = 0; t < np; t++) qc->data[q].mmpartners[t] = mmp[mmp_off[q] + t];
+            }
+        }
+    }
+    qc->nboundary = nb;
+    Selection *fx = NULL;
+    if (nfixed > 0) {
+        Integer *tmp = (Integer *)malloc(sizeof(Integer) * (size_t)nfixed);
+        for (int k = 0; k < nfixed; k++) tmp[k] = fixed[k];
+        fx = Selection_FromIntegerArray(nfixed, tmp);
+        free(tmp);
+    }
+    RefHandle *h = (RefHandle *)create_common(n, charges, ljtypes, ntypes, tA, tB, tA14, tB14, nexcl, excl, n14, p14,
+                                              opts, gen, nthreads, NULL, 1, 0, qc, fx, coupling);
+    /* The QC/MM and QC/QC interactions of NBModelABFS.CheckPairwiseInteractions
+     * (pMolecule.NBModelABFS.pyx:72-97): damping 0, the model's inner and outer cutoffs. */
+    for (int k = 0; k < 2; k++) {
+        PairwiseInteractionABFS *pw = PairwiseInteractionABFS_Allocate();
+        pw->dampingCutoff = 0.0;
+        pw->innerCutoff = opts[1];
+        pw->outerCutoff = opts[2];
+        Real1DArray *X = Real1DArray_Allocate(nspline, NULL), *Y = Real1DArray_Allocate(nspline, NULL);
+        for (int i = 0; i < nspline; i++) {
+            Real1DArray_Item(X, i) = spline_x[i];
+            Real1DArray_Item(Y, i) = spline_y[i];
+        }
+        CubicSpline_MakeFromReal1DArrays(&pw->electrostaticSpline, &X, &Y, 1, 0.0, 1, 0.0);
+        if (k == 0) h->pwqcmm = pw; else h->pwqcqc = pw;
+    }
+    return h;
+}
+
+/* The second derivatives h of the QC/MM spline as the reference builds them
+ * (CubicSpline_MakeFromReal1DArrays, dgtsv). */
+int ref_qcmm_spline(void *vh, int cap, double *h2)
+{
+    RefHandle *h = (RefHandle *)vh;
+    if (!h->pwqcmm || !h->pwqcmm->electrostaticSpline) return -1;
+    CubicSpline *sp = h->pwqcmm->electrostaticSpline;
+    for (int i = 0; i < sp->length && i < cap; i++) h2[i] = Real1DArray_Item(sp->h, i);
+    return sp->length;
+}
+
+static void copy_rows(Coordinates3 *c, double *out, int n)
+{
+    for (int i = 0; i < n; i++) {
+        double x, y, z;
+        Coordinates3_GetRow(c, i, x, y, z);
+        out[3 * i] = x; out[3 * i + 1] = y; out[3 * i + 2] = z;
+    }
+}
+
+/* One System.Energy call of a QC/MM system (pMolecule/System.py:297-304):
+ * NBModelABFSState_Initialize, NBModelABFS_Update, NBModelABFS_MMMMEnergy +
+ * NBModelABFS_QCMMEnergyLJ (NBModelABFS.Energy, pyx:108-122), then
+ * NBModelABFS_QCMMPotentials; the QC charges qcq are then set (as the QC model
+ * does) and NBModelABFS_QCMMGradients runs.  energies[6]: MM/MM Elect., LJ,
+ * 1-4 Elect., 1-4 LJ, QC/MM LJ, QC/MM 1-4 LJ; present[4]: nbmmmm, nbmmmm14,
+ * nbqcmmlj, nbqcmmlj14.  potentials[nqc] (atomic units); grads_e: gradients of
+ * the energy part, grads_q: those of NBModelABFS_QCMMGradients alone (either
+ * may be NULL; grads_q needs grads_e).  Returns 1 if the lists were rebuilt. */
+int ref_qcmm_energy(void *vh, const double *coords, const double *qcq, double *grads_e, double *grads_q,
+                    double *energies, int *present, double *potentials)
+{
+    RefHandle *h = (RefHandle *)vh;
+    Status status = Status_Continue;
+    if (!h->qc) return -1;
+    for (int i = 0; i < h->n; i++) {
+        Coordinates3_SetRow(h->crd, i, coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]);
+        Coordinates3_SetRow(h->grd, i, 0.0, 0.0, 0.0);
+    }
+    NBModelABFSState_Initialize(h->state, h->crd, NULL, grads_e ? h->grd : NULL, NULL);
+    Boolean up = NBModelABFS_Update(h->model, h->gen, h->state, &status);
+    if (status != Status_Continue) return -1;
+    NBModelABFS_MMMMEnergy(h->model, h->pw, h->state);
+    NBModelABFS_QCMMEnergyLJ(h->model, h->pw, h->state);
+    energies[0] = h->state->emmel;
+    energies[1] = h->state->emmlj;
+    energies[2] = h->state->emmel14;
+    energies[3] = h->state->emmlj14;
+    energies[4] = h->state->eqcmmlj;
+    energies[5] = h->state->eqcmmlj14;
+    present[0] = (h->state->nbmmmm != NULL);
+    present[1] = (h->state->nbmmmm14 != NULL);
+    present[2] = (h->state->nbqcmmlj != NULL);
+    present[3] = (h->state->nbqcmmlj14 != NULL);
+    NBModelABFS_QCMMPotentials(h->model, h->pwqcmm, h->pwqcqc, h->state);
+    for (int q = 0; q < h->qc->natoms; q++) potentials[q] = Real1DArray_Item(h->qcmmPotentials, q);
+    if (grads_e) {
+        copy_rows(h->grd, grads_e, h->n);
+        if (grads_q) {
+            Coordinates3_Set(h->grd, 0.0);
+            for (int q = 0; q < h->qc->natoms; q++) Real1DArray_Item(h->qcCharges, q) = qcq[q];
+            NBModelABFS_QCMMGradients(h->model, h->pwqcmm, h->pwqcqc, h->state);
+            copy_rows(h->grd, grads_q, h->n);
+        }
+    }
+    return up == True ? 1 : 0;
+}
+
+/* QC/MM lists in reference order: which = 0 nbqcmmel (QC atom position, MM
+ * site), 1 nbqcmmlj (atom, atom), 2 nbqcmmel14 (QC position, atom),
+ * 3 nbqcmmlj14 (atom, atom). */
+long ref_qcmm_pairs(void *vh, int which, int *oi, int *oj, long cap)
+{
+    RefHandle *h = (RefHandle *)vh;
+    PairList *pl = which == 0 ? h->state->nbqcmmel : which == 1 ? h->state->nbqcmmlj
+                 : which == 2 ? h->state->nbqcmmel14 : h->state->nbqcmmlj14;
+    return dump_list(pl, oi, oj, cap);
+}
+
+/* The QC atom coordinates and the MM sites of the QC/MM electrostatics (the
+ * atoms for MM coupling, the pure MM atoms plus the redistributed sites for
+ * RC/RD) with their charges, after the last call.  Returns the site count. */
+long ref_qcmm_sites(void *vh, double *qcxyz, double *mmxyz, double *mmq, long cap)
+{
+    RefHandle *h = (RefHandle *)vh;
+    NBModelABFSState *st = h->state;
+    long m = st->mmCharges ? st->mmCharges->length : 0;
+    if (qcxyz) copy_rows(st->qcCoordinates3, qcxyz, h->qc->natoms);
+    for (long k = 0; k < m && k < cap; k++) {
+        double x, y, z;
+        Coordinates3_GetRow(st->mmCoordinates3, k, x, y, z);
+        if (mmxyz) { mmxyz[3 * k] = x; mmxyz[3 * k + 1] = y; mmxyz[3 * k + 2] = z; }
+        if (mmq) mmq[k] = Real1DArray_Item(st->mmCharges, k);
+    }
+    return m;
 }
 
 /* One System.Energy-style call.  energies[4] = MM/MM Elect., LJ, 1-4 Elect.,
@@ -319,5 +508,11 @@ void ref_destroy(void *vh)
     Transformation3Container_Deallocate(&h->trans);
     SymmetryParameters_Deallocate(&h->sp);
     SymmetryParameterGradients_Deallocate(&h->spg);
+    QCAtomContainer_Deallocate(&h->qc);
+    Selection_Deallocate(&h->fixed);
+    Real1DArray_Deallocate(&h->qcCharges);
+    Real1DArray_Deallocate(&h->qcmmPotentials);
+    PairwiseInteractionABFS_Deallocate(&h->pwqcmm);
+    PairwiseInteractionABFS_Deallocate(&h->pwqcqc);
     free(h);
 }

@@ -260,7 +260,7 @@ __device__ __forceinline__ float box_gap2(float4 alo, float4 ahi, float4 blo, fl
 // super-cluster come out in a deterministic order.
 //
 // The reference pair predicate for atoms i, j (PairListGenerator.c:103-143,
-// 1020-1150): not excluded, OR selection (free atoms), and either the grid
+// 1020-1150): both atoms in the MM selection, not excluded, OR selection (free atoms), and either the grid
 // rule (both on grid; stencil flag 2 = whole cell pair within the cutoff,
 // 1 = FP64 distance test, 0 = out of range) or, for the direct method, the
 // FP64 distance test alone.
@@ -428,7 +428,12 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
                     const int li = c * NB_CL + a;
                     const int i = sIatom[li];
                     bool ok = (i >= 0) && (jd.atom >= 0) && (CROSS || (Jc < I0 + c) || (b < a));
-                    if (ok && P.free_flags) ok = (sIfree[li] | jd.free_) != 0;
+                    if (ok && P.free_flags) {
+                        // Atom flags: bit 0 free (the OR selection), bit 1 outside the MM
+                        // selection (pure QC atoms, NBModelABFSState.c:339-341).
+                        const int fl = sIfree[li] | jd.free_;
+                        ok = (fl & 1) && !(fl & 2);
+                    }
                     int flag = 1;
                     if (CROSS) {
                         if (ok && P.use_grid) ok = sIcell[li].w != 0;

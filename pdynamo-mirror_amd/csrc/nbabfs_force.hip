@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
             x = xyz[3 * (size_t)a];
             y = xyz[3 * (size_t)a + 1];
             z = xyz[3 * (size_t)a + 2];
-            if (chk.ref && (!chk.free_flags || chk.free_flags[a])) {
+            if (chk.ref && (!chk.free_flags || (chk.free_flags[a] & 1))) {
 #pragma clang fp contract(off)
                 const double dx = x - chk.ref[3 * (size_t)a];
                 const double dy = y - chk.ref[3 * (size_t)a + 1];
@@ -805,6 +805,8 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
         if (D.eimgpart) sum_range2(D.eimgpart, D.n_img_items, iel, ilj);
     }
     sum_range2(D.epart14, D.n14_blocks, el14, lj14);
+    double qlj = 0.0, qlj14 = 0.0;
+    if (D.qljpart) sum_range2(D.qljpart, D.n_qlj, qlj, qlj14);
     if (threadIdx.x == 0) {
         D.energies[0] = el;
         D.energies[1] = lj;
@@ -812,7 +814,73 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
         D.energies[3] = lj14;
         D.energies[4] = iel;
         D.energies[5] = ilj;
+        D.energies[6] = qlj;
+        D.energies[7] = qlj14;
     }
+}
+
+// ------------------------------------------------------------------------------------------------
+// QC/MM LJ (NBModelABFS_QCMMEnergyLJ, pMolecule-1.9.0/extensions/csource/NBModelABFS.c:306-378):
+// the analytic pair terms with no electrostatics over the QC/MM LJ list (the main LJ table) and
+// its 1-4 list (the 1-4 table), one workgroup per pure QC atom; gradients into the fixed-point
+// accumulators, the row's two energies into qljpart.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_qclj(const SysDesc *__restrict__ descs, AbfsFactors F)
+{
+#pragma clang fp contract(off)
+    const SysDesc &D = descs[0];
+    const int r = blockIdx.x;
+    const int i = D.qlj_atom[r];
+    const double xi = D.xyz[3 * (size_t)i], yi = D.xyz[3 * (size_t)i + 1], zi = D.xyz[3 * (size_t)i + 2];
+    const int ti = D.ntypes * D.atype[i];
+    const int si = D.atom_slot[i];
+    double e[2] = {0.0, 0.0}, g[3] = {0.0, 0.0, 0.0};
+    for (int list = 0; list < 2; list++) {
+        const int *ptr = list ? D.qlj14_ptr : D.qlj_ptr;
+        const int *idx = list ? D.qlj14_idx : D.qlj_idx;
+        const double *tA = list ? D.ljA14 : D.ljA;
+        const double *tB = list ? D.ljB14 : D.ljB;
+        for (int k = ptr[r] + threadIdx.x; k < ptr[r + 1]; k += 256) {
+            const int j = idx[k];
+            double xij = xi - D.xyz[3 * (size_t)j], yij = yi - D.xyz[3 * (size_t)j + 1],
+                   zij = zi - D.xyz[3 * (size_t)j + 2];
+            const double r2 = xij * xij + yij * yij + zij * zij;
+            const int tij = ti + D.atype[j];
+            double x1, x2, dF;
+            if (!pair_fp64(F, r2, 0.0, tA[tij], tB[tij], x1, x2, dF)) continue;
+            e[list] += x2;
+            xij *= (2.0 * dF);
+            yij *= (2.0 * dF);
+            zij *= (2.0 * dF);
+            g[0] += xij;
+            g[1] += yij;
+            g[2] += zij;
+            const int sj = D.atom_slot[j];
+            fx_atomic_add(D.facc, D.n_pad, sj, 0, -xij);
+            fx_atomic_add(D.facc, D.n_pad, sj, 1, -yij);
+            fx_atomic_add(D.facc, D.n_pad, sj, 2, -zij);
+        }
+    }
+    // block_sum2's shared arrays are reused: a barrier after each sum.
+    block_sum2<256>(e[0], e[1]);
+    __syncthreads();
+    double gz = g[2];
+    block_sum2<256>(g[0], g[1]);
+    __syncthreads();
+    double zz = 0.0;
+    block_sum2<256>(gz, zz);
+    if (threadIdx.x == 0) {
+        D.qljpart[2 * r] = e[0];
+        D.qljpart[2 * r + 1] = e[1];
+        fx_atomic_add(D.facc, D.n_pad, si, 0, g[0]);
+        fx_atomic_add(D.facc, D.n_pad, si, 1, g[1]);
+        fx_atomic_add(D.facc, D.n_pad, si, 2, gz);
+    }
+}
+
+void launch_qclj(const SysDesc *d_desc, int nrows, const AbfsFactors &F, hipStream_t s)
+{
+    if (nrows > 0) hipLaunchKernelGGL(k_qclj, dim3(nrows), dim3(256), 0, s, d_desc, F);
 }
 
 // ------------------------------------------------------------------------------------------------

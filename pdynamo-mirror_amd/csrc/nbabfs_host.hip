@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void k_check_batch(const CheckDesc *__restrict
     const CheckDesc &C = cd[blockIdx.y];
     int i = blockIdx.x * 256 + threadIdx.x;
     bool up = false;
-    if (C.ref && i < C.n && (!C.free_flags || C.free_flags[i])) {
+    if (C.ref && i < C.n && (!C.free_flags || (C.free_flags[i] & 1))) {
         double dx = C.xyz[3 * (size_t)i] - C.ref[3 * (size_t)i];
         double dy = C.xyz[3 * (size_t)i + 1] - C.ref[3 * (size_t)i + 1];
         double dz = C.xyz[3 * (size_t)i + 2] - C.ref[3 * (size_t)i + 2];
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(256) void k_max_displacement(const double *__restri
 #pragma clang fp contract(off)
     const int i = blockIdx.x * 256 + threadIdx.x;
     double r2 = 0.0;
-    if (i < n && (!free_flags || free_flags[i])) {
+    if (i < n && (!free_flags || (free_flags[i] & 1))) {
         const double dx = xyz[3 * (size_t)i] - ref[3 * (size_t)i];
         const double dy = xyz[3 * (size_t)i + 1] - ref[3 * (size_t)i + 1];
         const double dz = xyz[3 * (size_t)i + 2] - ref[3 * (size_t)i + 2];
@@ -316,13 +316,20 @@ struct nbabfs_state {
     double escale = 0, escale14 = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    std::vector<long long> p14_keys;   // canonical 1-4 set
+    std::vector<long long> p14_keys;   // canonical 1-4 set (after the MM and free selections)
+    std::vector<long long> p14_all;    // canonical 1-4 set as given
+    std::vector<unsigned char> fixed_h;
+    std::vector<int> xptr_h, xidx_h;   // exclusions CSR (host copy)
+    std::vector<double> charges_h;
+    const double *upd_xyz = nullptr;   // coordinates of the last update
 
     // Constant device data.
     DBuf<double> charge, ljA, ljB, ljA14, ljB14, ljrow;
     DBuf<int> atype, xptr, xidx, p14;
+    // Atom flags: bit 0 = free (not fixed), bit 1 = pure QC atom (outside the MM selection).
+    // has_flags: the list search reads them; has_fixed: some atom is fixed (displacement test).
     DBuf<unsigned char> freef;
-    bool has_fixed = false;
+    bool has_fixed = false, has_flags = false;
 
     // Coordinates.
     DBuf<double> xyz_stage, xyz_ref, grad_stage;
@@ -412,6 +419,24 @@ struct nbabfs_state {
     PinBuf<ImageXf> h_imgxf;
     PinBuf<long long> h_gacc;
     PinBuf<unsigned long long> h_inpairs;
+
+    // QC/MM (NBModelABFSState_SetUp with a QCAtomContainer): the QC atoms in
+    // ascending atom order, link (boundary) atoms with their QC partner, link
+    // factor and MM partners; the MM sites of the QC/MM electrostatics (the
+    // atoms for MM coupling; the pure MM atoms plus the redistributed charges
+    // for RC/RD); the lists of the last build and the electrostatic spline.
+    bool has_qc = false;
+    int nqc = 0, coupling = 0, linkRatio = 0, nsites = 0, nlj = 0;
+    std::vector<unsigned char> qc_pure;          // per atom: pure QC atom
+    std::vector<int> qc_index_h, qc_partner_h, lj_atom_h;
+    DBuf<int> qc_index, qc_partner, site_a, site_p, qex_ptr, qex_idx, el14_ptr, el14_idx, lj_atom, lj14_ptr,
+        lj14_idx, el_ptr, el_idx, lj_ptr, lj_idx, qcnt;
+    DBuf<double> qc_lf, site_q, spx, spy, sph, qpot, qcq, qgrad, qljpart, qxyz_out;
+    DBuf<long long> qgacc;
+    PinBuf<double> h_qpot, h_qgrad;
+    int nspline = 0;
+    long long n_el = 0, n_lj = 0, n_el14 = 0, n_lj14 = 0;
+    double sumQCMMElPairs = 0.0, sumQCMMLJPairs = 0.0;
 
     // Kernel timing.
     bool timing = false;
@@ -752,6 +777,8 @@ static int order_items(nbabfs_state *S)
     return NBABFS_STATUS_CONTINUE;
 }
 
+static int build_qcmm_lists(nbabfs_state *S, const double *d_xyz);
+
 // ------------------------------------------------------------------------------------------------
 // List build (NBModelABFSState_GridInitialize + GenerateLists).
 // ------------------------------------------------------------------------------------------------
@@ -867,7 +894,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     P.use_grid = g.use_grid;
     P.xptr = S->xptr.p;
     P.xidx = S->xidx.p;
-    P.free_flags = S->has_fixed ? S->freef.p : nullptr;
+    P.free_flags = S->has_flags ? S->freef.p : nullptr;
     P.reach = (float)(o.listCutoff + 0.01);
     P.cw = (float)cw;
     P.ncx = g.ncx;
@@ -944,6 +971,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
         S->csrValid = true;
     }
     HIPCHK(hipGetLastError());
+    if (S->has_qc) CHK(build_qcmm_lists(S, d_xyz));
     S->listValid = true;
     return NBABFS_STATUS_CONTINUE;
 }
@@ -970,6 +998,7 @@ static int ensure_desc(nbabfs_state *S, const double *d_xyz, double *d_grad)
 static int ensure_call_buffers(nbabfs_state *S)
 {
     CHK(S->energies.ensure(NB_NENERGIES));
+    if (S->has_qc) CHK(S->qljpart.ensure(2 * (size_t)std::max(S->nlj, 1)));
     if (S->opt.precision == NBABFS_PLAIN_FP64) CHK(S->eimgpart.ensure(2 * (size_t)std::max(S->nitems - S->nitems_main, 1)));
     CHK(S->eacc.ensure(NB_EACC_WORDS));
     CHK(S->vacc.ensure(NB_VACC_WORDS));
@@ -992,6 +1021,7 @@ static bool check_for_image_update(const nbabfs_state *S, double maxdisp);
 static int update(nbabfs_state *S, const double *d_xyz, int *rebuilt)
 {
     hipStream_t st = S->stream;
+    S->upd_xyz = d_xyz;
     bool doUpdate = S->isNew || !S->have_ref || !S->listValid;
     bool doImages = false;
     S->forced_xyz = nullptr;
@@ -1114,6 +1144,13 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.nimg = (int)S->slots.size();
     D.imgxf = D.nimg ? S->imgxf.p : nullptr;
     D.gacc = D.nimg ? S->gacc.p : nullptr;
+    D.n_qlj = S->has_qc ? S->nlj : 0;
+    D.qlj_atom = S->lj_atom.p;
+    D.qlj_ptr = S->lj_ptr.p;
+    D.qlj_idx = S->lj_idx.p;
+    D.qlj14_ptr = S->lj14_ptr.p;
+    D.qlj14_idx = S->lj14_idx.p;
+    D.qljpart = D.n_qlj > 0 ? S->qljpart.p : nullptr;
     D.n_img_items = S->nitems - S->nitems_main;
     D.eimgpart = (S->opt.precision == NBABFS_PLAIN_FP64 && D.n_img_items > 0) ? S->eimgpart.p : nullptr;
 }
@@ -1188,7 +1225,7 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
     // finalize pass, which reads the descriptor after the pair kernel.
     CHK(ensure_desc(S, d_xyz, d_grad));
     if (S->n == 0) {
-        launch_fill_int((int *)S->energies.p, 12, 0, st);
+        launch_fill_int((int *)S->energies.p, 2 * NB_NENERGIES, 0, st);
         HIPCHK(hipGetLastError());
         return NBABFS_STATUS_CONTINUE;
     }
@@ -1199,6 +1236,7 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
     }
     S->forced_xyz = nullptr;
     launch_pairs14(S->d_desc.p, 1, nblk(S->n14, 256), S->F, st);
+    if (S->has_qc) launch_qclj(S->d_desc.p, S->nlj, S->F, st);
     launch_finalize(S->d_desc.p, 1, S->n, plain ? 1 : 0, st);
     HIPCHK(hipGetLastError());
     return NBABFS_STATUS_CONTINUE;
@@ -1406,7 +1444,7 @@ static int build_images(nbabfs_state *S, const double *d_xyz)
             std::memset(&P, 0, sizeof(P));
             P.cutoff2 = cutoff * cutoff;
             P.use_grid = S->useGrid ? 1 : 0;
-            P.free_flags = S->has_fixed ? S->freef.p : nullptr;
+            P.free_flags = S->has_flags ? S->freef.p : nullptr;
             P.reach = (float)(cutoff + 0.01);
             P.ixyz = S->ixyz.p;
             P.iblo = S->iblo.p;
@@ -1618,6 +1656,30 @@ static int autotune(nbabfs_state *S, const double *d_xyz, int *best_epi, int *be
 // ------------------------------------------------------------------------------------------------
 // C ABI.
 // ------------------------------------------------------------------------------------------------
+// The MM/MM 1-4 list of the state: SelfPairList_FromSelfPairList (pCore-1.9.0/extensions/csource/
+// PairList.c:342-453) of the given 1-4 pairs with the MM selection (no pure QC atom) as the AND
+// selection and the free atoms as the OR selection (NBModelABFS.c:1113-1128).
+static int select_14(nbabfs_state *S)
+{
+    const int n = S->n;
+    std::vector<long long> keep;
+    keep.reserve(S->p14_all.size());
+    for (long long key : S->p14_all) {
+        const int i = (int)(key / n), j = (int)(key % n);
+        if (!S->qc_pure.empty() && (S->qc_pure[i] || S->qc_pure[j])) continue;
+        if (!S->fixed_h.empty() && S->fixed_h[i] && S->fixed_h[j]) continue;
+        keep.push_back(key);
+    }
+    S->p14_keys = keep;
+    S->n14 = (int)keep.size();
+    std::vector<int> p14(2 * std::max<size_t>(keep.size(), 1), 0);
+    for (size_t k = 0; k < keep.size(); k++) {
+        p14[2 * k] = (int)(keep[k] / n);
+        p14[2 * k + 1] = (int)(keep[k] % n);
+    }
+    return S->p14.upload(p14.data(), p14.size(), S->stream);
+}
+
 extern "C" int nbabfs_state_create(const nbabfs_options *opt, int natoms, const double *charges, const int *ljtypes,
                                    int ntypes, const double *tableA, const double *tableB, const double *tableA14,
                                    const double *tableB14, int nexclusions, const int *exclusions,
@@ -1747,13 +1809,9 @@ extern "C" int nbabfs_state_create(const nbabfs_options *opt, int natoms, const 
     }
     std::sort(pk.begin(), pk.end());
     pk.erase(std::unique(pk.begin(), pk.end()), pk.end());
-    S->p14_keys = pk;
-    S->n14 = (int)pk.size();
-    std::vector<int> p14(2 * pk.size());
-    for (size_t k = 0; k < pk.size(); k++) {
-        p14[2 * k] = (int)(pk[k] / natoms);
-        p14[2 * k + 1] = (int)(pk[k] % natoms);
-    }
+    S->p14_all = pk;
+    S->xptr_h = xptr;
+    S->xidx_h = xidx;
     if (fixedAtoms) {
         std::vector<unsigned char> fr(natoms);
         bool any = false;
@@ -1763,10 +1821,14 @@ extern "C" int nbabfs_state_create(const nbabfs_options *opt, int natoms, const 
         }
         if (any) {
             if ((rc = S->freef.upload(fr.data(), fr.size(), st)) != NBABFS_STATUS_CONTINUE) return bail(rc);
-            S->has_fixed = true;
+            S->has_fixed = S->has_flags = true;
+            S->fixed_h.assign(fixedAtoms, fixedAtoms + natoms);
         }
     }
+    // 1-4 list: SelfPairList_FromSelfPairList with the free atoms as the OR selection.
+    if ((rc = select_14(S)) != NBABFS_STATUS_CONTINUE) return bail(rc);
     std::vector<double> q(charges, charges + natoms);
+    S->charges_h = q;
     std::vector<int> t(ljtypes, ljtypes + natoms);
     if ((rc = S->charge.upload(q.data(), q.size(), st)) != NBABFS_STATUS_CONTINUE ||
         (rc = S->atype.upload(t.data(), t.size(), st)) != NBABFS_STATUS_CONTINUE ||
@@ -1776,8 +1838,7 @@ extern "C" int nbabfs_state_create(const nbabfs_options *opt, int natoms, const 
         (rc = S->ljA14.upload(A14.data(), nt2, st)) != NBABFS_STATUS_CONTINUE ||
         (rc = S->ljB14.upload(B14.data(), nt2, st)) != NBABFS_STATUS_CONTINUE ||
         (rc = S->xptr.upload(xptr.data(), xptr.size(), st)) != NBABFS_STATUS_CONTINUE ||
-        (rc = S->xidx.upload(xidx.data(), xidx.size(), st)) != NBABFS_STATUS_CONTINUE ||
-        (rc = S->p14.upload(p14.data(), p14.size(), st)) != NBABFS_STATUS_CONTINUE)
+        (rc = S->xidx.upload(xidx.data(), xidx.size(), st)) != NBABFS_STATUS_CONTINUE)
         return bail(rc);
     if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(NBABFS_STATUS_LOGIC_ERROR, "upload failed"));
     *out = S;
@@ -1894,6 +1955,10 @@ extern "C" int nbabfs_stats_get(nbabfs_state *S, nbabfs_stats *out)
     out->imagePairs = S->img_npairs;
     out->numberOfMMMMImageImages = S->sumImages;
     out->numberOfMMMMImagePairs = S->sumImagePairs;
+    out->qcmmElPairs = S->n_el;
+    out->qcmmLJPairs = S->n_lj;
+    out->numberOfQCMMElPairs = S->sumQCMMElPairs;
+    out->numberOfQCMMLJPairs = S->sumQCMMLJPairs;
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -1905,6 +1970,8 @@ extern "C" int nbabfs_set_transformations(nbabfs_state *S, int ntransformations,
 {
     if (!S) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state is NULL");
     if (!S->isNew) return fail(NBABFS_STATUS_LOGIC_ERROR, "transformations must be set before the first update");
+    if (S->has_qc && ntransformations > 0)
+        return fail(NBABFS_STATUS_VALUE_ERROR, "QC/MM image lists are not supported: the state has QC atoms");
     if (ntransformations < 0 || (ntransformations > 0 && (!rotations || !translations)) || imageExpandFactor < 0)
         return fail(NBABFS_STATUS_INVALID_ARGUMENT, "invalid transformations");
     S->has_sym = ntransformations > 0;
@@ -2243,6 +2310,10 @@ extern "C" int nbabfs_batch_create(nbabfs_state *const *states, int nsystems, in
             delete B;
             return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state %d has crystal symmetry; batches take non-periodic states", k);
         }
+        if (states[k]->has_qc) {
+            delete B;
+            return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state %d has QC atoms; batches take MM states", k);
+        }
         B->st.push_back(states[k]);
     }
     B->mixed = states[0]->opt.precision == NBABFS_MIXED;
@@ -2421,4 +2492,473 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
     for (int k = 0; k < ns; k++)
         std::memcpy(energies + 4 * (size_t)k, B->h_energies.p + NB_NENERGIES * (size_t)k, 4 * sizeof(double));
     return NBABFS_STATUS_CONTINUE;
+}
+
+// ------------------------------------------------------------------------------------------------
+// QC/MM (NBModelABFSState_SetUp with QC atoms, pMolecule-1.9.0/extensions/csource/
+// NBModelABFSState.c:323-361; GenerateLists / GenerateLists14, NBModelABFS.c:1050-1155;
+// NBModelABFS_QCMMEnergyLJ / _QCMMPotentials / _QCMMGradients, NBModelABFS.c:306-498).
+// ------------------------------------------------------------------------------------------------
+static const double kHartreesToKJ = 2625.5;   // UNITS_ENERGY_HARTREES_TO_KILOJOULES_PER_MOLE (Units.h:52)
+
+static QcmmGeom qcmm_geom(nbabfs_state *S, const double *d_xyz)
+{
+    QcmmGeom G;
+    G.xyz = d_xyz;
+    G.n = S->n;
+    G.qc_index = S->qc_index.p;
+    G.qc_partner = S->qc_partner.p;
+    G.qc_lf = S->qc_lf.p;
+    G.nqc = S->nqc;
+    G.linkRatio = S->linkRatio;
+    G.site_a = S->coupling == NBABFS_QCMM_MM ? nullptr : S->site_a.p;
+    G.site_p = S->coupling == NBABFS_QCMM_MM ? nullptr : S->site_p.p;
+    G.nsites = S->nsites;
+    return G;
+}
+
+static int upload_csr(DBuf<int> &ptr, DBuf<int> &idx, const std::vector<std::vector<int>> &rows, hipStream_t st)
+{
+    std::vector<int> p(rows.size() + 1, 0), x;
+    for (size_t r = 0; r < rows.size(); r++) {
+        p[r + 1] = p[r] + (int)rows[r].size();
+        x.insert(x.end(), rows[r].begin(), rows[r].end());
+    }
+    if (x.empty()) x.push_back(0);
+    CHK(ptr.upload(p.data(), p.size(), st));
+    CHK(idx.upload(x.data(), x.size(), st));
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_set_qc_atoms(nbabfs_state *S, int nqc, const int *index, const int *boundary,
+                                   const int *qcpartner, const double *linkfactor, int linkRatio,
+                                   const int *mmpartnerOffsets, const int *mmpartners, int coupling)
+{
+    if (!S) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state is NULL");
+    if (!S->isNew || S->listValid) return fail(NBABFS_STATUS_LOGIC_ERROR, "QC atoms must be set before the first update");
+    if (S->has_qc) return fail(NBABFS_STATUS_LOGIC_ERROR, "QC atoms are already set");
+    if (S->has_sym) return fail(NBABFS_STATUS_VALUE_ERROR, "QC/MM image lists are not supported: the state has symmetry");
+    if (nqc <= 0 || !index) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "no QC atoms");
+    if (coupling < NBABFS_QCMM_MM || coupling > NBABFS_QCMM_RD)
+        return fail(NBABFS_STATUS_VALUE_ERROR, "unknown QC/MM coupling %d", coupling);
+    const int n = S->n;
+    std::vector<unsigned char> full(n, 0), pure(n, 0), bnd(n, 0);
+    for (int q = 0; q < nqc; q++) {
+        if (index[q] < 0 || index[q] >= n || (q > 0 && index[q] <= index[q - 1]))
+            return fail(NBABFS_STATUS_INVALID_ARGUMENT, "QC atom indices must be ascending atoms in [0,%d)", n);
+        full[index[q]] = 1;
+    }
+    std::vector<int> part(nqc, -1);
+    std::vector<double> lf(nqc, 0.0);
+    std::vector<std::vector<int>> mmp(nqc);
+    for (int q = 0; q < nqc; q++) {
+        const bool b = boundary && boundary[q];
+        if (!b) {
+            pure[index[q]] = 1;
+            continue;
+        }
+        if (!qcpartner || !linkfactor || !mmpartnerOffsets)
+            return fail(NBABFS_STATUS_INVALID_ARGUMENT, "boundary atom %d without partner data", index[q]);
+        bnd[index[q]] = 1;
+        part[q] = qcpartner[q];
+        lf[q] = linkfactor[q];
+        const int o0 = mmpartnerOffsets[q], o1 = mmpartnerOffsets[q + 1];
+        if (o1 < o0 || (o1 > o0 && !mmpartners))
+            return fail(NBABFS_STATUS_INVALID_ARGUMENT, "bad MM partner range of boundary atom %d", index[q]);
+        if (o1 == o0 && coupling != NBABFS_QCMM_MM)
+            return fail(NBABFS_STATUS_VALUE_ERROR, "boundary atom %d has no MM partners (RC/RD coupling)", index[q]);
+        for (int k = o0; k < o1; k++) mmp[q].push_back(mmpartners[k]);
+    }
+    for (int q = 0; q < nqc; q++) {
+        if (part[q] < 0 && !(boundary && boundary[q])) continue;
+        if (part[q] < 0 || part[q] >= n || !full[part[q]] || bnd[part[q]])
+            return fail(NBABFS_STATUS_INVALID_ARGUMENT, "boundary atom %d: QC partner %d is not a pure QC atom",
+                        index[q], part[q]);
+        for (int p : mmp[q])
+            if (p < 0 || p >= n || pure[p])
+                return fail(NBABFS_STATUS_INVALID_ARGUMENT, "boundary atom %d: MM partner %d is not an MM atom",
+                            index[q], p);
+    }
+    hipStream_t st = S->stream;
+    S->nqc = nqc;
+    S->coupling = coupling;
+    S->linkRatio = linkRatio ? 1 : 0;
+    S->qc_pure = pure;
+    S->qc_index_h.assign(index, index + nqc);
+    S->qc_partner_h = part;
+    // Atom flags: bit 0 free, bit 1 pure QC.
+    std::vector<unsigned char> fl(n);
+    for (int i = 0; i < n; i++)
+        fl[i] = (unsigned char)(((S->fixed_h.empty() || !S->fixed_h[i]) ? 1 : 0) | (pure[i] ? 2 : 0));
+    CHK(S->freef.upload(fl.data(), fl.size(), st));
+    S->has_flags = true;
+    CHK(select_14(S));
+    CHK(S->qc_index.upload(index, nqc, st));
+    CHK(S->qc_partner.upload(part.data(), nqc, st));
+    CHK(S->qc_lf.upload(lf.data(), nqc, st));
+    // 1-4 neighbours and exclusion neighbours per atom.
+    std::vector<std::vector<int>> n14(n);
+    for (long long key : S->p14_all) {
+        const int i = (int)(key / n), j = (int)(key % n);
+        n14[i].push_back(j);
+        n14[j].push_back(i);
+    }
+    auto freeat = [&](int a) { return S->fixed_h.empty() || !S->fixed_h[a]; };
+    // MM sites and charges: MMAtomContainer_GetCharges of the active atoms (MM coupling) or
+    // QCAtomContainer_GetMMCharges (RC/RD, QCAtomContainer.c:465-525).
+    std::vector<double> sq;
+    std::vector<int> sa, sp;
+    if (coupling == NBABFS_QCMM_MM) {
+        sq = S->charges_h;
+        for (int i = 0; i < n; i++)
+            if (pure[i]) sq[i] = 0.0;
+        S->nsites = n;
+    } else {
+        std::vector<int> pos(n, -1);
+        for (int i = 0; i < n; i++)
+            if (!full[i]) {
+                pos[i] = (int)sa.size();
+                sa.push_back(i);
+                sp.push_back(-1);
+                sq.push_back(S->charges_h[i]);
+            }
+        for (int q = 0; q < nqc; q++) {
+            if (part[q] < 0) continue;
+            const int b = index[q];
+            const double q0 = S->charges_h[b] / (double)mmp[q].size();
+            for (int p : mmp[q]) {
+                sa.push_back(b);
+                sp.push_back(p);
+                if (bnd[p] || coupling == NBABFS_QCMM_RC) {
+                    sq.push_back(q0);
+                } else {
+                    sq.push_back(2.0 * q0);
+                    sq[pos[p]] -= q0;
+                }
+            }
+        }
+        S->nsites = (int)sa.size();
+        CHK(S->site_a.upload(sa.data(), sa.size(), st));
+        CHK(S->site_p.upload(sp.data(), sp.size(), st));
+    }
+    CHK(S->site_q.upload(sq.data(), sq.size(), st));
+    // qcmmExclusions (SelfPairList_ToCrossPairList of the exclusions, QC atoms x MM selection,
+    // the boundary atom itself included) and the QC/MM electrostatic 1-4 list (MM coupling only).
+    std::vector<std::vector<int>> qex(nqc), e14(nqc);
+    for (int q = 0; q < nqc; q++) {
+        const int i = index[q];
+        for (int k = S->xptr_h[i]; k < S->xptr_h[i + 1]; k++)
+            if (!pure[S->xidx_h[k]]) qex[q].push_back(S->xidx_h[k]);
+        if (!pure[i]) qex[q].push_back(i);
+        std::sort(qex[q].begin(), qex[q].end());
+        if (coupling == NBABFS_QCMM_MM)
+            for (int j : n14[i])
+                if (!pure[j] && j != i) e14[q].push_back(j);
+        std::sort(e14[q].begin(), e14[q].end());
+    }
+    if (coupling == NBABFS_QCMM_MM) CHK(upload_csr(S->qex_ptr, S->qex_idx, qex, st));
+    CHK(upload_csr(S->el14_ptr, S->el14_idx, e14, st));
+    S->n_el14 = 0;
+    for (auto &r : e14) S->n_el14 += (long long)r.size();
+    // QC/MM LJ rows (pure QC atoms) and their 1-4 lists (SelfPairList_ToCrossPairList with the
+    // free atoms as the OR selection).
+    std::vector<int> ljat;
+    std::vector<std::vector<int>> l14;
+    for (int q = 0; q < nqc; q++) {
+        const int i = index[q];
+        if (!pure[i]) continue;
+        ljat.push_back(i);
+        std::vector<int> r;
+        for (int j : n14[i])
+            if (!pure[j] && (freeat(i) || freeat(j))) r.push_back(j);
+        std::sort(r.begin(), r.end());
+        l14.push_back(r);
+    }
+    S->nlj = (int)ljat.size();
+    S->lj_atom_h = ljat;
+    if (ljat.empty()) ljat.push_back(0);
+    CHK(S->lj_atom.upload(ljat.data(), ljat.size(), st));
+    CHK(upload_csr(S->lj14_ptr, S->lj14_idx, l14, st));
+    S->n_lj14 = 0;
+    for (auto &r : l14) S->n_lj14 += (long long)r.size();
+    HIPCHK(hipStreamSynchronize(st));
+    S->has_qc = true;
+    S->desc_version = -1;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// Tridiagonal solve by Gaussian elimination with partial pivoting (LAPACK dgtsv's
+// published algorithm, the solver CubicSpline_MakeFromReal1DArrays calls): sub-diagonal dl
+// (n-1), diagonal d (n), super-diagonal du (n-1), right-hand side b (n, overwritten).
+static bool tridiagonal_solve(std::vector<double> dl, std::vector<double> d, std::vector<double> du,
+                              std::vector<double> &b)
+{
+#pragma clang fp contract(off)
+    const int n = (int)d.size();
+    for (int i = 0; i + 1 < n; i++) {
+        const bool last = i == n - 2;
+        if (std::fabs(d[i]) >= std::fabs(dl[i])) {
+            if (d[i] == 0.0) return false;
+            const double fact = dl[i] / d[i];
+            d[i + 1] = d[i + 1] - fact * du[i];
+            b[i + 1] = b[i + 1] - fact * b[i];
+            if (!last) dl[i] = 0.0;
+        } else {
+            const double fact = d[i] / dl[i];
+            d[i] = dl[i];
+            const double temp = d[i + 1];
+            d[i + 1] = du[i] - fact * temp;
+            if (!last) {
+                dl[i] = du[i + 1];
+                du[i + 1] = -fact * dl[i];
+            }
+            du[i] = temp;
+            const double tb = b[i];
+            b[i] = b[i + 1];
+            b[i + 1] = tb - fact * b[i + 1];
+        }
+    }
+    if (d[n - 1] == 0.0) return false;
+    b[n - 1] = b[n - 1] / d[n - 1];
+    if (n > 1) b[n - 2] = (b[n - 2] - du[n - 2] * b[n - 1]) / d[n - 2];
+    for (int i = n - 3; i >= 0; i--) b[i] = (b[i] - du[i] * b[i + 1] - dl[i] * b[i + 2]) / d[i];
+    return true;
+}
+
+extern "C" int nbabfs_set_qcmm_spline(nbabfs_state *S, int npoints, const double *x, const double *y)
+{
+#pragma clang fp contract(off)
+    if (!S || !x || !y) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
+    if (npoints < 2) return fail(NBABFS_STATUS_VALUE_ERROR, "a spline needs at least two points");
+    std::vector<double> X(x, x + npoints), Y(y, y + npoints);
+    // Decreasing abscissae are reversed, as the reference does.
+    if (X[0] > X[npoints - 1]) {
+        std::reverse(X.begin(), X.end());
+        std::reverse(Y.begin(), Y.end());
+    }
+    for (int i = 1; i < npoints; i++)
+        if (!(X[i] > X[i - 1])) return fail(NBABFS_STATUS_VALUE_ERROR, "spline abscissae must be strictly monotonic");
+    const int n = npoints;
+    std::vector<double> dl(n - 1), d(n), du(n - 1), b(n);
+    // Clamped ends (first derivatives 0) and the interior continuity conditions.
+    double h0 = X[1] - X[0];
+    d[0] = h0 / 3.0;
+    du[0] = h0 / 6.0;
+    b[0] = (Y[1] - Y[0]) / h0 - 0.0;
+    for (int i = 1; i < n - 1; i++) {
+        const double hl = X[i] - X[i - 1], hu = X[i + 1] - X[i];
+        dl[i - 1] = hl / 6.0;
+        d[i] = (hl + hu) / 3.0;
+        du[i] = hu / 6.0;
+        b[i] = (Y[i + 1] - Y[i]) / hu + (Y[i - 1] - Y[i]) / hl;
+    }
+    const double hn = X[n - 1] - X[n - 2];
+    dl[n - 2] = hn / 6.0;
+    d[n - 1] = hn / 3.0;
+    b[n - 1] = 0.0 - (Y[n - 1] - Y[n - 2]) / hn;
+    if (!tridiagonal_solve(dl, d, du, b)) return fail(NBABFS_STATUS_VALUE_ERROR, "singular spline system");
+    hipStream_t st = S->stream;
+    CHK(S->spx.upload(X.data(), n, st));
+    CHK(S->spy.upload(Y.data(), n, st));
+    CHK(S->sph.upload(b.data(), n, st));
+    HIPCHK(hipStreamSynchronize(st));
+    S->nspline = n;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// The QC/MM electrostatic list (QC atoms x MM sites, with the qcmmExclusions for MM coupling)
+// and the QC/MM LJ list (pure QC atoms x MM atoms, exclusions and the free-atom OR test), each
+// by a count pass, a scan and a fill pass.
+static int build_qcmm_lists(nbabfs_state *S, const double *d_xyz)
+{
+    hipStream_t st = S->stream;
+    const QcmmGeom G = qcmm_geom(S, d_xyz);
+    const double c2 = S->opt.listCutoff * S->opt.listCutoff;
+    const int rows_max = std::max(S->nqc, S->nlj) + 1;
+    CHK(S->qcnt.ensure(rows_max));
+    CHK(S->scantmp.ensure(scan_tmp_size(rows_max)));
+    CHK(S->h_int.ensure(2));
+    for (int list = 0; list < 2; list++) {
+        CrossListParams P{};
+        P.G = G;
+        P.cutoff2 = c2;
+        P.flags = S->freef.p;
+        DBuf<int> &ptr = list ? S->lj_ptr : S->el_ptr;
+        DBuf<int> &idx = list ? S->lj_idx : S->el_idx;
+        if (list == 0) {
+            P.rows_are_qc = 1;
+            P.nrows = S->nqc;
+            P.col_atoms = S->coupling == NBABFS_QCMM_MM;
+            if (P.col_atoms) {
+                P.ex_ptr = S->qex_ptr.p;
+                P.ex_idx = S->qex_idx.p;
+            }
+        } else {
+            P.rows_are_qc = 0;
+            P.row_atom = S->lj_atom.p;
+            P.nrows = S->nlj;
+            P.col_atoms = 1;
+            P.or_test = 1;
+            P.ex_ptr = S->xptr.p;
+            P.ex_idx = S->xidx.p;
+        }
+        CHK(ptr.ensure(P.nrows + 1));
+        long long total = 0;
+        if (P.nrows > 0) {
+            launch_fill_int(S->qcnt.p, P.nrows + 1, 0, st);
+            P.count = S->qcnt.p;
+            launch_cross_list(P, 0, st);
+            launch_scan(S->qcnt.p, ptr.p, P.nrows + 1, S->scantmp.p, st);
+            HIPCHK(hipMemcpyAsync(S->h_int.p, ptr.p + P.nrows, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            total = S->h_int.p[0];
+        } else {
+            launch_fill_int(ptr.p, 1, 0, st);
+        }
+        CHK(idx.ensure(std::max<long long>(total, 1)));
+        if (total > 0) {
+            P.ptr = ptr.p;
+            P.idx = idx.p;
+            launch_cross_list(P, 1, st);
+        }
+        HIPCHK(hipGetLastError());
+        if (list == 0) S->n_el = total;
+        else S->n_lj = total;
+    }
+    S->sumQCMMElPairs += (double)S->n_el;
+    S->sumQCMMLJPairs += (double)S->n_lj;
+    S->desc_version = -1;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+static int qcmm_ready(nbabfs_state *S, bool spline)
+{
+    if (!S) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state is NULL");
+    if (!S->has_qc) return fail(NBABFS_STATUS_LOGIC_ERROR, "the state has no QC atoms");
+    if (!S->listValid || !S->upd_xyz) return fail(NBABFS_STATUS_LOGIC_ERROR, "no lists: call an update first");
+    if (spline && S->nspline < 2) return fail(NBABFS_STATUS_LOGIC_ERROR, "no QC/MM spline: call nbabfs_set_qcmm_spline");
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_qcmm_terms(nbabfs_state *S, double *energies, int *present)
+{
+    if (!S || !energies) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
+    if (!S->has_qc) return fail(NBABFS_STATUS_LOGIC_ERROR, "the state has no QC atoms");
+    CHK(S->h_energies.ensure(NB_NENERGIES));
+    CHK(S->energies.ensure(NB_NENERGIES));
+    HIPCHK(hipMemcpyAsync(S->h_energies.p, S->energies.p, NB_NENERGIES * sizeof(double), hipMemcpyDeviceToHost,
+                          S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    energies[0] = S->h_energies.p[6];
+    energies[1] = S->h_energies.p[7];
+    if (present) {
+        present[0] = S->n_lj > 0;
+        present[1] = S->n_lj14 > 0;
+    }
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_qcmm_potentials(nbabfs_state *S, double *potentials)
+{
+    CHK(qcmm_ready(S, true));
+    if (!potentials) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "potentials is NULL");
+    hipStream_t st = S->stream;
+    const QcmmGeom G = qcmm_geom(S, S->upd_xyz);
+    const SplineDev sp{S->spx.p, S->spy.p, S->sph.p, S->nspline};
+    const bool mm = S->coupling == NBABFS_QCMM_MM;
+    const QcmmRows L{S->el_ptr.p, S->el_idx.p, mm ? S->el14_ptr.p : nullptr, mm ? S->el14_idx.p : nullptr};
+    double escale, escale14;
+    {
+#pragma clang fp contract(off)
+        escale = 1.0 / S->opt.dielectric;
+        escale14 = escale * S->opt.electrostaticScale14;
+    }
+    CHK(S->qpot.ensure(S->nqc));
+    CHK(S->h_qpot.ensure(S->nqc));
+    launch_qcmm_potentials(G, sp, L, S->site_q.p, escale, escale14, S->opt.outerCutoff * S->opt.outerCutoff,
+                           S->qpot.p, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(S->h_qpot.p, S->qpot.p, S->nqc * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int q = 0; q < S->nqc; q++) potentials[q] += S->h_qpot.p[q];
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_qcmm_gradients_host(nbabfs_state *S, const double *qcCharges, double *grad_inout)
+{
+    CHK(qcmm_ready(S, true));
+    if (!qcCharges || !grad_inout) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
+    hipStream_t st = S->stream;
+    const int n = S->n;
+    const QcmmGeom G = qcmm_geom(S, S->upd_xyz);
+    const SplineDev sp{S->spx.p, S->spy.p, S->sph.p, S->nspline};
+    const bool mm = S->coupling == NBABFS_QCMM_MM;
+    const QcmmRows L{S->el_ptr.p, S->el_idx.p, mm ? S->el14_ptr.p : nullptr, mm ? S->el14_idx.p : nullptr};
+    double scale, scale14;
+    {
+#pragma clang fp contract(off)
+        const double e = 1.0 / S->opt.dielectric;
+        scale = kHartreesToKJ * e;
+        scale14 = kHartreesToKJ * (e * S->opt.electrostaticScale14);
+    }
+    CHK(S->qcq.upload(qcCharges, S->nqc, st));
+    CHK(S->qgacc.ensure(6 * (size_t)std::max(n, 1)));
+    CHK(S->qgrad.ensure(3 * (size_t)std::max(n, 1)));
+    CHK(S->h_qgrad.ensure(3 * (size_t)std::max(n, 1)));
+    launch_fill_int((int *)S->qgacc.p, 12 * (size_t)n, 0, st);
+    launch_qcmm_gradients(G, sp, L, S->site_q.p, S->qcq.p, scale, scale14, S->opt.outerCutoff * S->opt.outerCutoff,
+                          S->qgacc.p, st);
+    launch_qcmm_grad_out(S->qgacc.p, n, S->qgrad.p, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(S->h_qgrad.p, S->qgrad.p, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (size_t k = 0; k < 3 * (size_t)n; k++) grad_inout[k] += S->h_qgrad.p[k];
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" long long nbabfs_export_qcmm_pairs(nbabfs_state *S, int which, long long *keys, long long cap)
+{
+    if (qcmm_ready(S, false) != NBABFS_STATUS_CONTINUE) return -1;
+    if (which < 0 || which > 3) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "which must be 0..3"), -1;
+    hipStream_t st = S->stream;
+    const bool rows_qc = which == 0 || which == 2;
+    const int nrows = rows_qc ? S->nqc : S->nlj;
+    DBuf<int> &ptr = which == 0 ? S->el_ptr : which == 1 ? S->lj_ptr : which == 2 ? S->el14_ptr : S->lj14_ptr;
+    DBuf<int> &idx = which == 0 ? S->el_idx : which == 1 ? S->lj_idx : which == 2 ? S->el14_idx : S->lj14_idx;
+    std::vector<int> p(nrows + 1, 0);
+    if (nrows > 0) HIPCHK(hipMemcpy(p.data(), ptr.p, (nrows + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    const long long total = p[nrows];
+    std::vector<int> x(std::max<long long>(total, 1));
+    if (total > 0) HIPCHK(hipMemcpy(x.data(), idx.p, total * sizeof(int), hipMemcpyDeviceToHost));
+    if (!keys) return total;
+    std::vector<long long> k;
+    k.reserve(total);
+    for (int r = 0; r < nrows; r++) {
+        const long long row = rows_qc ? r : S->lj_atom_h[r];
+        for (int e = p[r]; e < p[r + 1]; e++) k.push_back((row << 32) | (long long)x[e]);
+    }
+    std::sort(k.begin(), k.end());
+    for (long long e = 0; e < total && e < cap; e++) keys[e] = k[e];
+    (void)st;
+    return total;
+}
+
+extern "C" long long nbabfs_qcmm_sites(nbabfs_state *S, double *qcxyz, double *sitexyz, double *siteq, long long cap)
+{
+    if (qcmm_ready(S, false) != NBABFS_STATUS_CONTINUE) return -1;
+    hipStream_t st = S->stream;
+    const QcmmGeom G = qcmm_geom(S, S->upd_xyz);
+    const int m = S->nsites;
+    CHK(S->qxyz_out.ensure(3 * ((size_t)S->nqc + m)));
+    launch_qcmm_coords(G, S->qxyz_out.p, S->qxyz_out.p + 3 * (size_t)S->nqc, st);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(3 * ((size_t)S->nqc + m)), q(m);
+    HIPCHK(hipMemcpyAsync(h.data(), S->qxyz_out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(q.data(), S->site_q.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (qcxyz) std::memcpy(qcxyz, h.data(), 3 * (size_t)S->nqc * sizeof(double));
+    const long long w = std::min<long long>(cap, m);
+    if (sitexyz) std::memcpy(sitexyz, h.data() + 3 * (size_t)S->nqc, 3 * (size_t)w * sizeof(double));
+    if (siteq) std::memcpy(siteq, q.data(), (size_t)w * sizeof(double));
+    return m;
 }

@@ -114,6 +114,10 @@ struct SysDesc {
     int nimg;                   // images of the image table (gacc length / NB_GACC_WORDS_PER_IMAGE)
     int n_img_items;            // plain path: image work items (eimgpart holds 2 per item)
     double *eimgpart;           // plain path: FP64 image energies per work item, summed in order
+    // QC/MM LJ: rows = pure QC atoms, CSR lists of MM atoms (main and 1-4).
+    int n_qlj;
+    const int *qlj_atom, *qlj_ptr, *qlj_idx, *qlj14_ptr, *qlj14_idx;
+    double *qljpart;            // 2 per row (QC/MM LJ, QC/MM 1-4 LJ), summed in row order
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -170,8 +174,9 @@ __device__ __forceinline__ double fx_value(const long long *facc, int n_pad, int
            (double)facc[(size_t)comp * n_pad + slot] * (1.0 / NB_FX_LO_SCALE);
 }
 
-// Energies of one system: MM/MM Elect., LJ, 1-4 Elect., 1-4 LJ, Image Elect., Image LJ.
-#define NB_NENERGIES 6
+// Energies of one system: MM/MM Elect., LJ, 1-4 Elect., 1-4 LJ, Image Elect., Image LJ, QC/MM LJ,
+// QC/MM 1-4 LJ.
+#define NB_NENERGIES 8
 
 // Energy accumulators: [kind][lo word, hi word][slot], kinds 0 = Coulomb and
 // 1 = LJ of the main list, 2 and 3 of the image lists; a work item adds into

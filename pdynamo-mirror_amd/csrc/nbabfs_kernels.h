@@ -134,3 +134,53 @@ void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int n
                              ForceGuard g, hipStream_t s);
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s);
 void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s);
+
+// QC/MM electrostatics (nbabfs_qcmm.hip).  QC atoms in ascending atom order;
+// qc_partner[q] >= 0 marks a link atom (the QC partner of boundary atom
+// qc_index[q]) with link factor qc_lf[q].  MM sites: the atoms (site_a null)
+// or site k at atom site_a[k], or the midpoint of site_a[k] and site_p[k].
+struct QcmmGeom {
+    const double *xyz;
+    int n;
+    const int *qc_index, *qc_partner;
+    const double *qc_lf;
+    int nqc, linkRatio;
+    const int *site_a, *site_p;
+    int nsites;
+};
+
+struct CrossListParams {
+    QcmmGeom G;
+    int rows_are_qc;                    // rows: QC atoms (link coordinates), else atoms row_atom[r]
+    const int *row_atom;
+    int nrows;
+    const int *ex_ptr, *ex_idx;         // sorted excluded columns per row (QC rows) or per atom, or null
+    const unsigned char *flags;         // atom flags (bit 0 free, bit 1 pure QC) of atom columns, or null
+    int or_test;                        // apply the free-atom OR test
+    int col_atoms;                      // columns: atoms, else MM sites
+    double cutoff2;
+    int *count;
+    const int *ptr;
+    int *idx;
+};
+
+struct SplineDev {                      // CubicSpline: abscissae x (= r^2), ordinates y, second derivatives h
+    const double *x, *y, *h;
+    int n;
+};
+
+struct QcmmRows {                       // main and 1-4 lists of the QC atoms (CSR; the 1-4 pointers may be null)
+    const int *ptr, *idx, *ptr14, *idx14;
+};
+
+void launch_cross_list(const CrossListParams &P, int pass, hipStream_t s);
+void launch_qcmm_potentials(const QcmmGeom &G, const SplineDev &S, const QcmmRows &L, const double *site_q,
+                            double escale, double escale14, double r2off, double *pot, hipStream_t s);
+// gacc: six int64 arrays of n (x, y, z low words, then high words), fixed point as facc.
+void launch_qcmm_gradients(const QcmmGeom &G, const SplineDev &S, const QcmmRows &L, const double *site_q,
+                           const double *qcq, double scale, double scale14, double r2off, long long *gacc,
+                           hipStream_t s);
+void launch_qcmm_grad_out(const long long *gacc, int n, double *grad, hipStream_t s);
+void launch_qcmm_coords(const QcmmGeom &G, double *qcxyz, double *sitexyz, hipStream_t s);
+// QC/MM LJ (nbabfs_force.hip): the QC/MM LJ and 1-4 LJ lists of the descriptor's pure QC atoms.
+void launch_qclj(const SysDesc *d_desc, int nrows, const AbfsFactors &F, hipStream_t s);

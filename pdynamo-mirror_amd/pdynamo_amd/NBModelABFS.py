@@ -7,6 +7,11 @@ Same classes, option names, validation and labels as pDynamo 1.9.0:
 * ``NBModelABFS``       -- pMolecule-1.9.0/extensions/pyrex/pMolecule.NBModelABFS.pyx:15-302.
 * ``NBModelABFSState``  -- pMolecule-1.9.0/extensions/pyrex/pMolecule.NBModelABFSState.pyx:13-127.
 
+QC/MM systems (a ``QCAtomContainer`` passed to ``SetUp``): the MM/MM lists leave out the pure QC
+atoms, the QC/MM LJ terms join the energy, and ``QCMMPotentials`` / ``QCMMGradients`` give the
+QC model the electrostatic potentials at the QC atoms and add the gradients of its charges
+(pMolecule.NBModelABFS.pyx:72-97,108-138; NBModelABFS.c:306-498).  QC/MM image lists are not built.
+
 Energies and gradients are computed by libnbabfs.so on the GPU (cluster-pair
 list + FP32 pair kernel with fixed-point sums by default, or the plain FP64
 kernel with ``precision="fp64"``).  There is no CPU compute path.
@@ -19,6 +24,8 @@ import numpy as np
 from . import _native
 from ._native import CLibraryError
 from .LogFile import LogFileActive, logFile
+from .PairwiseInteraction import PairwiseInteractionABFS, UNITS_LENGTH_BOHRS_TO_ANGSTROMS
+from .QCAtoms import QCMMInteractionState
 
 _STATENAMES = ("nbState", "qcmmstate")
 
@@ -90,7 +97,9 @@ class NBModelABFSState:
         self.natoms = mmAtoms.size
         self.emmel = self.emmlj = self.emmel14 = self.emmlj14 = 0.0
         self.eimmmel = self.eimmmlj = 0.0
+        self.eqcmmlj = self.eqcmmlj14 = 0.0
         self.present = (False, False)
+        self.qcPresent = (False, False)
         self.imagesPresent = False
         self.lastRebuilt = False
 
@@ -102,6 +111,10 @@ class NBModelABFSState:
         if self.present[1]:
             energies.append(("MM/MM 1-4 Elect.", self.emmel14))
             energies.append(("MM/MM 1-4 LJ", self.emmlj14))
+        if self.qcPresent[0]:
+            energies.append(("QC/MM LJ", self.eqcmmlj))
+        if self.qcPresent[1]:
+            energies.append(("QC/MM 1-4 LJ", self.eqcmmlj14))
         if self.imagesPresent:
             energies.append(("MM/MM Image Elect.", self.eimmmel))
             energies.append(("MM/MM Image LJ", self.eimmmlj))
@@ -109,6 +122,10 @@ class NBModelABFSState:
     def Images(self):
         """[(a, b, c, transformation, scale, pairs)] of the MM/MM image list."""
         return self.native.images()
+
+    def QCMMPairs(self, which):
+        """A QC/MM list ('el', 'lj', 'el14' or 'lj14') as sorted keys row * 2^32 + column."""
+        return self.native.export_qcmm_pairs(which)
 
     def ImagePairs(self, image):
         """Pairs of one image as sorted keys i * n + j (i primary, j in the image)."""
@@ -126,6 +143,10 @@ class NBModelABFSState:
                 s.Entry("MM/MM Pairs", "{:d}".format(st["npairs"]))
             if st["npairs14"] > 0:
                 s.Entry("MM/MM 1-4 Pairs", "{:d}".format(st["npairs14"]))
+            if st["qcmmElPairs"] > 0:
+                s.Entry("QC/MM Elect. Pairs", "{:d}".format(st["qcmmElPairs"]))
+            if st["qcmmLJPairs"] > 0:
+                s.Entry("QC/MM LJ Pairs", "{:d}".format(st["qcmmLJPairs"]))
             if st["images"] > 0:
                 s.Entry("MM/MM Image Images", "{:d}".format(st["images"]))
                 s.Entry("MM/MM Image Pairs", "{:d}".format(st["imagePairs"]))
@@ -144,6 +165,10 @@ class NBModelABFSState:
                     s.Entry("<MM/MM Pairs>", "{:.1f}".format(st["numberOfMMMMPairs"] / n))
                 if st["npairs14"] > 0:
                     s.Entry("MM/MM 1-4 Pairs", "{:d}".format(st["npairs14"]))
+                if st["numberOfQCMMElPairs"] != 0.0:
+                    s.Entry("<QC/MM Elect. Pairs>", "{:.1f}".format(st["numberOfQCMMElPairs"] / n))
+                if st["numberOfQCMMLJPairs"] != 0.0:
+                    s.Entry("<QC/MM LJ Pairs>", "{:.1f}".format(st["numberOfQCMMLJPairs"] / n))
                 if st["numberOfMMMMImageImages"] != 0.0:
                     s.Entry("<MM/MM Image Images>", "{:.1f}".format(st["numberOfMMMMImageImages"] / n))
                 if st["numberOfMMMMImagePairs"] != 0.0:
@@ -233,6 +258,21 @@ class NBModelABFS:
                 self.dampingCutoff, self.innerCutoff, self.outerCutoff, self.listCutoff))
         self.CheckGenerator()
 
+    def CheckPairwiseInteractions(self):
+        """The QC/MM ("Delta/Gaussian", widths 2 bohr) and QC/QC ("Gaussian/Gaussian") interactions of
+        pMolecule.NBModelABFS.pyx:72-97 with their splines in atomic units."""
+        width = 2.0 * UNITS_LENGTH_BOHRS_TO_ANGSTROMS
+        if self.qcmmPairwiseInteraction is None:
+            self.qcmmPairwiseInteraction = PairwiseInteractionABFS.FromOptions(
+                dampingCutoff=0.0, electrostaticModel="Delta/Gaussian", innerCutoff=self.innerCutoff,
+                outerCutoff=self.outerCutoff, width2=width)
+        if self.qcmmPairwiseInteraction.splineX is None:
+            self.qcmmPairwiseInteraction.MakeSplines(lennardJones=False, useAtomicUnits=True)
+        if self.qcqcPairwiseInteraction is None:
+            self.qcqcPairwiseInteraction = PairwiseInteractionABFS.FromOptions(
+                dampingCutoff=0.0, electrostaticModel="Gaussian/Gaussian", innerCutoff=self.innerCutoff,
+                outerCutoff=self.outerCutoff, width1=width, width2=width)
+
     def CheckGenerator(self):
         """The ABFS default generator (pMolecule.NBModelABFS.pyx:59-71)."""
         if self.generator is None:
@@ -293,8 +333,9 @@ class NBModelABFS:
         """Create the state on first use, then run the list update (pyx:181-273)."""
         if configuration is None:
             return
-        if qcAtoms is not None and getattr(qcAtoms, "size", 0) > 0:
-            raise NotImplementedError("QC/MM interactions are not part of this implementation")
+        hasQC = qcAtoms is not None and getattr(qcAtoms, "size", 0) > 0
+        if hasQC and symmetry is not None:
+            raise NotImplementedError("QC/MM image lists are not part of this implementation")
         if self.useCentering and symmetry is not None:
             raise NotImplementedError("useCentering is not part of this implementation")
         transformations = None if symmetry is None else getattr(symmetry, "transformations", None)
@@ -313,6 +354,14 @@ class NBModelABFS:
             if transformations is not None:
                 rot, tra = transformations.Arrays()
                 native.set_transformations(rot, tra, self.checkForInverses, self.imageExpandFactor)
+            if hasQC:
+                index, boundary, partner, lf, off, mmp = qcAtoms.Arrays()
+                coupling = {"MM": _native.QCMM_MM, "RC": _native.QCMM_RC, "RD": _native.QCMM_RD}[self.qcmmCoupling]
+                native.set_qc_atoms(index, boundary, partner, lf, qcAtoms.useLinkRatio, off, mmp, coupling)
+                self.CheckPairwiseInteractions()
+                qi = self.qcmmPairwiseInteraction
+                native.set_qcmm_spline(qi.splineX, qi.splineY)
+                configuration.qcmmstate = QCMMInteractionState.WithExtent(qcAtoms.size)
             configuration.nbState = NBModelABFSState(native, mmAtoms)
         nbState = configuration.nbState
         xyz = configuration.coordinates3
@@ -325,6 +374,12 @@ class NBModelABFS:
         e, present, rebuilt = nbState.native.energy_host(xyz, grad)
         nbState.emmel, nbState.emmlj, nbState.emmel14, nbState.emmlj14 = (float(x) for x in e)
         nbState.present = (bool(present[0]), bool(present[1]))
+        if hasattr(configuration, "qcmmstate"):
+            # NBModelABFSState_Initialize zeroes the QC charges and potentials of each call.
+            configuration.qcmmstate.Initialize()
+            qe, qp = nbState.native.qcmm_terms()
+            nbState.eqcmmlj, nbState.eqcmmlj14 = float(qe[0]), float(qe[1])
+            nbState.qcPresent = (bool(qp[0]), bool(qp[1]))
         if transformations is not None:
             spg = getattr(configuration, "symmetryParameterGradients", None)
             ie, nbState.imagesPresent = nbState.native.image_terms(None if spg is None else spg.dEdM)
@@ -367,10 +422,18 @@ class NBModelABFS:
         return {"entriesPerItem": epi, "wavesPerCU": wpc, "kernelMs": ms}
 
     def QCMMGradients(self, configuration):
-        pass
+        """Gradients of the QC/MM electrostatics for the QC model's charges, added into
+        configuration.gradients3 (pyx:124-130)."""
+        if hasattr(configuration, "nbState") and hasattr(configuration, "qcmmstate"):
+            grad = getattr(configuration, "gradients3", None)
+            if grad is not None:
+                configuration.nbState.native.qcmm_gradients(configuration.qcmmstate.qcCharges, grad)
 
     def QCMMPotentials(self, configuration):
-        pass
+        """Electrostatic potentials of the MM charges at the QC atoms (atomic units), added into
+        configuration.qcmmstate.qcmmPotentials (pyx:132-138)."""
+        if hasattr(configuration, "nbState") and hasattr(configuration, "qcmmstate"):
+            configuration.nbState.native.qcmm_potentials(configuration.qcmmstate.qcmmPotentials)
 
     def Summary(self, log=logFile):
         if LogFileActive(log):

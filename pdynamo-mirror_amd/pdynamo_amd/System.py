@@ -106,6 +106,7 @@ class EnergyModel:
 
     def __init__(self):
         self.nbModel = None
+        self.qcModel = None
         self.mmAtoms = None
         self.qcAtoms = None
         self.ljParameters = None
@@ -133,7 +134,7 @@ class System:
 
     @classmethod
     def FromMMParameters(cls, coordinates3, mmAtoms, ljParameters, exclusions=None, interactions14=None,
-                         ljParameters14=None, electrostaticScale14=None, fixedAtoms=None, label=""):
+                         ljParameters14=None, electrostaticScale14=None, fixedAtoms=None, label="", qcAtoms=None):
         """A system from coordinates and MM parameters (what MMModel.BuildModel produces in pDynamo)."""
         xyz = np.ascontiguousarray(coordinates3, dtype=np.float64)
         if xyz.ndim != 2 or xyz.shape[1] != 3 or xyz.shape[0] != mmAtoms.size:
@@ -147,6 +148,7 @@ class System:
         em.exclusions = exclusions if (exclusions is None or isinstance(exclusions, SelfPairList)) else SelfPairList(exclusions, mmAtoms.size)
         em.interactions14 = interactions14 if (interactions14 is None or isinstance(interactions14, SelfPairList)) else SelfPairList(interactions14, mmAtoms.size)
         em.electrostaticScale14 = electrostaticScale14
+        em.qcAtoms = qcAtoms
         self.energyModel = em
         self.fixedAtoms = None if fixedAtoms is None else sorted(int(i) for i in fixedAtoms)
         return self
@@ -204,6 +206,17 @@ class System:
             self.symmetryParameters = crystalClass.CreateSymmetryParameters(a=a, b=b, c=c, alpha=alpha, beta=beta,
                                                                            gamma=gamma)
 
+    def DefineQCModel(self, qcModel, qcAtoms=None):
+        """Define the QC model (System.py:198-241) and, optionally, the QC atoms.  The QC model
+        supplies the QC charges and its energy from the QC/MM potentials (see QCModelPointCharges)."""
+        if self.energyModel is None:
+            raise ValueError("define the MM parameters first")
+        if self.energyModel.nbModel is not None:
+            self.energyModel.nbModel.Clear(self.configuration)
+        if qcAtoms is not None:
+            self.energyModel.qcAtoms = qcAtoms
+        self.energyModel.qcModel = qcModel
+
     def DefineNBModel(self, nbModel):
         """Define a non-bonding model (System.py:182-197)."""
         if isinstance(nbModel, NBModelABFS):
@@ -237,6 +250,13 @@ class System:
             em.nbModel.SetUp(em.mmAtoms, em.qcAtoms, em.ljParameters, em.ljParameters14, self.fixedAtoms,
                              em.interactions14, em.exclusions, self.symmetry, None, cfg, log=log)
             et.Extend(em.nbModel.Energy(cfg))
+        # QC and QC/MM electrostatic terms (System.py:300-304).
+        if em.qcModel is not None:
+            if em.nbModel is not None:
+                em.nbModel.QCMMPotentials(cfg)
+            et.Extend(em.qcModel.Energy(em.qcAtoms, cfg))
+            if em.nbModel is not None and doGradients:
+                em.nbModel.QCMMGradients(cfg)
         et.Total()
         grd = getattr(cfg, "gradients3", None)
         if self.fixedAtoms is not None and doGradients:

@@ -7,12 +7,17 @@ Drop-in names: ``System``, ``NBModelABFS``, ``NBModelABFSState``,
 ``EnergyTerms``; crystal symmetry for the periodic image lists:
 ``System.DefineSymmetry`` with the ``CrystalClass*`` classes,
 ``Transformation3Container`` and ``Transformation3_FromSymmetryOperationString``.
-Batches of images / replicas: ``BatchEnergy``.
+Batches of images / replicas: ``BatchEnergy``.  QC/MM systems: ``QCAtomContainer``
+(``QCAtomContainer_FromAtomContainer``) passed to ``System.FromMMParameters`` or
+``System.DefineQCModel``, ``PairwiseInteractionABFS``, ``QCMMInteractionState`` and the
+``QCModelPointCharges`` stand-in for a QC program.
 """
 
 from ._native import CLibraryError
 from .LogFile import LogFileActive, TextLogFile, logFile
 from .NBModelABFS import NBModelABFS, NBModelABFSState, PairListGenerator
+from .PairwiseInteraction import PairwiseInteractionABFS
+from .QCAtoms import QCAtomContainer, QCAtomContainer_FromAtomContainer, QCMMInteractionState, QCModelPointCharges
 from .Parameters import (LJParameterContainer, LJParameterContainer_FromEpsilonSigma,
                          LJParameterContainer_FromTableCoefficients, MMAtomContainer, SelfPairList,
                          SelfPairList_FromIntegerPairs, UNITS_ENERGY_E2ANGSTROMS_TO_KILOJOULES_PER_MOLE)
@@ -30,4 +35,5 @@ __all__ = ["CLibraryError", "LogFileActive", "TextLogFile", "logFile", "NBModelA
            "CrystalClassOrthorhombic", "CrystalClassRhombohedral", "CrystalClassTetragonal", "CrystalClassTriclinic",
            "Symmetry", "SymmetryParameterGradients", "SymmetryParameters", "Transformation3",
            "Transformation3Container", "Transformation3Container_Identity",
-           "Transformation3_FromSymmetryOperationString"]
+           "Transformation3_FromSymmetryOperationString", "PairwiseInteractionABFS", "QCAtomContainer",
+           "QCAtomContainer_FromAtomContainer", "QCMMInteractionState", "QCModelPointCharges"]

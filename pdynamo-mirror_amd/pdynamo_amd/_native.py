@@ -14,6 +14,7 @@ from .build import LIB
 
 STATUS_CONTINUE = 16
 MIXED, PLAIN_FP64 = 0, 1
+QCMM_MM, QCMM_RC, QCMM_RD = 0, 1, 2
 
 # Every symbol include/nbabfs.h declares.
 EXPORTED = ["nbabfs_default_options", "nbabfs_init", "nbabfs_last_error", "nbabfs_device_cu_count",
@@ -23,7 +24,9 @@ EXPORTED = ["nbabfs_default_options", "nbabfs_init", "nbabfs_last_error", "nbabf
             "nbabfs_batch_energy", "nbabfs_device_alloc", "nbabfs_device_free", "nbabfs_memcpy_htod",
             "nbabfs_memcpy_dtoh", "nbabfs_zero_async", "nbabfs_synchronize", "nbabfs_autotune",
             "nbabfs_virial", "nbabfs_set_transformations", "nbabfs_set_symmetry_parameters", "nbabfs_image_terms",
-            "nbabfs_images", "nbabfs_export_image_pairs"]
+            "nbabfs_images", "nbabfs_export_image_pairs", "nbabfs_set_qc_atoms", "nbabfs_set_qcmm_spline",
+            "nbabfs_qcmm_terms", "nbabfs_qcmm_potentials", "nbabfs_qcmm_gradients_host", "nbabfs_export_qcmm_pairs",
+            "nbabfs_qcmm_sites"]
 
 
 class CLibraryError(Exception):
@@ -44,7 +47,9 @@ class Stats(C.Structure):
                 ("useGridSearch", C.c_int), ("clusters", C.c_int), ("entries", C.c_longlong),
                 ("workItems", C.c_longlong), ("forceKernelMs", C.c_double), ("forceKernelLaunches", C.c_longlong),
                 ("images", C.c_int), ("reserved2", C.c_int), ("imagePairs", C.c_longlong),
-                ("numberOfMMMMImageImages", C.c_double), ("numberOfMMMMImagePairs", C.c_double)]
+                ("numberOfMMMMImageImages", C.c_double), ("numberOfMMMMImagePairs", C.c_double),
+                ("qcmmElPairs", C.c_longlong), ("qcmmLJPairs", C.c_longlong), ("numberOfQCMMElPairs", C.c_double),
+                ("numberOfQCMMLJPairs", C.c_double)]
 
 
 _lib = None
@@ -99,6 +104,15 @@ def load():
     L.nbabfs_images.argtypes = [vp, C.c_int, vp, vp, vp]
     L.nbabfs_export_image_pairs.argtypes = [vp, C.c_int, vp, C.c_longlong]
     L.nbabfs_export_image_pairs.restype = C.c_longlong
+    L.nbabfs_set_qc_atoms.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, vp, C.c_int]
+    L.nbabfs_set_qcmm_spline.argtypes = [vp, C.c_int, vp, vp]
+    L.nbabfs_qcmm_terms.argtypes = [vp, vp, vp]
+    L.nbabfs_qcmm_potentials.argtypes = [vp, vp]
+    L.nbabfs_qcmm_gradients_host.argtypes = [vp, vp, vp]
+    L.nbabfs_export_qcmm_pairs.argtypes = [vp, C.c_int, vp, C.c_longlong]
+    L.nbabfs_export_qcmm_pairs.restype = C.c_longlong
+    L.nbabfs_qcmm_sites.argtypes = [vp, vp, vp, vp, C.c_longlong]
+    L.nbabfs_qcmm_sites.restype = C.c_longlong
     _lib = L
     return L
 
@@ -272,6 +286,80 @@ class NativeState:
         if m2 < 0:
             check(int(-m2))
         return keys[:m]
+
+    # QC/MM ------------------------------------------------------------------------------------------
+    def set_qc_atoms(self, index, boundary=None, qcpartner=None, linkfactor=None, link_ratio=False,
+                     mmpartner_offsets=None, mmpartners=None, coupling=QCMM_RC):
+        """QC atoms in ascending atom order; boundary (link) atoms with their QC partner, link factor and MM
+        partners (CSR offsets per QC atom); coupling QCMM_MM, QCMM_RC or QCMM_RD."""
+        idx = np.ascontiguousarray(index, np.int32)
+        nqc = idx.size
+        b = np.zeros(nqc, np.int32) if boundary is None else np.ascontiguousarray(boundary, np.int32)
+        p = -np.ones(nqc, np.int32) if qcpartner is None else np.ascontiguousarray(qcpartner, np.int32)
+        lf = np.zeros(nqc) if linkfactor is None else np.ascontiguousarray(linkfactor, np.float64)
+        off = np.zeros(nqc + 1, np.int32) if mmpartner_offsets is None else \
+            np.ascontiguousarray(mmpartner_offsets, np.int32)
+        mmp = np.zeros(1, np.int32) if mmpartners is None or len(mmpartners) == 0 else \
+            np.ascontiguousarray(mmpartners, np.int32)
+        self.nqc = nqc
+        check(load().nbabfs_set_qc_atoms(self.h, nqc, _ptr(idx), _ptr(b), _ptr(p), _ptr(lf), 1 if link_ratio else 0,
+                                         _ptr(off), _ptr(mmp), int(coupling)))
+
+    def set_qcmm_spline(self, x, y):
+        """The QC/MM electrostatic spline through (x = r^2, y), clamped ends, atomic units."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        if x.size != y.size:
+            raise ValueError("spline x and y differ in length")
+        check(load().nbabfs_set_qcmm_spline(self.h, x.size, _ptr(x), _ptr(y)))
+
+    def qcmm_terms(self):
+        """((QC/MM LJ, QC/MM 1-4 LJ), (present, present 1-4)) of the last energy call."""
+        e = np.zeros(2)
+        present = np.zeros(2, np.int32)
+        check(load().nbabfs_qcmm_terms(self.h, _ptr(e), _ptr(present)))
+        return e, present.astype(bool)
+
+    def qcmm_potentials(self, potentials=None):
+        """QC/MM potentials (atomic units) at the QC atoms on the last update's coordinates, added into
+        potentials (a new zero array by default)."""
+        pot = np.zeros(self.nqc) if potentials is None else potentials
+        if pot.dtype != np.float64 or not pot.flags.c_contiguous or pot.size != self.nqc:
+            raise ValueError("potentials must be a contiguous float64 array of %d" % self.nqc)
+        check(load().nbabfs_qcmm_potentials(self.h, _ptr(pot)))
+        return pot
+
+    def qcmm_gradients(self, qc_charges, grad):
+        """QC/MM electrostatic gradients for the QC charges, added into grad (n, 3)."""
+        q = np.ascontiguousarray(qc_charges, np.float64)
+        if q.size != self.nqc:
+            raise ValueError("qc_charges must hold %d values" % self.nqc)
+        if grad.dtype != np.float64 or not grad.flags.c_contiguous or grad.shape != (self.n, 3):
+            raise ValueError("gradients must be a C-contiguous (%d, 3) float64 array" % self.n)
+        check(load().nbabfs_qcmm_gradients_host(self.h, _ptr(q), _ptr(grad)))
+
+    def export_qcmm_pairs(self, which):
+        """Sorted keys row * 2^32 + column of a QC/MM list: 'el', 'lj', 'el14' or 'lj14'."""
+        k = {"el": 0, "lj": 1, "el14": 2, "lj14": 3}[which]
+        L = load()
+        m = L.nbabfs_export_qcmm_pairs(self.h, k, None, 0)
+        if m < 0:
+            check(int(-m))
+        keys = np.zeros(max(m, 1), np.int64)
+        L.nbabfs_export_qcmm_pairs(self.h, k, _ptr(keys), m)
+        return keys[:m]
+
+    def qcmm_sites(self):
+        """(QC atom coordinates, MM site coordinates, MM site charges) of the last update."""
+        L = load()
+        m = L.nbabfs_qcmm_sites(self.h, None, None, None, 0)
+        if m < 0:
+            check(int(-m))
+        qx = np.zeros((self.nqc, 3))
+        sx = np.zeros((max(m, 1), 3))
+        sq = np.zeros(max(m, 1))
+        L.nbabfs_qcmm_sites(self.h, _ptr(qx), _ptr(sx), _ptr(sq), m)
+        return qx, sx[:m], sq[:m]
 
     def set_stream(self, stream_handle):
         check(load().nbabfs_set_stream(self.h, None if not stream_handle else C.c_void_p(stream_handle)))

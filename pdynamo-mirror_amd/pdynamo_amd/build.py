@@ -17,7 +17,7 @@ REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libnbabfs.so")
-SOURCES = ["nbabfs_build.hip", "nbabfs_force.hip", "nbabfs_host.hip"]
+SOURCES = ["nbabfs_build.hip", "nbabfs_force.hip", "nbabfs_host.hip", "nbabfs_qcmm.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else "hipcc")
 # Packed FP32 instructions stay disabled; the kernels use scalar FP32 and FP64.

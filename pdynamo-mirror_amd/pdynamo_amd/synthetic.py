@@ -320,3 +320,118 @@ def periodic_water_box(nwater, box, seed, jitter=0.3, name=None):
     rot = np.eye(3)[None]
     tra = np.zeros((1, 3))
     return s, (rot, tra), (float(box), float(box), float(box), 90.0, 90.0, 90.0)
+
+
+# ------------------------------------------------------------------------------------------------
+# A QC/MM system: a branched 17-atom chain in water with a QC region of six atoms and two link
+# (boundary) atoms, the layout QCAtomContainer_FromAtomContainer
+# (pMolecule-1.9.0/extensions/pyrex/pMolecule.QCAtomContainer.pyx:430-496) produces for it.
+# ------------------------------------------------------------------------------------------------
+CHAIN_BONDS = [(k, k + 1) for k in range(13)] + [(4, 14), (8, 15), (8, 16)]
+CHAIN_QC_PURE = (3, 4, 5, 6, 7, 14)
+CHAIN_BOUNDARY = {2: 3, 8: 7}        # boundary (link) atom -> its QC partner
+
+
+def _chain_geometry(rng, nat, bonds, bond=1.53):
+    """Heavy-atom chain with bond length 1.53 A, tetrahedral-like angles, no two atoms closer than 2.4 A
+    unless bonded or sharing a neighbour."""
+    nbr = [[] for _ in range(nat)]
+    for i, j in bonds:
+        nbr[i].append(j)
+        nbr[j].append(i)
+    x = np.zeros((nat, 3))
+    placed = [0]
+    order = []
+    seen = {0}
+    stack = [0]
+    while stack:
+        a = stack.pop(0)
+        for b in sorted(nbr[a]):
+            if b not in seen:
+                seen.add(b)
+                order.append((a, b))
+                stack.append(b)
+    for a, b in order:
+        for _ in range(2000):
+            v = rng.normal(size=3)
+            v /= np.linalg.norm(v)
+            prev = [p for p in nbr[a] if p in placed]
+            ok = all(np.dot(v, (x[p] - x[a]) / np.linalg.norm(x[p] - x[a])) < -0.2 for p in prev)
+            cand = x[a] + bond * v
+            close = [p for p in placed if p != a and p not in nbr[a] and np.linalg.norm(cand - x[p]) < 2.4]
+            if ok and not close:
+                x[b] = cand
+                break
+        else:
+            raise RuntimeError("chain geometry failed")
+        placed.append(b)
+    return x
+
+
+def _bond_paths(nat, bonds):
+    """1-2 and 1-3 pairs (the exclusions) and 1-4 pairs (not excluded) of a bond graph."""
+    nbr = [set() for _ in range(nat)]
+    for i, j in bonds:
+        nbr[i].add(j)
+        nbr[j].add(i)
+    e12 = {tuple(sorted(b)) for b in bonds}
+    e13 = set()
+    for a in range(nat):
+        for i in nbr[a]:
+            for j in nbr[a]:
+                if i < j:
+                    e13.add((i, j))
+    excl = e12 | e13
+    p14 = set()
+    for i, j in e12:
+        for a in nbr[i] - {j}:
+            for b in nbr[j] - {i}:
+                if a != b:
+                    key = tuple(sorted((a, b)))
+                    if key not in excl:
+                        p14.add(key)
+    return sorted(excl), sorted(p14)
+
+
+def qcmm_chain_in_water(nwater=1000, box=31.0, seed=20261207, name=None):
+    """Returns (MMSystem, qc): the chain (atoms 0-16) followed by the waters; qc describes the QC atoms as
+    pDynamo's QCAtomContainer holds them (index order, boundary flags, QC partners, MM partners of each
+    boundary atom)."""
+    rng = np.random.default_rng(seed)
+    nch = 17
+    xc = _chain_geometry(rng, nch, CHAIN_BONDS)
+    xc += 0.5 * box - xc.mean(0)
+    excl_c, p14_c = _bond_paths(nch, CHAIN_BONDS)
+    per = int(math.ceil((nwater * 1.6) ** (1.0 / 3.0)))
+    cen = _lattice_centres(rng, box, per, per ** 3, 0.25)
+    d = np.min(np.linalg.norm(cen[:, None, :] - xc[None, :, :], axis=2), axis=1)
+    cen = cen[d > 3.0]
+    cen = cen[np.sort(rng.choice(cen.shape[0], size=min(nwater, cen.shape[0]), replace=False))]
+    nw = cen.shape[0]
+    w = _water_sites(rng, cen)
+    _, qw, tw, ew = _water_params(nw)
+    xyz = np.concatenate([xc, w])
+    qch = rng.uniform(-0.45, 0.45, nch)
+    qch -= qch.mean()
+    charges = np.concatenate([qch, qw])
+    # LJ types: water O (0), water H (1), chain heavy atoms (2, 3).
+    tch = np.where(np.arange(nch) % 2 == 0, 2, 3).astype(np.int32)
+    ljtypes = np.concatenate([tch, tw]).astype(np.int32)
+    eps = [EPS_O_KCAL * KCAL_TO_KJ, 0.0, 0.35, 0.28]
+    sig = [SIGMA_O, 1.0, 3.5, 3.3]
+    tA, tB = lorentz_berthelot_table(eps, sig)
+    excl = np.concatenate([np.array(excl_c, np.int32).reshape(-1, 2), ew + nch]).astype(np.int32)
+    p14 = np.array(p14_c, np.int32).reshape(-1, 2)
+    s = MMSystem(name or f"qcmm_chain_{nw}w", np.ascontiguousarray(xyz), charges, ljtypes, 4, tA, tB, excl, p14)
+    nbr = [set() for _ in range(nch)]
+    for i, j in CHAIN_BONDS:
+        nbr[i].add(j)
+        nbr[j].add(i)
+    qc_atoms = sorted(set(CHAIN_QC_PURE) | set(CHAIN_BOUNDARY))
+    boundary = np.array([a in CHAIN_BOUNDARY for a in qc_atoms])
+    partner = np.array([CHAIN_BOUNDARY.get(a, -1) for a in qc_atoms], np.int32)
+    mmp = [sorted(nbr[a] - set(qc_atoms)) if a in CHAIN_BOUNDARY else [] for a in qc_atoms]
+    off = np.concatenate([[0], np.cumsum([len(m) for m in mmp])]).astype(np.int32)
+    qc = dict(index=np.array(qc_atoms, np.int32), boundary=boundary, partner=partner,
+              mmp_off=off, mmp=np.array(sum(mmp, []), np.int32))
+    return s, qc

@@ -51,7 +51,9 @@ def _has_key(path, key):
         return key in z.files
 
 
-ALL_FIXTURES = sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.npz")) if not p.endswith("dhfr_nb.npz"))
+ALL_FIXTURES = sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                      if not p.endswith("dhfr_nb.npz") and not os.path.basename(p).startswith("qcmm_"))
+QCMM_FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "qcmm_*.npz")))
 REFERENCE_FIXTURES = [p for p in ALL_FIXTURES if not _has_key(p, "rotations")]
 IMAGE_FIXTURES = [p for p in ALL_FIXTURES if _has_key(p, "rotations")]
 

@@ -1,0 +1,243 @@
+"""The QC/MM checker (oracle/qcmm_oracle.py) pinned against the reference and the mathematics.
+
+* Against the reference pDynamo C library's own outputs (tests/golden/qcmm_*.npz,
+  written by make_golden_qcmm.py): the QC/MM lists exactly (electrostatic, LJ
+  and both 1-4 lists), the MM/MM list of the MM selection (size and key
+  checksum) and its 1-4 list exactly, the QC atom coordinates and MM sites
+  exactly, the rebuild decisions, and to 1e-12 the six energies, the
+  potentials and both gradients, for MM, RC and RD coupling, link distances
+  and link ratios, fixed atoms and the direct search.
+* The electrostatic spline: its ordinates are erf(p r)/r plus a constant
+  below the inner cutoff and minus the integral of G(r) S(r) from r to the
+  outer cutoff above it (scipy.integrate.quad), zero at the outer cutoff, with
+  zero slope at both ends; its second derivatives equal the reference's.
+* Finite differences: the QC/MM gradients are the derivative of
+  2625.5 sum_q Q_q phi_q(x) (the link-atom chain rule and the RC/RD sites
+  included), the QC/MM LJ gradients that of the QC/MM LJ energy.
+* RC/RD coupling keeps the total MM charge.
+"""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from test_oracle import QCMM_FIXTURES
+
+QCMM_LABELS = ("MM/MM Elect.", "MM/MM LJ", "MM/MM 1-4 Elect.", "MM/MM 1-4 LJ", "QC/MM LJ", "QC/MM 1-4 LJ")
+KEY = 1 << 20
+
+
+@pytest.fixture(scope="module")
+def Q(oracle_mod):
+    import qcmm_oracle
+    return qcmm_oracle
+
+
+def fixture_system(fx):
+    from pdynamo_amd import synthetic as S
+    return S.MMSystem(str(fx["name"]), np.array(fx["coords"][0]), fx["charges"], fx["ljtypes"], int(fx["ntypes"]),
+                      fx["tableA"], fx["tableB"], fx["exclusions"], fx["interactions14"])
+
+
+def fixture_qc(Q, fx, n):
+    return Q.QCAtoms(n, fx["qc_index"], fx["qc_boundary"].astype(bool), fx["qc_partner"], fx["qc_linkfactor"],
+                     fx["qc_mmp_off"], fx["qc_mmp"], bool(fx["link_ratio"]))
+
+
+def fixture_fixed(fx):
+    return fx["fixed"].astype(bool) if fx["fixed"].size else None
+
+
+def rel_rms(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("path", QCMM_FIXTURES, ids=lambda p: os.path.basename(p)[:-4])
+def test_qcmm_oracle_matches_reference(Q, path):
+    fx = dict(np.load(path, allow_pickle=False))
+    s = fixture_system(fx)
+    qc = fixture_qc(Q, fx, s.natoms)
+    sp = Q.CubicSpline(fx["spline_x"], fx["spline_y"])
+    assert np.abs(sp.h - fx["spline_h"]).max() <= 1e-13 * np.abs(fx["spline_h"]).max()
+    orc = Q.QCMMOracle(s, qc, int(fx["coupling"]), sp, fixed=fixture_fixed(fx))
+    for k, xyz in enumerate(fx["coords"]):
+        e, pot, ge, gq, up = orc.call(xyz, fx["qc_charges"])
+        assert up == bool(fx["rebuilt"][k])
+        ref = fx["energies"][k]
+        for q, label in enumerate(QCMM_LABELS):
+            if np.isnan(ref[q]):
+                assert label not in e
+            else:
+                assert abs(e[label] - ref[q]) <= 1e-12 * max(abs(ref[q]), 1.0), (label, e[label], ref[q])
+        assert np.abs(pot - fx["potentials"][k]).max() <= 1e-13
+        assert rel_rms(ge, fx["grads_e"][k]) <= 1e-12
+        assert rel_rms(gq, fx["grads_q"][k]) <= 1e-12
+        if k == 0:
+            L = orc.lists
+            assert np.array_equal(Q.QCMMLists.keys(L.el, range(qc.nqc), KEY), fx["keys_el"])
+            assert np.array_equal(Q.QCMMLists.keys(L.el14, range(qc.nqc), KEY), fx["keys_el14"])
+            assert np.array_equal(Q.QCMMLists.keys(L.lj, L.lj_rows, KEY), fx["keys_lj"])
+            assert np.array_equal(Q.QCMMLists.keys(L.lj14, L.lj_rows, KEY), fx["keys_lj14"])
+            assert np.array_equal(L.qc_xyz, fx["qc_xyz"])
+            assert np.array_equal(L.site_xyz, fx["site_xyz"])
+            assert np.array_equal(L.site_q, fx["site_q"])
+            keys = orc.rec.canonical().astype(np.uint64)
+            w = np.arange(1, keys.size + 1, dtype=np.uint64)
+            assert int(fx["mmmm_checksum"][0]) == keys.size
+            assert int(fx["mmmm_checksum"][1]) == int(np.sum(keys * w, dtype=np.uint64))
+            assert np.array_equal(orc.rec14.canonical(), fx["keys_mmmm14"])
+
+
+def test_spline_closed_form_and_integral(Q):
+    """Below the inner cutoff y(r) = (erf(p r)/r + c) / a0 (a0 = bohr in A); above it y(r) = -int_r^12 G S / a0
+    with G = d/dr (erf(p r)/r) and S the switch; y(12) = 0; the clamped ends have zero slope."""
+    from scipy.integrate import quad
+    from scipy.special import erf
+    w2 = 2.0 * Q.BOHRS_TO_ANGSTROMS
+    x, y = Q.electrostatic_spline_points("Delta/Gaussian", 0.0, w2, 0.0, 8.0, 12.0)
+    p = 2.0 / math.sqrt(math.pi) / w2
+    r = np.sqrt(x)
+    a0 = 1.0 / Q.ANGSTROMS_TO_BOHRS
+
+    def G(t):
+        return p * (2.0 * math.exp(-(p * t) ** 2) / (math.sqrt(math.pi) * p * t) - erf(p * t) / (p * t) ** 2) * p
+
+    def S(t):
+        return (144.0 - t * t) ** 2 * (144.0 + 2.0 * t * t - 3.0 * 64.0) / (144.0 - 64.0) ** 3
+
+    tail = quad(lambda t: G(t) * S(t), 8.0, 12.0, epsabs=1e-14, epsrel=1e-13, limit=200)[0]
+    c = -tail - erf(p * 8.0) / 8.0
+    inner = (r > 0.0) & (r < 8.0)
+    assert np.allclose(y[inner], (erf(p * r[inner]) / r[inner] + c) * a0, rtol=0, atol=1e-13)
+    assert abs(y[0] - (2.0 * p / math.sqrt(math.pi) + c) * a0) <= 1e-13
+    for t in (8.0, 9.0, 10.5, 11.98):
+        k = int(np.argmin(np.abs(r - t)))
+        want = -quad(lambda u: G(u) * S(u), r[k], 12.0, epsabs=1e-15, epsrel=1e-13, limit=200)[0] * a0
+        assert abs(y[k] - want) <= 1e-13, (t, y[k], want)
+    assert y[-1] == 0.0 and x[-1] == 144.0 and x[0] == 0.0
+    sp = Q.CubicSpline(x, y)
+    # Clamped ends and the interior slope dy/dx = G S / (2 r a0^-1) in the switching region.
+    _, g0 = sp.evaluate(np.array([0.0]))
+    _, g1 = sp.evaluate(np.array([144.0]))
+    assert abs(g0[0]) < 1e-12 and abs(g1[0]) < 1e-12
+    for t in (9.0, 10.0, 11.0):
+        _, g = sp.evaluate(np.array([t * t]))
+        assert abs(g[0] - G(t) * S(t) / (2.0 * t) * a0) <= 1e-6 * abs(G(t) * S(t) / (2.0 * t) * a0)
+
+
+def _fd_setup(Q, coupling, link_ratio):
+    from pdynamo_amd import synthetic as S
+    s, qd = S.qcmm_chain_in_water(150, 16.0)
+    lf = np.where(qd["boundary"], 0.712 if link_ratio else 1.09, 0.0)
+    qc = Q.QCAtoms(s.natoms, qd["index"], qd["boundary"], qd["partner"], lf, qd["mmp_off"], qd["mmp"], link_ratio)
+    x, y = Q.electrostatic_spline_points("Delta/Gaussian", 0.0, 2.0 * Q.BOHRS_TO_ANGSTROMS, 0.0, 8.0, 12.0)
+    return s, qc, Q.CubicSpline(x, y)
+
+
+@pytest.mark.parametrize("coupling,link_ratio", [(0, False), (1, True), (2, False)])
+def test_qcmm_gradients_finite_difference(Q, coupling, link_ratio):
+    s, qc, sp = _fd_setup(Q, coupling, link_ratio)
+    xyz = s.coordinates3.copy()
+    L = Q.QCMMLists(qc, xyz, s.exclusions, s.interactions14, coupling, None, 13.5, s.charges)
+    qcq = np.linspace(-0.4, 0.5, qc.nqc)
+    rows14 = L.el14 if coupling == 0 else None
+
+    def energy(x):
+        qx = qc.coordinates(x)
+        sx, sq, _ = qc.mm_sites(x, s.charges, coupling)
+        pot = Q.potentials(sp, qx, sx, sq, L.el, 1.0)
+        if rows14 is not None:
+            pot = pot + Q.potentials(sp, qx, sx, sq, rows14, 1.0)
+        return Q.HARTREES_TO_KJ * float(np.dot(qcq, pot))
+
+    g = np.zeros_like(xyz)
+    sg = np.zeros((L.site_xyz.shape[0], 3))
+    Q.gradients(sp, qc, xyz, L.qc_xyz, L.site_xyz, L.site_q, L.el, 1.0, qcq, sg, g)
+    if rows14 is not None:
+        Q.gradients(sp, qc, xyz, L.qc_xyz, L.site_xyz, L.site_q, rows14, 1.0, qcq, sg, g)
+        g += sg
+    else:
+        for k, (a, p) in enumerate(L.sites):
+            if p < 0:
+                g[a] += sg[k]
+            else:
+                g[a] += 0.5 * sg[k]
+                g[p] += 0.5 * sg[k]
+    atoms = [0, 1, 2, 3, 6, 7, 8, 9, 15, 16, 17, 40, 200]
+    h = 1e-5
+    for a in atoms:
+        for c in range(3):
+            xp, xm = xyz.copy(), xyz.copy()
+            xp[a, c] += h
+            xm[a, c] -= h
+            fd = (energy(xp) - energy(xm)) / (2 * h)
+            assert abs(fd - g[a, c]) <= 1e-6 * max(1.0, abs(g[a, c])), (a, c, fd, g[a, c])
+
+
+def test_qcmm_lj_finite_difference(Q, oracle_mod):
+    O = oracle_mod
+    s, qc, _ = _fd_setup(Q, 0, False)
+    xyz = s.coordinates3.copy()
+    L = Q.QCMMLists(qc, xyz, s.exclusions, s.interactions14, 0, None, 13.5, s.charges)
+    f = O.factors()
+    rec = Q.records(L.lj_rows, L.lj, s.natoms)
+
+    def energy(x, g=None):
+        return O.mmmm_energy(f, x, s.charges, s.ljtypes, s.ntypes, s.tableA, s.tableB, rec, 0.0, 1.0, 1, g)[1]
+
+    g = np.zeros_like(xyz)
+    energy(xyz, g)
+    h = 1e-5
+    for a in [3, 4, 14, 17, 30, 300]:
+        for c in range(3):
+            xp, xm = xyz.copy(), xyz.copy()
+            xp[a, c] += h
+            xm[a, c] -= h
+            fd = (energy(xp) - energy(xm)) / (2 * h)
+            assert abs(fd - g[a, c]) <= 1e-6 * max(1.0, abs(g[a, c])), (a, c, fd, g[a, c])
+
+
+@pytest.mark.parametrize("coupling", [1, 2])
+def test_redistributed_charges_conserve_mm_charge(Q, coupling):
+    s, qc, _ = _fd_setup(Q, coupling, False)
+    _, sq, sites = qc.mm_sites(s.coordinates3, s.charges, coupling)
+    assert abs(sq.sum() - s.charges[~qc.pure].sum()) <= 1e-12
+    assert sites.shape[0] == int((~qc.full).sum()) + qc.mmp.size
+
+
+def test_product_spline_points_match_the_oracle(Q):
+    """The host-side ordinates (pdynamo_amd.PairwiseInteraction, what NBModelABFS hands the library) are
+    the oracle's, bit for bit, for both Gaussian models."""
+    from pdynamo_amd import NBModelABFS, PairwiseInteractionABFS
+    m = NBModelABFS()
+    m.CheckPairwiseInteractions()
+    x, y = Q.electrostatic_spline_points("Delta/Gaussian", 0.0, 2.0 * Q.BOHRS_TO_ANGSTROMS, 0.0, 8.0, 12.0)
+    assert np.array_equal(m.qcmmPairwiseInteraction.splineX, x)
+    assert np.array_equal(m.qcmmPairwiseInteraction.splineY, y)
+    w = 2.0 * Q.BOHRS_TO_ANGSTROMS
+    p = PairwiseInteractionABFS(dampingCutoff=0.3, electrostaticModel="Gaussian/Gaussian", width1=w, width2=0.7 * w,
+                                innerCutoff=7.0, outerCutoff=10.0, splinePointDensity=40)
+    p.MakeSplines(lennardJones=False, useAtomicUnits=False)
+    x, y = Q.electrostatic_spline_points("Gaussian/Gaussian", w, 0.7 * w, 0.3, 7.0, 10.0, density=40,
+                                         atomic_units=False)
+    assert np.array_equal(p.splineX, x) and np.array_equal(p.splineY, y)
+    with pytest.raises(ValueError):
+        PairwiseInteractionABFS(electrostaticModel="Gaussian/Delta")
+
+
+def test_qc_atom_container_from_boundary_atoms():
+    """QCAtomContainer_FromAtomContainer's layout (pMolecule.QCAtomContainer.pyx:430-496): QC and boundary
+    atoms merged in ascending order, MM partners the sorted union of boundary and MM partners, one QC partner."""
+    from pdynamo_amd import QCAtomContainer_FromAtomContainer
+    from pdynamo_amd import synthetic as S
+    _, qd = S.qcmm_chain_in_water(50, 12.0)
+    qc = QCAtomContainer_FromAtomContainer([3, 4, 5, 6, 7, 14], {2: ([], [1], [3]), 8: ([15], [9, 16], [7])},
+                                           {2: 1.09, 8: 1.09})
+    index, boundary, partner, lf, off, mmp = qc.Arrays()
+    assert np.array_equal(index, qd["index"]) and np.array_equal(boundary.astype(bool), qd["boundary"])
+    assert np.array_equal(partner, qd["partner"]) and np.array_equal(off, qd["mmp_off"])
+    assert np.array_equal(mmp, qd["mmp"])
+    with pytest.raises(ValueError):
+        QCAtomContainer_FromAtomContainer([3], {2: ([], [1], [3, 4])}, {2: 1.0})
