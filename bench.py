@@ -124,6 +124,92 @@ def make_workload(name):
     raise SystemExit("unknown workload " + name)
 
 
+def run_qcmm(args):
+    """QC/MM workload: the branched chain (6 pure QC atoms, 2 link atoms, RC coupling) plus the 40 nearest
+    waters as QC atoms, in 7,000 waters.  One step = a System.Energy of a QC/MM system without the QC
+    program: the MM/MM and QC/MM LJ energy and gradients (device arrays), the QC/MM potentials at the
+    QC atoms, then the QC/MM electrostatic gradients for the QC charges (host arrays, as a QC program
+    exchanges them)."""
+    from pdynamo_amd import _native, synthetic
+    from pdynamo_amd.NBModelABFS import NBModelABFS
+    mm, qd = synthetic.qcmm_chain_in_water(7000, 59.4, qc_waters=40)
+    lf = np.where(qd["boundary"], 1.09, 0.0)
+    model = NBModelABFS(precision=args.precision, qcmmCoupling="RC")
+    model.CheckPairwiseInteractions()
+    st = _native.NativeState(model.NativeOptions(), mm.charges, mm.ljtypes, mm.ntypes, mm.tableA, mm.tableB,
+                             exclusions=mm.exclusions, interactions14=mm.interactions14)
+    st.set_qc_atoms(qd["index"], qd["boundary"], qd["partner"], lf, False, qd["mmp_off"], qd["mmp"],
+                    _native.QCMM_RC)
+    qi = model.qcmmPairwiseInteraction
+    st.set_qcmm_spline(qi.splineX, qi.splineY)
+    nqc = qd["index"].size
+    qcq = np.linspace(-0.8, 0.8, nqc)
+    x = _native.DeviceArray.from_numpy(mm.coordinates3, st)
+    g = _native.DeviceArray(mm.coordinates3.shape)
+    hg = np.zeros_like(mm.coordinates3)
+    pot = np.zeros(nqc)
+
+    def step(qc=True):
+        g.zero(st)
+        st.update(x.ptr)
+        st.energy_async(x.ptr, g.ptr)
+        if qc:
+            pot[:] = 0.0
+            st.qcmm_potentials(pot)
+            st.qcmm_gradients(qcq, hg)
+
+    def timed(qc):
+        for _ in range(args.warmup):
+            step(qc)
+        _native.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(qc)
+        _native.synchronize()
+        return (time.perf_counter() - t0) / args.steps * 1e3
+
+    ms = timed(True)
+    ms_mm = timed(False)
+    s = st.stats()
+    e, _ = st.energies()
+    qe, _ = st.qcmm_terms()
+    n_list = int(s["npairs"]) + int(s["npairs14"]) + int(s["qcmmElPairs"]) + int(s["qcmmLJPairs"])
+    result = {
+        "metric": "pair interactions/s (reference list pairs d<=13.5 A per E+grad second)",
+        "value": n_list / (ms * 1e-3), "unit": "pair interactions/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (QC/MM) + the MM/MM path's precision", "data": "synthetic (seeded generator, pdynamo_amd.synthetic)",
+        "config": {"workload": "QC/MM chain + 40 QC waters in 7,000 waters (RC coupling, 2 link atoms)",
+                   "atoms": mm.natoms, "qc_atoms": nqc, "precision": args.precision, "list_pairs": n_list,
+                   "mmmm_pairs": int(s["npairs"]), "qcmm_el_pairs": int(s["qcmmElPairs"]),
+                   "qcmm_lj_pairs": int(s["qcmmLJPairs"]), "ms_mm_part_only": ms_mm,
+                   "ms_qcmm_potentials_and_gradients": ms - ms_mm,
+                   "energies_kJ_mol": [float(v) for v in list(e) + list(qe)],
+                   "potential_sum_au": float(pot.sum())},
+    }
+    if not args.no_cpu_baseline:
+        import oracle as O
+        if O.ref_available():
+            import qcmm_oracle as Q
+            threads = len(os.sched_getaffinity(0))
+            if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+                threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+            qc = Q.QCAtoms(mm.natoms, qd["index"], qd["boundary"], qd["partner"], lf, qd["mmp_off"], qd["mmp"])
+            ref = Q.ReferenceQCMM(mm, qc, 1, qi.splineX, qi.splineY, nthreads=threads)
+            ref.call(mm.coordinates3, qcq)
+            calls = max(1, args.cpu_calls // 10)
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                ref.call(mm.coordinates3, qcq)
+            dt = (time.perf_counter() - t0) / calls
+            ref.close()
+            result["cpu_baseline"] = {"value": n_list / dt, "unit": "pair interactions/s", "cores": threads,
+                                      "kind": "reference",
+                                      "sample": "%d QC/MM calls (Update, MMMMEnergy, QCMMEnergyLJ, QCMMPotentials, "
+                                                "QCMMGradients) after the first list build, %.1f ms each" % (calls, dt * 1e3)}
+    print(json.dumps(result))
+
+
 def make_batch(name):
     from pdynamo_amd import synthetic
     if name == "droplets64":
@@ -266,6 +352,9 @@ def main():
     from pdynamo_amd import _native
     from pdynamo_amd.NBModelABFS import NBModelABFS
 
+    if args.workload == "qcmm_solvated":
+        run_qcmm(args)
+        return
     systems, bdesc = make_batch(args.workload)
     if systems is not None:
         run_batch(args, systems, bdesc)

@@ -822,14 +822,14 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
 // ------------------------------------------------------------------------------------------------
 // QC/MM LJ (NBModelABFS_QCMMEnergyLJ, pMolecule-1.9.0/extensions/csource/NBModelABFS.c:306-378):
 // the analytic pair terms with no electrostatics over the QC/MM LJ list (the main LJ table) and
-// its 1-4 list (the 1-4 table), one workgroup per pure QC atom; gradients into the fixed-point
-// accumulators, the row's two energies into qljpart.
+// its 1-4 list (the 1-4 table), NB_QCMM_CHUNKS workgroups per pure QC atom; gradients into the fixed-point
+// accumulators, the two energies of each (row, chunk) into qljpart.
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_qclj(const SysDesc *__restrict__ descs, AbfsFactors F)
 {
 #pragma clang fp contract(off)
     const SysDesc &D = descs[0];
-    const int r = blockIdx.x;
+    const int r = blockIdx.x, c = blockIdx.y, stride = gridDim.y * 256;
     const int i = D.qlj_atom[r];
     const double xi = D.xyz[3 * (size_t)i], yi = D.xyz[3 * (size_t)i + 1], zi = D.xyz[3 * (size_t)i + 2];
     const int ti = D.ntypes * D.atype[i];
@@ -840,7 +840,7 @@ __global__ __launch_bounds__(256) void k_qclj(const SysDesc *__restrict__ descs,
         const int *idx = list ? D.qlj14_idx : D.qlj_idx;
         const double *tA = list ? D.ljA14 : D.ljA;
         const double *tB = list ? D.ljB14 : D.ljB;
-        for (int k = ptr[r] + threadIdx.x; k < ptr[r + 1]; k += 256) {
+        for (int k = ptr[r] + c * 256 + threadIdx.x; k < ptr[r + 1]; k += stride) {
             const int j = idx[k];
             double xij = xi - D.xyz[3 * (size_t)j], yij = yi - D.xyz[3 * (size_t)j + 1],
                    zij = zi - D.xyz[3 * (size_t)j + 2];
@@ -870,8 +870,8 @@ __global__ __launch_bounds__(256) void k_qclj(const SysDesc *__restrict__ descs,
     double zz = 0.0;
     block_sum2<256>(gz, zz);
     if (threadIdx.x == 0) {
-        D.qljpart[2 * r] = e[0];
-        D.qljpart[2 * r + 1] = e[1];
+        D.qljpart[2 * ((size_t)r * gridDim.y + c)] = e[0];
+        D.qljpart[2 * ((size_t)r * gridDim.y + c) + 1] = e[1];
         fx_atomic_add(D.facc, D.n_pad, si, 0, g[0]);
         fx_atomic_add(D.facc, D.n_pad, si, 1, g[1]);
         fx_atomic_add(D.facc, D.n_pad, si, 2, gz);
@@ -880,7 +880,7 @@ __global__ __launch_bounds__(256) void k_qclj(const SysDesc *__restrict__ descs,
 
 void launch_qclj(const SysDesc *d_desc, int nrows, const AbfsFactors &F, hipStream_t s)
 {
-    if (nrows > 0) hipLaunchKernelGGL(k_qclj, dim3(nrows), dim3(256), 0, s, d_desc, F);
+    if (nrows > 0) hipLaunchKernelGGL(k_qclj, dim3(nrows, NB_QCMM_CHUNKS), dim3(256), 0, s, d_desc, F);
 }
 
 // ------------------------------------------------------------------------------------------------

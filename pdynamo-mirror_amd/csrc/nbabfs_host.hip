@@ -998,7 +998,7 @@ static int ensure_desc(nbabfs_state *S, const double *d_xyz, double *d_grad)
 static int ensure_call_buffers(nbabfs_state *S)
 {
     CHK(S->energies.ensure(NB_NENERGIES));
-    if (S->has_qc) CHK(S->qljpart.ensure(2 * (size_t)std::max(S->nlj, 1)));
+    if (S->has_qc) CHK(S->qljpart.ensure(2 * (size_t)std::max(S->nlj, 1) * NB_QCMM_CHUNKS));
     if (S->opt.precision == NBABFS_PLAIN_FP64) CHK(S->eimgpart.ensure(2 * (size_t)std::max(S->nitems - S->nitems_main, 1)));
     CHK(S->eacc.ensure(NB_EACC_WORDS));
     CHK(S->vacc.ensure(NB_VACC_WORDS));
@@ -1144,7 +1144,7 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.nimg = (int)S->slots.size();
     D.imgxf = D.nimg ? S->imgxf.p : nullptr;
     D.gacc = D.nimg ? S->gacc.p : nullptr;
-    D.n_qlj = S->has_qc ? S->nlj : 0;
+    D.n_qlj = S->has_qc ? S->nlj * NB_QCMM_CHUNKS : 0;   // (row, chunk) partials
     D.qlj_atom = S->lj_atom.p;
     D.qlj_ptr = S->lj_ptr.p;
     D.qlj_idx = S->lj_idx.p;
@@ -2873,14 +2873,25 @@ extern "C" int nbabfs_qcmm_potentials(nbabfs_state *S, double *potentials)
         escale = 1.0 / S->opt.dielectric;
         escale14 = escale * S->opt.electrostaticScale14;
     }
-    CHK(S->qpot.ensure(S->nqc));
-    CHK(S->h_qpot.ensure(S->nqc));
-    launch_qcmm_potentials(G, sp, L, S->site_q.p, escale, escale14, S->opt.outerCutoff * S->opt.outerCutoff,
+    const int ch = NB_QCMM_CHUNKS;
+    const size_t np = 2 * (size_t)S->nqc * ch;
+    CHK(S->qpot.ensure(np));
+    CHK(S->h_qpot.ensure(np));
+    launch_qcmm_potentials(G, sp, L, S->site_q.p, escale, escale14, S->opt.outerCutoff * S->opt.outerCutoff, ch,
                            S->qpot.p, st);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(S->h_qpot.p, S->qpot.p, S->nqc * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(S->h_qpot.p, S->qpot.p, np * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    for (int q = 0; q < S->nqc; q++) potentials[q] += S->h_qpot.p[q];
+    // Chunk sums in chunk order; the main list's total, then the 1-4 list's, as the reference adds them.
+    for (int q = 0; q < S->nqc; q++) {
+        double m = 0.0, m14 = 0.0;
+        for (int c = 0; c < ch; c++) {
+            m += S->h_qpot.p[2 * ((size_t)q * ch + c)];
+            m14 += S->h_qpot.p[2 * ((size_t)q * ch + c) + 1];
+        }
+        potentials[q] += m;
+        potentials[q] += m14;
+    }
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -2907,7 +2918,7 @@ extern "C" int nbabfs_qcmm_gradients_host(nbabfs_state *S, const double *qcCharg
     CHK(S->h_qgrad.ensure(3 * (size_t)std::max(n, 1)));
     launch_fill_int((int *)S->qgacc.p, 12 * (size_t)n, 0, st);
     launch_qcmm_gradients(G, sp, L, S->site_q.p, S->qcq.p, scale, scale14, S->opt.outerCutoff * S->opt.outerCutoff,
-                          S->qgacc.p, st);
+                          NB_QCMM_CHUNKS, S->qgacc.p, st);
     launch_qcmm_grad_out(S->qgacc.p, n, S->qgrad.p, st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(S->h_qgrad.p, S->qgrad.p, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));

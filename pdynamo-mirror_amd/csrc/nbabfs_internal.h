@@ -117,7 +117,7 @@ struct SysDesc {
     // QC/MM LJ: rows = pure QC atoms, CSR lists of MM atoms (main and 1-4).
     int n_qlj;
     const int *qlj_atom, *qlj_ptr, *qlj_idx, *qlj14_ptr, *qlj14_idx;
-    double *qljpart;            // 2 per row (QC/MM LJ, QC/MM 1-4 LJ), summed in row order
+    double *qljpart;            // 2 per (row, chunk) (QC/MM LJ, QC/MM 1-4 LJ), summed in that order
 };
 
 // ------------------------------------------------------------------------------------------------

@@ -174,13 +174,16 @@ struct QcmmRows {                       // main and 1-4 lists of the QC atoms (C
 };
 
 void launch_cross_list(const CrossListParams &P, int pass, hipStream_t s);
+// chunks workgroups per QC atom; part: 2 per (QC atom, chunk), main and 1-4 sums.
 void launch_qcmm_potentials(const QcmmGeom &G, const SplineDev &S, const QcmmRows &L, const double *site_q,
-                            double escale, double escale14, double r2off, double *pot, hipStream_t s);
+                            double escale, double escale14, double r2off, int chunks, double *part, hipStream_t s);
 // gacc: six int64 arrays of n (x, y, z low words, then high words), fixed point as facc.
 void launch_qcmm_gradients(const QcmmGeom &G, const SplineDev &S, const QcmmRows &L, const double *site_q,
-                           const double *qcq, double scale, double scale14, double r2off, long long *gacc,
+                           const double *qcq, double scale, double scale14, double r2off, int chunks, long long *gacc,
                            hipStream_t s);
 void launch_qcmm_grad_out(const long long *gacc, int n, double *grad, hipStream_t s);
 void launch_qcmm_coords(const QcmmGeom &G, double *qcxyz, double *sitexyz, hipStream_t s);
-// QC/MM LJ (nbabfs_force.hip): the QC/MM LJ and 1-4 LJ lists of the descriptor's pure QC atoms.
+// QC/MM LJ (nbabfs_force.hip): the QC/MM LJ and 1-4 LJ lists of the descriptor's pure QC atoms,
+// NB_QCMM_CHUNKS workgroups per row.
+#define NB_QCMM_CHUNKS 4
 void launch_qclj(const SysDesc *d_desc, int nrows, const AbfsFactors &F, hipStream_t s);

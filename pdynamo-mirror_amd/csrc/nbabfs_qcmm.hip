@@ -95,6 +95,21 @@ __device__ __forceinline__ void spline_eval(const SplineDev &S, double x, double
     if (g) *g = (yu - yl) / d + (-(3.0 * t * t - 1.0) * hl + (3.0 * s * s - 1.0) * hu);
 }
 
+// The spline's arrays staged in LDS when they fit (every pair's bisection
+// then reads LDS instead of L2); otherwise the global arrays are used.
+#define NB_SPLINE_LDS 1024
+__device__ __forceinline__ SplineDev stage_spline(const SplineDev &S, double *lds)
+{
+    if (S.n > NB_SPLINE_LDS) return S;
+    for (int k = threadIdx.x; k < S.n; k += blockDim.x) {
+        lds[k] = S.x[k];
+        lds[NB_SPLINE_LDS + k] = S.y[k];
+        lds[2 * NB_SPLINE_LDS + k] = S.h[k];
+    }
+    __syncthreads();
+    return SplineDev{lds, lds + NB_SPLINE_LDS, lds + 2 * NB_SPLINE_LDS, S.n};
+}
+
 // Binary search in a sorted range [lo, hi).
 __device__ __forceinline__ bool in_sorted(const int *__restrict__ a, int lo, int hi, int v)
 {
@@ -198,14 +213,19 @@ __global__ __launch_bounds__(256) void k_cross_list(CrossListParams P)
 
 // ------------------------------------------------------------------------------------------------
 // Potentials (atomic units): pot[q] = sum_main (eScale q_m) f(r^2) + sum_1-4
-// (eScale14 q_m) f(r^2) over r^2 <= outerCutoff^2.
+// (eScale14 q_m) f(r^2) over r^2 <= outerCutoff^2.  Each QC atom's lists are
+// shared by gridDim.y workgroups (chunk c takes entries c*256+t, stride
+// gridDim.y*256); chunk sums go to part[2*(q*chunks + c) + list], added in
+// chunk order by the host.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_qcmm_potentials(QcmmGeom G, SplineDev S, QcmmRows L, const double *site_q,
-                                                        double escale, double escale14, double r2off, double *pot)
+__global__ __launch_bounds__(256) void k_qcmm_potentials(QcmmGeom G, SplineDev S0, QcmmRows L, const double *site_q,
+                                                        double escale, double escale14, double r2off, double *part)
 {
 #pragma clang fp contract(off)
     __shared__ double sh[8];
-    const int q = blockIdx.x;
+    __shared__ double sS[3 * NB_SPLINE_LDS];
+    const SplineDev S = stage_spline(S0, sS);
+    const int q = blockIdx.x, c = blockIdx.y, stride = gridDim.y * 256;
     double xq[3];
     qc_coord(G, q, xq);
     const bool atoms = G.site_a == nullptr;
@@ -215,7 +235,7 @@ __global__ __launch_bounds__(256) void k_qcmm_potentials(QcmmGeom G, SplineDev S
         const int *idx = list ? L.idx14 : L.idx;
         if (!ptr) continue;
         const double sc = list ? escale14 : escale;
-        for (int k = ptr[q] + threadIdx.x; k < ptr[q + 1]; k += 256) {
+        for (int k = ptr[q] + c * 256 + threadIdx.x; k < ptr[q + 1]; k += stride) {
             const int m = idx[k];
             double xm[3];
             site_coord(G, m, atoms, xm);
@@ -228,27 +248,34 @@ __global__ __launch_bounds__(256) void k_qcmm_potentials(QcmmGeom G, SplineDev S
         }
     }
     block_sum_n(v, 2, sh);
-    if (threadIdx.x == 0) pot[q] = v[0] + v[1];
+    if (threadIdx.x == 0) {
+        part[2 * ((size_t)q * gridDim.y + c)] = v[0];
+        part[2 * ((size_t)q * gridDim.y + c) + 1] = v[1];
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
 // Gradients for the QC charges: per pair v = (x_q - x_m) * 2 f'(r^2) (scale
 // qc_q) q_m; the MM site gets -v (onto its atom, or half each onto the
 // boundary atom and partner), the QC atom the sum of v, passed through the
-// link-atom rule (QCAtomContainer_SetAtomGradients3).
+// link-atom rule (QCAtomContainer_SetAtomGradients3, linear in the sum, so each
+// chunk of a QC atom's lists applies it to its own sum).  All sums are fixed
+// point, so the chunking does not change the result.
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ void gacc_add(long long *gacc, int n, int a, const double v[3])
 {
     for (int k = 0; k < 3; k++) fx_add(&gacc[(size_t)k * n + a], &gacc[(size_t)(k + 3) * n + a], v[k]);
 }
 
-__global__ __launch_bounds__(256) void k_qcmm_gradients(QcmmGeom G, SplineDev S, QcmmRows L, const double *site_q,
+__global__ __launch_bounds__(256) void k_qcmm_gradients(QcmmGeom G, SplineDev S0, QcmmRows L, const double *site_q,
                                                        const double *qcq, double scale, double scale14, double r2off,
                                                        long long *gacc)
 {
 #pragma clang fp contract(off)
     __shared__ double sh[12];
-    const int q = blockIdx.x;
+    __shared__ double sS[3 * NB_SPLINE_LDS];
+    const SplineDev S = stage_spline(S0, sS);
+    const int q = blockIdx.x, c = blockIdx.y, stride = gridDim.y * 256;
     double xq[3];
     qc_coord(G, q, xq);
     const bool atoms = G.site_a == nullptr;
@@ -258,7 +285,7 @@ __global__ __launch_bounds__(256) void k_qcmm_gradients(QcmmGeom G, SplineDev S,
         const int *idx = list ? L.idx14 : L.idx;
         if (!ptr) continue;
         const double qq = (list ? scale14 : scale) * qcq[q];
-        for (int k = ptr[q] + threadIdx.x; k < ptr[q + 1]; k += 256) {
+        for (int k = ptr[q] + c * 256 + threadIdx.x; k < ptr[q + 1]; k += stride) {
             const int m = idx[k];
             const double qqm = qq * site_q[m];
             double xm[3];
@@ -346,20 +373,20 @@ void launch_cross_list(const CrossListParams &P, int pass, hipStream_t s)
 }
 
 void launch_qcmm_potentials(const QcmmGeom &G, const SplineDev &S, const QcmmRows &L, const double *site_q,
-                            double escale, double escale14, double r2off, double *pot, hipStream_t s)
+                            double escale, double escale14, double r2off, int chunks, double *part, hipStream_t s)
 {
     if (G.nqc > 0)
-        hipLaunchKernelGGL(k_qcmm_potentials, dim3(G.nqc), dim3(256), 0, s, G, S, L, site_q, escale, escale14, r2off,
-                           pot);
+        hipLaunchKernelGGL(k_qcmm_potentials, dim3(G.nqc, chunks), dim3(256), 0, s, G, S, L, site_q, escale, escale14,
+                           r2off, part);
 }
 
 void launch_qcmm_gradients(const QcmmGeom &G, const SplineDev &S, const QcmmRows &L, const double *site_q,
-                           const double *qcq, double scale, double scale14, double r2off, long long *gacc,
+                           const double *qcq, double scale, double scale14, double r2off, int chunks, long long *gacc,
                            hipStream_t s)
 {
     if (G.nqc > 0)
-        hipLaunchKernelGGL(k_qcmm_gradients, dim3(G.nqc), dim3(256), 0, s, G, S, L, site_q, qcq, scale, scale14,
-                           r2off, gacc);
+        hipLaunchKernelGGL(k_qcmm_gradients, dim3(G.nqc, chunks), dim3(256), 0, s, G, S, L, site_q, qcq, scale,
+                           scale14, r2off, gacc);
 }
 
 void launch_qcmm_grad_out(const long long *gacc, int n, double *grad, hipStream_t s)

@@ -393,10 +393,10 @@ def _bond_paths(nat, bonds):
     return sorted(excl), sorted(p14)
 
 
-def qcmm_chain_in_water(nwater=1000, box=31.0, seed=20261207, name=None):
+def qcmm_chain_in_water(nwater=1000, box=31.0, seed=20261207, name=None, qc_waters=0):
     """Returns (MMSystem, qc): the chain (atoms 0-16) followed by the waters; qc describes the QC atoms as
     pDynamo's QCAtomContainer holds them (index order, boundary flags, QC partners, MM partners of each
-    boundary atom)."""
+    boundary atom).  qc_waters > 0 adds that many whole waters nearest the chain to the QC region."""
     rng = np.random.default_rng(seed)
     nch = 17
     xc = _chain_geometry(rng, nch, CHAIN_BONDS)
@@ -427,7 +427,12 @@ def qcmm_chain_in_water(nwater=1000, box=31.0, seed=20261207, name=None):
     for i, j in CHAIN_BONDS:
         nbr[i].add(j)
         nbr[j].add(i)
-    qc_atoms = sorted(set(CHAIN_QC_PURE) | set(CHAIN_BOUNDARY))
+    qc_set = set(CHAIN_QC_PURE) | set(CHAIN_BOUNDARY)
+    if qc_waters > 0:
+        dw = np.linalg.norm(w[0::3] - xc.mean(0), axis=1)
+        for k in np.argsort(dw)[:qc_waters]:
+            qc_set |= {nch + 3 * int(k), nch + 3 * int(k) + 1, nch + 3 * int(k) + 2}
+    qc_atoms = sorted(qc_set)
     boundary = np.array([a in CHAIN_BOUNDARY for a in qc_atoms])
     partner = np.array([CHAIN_BOUNDARY.get(a, -1) for a in qc_atoms], np.int32)
     mmp = [sorted(nbr[a] - set(qc_atoms)) if a in CHAIN_BOUNDARY else [] for a in qc_atoms]
