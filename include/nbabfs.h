@@ -142,7 +142,9 @@ int nbabfs_energies(nbabfs_state *state, double *energies, int *present);
  * list and, for the plain path, its rows), o being the state's cluster-grid
  * origin; for the non-periodic pair model this equals sum over pairs of
  * r_ij,a * g_ij,b.  No counterpart in the reference, which computes no virial
- * for non-periodic systems; synchronises the state's stream. */
+ * for non-periodic systems; computed on request from the call's fixed-point
+ * gradient sums, so it is valid until the state's next update or energy call;
+ * synchronises the state's stream. */
 int nbabfs_virial(nbabfs_state *state, double *virial);
 
 /* Work-item autotuner (no counterpart in the reference, which has no work

@@ -127,8 +127,6 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
     if (heads && blockIdx.y == 0 && blockIdx.x == 0 && threadIdx.x < 8) heads[32 * threadIdx.x] = 0;
     if (blockIdx.x == 0 && D.eacc)
         for (int k = threadIdx.x; k < NB_EACC_WORDS; k += 256) D.eacc[k] = 0;
-    if (blockIdx.x == 0 && D.vacc)
-        for (int k = threadIdx.x; k < NB_VACC_WORDS; k += 256) D.vacc[k] = 0;
     if (D.gacc)
         for (int k = blockIdx.x * 256 + threadIdx.x; k < NB_GACC_WORDS_PER_IMAGE * D.nimg; k += gridDim.x * 256)
             D.gacc[k] = 0;
@@ -741,29 +739,20 @@ __device__ void sum_range2(const double *__restrict__ p, int m, double &x, doubl
 __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ descs, int plain)
 {
     const SysDesc &D = descs[blockIdx.y];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    double g[3] = {0.0, 0.0, 0.0}, x[3] = {0.0, 0.0, 0.0};
-    if (i < D.n) {
-        const int s = D.atom_slot[i];
-        for (int k = 0; k < 3; k++) {
-            g[k] = fx_value(D.facc, D.n_pad, s, k);
-            if (plain) g[k] += D.gplain[3 * (size_t)i + k];
-            x[k] = D.xyz[3 * (size_t)i + k] - D.origin[k];
-            if (D.grad != nullptr) D.grad[3 * (size_t)i + k] += g[k];
+    // The last block of each system sums its energies while the others add
+    // the gradients (the two are independent, so neither waits for the other).
+    if (blockIdx.x + 1 != gridDim.x) {
+        const int i = blockIdx.x * 256 + threadIdx.x;
+        if (i < D.n && D.grad != nullptr) {
+            const int s = D.atom_slot[i];
+            for (int k = 0; k < 3; k++) {
+                double g = fx_value(D.facc, D.n_pad, s, k);
+                if (plain) g += D.gplain[3 * (size_t)i + k];
+                D.grad[3 * (size_t)i + k] += g;
+            }
         }
+        return;
     }
-    // Virial of this call's gradient, W_ab = sum_i x_i,a g_i,b: wave sums in
-    // FP64, added into fixed point (slot = wave index modulo 64).
-    if (D.vacc) {
-        const int slot = ((blockIdx.x * 4 + (threadIdx.x >> 6)) & (NB_ESLOTS - 1));
-#pragma unroll
-        for (int ab = 0; ab < 9; ab++) {
-            const double w = wave_sum_f64(x[ab / 3] * g[ab % 3]);
-            if ((threadIdx.x & 63) == ab && w != 0.0)
-                fx_add(&D.vacc[(2 * ab) * NB_ESLOTS + slot], &D.vacc[(2 * ab + 1) * NB_ESLOTS + slot], w);
-        }
-    }
-    if (blockIdx.x != 0) return;
     double el, lj, el14, lj14, iel = 0.0, ilj = 0.0;
     {
         // Exact: each slot's low word is split at 2^48 (its upper part carried
@@ -935,5 +924,38 @@ void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const A
 void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s)
 {
     if (nsys <= 0) return;
-    hipLaunchKernelGGL(k_finalize, dim3(std::max(nblk(max_n, 256), 1), nsys), dim3(256), 0, s, d_desc, plain);
+    hipLaunchKernelGGL(k_finalize, dim3(nblk(max_n, 256) + 1, nsys), dim3(256), 0, s, d_desc, plain);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Virial of the last call's gradient (still in the fixed-point accumulators,
+// plus the plain path's rows), W_ab = sum_i (x_i - o)_a g_i,b: wave sums in
+// FP64 added into fixed point (slot = wave index modulo 64), so the total
+// does not depend on which waves shared a slot.  vacc must be zero.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_virial(const SysDesc *__restrict__ descs, int plain)
+{
+    const SysDesc &D = descs[0];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double g[3] = {0.0, 0.0, 0.0}, x[3] = {0.0, 0.0, 0.0};
+    if (i < D.n) {
+        const int s = D.atom_slot[i];
+        for (int k = 0; k < 3; k++) {
+            g[k] = fx_value(D.facc, D.n_pad, s, k);
+            if (plain) g[k] += D.gplain[3 * (size_t)i + k];
+            x[k] = D.xyz[3 * (size_t)i + k] - D.origin[k];
+        }
+    }
+    const int slot = ((blockIdx.x * 4 + (threadIdx.x >> 6)) & (NB_ESLOTS - 1));
+#pragma unroll
+    for (int ab = 0; ab < 9; ab++) {
+        const double w = wave_sum_f64(x[ab / 3] * g[ab % 3]);
+        if ((threadIdx.x & 63) == ab && w != 0.0)
+            fx_add(&D.vacc[(2 * ab) * NB_ESLOTS + slot], &D.vacc[(2 * ab + 1) * NB_ESLOTS + slot], w);
+    }
+}
+
+void launch_virial(const SysDesc *d_desc, int n, int plain, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_virial, dim3(nblk(n, 256)), dim3(256), 0, s, d_desc, plain);
 }

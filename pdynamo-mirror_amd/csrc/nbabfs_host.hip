@@ -2218,7 +2218,11 @@ extern "C" int nbabfs_virial(nbabfs_state *S, double *virial)
 {
     if (!S || !virial) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
     for (int k = 0; k < 9; k++) virial[k] = 0.0;
-    if (!S->vacc.p || S->n == 0) return NBABFS_STATUS_CONTINUE;
+    if (!S->vacc.p || S->n == 0 || !S->listValid || S->desc_version < 0) return NBABFS_STATUS_CONTINUE;
+    // From the last call's gradient, still in the fixed-point accumulators.
+    launch_fill_int((int *)S->vacc.p, 2 * (size_t)NB_VACC_WORDS, 0, S->stream);
+    launch_virial(S->d_desc.p, S->n, S->opt.precision == NBABFS_PLAIN_FP64 ? 1 : 0, S->stream);
+    HIPCHK(hipGetLastError());
     std::vector<long long> w(NB_VACC_WORDS);
     HIPCHK(hipMemcpyAsync(w.data(), S->vacc.p, NB_VACC_WORDS * sizeof(long long), hipMemcpyDeviceToHost, S->stream));
     HIPCHK(hipStreamSynchronize(S->stream));

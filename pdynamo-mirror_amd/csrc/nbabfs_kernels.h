@@ -134,6 +134,8 @@ void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int n
                              ForceGuard g, hipStream_t s);
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s);
 void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s);
+// Virial of the single system's last call (NB_VACC_WORDS in D.vacc, zeroed by the caller).
+void launch_virial(const SysDesc *d_desc, int n, int plain, hipStream_t s);
 
 // QC/MM electrostatics (nbabfs_qcmm.hip).  QC atoms in ascending atom order;
 // qc_partner[q] >= 0 marks a link atom (the QC partner of boundary atom
