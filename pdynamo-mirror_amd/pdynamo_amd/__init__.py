@@ -7,13 +7,15 @@ Drop-in names: ``System``, ``NBModelABFS``, ``NBModelABFSState``,
 ``EnergyTerms``; crystal symmetry for the periodic image lists:
 ``System.DefineSymmetry`` with the ``CrystalClass*`` classes,
 ``Transformation3Container`` and ``Transformation3_FromSymmetryOperationString``.
-Batches of images / replicas: ``BatchEnergy``.  QC/MM systems: ``QCAtomContainer``
+Batches of images / replicas: ``SGOFProcessPoolGPU`` (the pool interface of pDynamo's
+chain-of-states optimizer) and ``BatchEnergy``.  QC/MM systems: ``QCAtomContainer``
 (``QCAtomContainer_FromAtomContainer``) passed to ``System.FromMMParameters`` or
 ``System.DefineQCModel``, ``PairwiseInteractionABFS``, ``QCMMInteractionState`` and the
 ``QCModelPointCharges`` stand-in for a QC program.
 """
 
 from ._native import CLibraryError
+from .BatchPool import BatchEnergy, SGOFProcessPoolGPU
 from .LogFile import LogFileActive, TextLogFile, logFile
 from .NBModelABFS import NBModelABFS, NBModelABFSState, PairListGenerator
 from .PairwiseInteraction import PairwiseInteractionABFS
@@ -36,4 +38,5 @@ __all__ = ["CLibraryError", "LogFileActive", "TextLogFile", "logFile", "NBModelA
            "Symmetry", "SymmetryParameterGradients", "SymmetryParameters", "Transformation3",
            "Transformation3Container", "Transformation3Container_Identity",
            "Transformation3_FromSymmetryOperationString", "PairwiseInteractionABFS", "QCAtomContainer",
-           "QCAtomContainer_FromAtomContainer", "QCMMInteractionState", "QCModelPointCharges"]
+           "QCAtomContainer_FromAtomContainer", "QCMMInteractionState", "QCModelPointCharges", "BatchEnergy",
+           "SGOFProcessPoolGPU"]

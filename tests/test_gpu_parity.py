@@ -296,3 +296,26 @@ def test_dhfr(oracle_mod, dhfr, precision, etol, gtol):
                               nthreads=8)
     _, rgrad, _ = o.energy(dhfr.mm.coordinates3)
     assert np.linalg.norm(grad - rgrad) <= gtol * np.linalg.norm(rgrad)
+
+
+def test_sgof_pool_matches_system_energy():
+    """The GPU image pool (pDynamo's SGOFProcessPool interface) gives each image the energy and gradients of a
+    System.Energy call on that image, bit for bit, and keeps each image's list between calls."""
+    from pdynamo_amd import NBModelABFS, SGOFProcessPoolGPU, System, synthetic
+    mm = synthetic.random_small_system(1500, 45.0, 5)
+    s = System.FromSynthetic(mm)
+    s.DefineNBModel(NBModelABFS())
+    rng = np.random.default_rng(3)
+    images = [mm.coordinates3 + rng.normal(0.0, 0.1, mm.coordinates3.shape) for _ in range(5)]
+    pool = SGOFProcessPoolGPU.FromSystem(s)
+    for rep in range(2):
+        g = [np.zeros((mm.natoms, 3)) for _ in images]
+        f = pool.FunctionGradients(images, g)
+        assert pool.rebuilt.all() == (rep == 0)
+        for k, x in enumerate(images):
+            t = System.FromSynthetic(mm.with_coordinates(x))
+            t.DefineNBModel(NBModelABFS())
+            e = t.Energy(log=None, doGradients=True)
+            assert f[k] == e, (k, f[k], e)
+            assert np.array_equal(g[k], t.configuration.gradients3)
+    pool.Finalize()
