@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void k_bucket_sort(const int *__restrict__ off
     }
 }
 
-// Column sizes padded to whole clusters.
+// Column sizes padded to whole super-clusters.
 __global__ __launch_bounds__(256) void k_colpad(const int *__restrict__ off, int ncol, int nzb, int n,
                                                 int *__restrict__ padded)
 {
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_colpad(const int *__restrict__ off, int
     int b = off[c * nzb];
     int e = (c + 1 < ncol) ? off[(c + 1) * nzb] : n;
     int cnt = e - b;
-    padded[c] = (cnt + NB_CL - 1) / NB_CL * NB_CL;
+    padded[c] = (cnt + NB_SC - 1) / NB_SC * NB_SC;
 }
 
 __global__ __launch_bounds__(256) void k_fill_int(int *__restrict__ p, size_t m, int v)
@@ -187,6 +187,62 @@ __global__ __launch_bounds__(256) void k_slots(const int *__restrict__ sorted, c
     int slot = colstart[col] + (p - off[col * nzb]);
     slot_atom[slot] = a;
     atom_slot[a] = slot;
+}
+
+// Super-cluster shape.  Columns are one super-cluster wide and padded to whole
+// super-clusters, so the 64 slots of a super-cluster hold atoms that are
+// consecutive in z within one column: a roughly cubic block.  One wavefront
+// per super-cluster splits it into 2 x 2 x 2 clusters: its m atoms are ranked
+// by x and cut into halves of min(32, m) and the rest, each half ranked by y
+// and cut at 16, each quarter ranked by z and cut at 8; full clusters come
+// first.  Ties are broken by atom index, so the order is deterministic.  The
+// pair set does not depend on it (every pair is tested atom by atom).
+__device__ __forceinline__ bool before(double u, int a, double v, int b)
+{
+    return u < v || (u == v && a < b);
+}
+
+__global__ __launch_bounds__(64) void k_sc_split(const double *__restrict__ xyz, int *__restrict__ slot_atom,
+                                                 int *__restrict__ atom_slot)
+{
+    __shared__ double sx[NB_SC], sy[NB_SC], sz[NB_SC];
+    __shared__ int sa[NB_SC], sg[NB_SC], sOut[NB_SC];
+    const int l = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * NB_SC;
+    const int a = slot_atom[base + l];
+    const bool valid = a >= 0;
+    sa[l] = a;
+    sx[l] = valid ? xyz[3 * (size_t)a] : 0.0;
+    sy[l] = valid ? xyz[3 * (size_t)a + 1] : 0.0;
+    sz[l] = valid ? xyz[3 * (size_t)a + 2] : 0.0;
+    sg[l] = valid ? 0 : -1;
+    sOut[l] = -1;
+    __syncthreads();
+    int pos = 0;
+    // Level 0 ranks by x, level 1 by y within the x half, level 2 by z within the quarter.
+    for (int lev = 0; lev < 3; lev++) {
+        const double *sc = lev == 0 ? sx : (lev == 1 ? sy : sz);
+        const double u = sc[l];
+        const int g = sg[l];
+        int rank = 0, size = 0;
+        for (int k = 0; k < NB_SC; k++) {
+            if (sg[k] != g) continue;
+            size++;
+            if (before(sc[k], sa[k], u, a)) rank++;
+        }
+        const int cut = min(32 >> lev, size);
+        const int hi = rank >= cut ? 1 : 0;
+        pos += hi * (32 >> lev);
+        __syncthreads();
+        if (valid) sg[l] = 2 * g + hi;
+        __syncthreads();
+        if (lev == 2) pos += rank - (hi ? cut : 0);
+    }
+    if (valid && pos >= 0 && pos < NB_SC) sOut[pos] = a;
+    __syncthreads();
+    const int o = sOut[l];
+    slot_atom[base + l] = o;
+    if (o >= 0) atom_slot[o] = (int)(base + l);
 }
 
 // Per-column cluster range [first, last) for the search.
@@ -966,11 +1022,14 @@ void launch_sort_slots(const int *key, int n, int nkeys, int ncol, int nzb, int 
 }
 
 void launch_slots(const int *sorted, const int *key, int n, int nzb, const int *off, const int *colstart,
-                  const int *padded, int ncol, int *slot_atom, int *atom_slot, int2 *colcl, hipStream_t s)
+                  const int *padded, int ncol, const double *xyz, int nsc, int *slot_atom, int *atom_slot, int2 *colcl,
+                  hipStream_t s)
 {
-    if (n > 0)
+    if (n > 0) {
         hipLaunchKernelGGL(k_slots, dim3(nblk(n, 256)), dim3(256), 0, s, sorted, key, n, nzb, off, colstart, slot_atom,
                            atom_slot);
+        hipLaunchKernelGGL(k_sc_split, dim3(nsc), dim3(64), 0, s, xyz, slot_atom, atom_slot);
+    }
     hipLaunchKernelGGL(k_colclusters, dim3(nblk(ncol, 256)), dim3(256), 0, s, colstart, padded, ncol, colcl);
 }
 

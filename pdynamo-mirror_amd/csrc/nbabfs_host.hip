@@ -828,19 +828,20 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     }
     CHK(S->stencil.upload(flags.data(), flags.size(), st));
 
-    // Cluster columns: width so that an 8-atom cluster is roughly cubic.
+    // Super-cluster columns: width so that 64 consecutive atoms of a column
+    // (one super-cluster, split into 2 x 2 x 2 clusters) fill a rough cube.
     double ext[3], vol = 1.0;
     for (int d = 0; d < 3; d++) {
         ext[d] = bb[3 + d] - bb[d];
         vol *= std::max(ext[d], 1.0);
         g.corg[d] = bb[d];
     }
-    double cw = std::cbrt((double)NB_CL * vol / (double)n);
-    cw = std::min(std::max(cw, 1.5), 1.0e4);
+    double cw = std::cbrt((double)NB_SC * vol / (double)n);
+    cw = std::min(std::max(cw, 3.0), 1.0e4);
     g.cw = cw;
     g.ncx = std::max(1, (int)std::ceil(ext[0] / cw));
     g.ncy = std::max(1, (int)std::ceil(ext[1] / cw));
-    double dz = 0.25 * cw;
+    double dz = 0.125 * cw;
     long long maxkeys = 8LL * n + 4096;
     if ((long long)g.ncx * g.ncy * (long long)std::ceil(ext[2] / dz + 1.0) > maxkeys)
         dz = ext[2] / std::max(1.0, (double)(maxkeys / ((long long)g.ncx * g.ncy)) - 1.0);
@@ -879,8 +880,8 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     CHK(S->slot_atom.ensure(S->n_pad));
     CHK(S->atom_slot.ensure(n));
     launch_fill_int(S->slot_atom.p, S->n_pad, -1, st);
-    launch_slots(S->sorted.p, S->key.p, n, g.nzb, S->off.p, S->colstart.p, S->padded.p, ncol, S->slot_atom.p,
-                 S->atom_slot.p, S->colcl.p, st);
+    launch_slots(S->sorted.p, S->key.p, n, g.nzb, S->off.p, S->colstart.p, S->padded.p, ncol, d_xyz, S->nsc,
+                 S->slot_atom.p, S->atom_slot.p, S->colcl.p, st);
     CHK(S->blo.ensure(S->ncl));
     CHK(S->bhi.ensure(S->ncl));
     CHK(S->centre.ensure(S->ncl));

@@ -57,7 +57,8 @@ void launch_fill_int(int *p, size_t m, int v, hipStream_t s);
 void launch_sort_slots(const int *key, int n, int nkeys, int ncol, int nzb, int *off, int *cursor, int *sorted,
                        int *padded, hipStream_t s);
 void launch_slots(const int *sorted, const int *key, int n, int nzb, const int *off, const int *colstart,
-                  const int *padded, int ncol, int *slot_atom, int *atom_slot, int2 *colcl, hipStream_t s);
+                  const int *padded, int ncol, const double *xyz, int nsc, int *slot_atom, int *atom_slot, int2 *colcl,
+                  hipStream_t s);
 void launch_cluster_geom(const double *xyz, const int *slot_atom, int ncl, const double *origin, float4 *blo,
                          float4 *bhi, float4 *centre, hipStream_t s);
 void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4 *cellv, const int *slot_atom,
