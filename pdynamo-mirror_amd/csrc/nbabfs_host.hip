@@ -364,6 +364,7 @@ struct nbabfs_state {
     PinBuf<SysDesc> h_desc;
     PinBuf<double> h_energies, h_grad;
     long desc_version = -1;
+    long items_version = 0;        // bumped whenever the work items are cut or ordered anew
     const double *last_xyz = nullptr;
     double *last_grad = nullptr;
     // Coordinates for which the last update already queued the prepare pass
@@ -771,6 +772,7 @@ static int order_items(nbabfs_state *S)
         HIPCHK(hipGetLastError());
         S->items.swap(S->items2);
     }
+    S->items_version++;
     qs[0] = 0;
     for (int q = 0; q < 8; q++) qs[q + 1] = interleave ? qs[q] + (n - q + 7) / 8 * (n > q) : (int)((long long)n * (q + 1) / 8);
     CHK(S->qstarts.upload(qs.data(), 9, st));
@@ -1177,6 +1179,8 @@ static int n_persistent_blocks(const nbabfs_options &o, int nitems)
 static int launch_force(nbabfs_state *S, ForceGuard g)
 {
     hipStream_t st = S->stream;
+    // The kernels write through the descriptor's pointers: it must describe the current buffers.
+    if (S->desc_version != g_alloc_version) return fail(NBABFS_STATUS_LOGIC_ERROR, "pair kernel launched with an out-of-date descriptor");
     if (S->timing) CHK(record_event(S));
     if (S->opt.precision != NBABFS_PLAIN_FP64) {
         const int nm = S->nitems_main, ni = S->nitems - S->nitems_main;
@@ -1249,6 +1253,7 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
 static int recut_items(nbabfs_state *S, int per_item)
 {
     hipStream_t st = S->stream;
+    per_item = std::max(1, per_item > 0 ? per_item : 16);
     const int ipi = image_per_item(S->opt);
     if (S->nsc <= 0) return NBABFS_STATUS_CONTINUE;
     const int nb = (int)S->slots.size() * S->nsc;
@@ -1614,6 +1619,8 @@ static int autotune(nbabfs_state *S, const double *d_xyz, int *best_epi, int *be
         S->opt.entriesPerItem = cand[c][0];
         S->opt.wavesPerCU = cand[c][1];
         rc = recut_items(S, cand[c][0]);
+        // The launches below read the descriptor: refresh it after the new cut.
+        if (rc == NBABFS_STATUS_CONTINUE) rc = ensure_desc(S, d_xyz, S->last_grad);
         // Each launch is timed alone; persistent queues are reset before it.
         for (int r = -1; r < reps && rc == NBABFS_STATUS_CONTINUE; r++) {
             if (S->opt.wavesPerCU > 0) launch_fill_int(S->heads.p, 8 * 32, 0, st);
@@ -2290,6 +2297,7 @@ struct nbabfs_batch {
     long layout_version = -1;
     std::vector<const double *> last_xyz;
     std::vector<double *> last_grad;
+    std::vector<long> last_items;
     hipGraphExec_t gexec = nullptr;
     bool graph_valid = false;
     bool mixed = true;
@@ -2334,6 +2342,7 @@ extern "C" int nbabfs_batch_create(nbabfs_state *const *states, int nsystems, in
     }
     B->last_xyz.assign(nsystems, nullptr);
     B->last_grad.assign(nsystems, nullptr);
+    B->last_items.assign(nsystems, -1);
     *out = B;
     return NBABFS_STATUS_CONTINUE;
 }
@@ -2377,6 +2386,7 @@ static int batch_layout(nbabfs_batch *B, const double *const *d_xyz, double *con
         first += s->nitems;
         B->last_xyz[k] = d_xyz[k];
         B->last_grad[k] = d_grad ? d_grad[k] : nullptr;
+        B->last_items[k] = s->items_version;
     }
     HIPCHK(hipMemcpyAsync(B->d_desc.p, B->h_desc.p, ns * sizeof(SysDesc), hipMemcpyHostToDevice, st));
     std::vector<int> qs(9);
@@ -2473,7 +2483,8 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
     }
     bool relayout = any || B->layout_version != g_alloc_version;
     for (int k = 0; k < ns && !relayout; k++)
-        relayout = B->last_xyz[k] != d_xyz[k] || B->last_grad[k] != (d_grad ? d_grad[k] : nullptr);
+        relayout = B->last_xyz[k] != d_xyz[k] || B->last_grad[k] != (d_grad ? d_grad[k] : nullptr) ||
+                   B->last_items[k] != B->st[k]->items_version;
     if (relayout) CHK(batch_layout(B, d_xyz, d_grad));
 
     if (B->use_graph) {
