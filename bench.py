@@ -149,8 +149,9 @@ def run_qcmm(args):
     hg = np.zeros_like(mm.coordinates3)
     pot = np.zeros(nqc)
 
+    st.set_gradient_mode(_native.GRADIENT_ASSIGN)
+
     def step(qc=True):
-        g.zero(st)
         st.update(x.ptr)
         st.energy_async(x.ptr, g.ptr)
         if qc:
@@ -227,8 +228,11 @@ def run_batch(args, systems, wdesc):
     def states_for(precision):
         model = NBModelABFS(precision=precision, **({"entriesPerItem": args.entries_per_item}
                                                     if args.entries_per_item else {}))
-        return [_native.NativeState(model.NativeOptions(), m.charges, m.ljtypes, m.ntypes, m.tableA, m.tableB,
-                                    exclusions=m.exclusions) for m in systems]
+        sts = [_native.NativeState(model.NativeOptions(), m.charges, m.ljtypes, m.ntypes, m.tableA, m.tableB,
+                                   exclusions=m.exclusions) for m in systems]
+        for st in sts:
+            st.set_gradient_mode(_native.GRADIENT_ASSIGN)   # the batch writes every gradient row
+        return sts
 
     sizes = [m.natoms for m in systems]
     offs = np.concatenate([[0], np.cumsum(sizes)])
@@ -241,12 +245,10 @@ def run_batch(args, systems, wdesc):
     states = states_for(args.precision)
     batch = _native.NativeBatch(states, use_graph=True)
     for _ in range(args.warmup):
-        G.zero()
         batch.energy(xp, gp)
     _native.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        G.zero()
         e, rebuilt = batch.energy(xp, gp)
     _native.synchronize()
     ms = (time.perf_counter() - t0) / args.steps * 1e3
@@ -262,7 +264,6 @@ def run_batch(args, systems, wdesc):
         reps = max(1, args.steps // 5)
         t0 = time.perf_counter()
         for _ in range(reps):
-            G.zero()
             for st, x, g in zip(seq, xp, gp):
                 st.update(x)
                 st.energy_async(x, g)
@@ -391,11 +392,13 @@ def main():
     counter = [0]
     rebuilds = [0]
 
+    # System.Energy zeroes the gradients before the model adds to them; with the model as
+    # the first gradient term that zero fill is folded into its finalize pass (assign mode).
+    st.set_gradient_mode(_native.GRADIENT_ASSIGN)
+
     def step():
-        # System.Energy zeroes the gradients; the model increments them.
         x = xs[counter[0] % len(xs)]
         counter[0] += 1
-        g.zero(st)
         rebuilds[0] += st.update(x.ptr)
         st.energy_async(x.ptr, g.ptr)
 

@@ -137,6 +137,19 @@ int nbabfs_update(nbabfs_state *state, const double *d_xyz, int *rebuilt);
 int nbabfs_mmmm_energy_async(nbabfs_state *state, const double *d_xyz, double *d_grad);
 int nbabfs_energies(nbabfs_state *state, double *energies, int *present);
 
+/* How the energy calls of a state hand over its gradient:
+ * NBABFS_GRADIENT_ADD (the default, and the reference's convention: the model
+ * increments the caller's array, NBModelABFSState_AccumulateGradients,
+ * pMx/csource/NBModelABFSState.c:19-38) or NBABFS_GRADIENT_ASSIGN (every
+ * atom's row is overwritten with this model's gradient).  No counterpart in
+ * the reference: for callers whose first gradient term is this model, so the
+ * zero fill they would issue before it is folded into the model's last pass.
+ * Applies to nbabfs_energy_host, nbabfs_mmmm_energy_async and to batches
+ * (per state).  Synchronises the state's stream. */
+#define NBABFS_GRADIENT_ADD 0
+#define NBABFS_GRADIENT_ASSIGN 1
+int nbabfs_set_gradient_mode(nbabfs_state *state, int mode);
+
 /* Virial of the last energy call's gradient contribution, row-major 3x3:
  * W[3a+b] = sum_i (x_i - o)_a * dE/dx_i,b over the atoms (main list, 1-4
  * list and, for the plain path, its rows), o being the state's cluster-grid

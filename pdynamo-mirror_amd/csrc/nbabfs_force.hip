@@ -720,7 +720,7 @@ __global__ __launch_bounds__(256) void k_pairs14(const SysDesc *__restrict__ des
 
 // ------------------------------------------------------------------------------------------------
 // Finalize: gradients (fixed point -> FP64, plus the plain-path rows) added
-// into the caller's array; block 0 of each system also sums the energies in a
+// into the caller's array (or written over it in assign mode); block 0 of each system also sums the energies in a
 // fixed order.
 // ------------------------------------------------------------------------------------------------
 __device__ void sum_range2(const double *__restrict__ p, int m, double &x, double &y)
@@ -748,7 +748,7 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
             for (int k = 0; k < 3; k++) {
                 double g = fx_value(D.facc, D.n_pad, s, k);
                 if (plain) g += D.gplain[3 * (size_t)i + k];
-                D.grad[3 * (size_t)i + k] += g;
+                D.grad[3 * (size_t)i + k] = D.grad_assign ? g : D.grad[3 * (size_t)i + k] + g;
             }
         }
         return;

@@ -365,6 +365,7 @@ struct nbabfs_state {
     PinBuf<double> h_energies, h_grad;
     long desc_version = -1;
     long items_version = 0;        // bumped whenever the work items are cut or ordered anew
+    int grad_assign = 0;           // NBABFS_GRADIENT_ASSIGN: energy calls overwrite the gradient
     const double *last_xyz = nullptr;
     double *last_grad = nullptr;
     // Coordinates for which the last update already queued the prepare pass
@@ -1125,6 +1126,7 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.ljA14 = S->ljA14.p;
     D.ljB14 = S->ljB14.p;
     D.entries = S->entries.p;
+    D.grad_assign = S->grad_assign;
     D.p14 = S->p14.p;
     D.facc = S->facc.p;
     D.gplain = S->gplain.p;
@@ -1917,7 +1919,7 @@ extern "C" int nbabfs_energy_host(nbabfs_state *S, const double *xyz, double *gr
     double *dg = nullptr;
     if (grad_inout && m) {
         CHK(S->grad_stage.ensure(m));
-        launch_fill_int((int *)S->grad_stage.p, 2 * m, 0, st);
+        if (!S->grad_assign) launch_fill_int((int *)S->grad_stage.p, 2 * m, 0, st);
         dg = S->grad_stage.p;
     }
     CHK(mmmm_async(S, S->xyz_stage.p, dg));
@@ -1926,8 +1928,22 @@ extern "C" int nbabfs_energy_host(nbabfs_state *S, const double *xyz, double *gr
         HIPCHK(hipMemcpyAsync(S->h_grad.p, dg, m * sizeof(double), hipMemcpyDeviceToHost, st));
     }
     CHK(nbabfs_energies(S, energies, present));
-    if (dg)
-        for (size_t k = 0; k < m; k++) grad_inout[k] += S->h_grad.p[k];
+    if (dg) {
+        if (S->grad_assign) std::memcpy(grad_inout, S->h_grad.p, m * sizeof(double));
+        else
+            for (size_t k = 0; k < m; k++) grad_inout[k] += S->h_grad.p[k];
+    }
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_set_gradient_mode(nbabfs_state *S, int mode)
+{
+    if (!S) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state is NULL");
+    if (mode != NBABFS_GRADIENT_ADD && mode != NBABFS_GRADIENT_ASSIGN)
+        return fail(NBABFS_STATUS_INVALID_ARGUMENT, "unknown gradient mode %d", mode);
+    HIPCHK(hipStreamSynchronize(S->stream));
+    S->grad_assign = mode == NBABFS_GRADIENT_ASSIGN ? 1 : 0;
+    S->desc_version = -1;
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -2298,6 +2314,7 @@ struct nbabfs_batch {
     std::vector<const double *> last_xyz;
     std::vector<double *> last_grad;
     std::vector<long> last_items;
+    std::vector<int> last_assign;
     hipGraphExec_t gexec = nullptr;
     bool graph_valid = false;
     bool mixed = true;
@@ -2343,6 +2360,7 @@ extern "C" int nbabfs_batch_create(nbabfs_state *const *states, int nsystems, in
     B->last_xyz.assign(nsystems, nullptr);
     B->last_grad.assign(nsystems, nullptr);
     B->last_items.assign(nsystems, -1);
+    B->last_assign.assign(nsystems, -1);
     *out = B;
     return NBABFS_STATUS_CONTINUE;
 }
@@ -2387,6 +2405,7 @@ static int batch_layout(nbabfs_batch *B, const double *const *d_xyz, double *con
         B->last_xyz[k] = d_xyz[k];
         B->last_grad[k] = d_grad ? d_grad[k] : nullptr;
         B->last_items[k] = s->items_version;
+        B->last_assign[k] = s->grad_assign;
     }
     HIPCHK(hipMemcpyAsync(B->d_desc.p, B->h_desc.p, ns * sizeof(SysDesc), hipMemcpyHostToDevice, st));
     std::vector<int> qs(9);
@@ -2484,7 +2503,7 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
     bool relayout = any || B->layout_version != g_alloc_version;
     for (int k = 0; k < ns && !relayout; k++)
         relayout = B->last_xyz[k] != d_xyz[k] || B->last_grad[k] != (d_grad ? d_grad[k] : nullptr) ||
-                   B->last_items[k] != B->st[k]->items_version;
+                   B->last_items[k] != B->st[k]->items_version || B->last_assign[k] != B->st[k]->grad_assign;
     if (relayout) CHK(batch_layout(B, d_xyz, d_grad));
 
     if (B->use_graph) {

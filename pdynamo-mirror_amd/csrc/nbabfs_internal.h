@@ -118,6 +118,7 @@ struct SysDesc {
     int n_qlj;
     const int *qlj_atom, *qlj_ptr, *qlj_idx, *qlj14_ptr, *qlj14_idx;
     double *qljpart;            // 2 per (row, chunk) (QC/MM LJ, QC/MM 1-4 LJ), summed in that order
+    int grad_assign;            // 1: the finalize pass overwrites grad instead of adding to it
 };
 
 // ------------------------------------------------------------------------------------------------

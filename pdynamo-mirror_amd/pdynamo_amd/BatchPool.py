@@ -49,10 +49,13 @@ class SGOFProcessPoolGPU:
         if self.system.fixedAtoms is not None:
             fixed = np.zeros(self.natoms, np.uint8)
             fixed[np.asarray(self.system.fixedAtoms, np.int64)] = 1
-        return _native.NativeState(self.nbModel.NativeOptions(), em.mmAtoms.charges, em.mmAtoms.ljtypes, lj.ntypes,
-                                   lj.tableA, lj.tableB, lj14.tableA, lj14.tableB,
-                                   None if em.exclusions is None else em.exclusions.pairs,
-                                   None if em.interactions14 is None else em.interactions14.pairs, fixed)
+        st = _native.NativeState(self.nbModel.NativeOptions(), em.mmAtoms.charges, em.mmAtoms.ljtypes, lj.ntypes,
+                                 lj.tableA, lj.tableB, lj14.tableA, lj14.tableB,
+                                 None if em.exclusions is None else em.exclusions.pairs,
+                                 None if em.interactions14 is None else em.interactions14.pairs, fixed)
+        # The pool's gradient rows get this model's gradient only: written, not added (no zero fill).
+        st.set_gradient_mode(_native.GRADIENT_ASSIGN)
+        return st
 
     def _resize(self, m):
         if self.batch is not None and len(self.states) == m:
@@ -72,7 +75,6 @@ class SGOFProcessPoolGPU:
         X = np.concatenate([np.asarray(xi, np.float64).reshape(self.natoms, 3) for xi in x])
         self._resize(m)
         self.x.upload(X)
-        self.g.zero()
         stride = 24 * self.natoms
         xp = [self.x.ptr + k * stride for k in range(m)]
         gp = [self.g.ptr + k * stride for k in range(m)] if g is not None else None

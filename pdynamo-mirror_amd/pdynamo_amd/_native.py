@@ -13,6 +13,7 @@ import numpy as np
 from .build import LIB
 
 STATUS_CONTINUE = 16
+GRADIENT_ADD, GRADIENT_ASSIGN = 0, 1           # nbabfs_set_gradient_mode
 MIXED, PLAIN_FP64 = 0, 1
 QCMM_MM, QCMM_RC, QCMM_RD = 0, 1, 2
 
@@ -26,7 +27,7 @@ EXPORTED = ["nbabfs_default_options", "nbabfs_init", "nbabfs_last_error", "nbabf
             "nbabfs_virial", "nbabfs_set_transformations", "nbabfs_set_symmetry_parameters", "nbabfs_image_terms",
             "nbabfs_images", "nbabfs_export_image_pairs", "nbabfs_set_qc_atoms", "nbabfs_set_qcmm_spline",
             "nbabfs_qcmm_terms", "nbabfs_qcmm_potentials", "nbabfs_qcmm_gradients_host", "nbabfs_export_qcmm_pairs",
-            "nbabfs_qcmm_sites"]
+            "nbabfs_qcmm_sites", "nbabfs_set_gradient_mode"]
 
 
 class CLibraryError(Exception):
@@ -113,6 +114,7 @@ def load():
     L.nbabfs_export_qcmm_pairs.restype = C.c_longlong
     L.nbabfs_qcmm_sites.argtypes = [vp, vp, vp, vp, C.c_longlong]
     L.nbabfs_qcmm_sites.restype = C.c_longlong
+    L.nbabfs_set_gradient_mode.argtypes = [vp, C.c_int]
     _lib = L
     return L
 
@@ -209,7 +211,8 @@ class NativeState:
         self.options = options
 
     def energy_host(self, xyz, grad=None):
-        """One Initialize + Update + MMMMEnergy on host arrays; grad is incremented."""
+        """One Initialize + Update + MMMMEnergy on host arrays; grad is incremented (overwritten in
+        assign mode)."""
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
         if xyz.shape != (self.n, 3):
             raise ValueError("coordinates must be (%d, 3)" % self.n)
@@ -363,6 +366,11 @@ class NativeState:
 
     def set_stream(self, stream_handle):
         check(load().nbabfs_set_stream(self.h, None if not stream_handle else C.c_void_p(stream_handle)))
+
+    def set_gradient_mode(self, mode):
+        """GRADIENT_ADD (default: energy calls add into the gradient array) or GRADIENT_ASSIGN
+        (they overwrite it, for a caller whose first gradient term this is)."""
+        check(load().nbabfs_set_gradient_mode(self.h, int(mode)))
 
     def virial(self):
         """3x3 virial W_ab = sum_i x_i,a g_i,b of the last energy call's gradient."""

@@ -174,6 +174,61 @@ def test_batch_matches_single():
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_gradient_assign_mode(precision):
+    """nbabfs_set_gradient_mode(ASSIGN): every gradient row is overwritten with the model's
+    gradient, bit for bit what ADD gives on a zeroed array, for the device and host entry points
+    and for a batch; ADD again adds."""
+    from pdynamo_amd import _native, synthetic
+    from pdynamo_amd.NBModelABFS import NBModelABFS
+    systems = [synthetic.random_small_system(n, b, sd) for n, b, sd in [(900, 40.0, 5), (2500, 55.0, 6)]]
+    opts = NBModelABFS(precision=precision).NativeOptions()
+
+    def state(mm):
+        return _native.NativeState(opts, mm.charges, mm.ljtypes, mm.ntypes, mm.tableA, mm.tableB,
+                                   exclusions=mm.exclusions)
+    ref = []
+    for mm in systems:
+        st = state(mm)
+        g = np.zeros((mm.natoms, 3))
+        e, _, _ = st.energy_host(mm.coordinates3, g)
+        ref.append((e, g))
+        st.close()
+    for mm, (er, gr) in zip(systems, ref):
+        st = state(mm)
+        st.set_gradient_mode(_native.GRADIENT_ASSIGN)
+        x = _native.DeviceArray.from_numpy(mm.coordinates3)
+        g = _native.DeviceArray.from_numpy(np.full((mm.natoms, 3), 123.25))
+        for _ in range(2):
+            st.update(x.ptr)
+            st.energy_async(x.ptr, g.ptr)
+            e, _ = st.energies()
+            assert np.array_equal(e, er)
+            assert np.array_equal(g.numpy(), gr)
+        h = np.full((mm.natoms, 3), -7.5)
+        e, _, _ = st.energy_host(mm.coordinates3, h)
+        assert np.array_equal(h, gr)
+        st.set_gradient_mode(_native.GRADIENT_ADD)
+        st.update(x.ptr)
+        st.energy_async(x.ptr, g.ptr)
+        assert np.array_equal(g.numpy(), gr + gr)
+        with pytest.raises(_native.CLibraryError):
+            st.set_gradient_mode(2)
+        st.close()
+    states = [state(mm) for mm in systems]
+    for st in states:
+        st.set_gradient_mode(_native.GRADIENT_ASSIGN)
+    xs = [_native.DeviceArray.from_numpy(mm.coordinates3) for mm in systems]
+    gs = [_native.DeviceArray.from_numpy(np.full((mm.natoms, 3), 9.0)) for mm in systems]
+    batch = _native.NativeBatch(states, use_graph=True)
+    for _ in range(3):
+        e, _ = batch.energy([x.ptr for x in xs], [g.ptr for g in gs])
+        for k, (er, gr) in enumerate(ref):
+            assert np.array_equal(e[k], er)
+            assert np.array_equal(gs[k].numpy(), gr)
+    batch.close()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_call_orders_match(precision):
     """Update and energy calls in other orders give the same bits as update-then-energy.
 
