@@ -77,9 +77,10 @@ __device__ __forceinline__ bool pair_fp64(const AbfsFactors &F, double r2, doubl
 // index instead of two evaluated branches.  The Coulomb switch keeps the
 // reference polynomial with the region selected per lane.  The damped region
 // (r < dampingCutoff) is a wave-uniform rare path on the raw A, B.
-__device__ __forceinline__ void pair_rows(const AbfsFactors &F, double r2, double qq, double4 C, bool sw, bool act,
-                                          const GAS double *__restrict__ ljA, const GAS double *__restrict__ ljB,
-                                          int tij, double &fs, double &eel, double &elj)
+__device__ __forceinline__ void pair_rows(const AbfsFactors &F, double r2, double qq, double4 C, bool sw,
+                                          unsigned long long mact, const GAS double *__restrict__ ljA,
+                                          const GAS double *__restrict__ ljB, int tij, double &fs, double &eel,
+                                          double &elj)
 {
     const double y = (double)__builtin_amdgcn_rsqf((float)r2);
     const double h = fma(-(r2 * y), y, 1.0);
@@ -91,24 +92,25 @@ __device__ __forceinline__ void pair_rows(const AbfsFactors &F, double r2, doubl
     const double Q = sw ? fma(r2, fma(r2, fma(5.0 * F.d, r2, 3.0 * F.c), F.b), F.a) : 1.0;
     const double K = sw ? F.qShift2 : F.qShift1;
     double e1 = qq * fma(s, P, K);
-    double d1 = (-0.5 * qq) * s3 * Q;
-    // 0.5 (1/r) dE/dr of the polynomial = -1.5 s^5 (s^3 (4 C12 s^6 + 2 C6) + C3).
+    // d1, d2: twice dE/d(r^2) (the factor 2 of the gradient folded into the
+    // constants; scaling by 2 is exact, so fs = d1 + d2 equals 2 (dF1 + dF2)).
+    double d1 = (-qq) * s3 * Q;
+    // (1/r) dE/dr of the polynomial = -3 s^5 (s^3 (4 C12 s^6 + 2 C6) + C3).
     double e2 = fma(fma(C.x, s6, C.y), s6, fma(C.z, s3, C.w));
     const double inner = fma(fma(4.0 * C.x, s6, 2.0 * C.y), s3, C.z);
-    double d2 = (-1.5 * s2) * s3 * inner;
-    const bool dmp = r2 < F.r2Damp;
-    if (__builtin_amdgcn_ballot_w64(act && dmp) != 0ull) {
-        if (dmp) {
+    double d2 = (-3.0 * s2) * s3 * inner;
+    if (__builtin_amdgcn_ballot_w64(r2 < F.r2Damp) & mact) {
+        if (r2 < F.r2Damp) {
             const double A = ljA[tij], B = ljB[tij];
             e1 = qq * fma(-F.qAlpha, r2, F.qF0);
-            d1 = -qq * F.qAlpha;
+            d1 = (-qq * F.qAlpha) * 2.0;
             e2 = A * fma(-F.aAlpha, r2, F.aF0) - B * fma(-F.bAlpha, r2, F.bF0);
-            d2 = -A * F.aAlpha + B * F.bAlpha;
+            d2 = (-A * F.aAlpha + B * F.bAlpha) * 2.0;
         }
     }
-    fs = act ? 2.0 * (d1 + d2) : 0.0;
-    eel = act ? e1 : 0.0;
-    elj = act ? e2 : 0.0;
+    fs = d1 + d2;
+    eel = e1;
+    elj = e2;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -372,25 +374,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
 #pragma unroll
             for (int c = 0; c < NB_SCC; c++) {
                 if (cb & (1 << c)) {
-                    const bool inl = (lb & (1 << c)) != 0;
+                    const bool inl = __builtin_amdgcn_ubfe(lb, c, 1) != 0;
                     const double4 xi = sI[c * NB_CL + a];
                     const int roff = sT[c * NB_CL + a] + tj;   // byte offset of the pair's plain row
                     const double dx = xi.x - xjt.x, dy = xi.y - xjt.y, dz = xi.z - xjt.z;
                     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-                    const bool act = inl && (r2 <= F.r2Off);
-                    if (__builtin_amdgcn_ballot_w64(act) != 0ull) {
+                    const bool inr = r2 <= F.r2Off;
+                    const bool act = inl && inr;
+                    // Ballots of the two compares (lane masks as they stand), combined in scalar registers.
+                    const unsigned long long mact = __builtin_amdgcn_ballot_w64(inl) & __builtin_amdgcn_ballot_w64(inr);
+                    if (mact != 0ull) {
                         const bool sw = r2 > F.r2On;
                         const int off = roff + (sw ? 32 : 0);
                         const double4 C = ROWS_LDS ? *(const double4 *)((const unsigned char *)sR + off)
                                                    : load_slot(rows, off >> 5);
                         double fs, x1, x2;
-                        pair_rows(F, r2, xi.w * xjt.w, C, sw, act, ljA, ljB, roff >> 6, fs, x1, x2);
-                        const float f = (float)fs;
-                        const float px = f * (float)dx, py = f * (float)dy, pz = f * (float)dz;
-                        fx[c] += px; fy[c] += py; fz[c] += pz;
-                        gx -= px; gy -= py; gz -= pz;
-                        eel += x1;
-                        elj += x2;
+                        pair_rows(F, r2, xi.w * xjt.w, C, sw, mact, ljA, ljB, roff >> 6, fs, x1, x2);
+                        // The sums take the active lanes only (the others may hold any value).
+                        if (act) {
+                            const float f = (float)fs;
+                            const float px = f * (float)dx, py = f * (float)dy, pz = f * (float)dz;
+                            fx[c] += px; fy[c] += py; fz[c] += pz;
+                            gx -= px; gy -= py; gz -= pz;
+                            eel += x1;
+                            elj += x2;
+                        }
                     }
                 }
             }
