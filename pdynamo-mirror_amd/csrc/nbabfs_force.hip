@@ -21,6 +21,8 @@
 #include "nbabfs_internal.h"
 #include "nbabfs_kernels.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 
 static inline int nblk(size_t m, int b) { return (int)((m + b - 1) / b); }
@@ -892,35 +894,43 @@ void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, in
 }
 
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
-                        int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images)
+                        int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images,
+                        hipEvent_t ev_start, hipEvent_t ev_stop)
 {
     if (nblocks <= 0) return;
     const bool rows_lds = tiled_lds_bytes(max_ntypes, true, max_count) <= NB_LDS_ROW_LIMIT;
     const int head = (int)tiled_lds_head(max_ntypes, rows_lds);
     const size_t bytes = tiled_lds_bytes(max_ntypes, rows_lds, max_count);
+    const dim3 grid(nblocks), block(64);
     if (images) {
         if (rows_lds)
-            hipLaunchKernelGGL((k_force_tiled<true, true>), dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
+            hipExtLaunchKernelGGL((k_force_tiled<true, true>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items, q,
+                                  F, head, g);
         else
-            hipLaunchKernelGGL((k_force_tiled<false, true>), dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
+            hipExtLaunchKernelGGL((k_force_tiled<false, true>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items, q,
+                                  F, head, g);
     } else {
         if (rows_lds)
-            hipLaunchKernelGGL((k_force_tiled<true, false>), dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
+            hipExtLaunchKernelGGL((k_force_tiled<true, false>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items, q,
+                                  F, head, g);
         else
-            hipLaunchKernelGGL((k_force_tiled<false, false>), dim3(nblocks), dim3(64), bytes, s, d_desc, items, q, F, head, g);
+            hipExtLaunchKernelGGL((k_force_tiled<false, false>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items,
+                                  q, F, head, g);
     }
 }
 
 void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
-                        const int *cols, ForceGuard g, hipStream_t s)
+                        const int *cols, ForceGuard g, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop)
 {
-    hipLaunchKernelGGL(k_force_plain, dim3(max(nblk(n, 256), 1)), dim3(256), 0, s, d_desc, sys, F, rowptr, cols, g);
+    hipExtLaunchKernelGGL(k_force_plain, dim3(max(nblk(n, 256), 1)), dim3(256), 0, s, ev_start, ev_stop, 0, d_desc, sys, F,
+                          rowptr, cols, g);
 }
 
 void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int nitems, const AbfsFactors &F,
-                             ForceGuard g, hipStream_t s)
+                             ForceGuard g, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop)
 {
-    if (nitems > 0) hipLaunchKernelGGL(k_force_image_fp64, dim3(nitems), dim3(64), 0, s, d_desc, items, F, g);
+    if (nitems > 0)
+        hipExtLaunchKernelGGL(k_force_image_fp64, dim3(nitems), dim3(64), 0, s, ev_start, ev_stop, 0, d_desc, items, F, g);
 }
 
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s)

@@ -444,6 +444,7 @@ struct nbabfs_state {
     bool timing = false;
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
+    std::vector<int> ev_group;     // events per timed pass (2 or 4)
     double force_ms = 0.0;
     long long force_launches = 0;
 
@@ -463,23 +464,40 @@ struct nbabfs_state {
 };
 
 // Sum of timed force launches recorded so far (synchronises on the last event).
+// Timed pair passes: each holds a group of 2 events (start, stop of one
+// dispatch or of a same-stream sequence) or 4 (two dispatches on two streams,
+// start/stop each); its time is the span from the first start to the last
+// stop.  The events are the runtime's dispatch timestamps (hipExtLaunchKernel),
+// so the time is that of the kernels themselves, not of the gaps before them.
 static int collect_timing(nbabfs_state *S)
 {
     if (S->ev_used == 0) return NBABFS_STATUS_CONTINUE;
-    HIPCHK(hipEventSynchronize(S->ev[S->ev_used - 1]));
-    for (size_t k = 0; k + 1 < S->ev_used; k += 2) {
-        float ms = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms, S->ev[k], S->ev[k + 1]));
-        S->force_ms += ms;
+    HIPCHK(hipDeviceSynchronize());
+    size_t k = 0;
+    for (int g : S->ev_group) {
+        float t1 = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t1, S->ev[k], S->ev[k + 1]));
+        float lo = 0.0f, hi = t1;
+        if (g == 4) {
+            float t2 = 0.0f, t3 = 0.0f;
+            HIPCHK(hipEventElapsedTime(&t2, S->ev[k], S->ev[k + 2]));
+            HIPCHK(hipEventElapsedTime(&t3, S->ev[k], S->ev[k + 3]));
+            lo = std::min(lo, t2);
+            hi = std::max(hi, t3);
+        }
+        S->force_ms += hi - lo;
         S->force_launches++;
+        k += g;
     }
     S->ev_used = 0;
+    S->ev_group.clear();
     return NBABFS_STATUS_CONTINUE;
 }
 
-static int record_event(nbabfs_state *S)
+// Reserve the events of one timed pass (2 or 4).
+static int timing_events(nbabfs_state *S, int count, hipEvent_t *out)
 {
-    if (S->ev_used == S->ev.size()) {
+    if (S->ev_used + count > S->ev.size()) {
         if (S->ev.size() >= 4096) {
             CHK(collect_timing(S));
         } else {
@@ -490,7 +508,8 @@ static int record_event(nbabfs_state *S)
             }
         }
     }
-    HIPCHK(hipEventRecord(S->ev[S->ev_used++], S->stream));
+    for (int k = 0; k < count; k++) out[k] = S->ev[S->ev_used++];
+    S->ev_group.push_back(count);
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -1073,7 +1092,10 @@ static int update(nbabfs_state *S, const double *d_xyz, int *rebuilt)
         doUpdate = *(volatile int *)S->upflag.h == S->epoch;
         if (doUpdate && S->forced_xyz) {
             // The guarded launch returns at once; keep it out of the timing.
-            if (S->timing && S->ev_used >= 2) S->ev_used -= 2;
+            if (S->timing && !S->ev_group.empty()) {
+                S->ev_used -= S->ev_group.back();
+                S->ev_group.pop_back();
+            }
             S->forced_xyz = nullptr;
         }
         if (!doUpdate && cellCheck) {
@@ -1183,7 +1205,7 @@ static int launch_force(nbabfs_state *S, ForceGuard g)
     hipStream_t st = S->stream;
     // The kernels write through the descriptor's pointers: it must describe the current buffers.
     if (S->desc_version != g_alloc_version) return fail(NBABFS_STATUS_LOGIC_ERROR, "pair kernel launched with an out-of-date descriptor");
-    if (S->timing) CHK(record_event(S));
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if (S->opt.precision != NBABFS_PLAIN_FP64) {
         const int nm = S->nitems_main, ni = S->nitems - S->nitems_main;
         // The image launch runs on the second stream, overlapping the main one.
@@ -1198,26 +1220,28 @@ static int launch_force(nbabfs_state *S, ForceGuard g)
             HIPCHK(hipStreamWaitEvent(S->stream2, S->ev_fork, 0));
             sti = S->stream2;
         }
+        if (S->timing && nm + ni > 0) CHK(timing_events(S, (ni > 0 && nm > 0) ? 4 : 2, ev));
         if (ni > 0) {
             QueueParams q{nullptr, S->qstarts.p};
+            const int o = nm > 0 ? 2 : 0;
             launch_force_tiled(S->d_desc.p, S->items.p + nm, q, S->F, ni, S->ntypes, image_per_item(S->opt), g, sti,
-                               true);
+                               true, ev[o], ev[o + 1]);
         }
         if (nm > 0) {
             QueueParams q{S->opt.wavesPerCU > 0 ? S->heads.p : nullptr, S->qstarts.p};
             launch_force_tiled(S->d_desc.p, S->items.p, q, S->F, n_persistent_blocks(S->opt, nm), S->ntypes,
-                               S->opt.entriesPerItem, g, st);
+                               S->opt.entriesPerItem, g, st, false, ev[0], ev[1]);
         }
         if (sti != st) {
             HIPCHK(hipEventRecord(S->ev_join, sti));
             HIPCHK(hipStreamWaitEvent(st, S->ev_join, 0));
         }
     } else {
-        launch_force_plain(S->d_desc.p, 0, S->n, S->F, S->rowptr.p, S->cols.p, g, st);
         const int nm = S->nitems_main, ni = S->nitems - S->nitems_main;
-        if (ni > 0) launch_force_image_fp64(S->d_desc.p, S->items.p + nm, ni, S->F, g, st);
+        if (S->timing) CHK(timing_events(S, 2, ev));
+        launch_force_plain(S->d_desc.p, 0, S->n, S->F, S->rowptr.p, S->cols.p, g, st, ev[0], ni > 0 ? nullptr : ev[1]);
+        if (ni > 0) launch_force_image_fp64(S->d_desc.p, S->items.p + nm, ni, S->F, g, st, nullptr, ev[1]);
     }
-    if (S->timing) CHK(record_event(S));
     HIPCHK(hipGetLastError());
     return NBABFS_STATUS_CONTINUE;
 }

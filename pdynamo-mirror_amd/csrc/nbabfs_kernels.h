@@ -125,14 +125,18 @@ struct ForceGuard {
 };
 void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, int *heads, const PrepareCheck &chk,
                     hipStream_t s);
-// images: the launch over image-list items (one workgroup per item).
+// images: the launch over image-list items (one workgroup per item).  The pair
+// launchers take optional timing events (start, stop: recorded by the runtime
+// at the dispatch's own begin and end; null = not timed).
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
-                        int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images = false);
+                        int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images = false,
+                        hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
-                        const int *cols, ForceGuard g, hipStream_t s);
+                        const int *cols, ForceGuard g, hipStream_t s, hipEvent_t ev_start = nullptr,
+                        hipEvent_t ev_stop = nullptr);
 // Plain FP64 image pass over the image lists' work items.
 void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int nitems, const AbfsFactors &F,
-                             ForceGuard g, hipStream_t s);
+                             ForceGuard g, hipStream_t s, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s);
 void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s);
 // Virial of the single system's last call (NB_VACC_WORDS in D.vacc, zeroed by the caller).

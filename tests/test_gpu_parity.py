@@ -173,6 +173,41 @@ def test_batch_matches_single():
     batch.close()
 
 
+@pytest.mark.parametrize("precision,periodic", [("mixed", False), ("fp64", False), ("mixed", True), ("fp64", True)])
+def test_kernel_timing_passes(precision, periodic):
+    """nbabfs_set_timing: one timed pass per evaluated pair pass (a guarded pass that the
+    update discards is not counted), with a positive mean, for both paths, with and without
+    an image launch beside the main one."""
+    from pdynamo_amd import _native, synthetic
+    from pdynamo_amd.NBModelABFS import NBModelABFS
+    if periodic:
+        mm, (rot, tra), cell = synthetic.periodic_water_box(300, 21.0, 3)
+    else:
+        mm = synthetic.random_small_system(1500, 45.0, 8)
+    st = _native.NativeState(NBModelABFS(precision=precision).NativeOptions(), mm.charges, mm.ljtypes, mm.ntypes,
+                             mm.tableA, mm.tableB, exclusions=mm.exclusions)
+    if periodic:
+        st.set_transformations(rot, tra)
+        st.set_symmetry_parameters(cell)
+    x = _native.DeviceArray.from_numpy(mm.coordinates3)
+    g = _native.DeviceArray(mm.coordinates3.shape)
+    st.update(x.ptr)
+    st.energy_async(x.ptr, g.ptr)
+    st.set_timing(True)
+    for _ in range(5):
+        st.update(x.ptr)
+        st.energy_async(x.ptr, g.ptr)
+    moved = _native.DeviceArray.from_numpy(mm.coordinates3 + 1.0)    # every atom beyond the half skin
+    assert st.update(moved.ptr)
+    st.energy_async(moved.ptr, g.ptr)
+    _native.synchronize()
+    s = st.stats()
+    st.set_timing(False)
+    assert s["forceKernelLaunches"] == 6
+    assert s["forceKernelMs"] > 0.0
+    st.close()
+
+
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_gradient_assign_mode(precision):
     """nbabfs_set_gradient_mode(ASSIGN): every gradient row is overwritten with the model's
