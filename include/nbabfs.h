@@ -244,6 +244,14 @@ int nbabfs_qcmm_potentials(nbabfs_state *state, double *potentials);
  * host, kJ/mol/A) is incremented. */
 int nbabfs_qcmm_gradients_host(nbabfs_state *state, const double *qcCharges, double *grad_inout);
 
+/* Device-array variants of the two calls above (the same NBModelABFS.h:44-45
+ * entry points, for a QC program whose charges and potentials live in GPU
+ * memory): enqueued on the state's stream, no host copy and no
+ * synchronisation; d_potentials[nqc] and d_grad (n x 3) are incremented with
+ * bit for bit the values the host calls add. */
+int nbabfs_qcmm_potentials_async(nbabfs_state *state, double *d_potentials);
+int nbabfs_qcmm_gradients_async(nbabfs_state *state, const double *d_qcCharges, double *d_grad);
+
 /* The QC/MM lists of the last build as sorted keys row * 2^32 + column.
  * which: 0 electrostatic (row = QC atom position, column = MM site: the atom
  * for MM coupling, else the site of nbabfs_qcmm_sites), 1 LJ (atom, atom),

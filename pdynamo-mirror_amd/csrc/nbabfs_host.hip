@@ -2917,10 +2917,10 @@ extern "C" int nbabfs_qcmm_terms(nbabfs_state *S, double *energies, int *present
     return NBABFS_STATUS_CONTINUE;
 }
 
-extern "C" int nbabfs_qcmm_potentials(nbabfs_state *S, double *potentials)
+// QC/MM potentials (NBModelABFS_QCMMPotentials) into the per-(QC atom, chunk)
+// partials on the state's stream.
+static int qcmm_potentials_enqueue(nbabfs_state *S)
 {
-    CHK(qcmm_ready(S, true));
-    if (!potentials) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "potentials is NULL");
     hipStream_t st = S->stream;
     const QcmmGeom G = qcmm_geom(S, S->upd_xyz);
     const SplineDev sp{S->spx.p, S->spy.p, S->sph.p, S->nspline};
@@ -2932,13 +2932,22 @@ extern "C" int nbabfs_qcmm_potentials(nbabfs_state *S, double *potentials)
         escale = 1.0 / S->opt.dielectric;
         escale14 = escale * S->opt.electrostaticScale14;
     }
+    CHK(S->qpot.ensure(2 * (size_t)S->nqc * NB_QCMM_CHUNKS));
+    launch_qcmm_potentials(G, sp, L, S->site_q.p, escale, escale14, S->opt.outerCutoff * S->opt.outerCutoff,
+                           NB_QCMM_CHUNKS, S->qpot.p, st);
+    HIPCHK(hipGetLastError());
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_qcmm_potentials(nbabfs_state *S, double *potentials)
+{
+    CHK(qcmm_ready(S, true));
+    if (!potentials) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "potentials is NULL");
+    hipStream_t st = S->stream;
     const int ch = NB_QCMM_CHUNKS;
     const size_t np = 2 * (size_t)S->nqc * ch;
-    CHK(S->qpot.ensure(np));
+    CHK(qcmm_potentials_enqueue(S));
     CHK(S->h_qpot.ensure(np));
-    launch_qcmm_potentials(G, sp, L, S->site_q.p, escale, escale14, S->opt.outerCutoff * S->opt.outerCutoff, ch,
-                           S->qpot.p, st);
-    HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(S->h_qpot.p, S->qpot.p, np * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     // Chunk sums in chunk order; the main list's total, then the 1-4 list's, as the reference adds them.
@@ -2954,10 +2963,21 @@ extern "C" int nbabfs_qcmm_potentials(nbabfs_state *S, double *potentials)
     return NBABFS_STATUS_CONTINUE;
 }
 
-extern "C" int nbabfs_qcmm_gradients_host(nbabfs_state *S, const double *qcCharges, double *grad_inout)
+extern "C" int nbabfs_qcmm_potentials_async(nbabfs_state *S, double *d_potentials)
 {
     CHK(qcmm_ready(S, true));
-    if (!qcCharges || !grad_inout) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
+    if (!d_potentials) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "potentials is NULL");
+    CHK(qcmm_potentials_enqueue(S));
+    launch_qcmm_pot_sum(S->qpot.p, S->nqc, NB_QCMM_CHUNKS, d_potentials, S->stream);
+    HIPCHK(hipGetLastError());
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// QC/MM electrostatic gradients (NBModelABFS_QCMMGradients) for the QC charges
+// at d_q (device) into the state's fixed-point accumulators, then out: written
+// to out (add = false) or added into it.
+static int qcmm_gradients_enqueue(nbabfs_state *S, const double *d_q, double *out, bool add)
+{
     hipStream_t st = S->stream;
     const int n = S->n;
     const QcmmGeom G = qcmm_geom(S, S->upd_xyz);
@@ -2971,19 +2991,36 @@ extern "C" int nbabfs_qcmm_gradients_host(nbabfs_state *S, const double *qcCharg
         scale = kHartreesToKJ * e;
         scale14 = kHartreesToKJ * (e * S->opt.electrostaticScale14);
     }
-    CHK(S->qcq.upload(qcCharges, S->nqc, st));
     CHK(S->qgacc.ensure(6 * (size_t)std::max(n, 1)));
+    launch_fill_int((int *)S->qgacc.p, 12 * (size_t)n, 0, st);
+    launch_qcmm_gradients(G, sp, L, S->site_q.p, d_q, scale, scale14, S->opt.outerCutoff * S->opt.outerCutoff,
+                          NB_QCMM_CHUNKS, S->qgacc.p, st);
+    launch_qcmm_grad_out(S->qgacc.p, n, out, add, st);
+    HIPCHK(hipGetLastError());
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_qcmm_gradients_host(nbabfs_state *S, const double *qcCharges, double *grad_inout)
+{
+    CHK(qcmm_ready(S, true));
+    if (!qcCharges || !grad_inout) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
+    hipStream_t st = S->stream;
+    const int n = S->n;
+    CHK(S->qcq.upload(qcCharges, S->nqc, st));
     CHK(S->qgrad.ensure(3 * (size_t)std::max(n, 1)));
     CHK(S->h_qgrad.ensure(3 * (size_t)std::max(n, 1)));
-    launch_fill_int((int *)S->qgacc.p, 12 * (size_t)n, 0, st);
-    launch_qcmm_gradients(G, sp, L, S->site_q.p, S->qcq.p, scale, scale14, S->opt.outerCutoff * S->opt.outerCutoff,
-                          NB_QCMM_CHUNKS, S->qgacc.p, st);
-    launch_qcmm_grad_out(S->qgacc.p, n, S->qgrad.p, st);
-    HIPCHK(hipGetLastError());
+    CHK(qcmm_gradients_enqueue(S, S->qcq.p, S->qgrad.p, false));
     HIPCHK(hipMemcpyAsync(S->h_qgrad.p, S->qgrad.p, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     for (size_t k = 0; k < 3 * (size_t)n; k++) grad_inout[k] += S->h_qgrad.p[k];
     return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_qcmm_gradients_async(nbabfs_state *S, const double *d_qcCharges, double *d_grad)
+{
+    CHK(qcmm_ready(S, true));
+    if (!d_qcCharges || !d_grad) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
+    return qcmm_gradients_enqueue(S, d_qcCharges, d_grad, true);
 }
 
 extern "C" long long nbabfs_export_qcmm_pairs(nbabfs_state *S, int which, long long *keys, long long cap)

@@ -188,7 +188,8 @@ void launch_qcmm_potentials(const QcmmGeom &G, const SplineDev &S, const QcmmRow
 void launch_qcmm_gradients(const QcmmGeom &G, const SplineDev &S, const QcmmRows &L, const double *site_q,
                            const double *qcq, double scale, double scale14, double r2off, int chunks, long long *gacc,
                            hipStream_t s);
-void launch_qcmm_grad_out(const long long *gacc, int n, double *grad, hipStream_t s);
+void launch_qcmm_grad_out(const long long *gacc, int n, double *grad, bool add, hipStream_t s);
+void launch_qcmm_pot_sum(const double *part, int nqc, int chunks, double *pot, hipStream_t s);
 void launch_qcmm_coords(const QcmmGeom &G, double *qcxyz, double *sitexyz, hipStream_t s);
 // QC/MM LJ (nbabfs_force.hip): the QC/MM LJ and 1-4 LJ lists of the descriptor's pure QC atoms,
 // NB_QCMM_CHUNKS workgroups per row.

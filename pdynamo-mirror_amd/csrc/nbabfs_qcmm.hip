@@ -338,14 +338,37 @@ __global__ __launch_bounds__(256) void k_qcmm_gradients(QcmmGeom G, SplineDev S0
     gacc_add(gacc, G.n, p, mt);
 }
 
+// add = 0: grad = the accumulated gradient; 1: grad += it.
 __global__ __launch_bounds__(256) void k_qcmm_grad_out(const long long *__restrict__ gacc, int n,
-                                                      double *__restrict__ grad)
+                                                      double *__restrict__ grad, int add)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    for (int k = 0; k < 3; k++)
-        grad[3 * (size_t)i + k] = (double)gacc[(size_t)(k + 3) * n + i] * NB_FX_HI_UNIT +
-                                  (double)gacc[(size_t)k * n + i] * (1.0 / NB_FX_LO_SCALE);
+    for (int k = 0; k < 3; k++) {
+        const double g = (double)gacc[(size_t)(k + 3) * n + i] * NB_FX_HI_UNIT +
+                         (double)gacc[(size_t)k * n + i] * (1.0 / NB_FX_LO_SCALE);
+        grad[3 * (size_t)i + k] = add ? grad[3 * (size_t)i + k] + g : g;
+    }
+}
+
+// The potentials from the per-(QC atom, chunk) partials, in the host call's
+// order: the chunk sums of the main list, then of the 1-4 list, each added in
+// turn to pot[q].
+__global__ __launch_bounds__(256) void k_qcmm_pot_sum(const double *__restrict__ part, int nqc, int chunks,
+                                                     double *__restrict__ pot)
+{
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= nqc) return;
+    double m = 0.0, m14 = 0.0;
+    for (int c = 0; c < chunks; c++) {
+        m += part[2 * ((size_t)q * chunks + c)];
+        m14 += part[2 * ((size_t)q * chunks + c) + 1];
+    }
+    double p = pot[q];
+    p += m;
+    p += m14;
+    pot[q] = p;
 }
 
 __global__ __launch_bounds__(256) void k_qcmm_coords(QcmmGeom G, double *qcxyz, double *sitexyz)
@@ -389,9 +412,14 @@ void launch_qcmm_gradients(const QcmmGeom &G, const SplineDev &S, const QcmmRows
                            scale14, r2off, gacc);
 }
 
-void launch_qcmm_grad_out(const long long *gacc, int n, double *grad, hipStream_t s)
+void launch_qcmm_grad_out(const long long *gacc, int n, double *grad, bool add, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_qcmm_grad_out, dim3((n + 255) / 256), dim3(256), 0, s, gacc, n, grad);
+    if (n > 0) hipLaunchKernelGGL(k_qcmm_grad_out, dim3((n + 255) / 256), dim3(256), 0, s, gacc, n, grad, add ? 1 : 0);
+}
+
+void launch_qcmm_pot_sum(const double *part, int nqc, int chunks, double *pot, hipStream_t s)
+{
+    if (nqc > 0) hipLaunchKernelGGL(k_qcmm_pot_sum, dim3((nqc + 255) / 256), dim3(256), 0, s, part, nqc, chunks, pot);
 }
 
 void launch_qcmm_coords(const QcmmGeom &G, double *qcxyz, double *sitexyz, hipStream_t s)

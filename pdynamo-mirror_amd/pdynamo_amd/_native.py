@@ -27,7 +27,8 @@ EXPORTED = ["nbabfs_default_options", "nbabfs_init", "nbabfs_last_error", "nbabf
             "nbabfs_virial", "nbabfs_set_transformations", "nbabfs_set_symmetry_parameters", "nbabfs_image_terms",
             "nbabfs_images", "nbabfs_export_image_pairs", "nbabfs_set_qc_atoms", "nbabfs_set_qcmm_spline",
             "nbabfs_qcmm_terms", "nbabfs_qcmm_potentials", "nbabfs_qcmm_gradients_host", "nbabfs_export_qcmm_pairs",
-            "nbabfs_qcmm_sites", "nbabfs_set_gradient_mode"]
+            "nbabfs_qcmm_sites", "nbabfs_set_gradient_mode", "nbabfs_qcmm_potentials_async",
+            "nbabfs_qcmm_gradients_async"]
 
 
 class CLibraryError(Exception):
@@ -115,6 +116,8 @@ def load():
     L.nbabfs_qcmm_sites.argtypes = [vp, vp, vp, vp, C.c_longlong]
     L.nbabfs_qcmm_sites.restype = C.c_longlong
     L.nbabfs_set_gradient_mode.argtypes = [vp, C.c_int]
+    L.nbabfs_qcmm_potentials_async.argtypes = [vp, vp]
+    L.nbabfs_qcmm_gradients_async.argtypes = [vp, vp, vp]
     _lib = L
     return L
 
@@ -340,6 +343,14 @@ class NativeState:
         if grad.dtype != np.float64 or not grad.flags.c_contiguous or grad.shape != (self.n, 3):
             raise ValueError("gradients must be a C-contiguous (%d, 3) float64 array" % self.n)
         check(load().nbabfs_qcmm_gradients_host(self.h, _ptr(q), _ptr(grad)))
+
+    def qcmm_potentials_async(self, d_potentials):
+        """Add the QC/MM potentials into a device array (nqc FP64), on the state's stream."""
+        check(load().nbabfs_qcmm_potentials_async(self.h, C.c_void_p(d_potentials)))
+
+    def qcmm_gradients_async(self, d_qc_charges, d_grad):
+        """Add the QC/MM electrostatic gradients of the device QC charges into a device gradient array."""
+        check(load().nbabfs_qcmm_gradients_async(self.h, C.c_void_p(d_qc_charges), C.c_void_p(d_grad)))
 
     def export_qcmm_pairs(self, which):
         """Sorted keys row * 2^32 + column of a QC/MM list: 'el', 'lj', 'el14' or 'lj14'."""

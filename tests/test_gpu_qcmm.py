@@ -120,3 +120,30 @@ def test_qcmm_system_point_charges():
     g = s.configuration.gradients3
     assert rel_rms(g, fx["grads_e"][0] + fx["grads_q"][0]) <= 1e-10
     assert np.allclose(s.configuration.qcmmstate.qcmmPotentials, fx["potentials"][0], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("path", QCMM_FIXTURES, ids=lambda p: os.path.basename(p)[:-4])
+def test_qcmm_device_variants(path):
+    """nbabfs_qcmm_potentials_async / nbabfs_qcmm_gradients_async add, into device arrays, bit for bit
+    what the host calls add into host arrays."""
+    from pdynamo_amd import _native
+    fx = dict(np.load(path, allow_pickle=False))
+    st = _state(fx, "mixed")
+    xyz = fx["coords"][0]
+    st.energy_host(xyz, np.zeros_like(xyz))
+    nqc = fx["qc_index"].size
+    p0 = np.linspace(-1.0, 1.0, nqc)
+    g0 = np.full_like(xyz, 0.25)
+    ph = p0.copy()
+    st.qcmm_potentials(ph)
+    gh = g0.copy()
+    st.qcmm_gradients(fx["qc_charges"], gh)
+    dp = _native.DeviceArray.from_numpy(p0)
+    dq = _native.DeviceArray.from_numpy(np.ascontiguousarray(fx["qc_charges"], dtype=np.float64))
+    dg = _native.DeviceArray.from_numpy(g0)
+    st.qcmm_potentials_async(dp.ptr)
+    st.qcmm_gradients_async(dq.ptr, dg.ptr)
+    _native.synchronize()
+    assert np.array_equal(dp.numpy(), ph)
+    assert np.array_equal(dg.numpy(), gh)
+    st.close()
