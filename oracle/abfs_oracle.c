@@ -253,6 +253,15 @@ long orc_self_pair_list(int n, const double *xyz, const int *xptr, const int *xi
         double lower[3];
         int *cell = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
         long ncells = orc_make_grid(n, xyz, cellSize, bins, lower, cell);
+        /* A grid axis with no cells (an exactly zero extent) has no occupancy:
+         * RegularGridOccupancy_Allocate rejects zero cells with
+         * Status_InvalidArgument (pCx/csource/RegularGridOccupancy.c:29,54). */
+        if (ncells <= 0) {
+            free(cell);
+            free(ind);
+            free(incl);
+            return ORC_INVALID_ARGUMENT;
+        }
         /* Occupancy by counting sort (pCx/csource/RegularGridOccupancy.c:95-126). */
         long *first = (long *)calloc((size_t)ncells + 1, sizeof(long));
         long *total = (long *)calloc((size_t)ncells + 1, sizeof(long));

@@ -119,6 +119,9 @@ def exclusion_csr(n, pairs):
     return np.cumsum(ptr).astype(np.int32), cols.astype(np.int32)
 
 
+ORC_INVALID_ARGUMENT = -(1 << 62)      # abfs_oracle.h
+
+
 def determine_method(xyz, cutoff=13.5, factor=0.5, min_extent=2, min_points=500):
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     return bool(lib().orc_determine_method(xyz.shape[0], xyz, cutoff, factor, min_extent, min_points))
@@ -179,6 +182,8 @@ def self_pair_list(xyz, xptr, xidx, cutoff=13.5, cell_size=6.75, method=1):
         if npairs >= 0:
             r = nrec.value
             return PairRecords(rec_i[:r].copy(), rec_ptr[:r + 1].copy(), js[:npairs].copy(), n)
+        if npairs == ORC_INVALID_ARGUMENT:
+            raise ValueError("Status_InvalidArgument: the cell grid has an axis with no cells (zero extent)")
         cap_p = max(cap_p, -npairs) + 16
 
 

@@ -842,6 +842,11 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     const double cell = o.cutoffCellSizeFactor * o.listCutoff;
     if (S->useGrid) {
         reference_grid(bb, cell, g);
+        // A zero extent gives an axis with no cells, which RegularGridOccupancy_Allocate
+        // rejects (pCore-1.9.0/extensions/csource/RegularGridOccupancy.c:29,54).
+        if (g.bins[0] <= 0 || g.bins[1] <= 0 || g.bins[2] <= 0)
+            return fail(NBABFS_STATUS_INVALID_ARGUMENT, "the cell grid has an axis with no cells (zero extent %s)",
+                        g.bins[0] <= 0 ? "in x" : (g.bins[1] <= 0 ? "in y" : "in z"));
         make_stencil(cell, o.listCutoff, S->u, flags);
     } else {
         g.h = 1.0;

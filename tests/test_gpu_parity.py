@@ -63,6 +63,46 @@ def test_random_mixtures(oracle_mod, natoms, box, seed, precision, etol, gtol):
     _check(terms, grad, rterms, rgrad, etol, gtol)
 
 
+def _shaped(shape):
+    """Atoms on a jittered 2.8 A lattice filling a slab, a rod, a 0.5 A thin plane, or an exact line."""
+    from pdynamo_amd import synthetic
+    n, ext = {"slab": (900, (60.0, 60.0, 4.0)), "rod": (600, (9.0, 9.0, 170.0)),
+              "thin_plane": (700, (80.0, 0.5, 80.0)), "line": (600, (1700.0, 0.0, 0.0))}[shape]
+    mm = synthetic.random_small_system(n, 40.0, 17)
+    rng = np.random.default_rng(3)
+    axes = [np.arange(0.0, max(e, 1e-9), 2.8) if e > 0 else np.zeros(1) for e in ext]
+    pts = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+    pts = pts[rng.permutation(len(pts))[:n]]
+    e = np.array(ext)
+    jit = np.where(e >= 1.0, rng.uniform(-0.3, 0.3, pts.shape), rng.uniform(0.0, 1.0, pts.shape) * e)
+    mm.coordinates3 = np.ascontiguousarray(pts + jit)
+    assert len(mm.coordinates3) == n
+    return mm
+
+
+@pytest.mark.parametrize("shape", ["slab", "rod", "thin_plane", "line"])
+@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("fp64", 1e-12, 1e-12)])
+def test_extreme_shapes(oracle_mod, shape, precision, etol, gtol):
+    """Flat and thin systems: super-cluster columns much wider than the system in some direction,
+    columns padded to whole super-clusters; same pair set and terms as the oracle.  An exact line of
+    more than 500 atoms (grid method, two zero extents) is an invalid argument, as in the reference,
+    whose grid then has no cells (RegularGridOccupancy.c:29,54)."""
+    from pdynamo_amd import CLibraryError
+    mm = _shaped(shape)
+    s = _system(mm, precision=precision)
+    o = oracle_mod.OracleABFS(mm)
+    if shape == "line":
+        with pytest.raises(CLibraryError):
+            _run(s)
+        with pytest.raises(ValueError):
+            o.energy(mm.coordinates3)
+        return
+    _, terms, grad = _run(s)
+    rterms, rgrad, _ = o.energy(mm.coordinates3)
+    assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
+    _check(terms, grad, rterms, rgrad, etol, gtol)
+
+
 def test_options_14_dielectric(oracle_mod):
     """1-4 list with its own LJ table, scale14 and a dielectric (NBModelABFS.c:228-301)."""
     from pdynamo_amd import (LJParameterContainer_FromTableCoefficients, MMAtomContainer, NBModelABFS, System,

@@ -426,3 +426,28 @@ def test_image_energy_equals_expanded_p1_cell(oracle_mod, group):
     terms1, g1, dm1, _ = orc1.energy_sym(big.coordinates3, sp)
     e_p1 = sum(v for _, v in terms1)
     assert e_p1 == pytest.approx(nt * e_sym, rel=1e-10)
+
+
+def test_zero_extent_grid_is_invalid_argument(oracle_mod):
+    """More than 500 atoms on an exact line take the grid method, whose grid then has no cells on the
+    two flat axes: the reference fails (RegularGridOccupancy_Allocate, Status_InvalidArgument,
+    pCore-1.9.0/extensions/csource/RegularGridOccupancy.c:29,54) and so does the oracle.  200 atoms on
+    the same line take the direct method and give the same pair set in both."""
+    from pdynamo_amd import synthetic
+    for n, grid in ((600, True), (200, False)):
+        mm = synthetic.random_small_system(n, 40.0, 17)
+        mm.coordinates3 = np.ascontiguousarray(np.stack([np.arange(n) * 2.8, np.zeros(n), np.zeros(n)], 1))
+        o = oracle_mod.OracleABFS(mm)
+        if grid:
+            with pytest.raises(ValueError):
+                o.energy(mm.coordinates3)
+        else:
+            o.energy(mm.coordinates3)
+        if oracle_mod.ref_available():
+            r = oracle_mod.ReferenceABFS(mm)
+            if grid:
+                with pytest.raises(RuntimeError):
+                    r.energy(mm.coordinates3)
+            else:
+                r.energy(mm.coordinates3)
+                assert np.array_equal(r.canonical(), o.records.canonical())
