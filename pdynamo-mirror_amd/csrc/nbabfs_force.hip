@@ -207,6 +207,48 @@ __device__ __forceinline__ double4 image_slot(double4 v, const volatile double *
 }
 
 // ------------------------------------------------------------------------------------------------
+// The 1-4 list (the second pass of NBModelABFS_MMMMEnergy, pMolecule-1.9.0/
+// extensions/csource/NBModelABFS.c:228-301): 64 pairs per block (one wave),
+// FP64 in the reference's expression order, forces into the fixed-point
+// accumulators, the block's two energies (wave sums) into epart14[blk].  Run
+// by k_pairs14, or by extra workgroups of the main pair launch.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void pairs14_block(const SysDesc &D, const AbfsFactors &F, int blk)
+{
+#pragma clang fp contract(off)
+    const int k = blk * 64 + (int)threadIdx.x;
+    double e1 = 0.0, e2 = 0.0;
+    if (k < D.n14) {
+        const int i = D.p14[2 * k], j = D.p14[2 * k + 1];
+        double xij = D.xyz[3 * (size_t)i] - D.xyz[3 * (size_t)j];
+        double yij = D.xyz[3 * (size_t)i + 1] - D.xyz[3 * (size_t)j + 1];
+        double zij = D.xyz[3 * (size_t)i + 2] - D.xyz[3 * (size_t)j + 2];
+        const double r2 = (xij * xij + yij * yij + zij * zij);
+        const double qij = (D.escale14 * D.charge[i]) * D.charge[j];
+        const int tij = D.ntypes * D.atype[i] + D.atype[j];
+        double x1, x2, dF;
+        if (pair_fp64(F, r2, qij, D.ljA14[tij], D.ljB14[tij], x1, x2, dF)) {
+            e1 = x1;
+            e2 = x2;
+            const int si = D.atom_slot[i], sj = D.atom_slot[j];
+            const double px = xij * (2.0 * dF), py = yij * (2.0 * dF), pz = zij * (2.0 * dF);
+            fx_atomic_add(D.facc, D.n_pad, si, 0, px);
+            fx_atomic_add(D.facc, D.n_pad, si, 1, py);
+            fx_atomic_add(D.facc, D.n_pad, si, 2, pz);
+            fx_atomic_add(D.facc, D.n_pad, sj, 0, -px);
+            fx_atomic_add(D.facc, D.n_pad, sj, 1, -py);
+            fx_atomic_add(D.facc, D.n_pad, sj, 2, -pz);
+        }
+    }
+    e1 = wave_sum_f64(e1);
+    e2 = wave_sum_f64(e2);
+    if (threadIdx.x == 0) {
+        D.epart14[2 * blk] = e1;
+        D.epart14[2 * blk + 1] = e2;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
 // Tiled force kernel: one work item per workgroup, or a persistent loop over
 // per-XCD queues with stealing.  The LJ row table of the item's system sits
 // in LDS when it fits; the next entry's j-cluster record and masks are
@@ -277,9 +319,14 @@ static inline size_t tiled_lds_bytes(int ntypes, bool rows_lds, int max_count)
 template <bool ROWS_LDS, bool IMG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4))) void k_force_tiled(const SysDesc *__restrict__ descs,
                                                     const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F,
-                                                    int lds_head, ForceGuard guard)
+                                                    int lds_head, ForceGuard guard, int nforce)
 {
     if (guard.flag && *(const volatile int *)guard.flag == guard.epoch) return;
+    // Workgroups past the pair blocks evaluate the 1-4 list of system 0 (single-state launches).
+    if (!IMG && (int)blockIdx.x >= nforce) {
+        pairs14_block(descs[0], F, (int)blockIdx.x - nforce);
+        return;
+    }
     // Lane l = 8b + a: a is the i-atom within an i-cluster, b the j-atom of
     // the entry's j-cluster.  The entry masks are stored per l' = 8a + b.
     const int lane = threadIdx.x;
@@ -692,40 +739,11 @@ __global__ __launch_bounds__(64) void k_force_image_fp64(const SysDesc *__restri
 // 1-4 list (NBModelABFS_MMMMEnergy's second pass, NBModelABFS.c:276-294): FP64
 // with scale14 and the 1-4 LJ table; gradients into the fixed-point arrays.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_pairs14(const SysDesc *__restrict__ descs, AbfsFactors F)
+__global__ __launch_bounds__(64) void k_pairs14(const SysDesc *__restrict__ descs, AbfsFactors F)
 {
-#pragma clang fp contract(off)
     const SysDesc &D = descs[blockIdx.y];
     if ((int)blockIdx.x >= D.n14_blocks) return;
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    double e1 = 0.0, e2 = 0.0;
-    if (k < D.n14) {
-        const int i = D.p14[2 * k], j = D.p14[2 * k + 1];
-        double xij = D.xyz[3 * (size_t)i] - D.xyz[3 * (size_t)j];
-        double yij = D.xyz[3 * (size_t)i + 1] - D.xyz[3 * (size_t)j + 1];
-        double zij = D.xyz[3 * (size_t)i + 2] - D.xyz[3 * (size_t)j + 2];
-        const double r2 = (xij * xij + yij * yij + zij * zij);
-        const double qij = (D.escale14 * D.charge[i]) * D.charge[j];
-        const int tij = D.ntypes * D.atype[i] + D.atype[j];
-        double x1, x2, dF;
-        if (pair_fp64(F, r2, qij, D.ljA14[tij], D.ljB14[tij], x1, x2, dF)) {
-            e1 = x1;
-            e2 = x2;
-            const int si = D.atom_slot[i], sj = D.atom_slot[j];
-            const double px = xij * (2.0 * dF), py = yij * (2.0 * dF), pz = zij * (2.0 * dF);
-            fx_atomic_add(D.facc, D.n_pad, si, 0, px);
-            fx_atomic_add(D.facc, D.n_pad, si, 1, py);
-            fx_atomic_add(D.facc, D.n_pad, si, 2, pz);
-            fx_atomic_add(D.facc, D.n_pad, sj, 0, -px);
-            fx_atomic_add(D.facc, D.n_pad, sj, 1, -py);
-            fx_atomic_add(D.facc, D.n_pad, sj, 2, -pz);
-        }
-    }
-    block_sum2<256>(e1, e2);
-    if (threadIdx.x == 0) {
-        D.epart14[2 * blockIdx.x] = e1;
-        D.epart14[2 * blockIdx.x + 1] = e2;
-    }
+    pairs14_block(D, F, blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -895,27 +913,28 @@ void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, in
 
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
                         int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images,
-                        hipEvent_t ev_start, hipEvent_t ev_stop)
+                        hipEvent_t ev_start, hipEvent_t ev_stop, int n14_blocks)
 {
-    if (nblocks <= 0) return;
+    if (images) n14_blocks = 0;
+    if (nblocks + n14_blocks <= 0) return;
     const bool rows_lds = tiled_lds_bytes(max_ntypes, true, max_count) <= NB_LDS_ROW_LIMIT;
     const int head = (int)tiled_lds_head(max_ntypes, rows_lds);
     const size_t bytes = tiled_lds_bytes(max_ntypes, rows_lds, max_count);
-    const dim3 grid(nblocks), block(64);
+    const dim3 grid(nblocks + n14_blocks), block(64);
     if (images) {
         if (rows_lds)
             hipExtLaunchKernelGGL((k_force_tiled<true, true>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items, q,
-                                  F, head, g);
+                                  F, head, g, nblocks);
         else
             hipExtLaunchKernelGGL((k_force_tiled<false, true>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items, q,
-                                  F, head, g);
+                                  F, head, g, nblocks);
     } else {
         if (rows_lds)
             hipExtLaunchKernelGGL((k_force_tiled<true, false>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items, q,
-                                  F, head, g);
+                                  F, head, g, nblocks);
         else
             hipExtLaunchKernelGGL((k_force_tiled<false, false>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items,
-                                  q, F, head, g);
+                                  q, F, head, g, nblocks);
     }
 }
 
@@ -936,7 +955,7 @@ void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int n
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s)
 {
     if (nsys > 0 && max_n14_blocks > 0)
-        hipLaunchKernelGGL(k_pairs14, dim3(max_n14_blocks, nsys), dim3(256), 0, s, d_desc, F);
+        hipLaunchKernelGGL(k_pairs14, dim3(max_n14_blocks, nsys), dim3(64), 0, s, d_desc, F);
 }
 
 void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s)

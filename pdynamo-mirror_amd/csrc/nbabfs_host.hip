@@ -1030,7 +1030,7 @@ static int ensure_call_buffers(nbabfs_state *S)
     if (S->opt.precision == NBABFS_PLAIN_FP64) CHK(S->eimgpart.ensure(2 * (size_t)std::max(S->nitems - S->nitems_main, 1)));
     CHK(S->eacc.ensure(NB_EACC_WORDS));
     CHK(S->vacc.ensure(NB_VACC_WORDS));
-    CHK(S->epart14.ensure(2 * (size_t)std::max(nblk(S->n14, 256), 1)));
+    CHK(S->epart14.ensure(2 * (size_t)std::max(nblk(S->n14, NB_P14_BLOCK), 1)));
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -1172,7 +1172,7 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.ntypes = S->ntypes;
     D.n14 = S->n14;
     D.n_plain_blocks = nblk(S->n, 256);
-    D.n14_blocks = nblk(S->n14, 256);
+    D.n14_blocks = nblk(S->n14, NB_P14_BLOCK);
     D.nimg = (int)S->slots.size();
     D.imgxf = D.nimg ? S->imgxf.p : nullptr;
     D.gacc = D.nimg ? S->gacc.p : nullptr;
@@ -1235,7 +1235,11 @@ static int launch_force(nbabfs_state *S, ForceGuard g)
         if (nm > 0) {
             QueueParams q{S->opt.wavesPerCU > 0 ? S->heads.p : nullptr, S->qstarts.p};
             launch_force_tiled(S->d_desc.p, S->items.p, q, S->F, n_persistent_blocks(S->opt, nm), S->ntypes,
-                               S->opt.entriesPerItem, g, st, false, ev[0], ev[1]);
+                               S->opt.entriesPerItem, g, st, false, ev[0], ev[1], nblk(S->n14, NB_P14_BLOCK));
+        } else if (S->n14 > 0) {
+            QueueParams q{nullptr, S->qstarts.p};
+            launch_force_tiled(S->d_desc.p, S->items.p, q, S->F, 0, S->ntypes, S->opt.entriesPerItem, g, st, false,
+                               ni > 0 ? nullptr : ev[0], ni > 0 ? nullptr : ev[1], nblk(S->n14, NB_P14_BLOCK));
         }
         if (sti != st) {
             HIPCHK(hipEventRecord(S->ev_join, sti));
@@ -1271,7 +1275,8 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
         CHK(launch_force(S, ForceGuard{nullptr, 0}));
     }
     S->forced_xyz = nullptr;
-    launch_pairs14(S->d_desc.p, 1, nblk(S->n14, 256), S->F, st);
+    // The tiled pair launch evaluates the 1-4 list in its extra workgroups.
+    if (plain) launch_pairs14(S->d_desc.p, 1, nblk(S->n14, NB_P14_BLOCK), S->F, st);
     if (S->has_qc) launch_qclj(S->d_desc.p, S->nlj, S->F, st);
     launch_finalize(S->d_desc.p, 1, S->n, plain ? 1 : 0, st);
     HIPCHK(hipGetLastError());
@@ -2412,7 +2417,7 @@ static int batch_layout(nbabfs_batch *B, const double *const *d_xyz, double *con
         total += s->nitems;
         B->max_npad = std::max(B->max_npad, s->n_pad);
         B->max_n = std::max(B->max_n, s->n);
-        B->max_n14b = std::max(B->max_n14b, nblk(s->n14, 256));
+        B->max_n14b = std::max(B->max_n14b, nblk(s->n14, NB_P14_BLOCK));
     }
     B->nitems = total;
     CHK(B->items.ensure(std::max(total, 1)));

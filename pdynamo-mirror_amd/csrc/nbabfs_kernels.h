@@ -130,7 +130,9 @@ void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, in
 // at the dispatch's own begin and end; null = not timed).
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
                         int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images = false,
-                        hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
+                        hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr, int n14_blocks = 0);
+// The 1-4 list in blocks of NB_P14_BLOCK pairs (k_pairs14, or extra workgroups of a single-state pair launch).
+#define NB_P14_BLOCK 64
 void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
                         const int *cols, ForceGuard g, hipStream_t s, hipEvent_t ev_start = nullptr,
                         hipEvent_t ev_stop = nullptr);
