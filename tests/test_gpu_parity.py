@@ -449,3 +449,38 @@ def test_sgof_pool_matches_system_energy():
             assert f[k] == e, (k, f[k], e)
             assert np.array_equal(g[k], t.configuration.gradients3)
     pool.Finalize()
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_randomized_configurations(oracle_mod, seed):
+    """Seeded random systems, box shapes and options (cutoffs, dielectric, 1-4 scaling), both paths: the
+    pair set bit for bit and the terms within tolerance of the oracle."""
+    from pdynamo_amd import (LJParameterContainer_FromTableCoefficients, MMAtomContainer, NBModelABFS, System,
+                             synthetic)
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.integers(60, 3000))
+    mm = synthetic.random_small_system(n, 40.0, 500 + seed)
+    aspect = rng.uniform(0.3, 2.5, 3)
+    mm.coordinates3 = np.ascontiguousarray(mm.coordinates3 * aspect)
+    k = int(rng.integers(0, n // 4 + 1))
+    i = rng.integers(8, n, k) if n > 8 else np.zeros(0, np.int64)
+    mm.interactions14 = np.stack([i, i - rng.integers(3, 8, i.size)], 1).astype(np.int32).reshape(-1, 2)
+    inner = float(rng.uniform(6.0, 9.0))
+    outer = inner + float(rng.uniform(2.0, 4.0))
+    lst = outer + float(rng.uniform(0.5, 2.0))
+    damp = float(rng.uniform(0.3, 1.0))
+    diel = float(rng.choice([1.0, 2.0, 4.0]))
+    sc14 = float(rng.choice([1.0, 0.5, 0.8333]))
+    lj = LJParameterContainer_FromTableCoefficients(mm.ntypes, mm.tableA, mm.tableB)
+    o = oracle_mod.OracleABFS(mm, damping=damp, inner=inner, outer=outer, list_cutoff=lst, dielectric=diel,
+                              scale14=sc14)
+    rterms, rgrad, _ = o.energy(mm.coordinates3)
+    for precision, etol, gtol in (("mixed", 1e-6, 1e-5), ("fp64", 1e-12, 1e-12)):
+        s = System.FromMMParameters(mm.coordinates3, MMAtomContainer(mm.charges, mm.ljtypes), lj, mm.exclusions,
+                                    mm.interactions14)
+        s.DefineNBModel(NBModelABFS(dielectric=diel, electrostaticScale14=sc14, innerCutoff=inner, outerCutoff=outer,
+                                    listCutoff=lst, dampingCutoff=damp, precision=precision))
+        print("seed", seed, "n", n, precision, flush=True)
+        _, terms, grad = _run(s)
+        assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
+        _check(terms, grad, rterms, rgrad, etol, gtol)
