@@ -1,15 +1,19 @@
 """Benchmark of the NBModelABFS energy + gradient on one MI355X.
 
-One step = NBModelABFS_Update (the half-skin rebuild check, rebuilding when it
-fires) + one energy/gradient evaluation of the whole system, coordinates and
-gradients resident in HBM.  The default workload is the 30,000-atom
-solute-in-water system (9,667 waters + 999 random solute atoms, 67 A cube,
-8 Lorentz-Berthelot solute LJ types), tiled mixed-precision kernel.
+The default workload is the largest single-GPU configuration: 16 images of the
+30,000-atom solute-in-water system (9,667 waters + 999 random solute atoms, 67 A
+cube, 8 Lorentz-Berthelot solute LJ types), each with independent N(0, 0.1 A)
+perturbations, 480,000 atoms evaluated as one batch.  One step = the half-skin
+rebuild checks of all images + one batched energy/gradient evaluation (tiled
+mixed-precision kernel), coordinates and gradients resident in HBM.  The other
+configurations are named workloads (--workload): per system, one step =
+NBModelABFS_Update (rebuilding when the half-skin rule fires) + one E+grad.
 
 Prints ONE JSON line: pair interactions/s (reference list pairs, d <= 13.5 A,
 per second), ms per E+grad, the roofline of the pair kernel (algorithmic
-flops per launch over the launch time measured with HIP events on the launch
-stream) and the reference C library timed on this host's cores.
+flops per launch over the launch time measured with HIP dispatch events on the
+launch stream), the plain FP64 comparisons, and the reference C library timed
+on this host's cores.
 """
 
 import argparse
@@ -26,6 +30,7 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6     # MI355X published FP64 vector peak
+DTYPE = {"mixed": "f64 pair core + f32 rsqrt seed/force vectors + int64 fixed-point sums", "fp64": "f64"}
 
 
 def algorithmic_flops(xyz, keys, n):
@@ -233,8 +238,9 @@ def make_batch(name):
     return None, None
 
 
-def run_batch(args, systems, wdesc):
-    """One step = rebuild checks of every system + one batched E+grad launch (hipGraph replay)."""
+def run_batch(args, systems, wdesc, workload):
+    """One step = the rebuild checks of every system + one batched E+grad of all of them (hipGraph replay of
+    prepare, pair, 1-4 and finalize passes over the union of the systems' work items)."""
     from pdynamo_amd import _native
     from pdynamo_amd.NBModelABFS import NBModelABFS
 
@@ -255,17 +261,31 @@ def run_batch(args, systems, wdesc):
     xp = [X.ptr + 24 * int(o) for o in offs[:-1]]
     gp = [G.ptr + 24 * int(o) for o in offs[:-1]]
 
-    states = states_for(args.precision)
-    batch = _native.NativeBatch(states, use_graph=True)
-    for _ in range(args.warmup):
-        batch.energy(xp, gp)
-    _native.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e, rebuilt = batch.energy(xp, gp)
-    _native.synchronize()
-    ms = (time.perf_counter() - t0) / args.steps * 1e3
-    npairs = int(sum(st.stats()["npairs"] for st in states))
+    def timed_batch(precision, steps, warmup, kernel_pass=True):
+        states = states_for(precision)
+        batch = _native.NativeBatch(states, use_graph=True)
+        for _ in range(warmup):
+            batch.energy(xp, gp)
+        _native.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            e, rebuilt = batch.energy(xp, gp)
+        _native.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        kms = None
+        if kernel_pass:
+            # The pair launch's own duration: dispatch events in a pass of its own (no graph meanwhile), so
+            # that the events do not add to the step time above.
+            batch.set_timing(True)
+            for _ in range(min(steps, 20)):
+                batch.energy(xp, gp)
+            kms, nk = batch.kernel_timing()
+            batch.set_timing(False)
+        return ms, kms, e, states, batch
+
+    ms, kms, e, states, batch = timed_batch(args.precision, args.steps, args.warmup)
+    stats = [st.stats() for st in states]
+    npairs = int(sum(s_["npairs"] for s_ in stats))
 
     def sequential(precision):
         # The same systems as separate single-state calls, one after another, on one stream.
@@ -274,29 +294,110 @@ def run_batch(args, systems, wdesc):
             st.update(x)
             st.energy_async(x, g)
         _native.synchronize()
-        reps = max(1, args.steps // 5)
+        reps = max(1, min(args.steps, 20) // 4)
         t0 = time.perf_counter()
         for _ in range(reps):
             for st, x, g in zip(seq, xp, gp):
                 st.update(x)
                 st.energy_async(x, g)
             _native.synchronize()
-        return (time.perf_counter() - t0) / reps * 1e3
+        out = (time.perf_counter() - t0) / reps * 1e3
+        for st in seq:
+            st.close()
+        return out
 
-    ms_plain = sequential("fp64")
-    ms_seq_mixed = sequential("mixed")
+    other = "fp64" if args.precision == "mixed" else "mixed"
+    cmp = {}
+    if not args.no_compare:
+        ms_other_batch, kms_other, _, st_other, b_other = timed_batch(other, max(5, args.steps // 5), 2)
+        b_other.close()
+        for st in st_other:
+            st.close()
+        ms_seq_plain = sequential("fp64")
+        ms_seq_mixed = sequential("mixed")
+        cmp = {"batched_%s_ms" % other: ms_other_batch, "pair_kernel_ms_%s" % other: kms_other,
+               "sequential_plain_fp64_ms": ms_seq_plain, "sequential_mixed_ms": ms_seq_mixed,
+               "speedup_vs_sequential_plain_fp64": ms_seq_plain / ms,
+               "speedup_vs_batched_%s" % other: ms_other_batch / ms}
     result = {
         "metric": "pair interactions/s (reference list pairs d<=13.5 A per E+grad second)",
         "value": npairs / (ms * 1e-3), "unit": "pair interactions/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 pair core + f32 rsqrt seed/force vectors + int64 fixed-point sums",
-        "data": "synthetic (seeded generator, pdynamo_amd.synthetic)",
+        "dtype": DTYPE[args.precision], "data": "synthetic (seeded generator, pdynamo_amd.synthetic)",
         "config": {"workload": wdesc, "systems": len(systems), "atoms": int(offs[-1]), "list_pairs": npairs,
                    "min_atoms": int(min(sizes)), "max_atoms": int(max(sizes)), "precision": args.precision,
-                   "sequential_plain_fp64_ms": ms_plain, "sequential_mixed_ms": ms_seq_mixed,
-                   "speedup_vs_sequential_plain_fp64": ms_plain / ms},
+                   "work_items": int(sum(s_["workItems"] for s_ in stats)),
+                   "cluster_entries": int(sum(s_["entries"] for s_ in stats)),
+                   "energies_kJ_mol_system0": [float(v) for v in e[0]], **cmp},
     }
+    if not args.no_roofline and kms:
+        flops = n_plain = n_switch = 0
+        for st, m in zip(states, systems):
+            f_, _, p_, s2_ = algorithmic_flops(m.coordinates3, st.export_pairs(0), m.natoms)
+            flops, n_plain, n_switch = flops + f_, n_plain + p_, n_switch + s2_
+        achieved = flops / (kms * 1e-3) / 1e12
+        peak = PEAK_FP32_TFLOPS if args.precision == "mixed" else PEAK_FP64_TFLOPS
+        tpath = args.traffic_summary
+        if tpath is None and args.precision == "mixed":
+            tpath = os.path.join(ROOT, "profiles", "traffic_k_force_tiled_%s.md" % workload)
+        result["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                              "frac": achieved / peak, "traffic": traffic_from_summary(tpath),
+                              "kernel": "k_force_tiled" if args.precision == "mixed" else "k_force_plain",
+                              "kernel_ms": kms, "launches": min(args.steps, 20), "flops_per_launch": flops,
+                              "pairs_d_le_8": n_plain, "pairs_8_12": n_switch}
+    batch.close()
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline_batch(systems, npairs, args.cpu_image_calls)
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
     print(json.dumps(result))
+
+
+def host_threads():
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+    return threads
+
+
+def cpu_baseline_batch(systems, npairs, calls):
+    """The reference pDynamo C library (oracle/_ref, -O3 -fopenmp) over the same systems, one configuration
+    after another as pDynamo's serial image pool evaluates them, in the two buckets of SystemWithTimings
+    ("NB Set Up": the list build of the first call; "NB Evaluation": the E+grad of the calls after it), at
+    the host's thread count and at one thread.  Bounded sample: per system one build and `calls` E+grad
+    calls (one at one thread)."""
+    import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+
+    def run(nthreads, ncalls):
+        t_build = t_eval = 0.0
+        for m in systems:
+            if kind == "reference":
+                eng = O.ReferenceABFS(m, nthreads=nthreads)
+            else:
+                eng = O.OracleABFS(m, nthreads=nthreads)
+            t0 = time.perf_counter()
+            eng.energy(m.coordinates3)
+            t1 = time.perf_counter()
+            for _ in range(ncalls):
+                eng.energy(m.coordinates3)
+            t2 = time.perf_counter()
+            per = (t2 - t1) / ncalls
+            t_eval += per
+            t_build += max((t1 - t0) - per, 0.0)
+            if kind == "reference":
+                eng.close()
+        return t_build, t_eval
+
+    threads = host_threads()
+    b_n, e_n = run(threads, calls)
+    b_1, e_1 = run(1, 1)
+    return {"value": npairs / e_n, "unit": "pair interactions/s", "cores": threads, "kind": kind,
+            "sample": "each of the %d systems: one list build and %d E+grad calls at %d threads, one build and one "
+                      "E+grad call at 1 thread" % (len(systems), calls, threads),
+            "eval_ms_per_step": e_n * 1e3, "list_build_ms_per_step": b_n * 1e3,
+            "one_thread": {"value": npairs / e_1, "unit": "pair interactions/s", "cores": 1,
+                           "eval_ms_per_step": e_1 * 1e3, "list_build_ms_per_step": b_1 * 1e3}}
 
 
 def cpu_baseline(mm, calls, threads, extra=None):
@@ -339,16 +440,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="solute30000")
+    ap.add_argument("--workload", default="images16",
+                    help="images16 (default: 16 perturbed images of the 30,000-atom system, one batch), droplets64, "
+                         "solute30000, water200001, water648, dhfr, dhfr_periodic, qcmm_solvated")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--entries-per-item", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=-1)
     ap.add_argument("--image-entries-per-item", type=int, default=0)
     ap.add_argument("--cpu-calls", type=int, default=100)
+    ap.add_argument("--cpu-image-calls", type=int, default=3,
+                    help="E+grad calls per system after its list build in the batch workloads' CPU baseline")
     ap.add_argument("--moves", type=float, default=0.0,
                     help="Gaussian displacement (A) applied before every call, cumulative; rebuilds follow the "
                          "half-skin rule (the 200,001-atom protocol uses 0.05)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="batch workloads: skip the other precision and the sequential single-state comparisons "
+                         "(profiler runs)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the library's work-item autotuner (entriesPerItem / wavesPerCU) before timing")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -371,7 +479,7 @@ def main():
         return
     systems, bdesc = make_batch(args.workload)
     if systems is not None:
-        run_batch(args, systems, bdesc)
+        run_batch(args, systems, bdesc, args.workload)
         return
     mm, wdesc, extra = make_workload(args.workload)
     opts = {"precision": args.precision}
@@ -455,8 +563,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 pair core + f32 rsqrt seed/force vectors + int64 fixed-point sums" if args.precision == "mixed"
-                 else "f64",
+        "dtype": DTYPE[args.precision],
         "data": "synthetic (seeded generator, pdynamo_amd.synthetic)",
         "config": {"workload": wdesc, "atoms": mm.natoms, "list_pairs": n_list, "precision": args.precision,
                    "list_cutoff": 13.5, "cutoffs": [0.5, 8.0, 12.0],
@@ -485,10 +592,7 @@ def main():
                               "kernel_ms": kms, "launches": int(stats["forceKernelLaunches"]),
                               "flops_per_launch": flops, "pairs_d_le_8": n_plain, "pairs_8_12": n_switch}
     if not args.no_cpu_baseline:
-        threads = len(os.sched_getaffinity(0))
-        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
-            threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
-        result["cpu_baseline"] = cpu_baseline(mm, args.cpu_calls, threads, extra)
+        result["cpu_baseline"] = cpu_baseline(mm, args.cpu_calls, host_threads(), extra)
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
     print(json.dumps(result))

@@ -38,8 +38,11 @@ extern "C" {
 #define NBABFS_STATUS_LOGIC_ERROR       10
 
 /* Arithmetic of the pair kernel. */
-#define NBABFS_MIXED      0   /* FP32 pair math, 64-bit fixed-point sums, FP64 totals */
-#define NBABFS_PLAIN_FP64 1   /* FP64 pair math in the reference expression order    */
+#define NBABFS_MIXED      0   /* tiled kernel: FP64 pair core with 1/r from the FP32
+                                 rsqrt seed plus one FP64 Newton step, FP32 force
+                                 vectors, 64-bit fixed-point sums, FP64 totals      */
+#define NBABFS_PLAIN_FP64 1   /* one thread per atom over a full CSR list, FP64 in
+                                 the reference expression order (divide, sqrt)      */
 
 /* Upper bound of nbabfs_options.entriesPerItem (per-item j-force staging in LDS). */
 #define NBABFS_MAX_ENTRIES_PER_ITEM 128
@@ -114,6 +117,17 @@ void nbabfs_state_destroy(nbabfs_state *state);
 /* Stream used by every launch of this state (a hipStream_t; NULL = the
  * library's own stream). */
 int nbabfs_set_stream(nbabfs_state *state, void *hip_stream);
+
+/* Errors of an energy call.  Where the reference would return NaN or an
+ * overflowed value with Status_Continue, these calls return
+ * NBABFS_STATUS_VALUE_ERROR: a NaN or infinite coordinate (found by the list
+ * build, or by the per-call pass that reads every coordinate), a gradient or
+ * energy contribution that is NaN or beyond the +-2^66 range of the
+ * fixed-point sums, coordinates spanning more than 1e4 A in one axis, and, at
+ * state creation, non-finite charges or LJ parameters.  The energy outputs are
+ * still written; the gradients of such a call are not defined.  The error of
+ * an asynchronous energy call is returned by nbabfs_energies (and
+ * nbabfs_image_terms); a batch returns it from nbabfs_batch_energy. */
 
 /* NBModelABFSState_Initialize + NBModelABFS_Update + NBModelABFS_MMMMEnergy on
  * host arrays (coordinates n*3 FP64 row-major).  grad_inout (n*3) is
@@ -277,7 +291,8 @@ long long nbabfs_export_pairs(nbabfs_state *state, int which, long long *keys, l
  * framework that bundles its own libamdhip64 cannot share streams with this
  * library).  No counterpart in the reference, whose arrays live in host
  * memory.  Copies and the zero fill are ordered on the state's stream when a
- * state is given (NULL = synchronous on the device). */
+ * state is given; with NULL they first wait for all work on the device (every
+ * state's stream), then run synchronously. */
 int nbabfs_device_alloc(size_t bytes, void **out);
 int nbabfs_device_free(void *ptr);
 int nbabfs_memcpy_htod(nbabfs_state *state, void *dst, const void *src, size_t bytes);
@@ -294,6 +309,19 @@ void nbabfs_batch_destroy(nbabfs_batch *batch);
  * NULL); energies (4*nsystems) and rebuilt (nsystems, may be NULL) on host. */
 int nbabfs_batch_energy(nbabfs_batch *batch, const double *const *d_xyz, double *const *d_grad, double *energies,
                         int *rebuilt);
+/* Bookkeeping of a batch (no counterpart in the reference): hipGraph captures,
+ * batched launches and re-layouts (concatenations of the systems' work items,
+ * after a rebuild of any system or a change of its arrays) so far, and the
+ * mean duration (ms) of the batched pair launch over the timed launches.
+ * Outputs may be NULL. */
+int nbabfs_batch_info(nbabfs_batch *batch, long long *graph_captures, long long *launches, long long *layouts,
+                      double *force_kernel_ms, long long *timed_launches);
+/* Timing of the batched pair launch (no counterpart in the reference): while
+ * enable != 0 the batch's calls enqueue their passes directly instead of
+ * replaying the graph, with the runtime's dispatch start/stop events around
+ * the pair kernel (the plain path: from the first system's pair kernel to the
+ * last's); enabling resets the mean. */
+int nbabfs_batch_set_timing(nbabfs_batch *batch, int enable);
 
 #ifdef __cplusplus
 }

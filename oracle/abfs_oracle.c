@@ -404,6 +404,46 @@ void orc_mmmm_energy(const double *f, int n, const double *xyz, const double *ch
     *elj = eLJ;
 }
 
+/* Sums of the magnitudes of the pair terms, sum |e_el| and sum |e_lj| over the
+ * same records as orc_mmmm_energy: the scale of the rounding error of any FP64
+ * summation order of a cancelling total (checker tolerances; no counterpart in
+ * the reference). */
+void orc_mmmm_abs_sums(const double *f, int n, const double *xyz, const double *charge,
+                       const int *ljtype, int ntypes, const double *tableA, const double *tableB,
+                       const int *tableindex, double electrostaticScale, double ljScale,
+                       long nrec, const int *rec_i, const long *rec_ptr, const int *js,
+                       int nthreads, double *sel, double *slj)
+{
+    double aQQ = 0.0, aLJ = 0.0;
+    double escale = electrostaticScale * orc_coulomb_factor();
+    int nt = 1;
+#ifdef _OPENMP
+    nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#endif
+    (void)n;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) reduction(+ : aQQ, aLJ) schedule(dynamic)
+#endif
+    for (long r = 0; r < nrec; r++) {
+        int i = rec_i[r];
+        double qi = escale * charge[i];
+        int ti = ntypes * ljtype[i];
+        for (long k = rec_ptr[r]; k < rec_ptr[r + 1]; k++) {
+            int j = js[k];
+            double xij = xyz[3 * i] - xyz[3 * j], yij = xyz[3 * i + 1] - xyz[3 * j + 1], zij = xyz[3 * i + 2] - xyz[3 * j + 2];
+            double r2 = (xij * xij + yij * yij + zij * zij);
+            int tij = tableindex[ti + ljtype[j]];
+            double e1, e2, dF;
+            if (!orc_pair_terms(f, r2, qi * charge[j], tableA[tij] * ljScale, tableB[tij] * ljScale, &e1, &e2, &dF))
+                continue;
+            aQQ += fabs(e1);
+            aLJ += fabs(e2);
+        }
+    }
+    *sel = aQQ;
+    *slj = aLJ;
+}
+
 /* pMx/csource/LJParameterContainer.c:184-210 (MakeTable). */
 void orc_lj_make_table(int ntypes, const double *eps, const double *sigma, int geometric,
                        int four_eps, double *tableA, double *tableB, int *tableindex)

@@ -79,6 +79,14 @@ void orc_mmmm_energy(const double *f, int n, const double *xyz, const double *ch
                      long nrec, const int *rec_i, const long *rec_ptr, const int *js,
                      int nthreads, double *eel, double *elj, double *grad);
 
+/* Sum |e_el| and sum |e_lj| of the pair terms over the same records (the
+ * rounding scale of a cancelling FP64 total; checker tolerances only). */
+void orc_mmmm_abs_sums(const double *f, int n, const double *xyz, const double *charge,
+                       const int *ljtype, int ntypes, const double *tableA, const double *tableB,
+                       const int *tableindex, double electrostaticScale, double ljScale,
+                       long nrec, const int *rec_i, const long *rec_ptr, const int *js,
+                       int nthreads, double *sel, double *slj);
+
 /* LJ tables from epsilon/sigma (pMx/csource/LJParameterContainer.c:184-210).
  * geometric: 1 = geometric sigma (OPLS), 0 = arithmetic (AMBER); four_eps: 1 =
  * A=4 eps s^12 form (OPLS), 0 = B = eps 2 s^6 form (AMBER). */

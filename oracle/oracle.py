@@ -59,6 +59,9 @@ def lib():
         L.orc_mmmm_energy.argtypes = [_dp, C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp, _ip, C.c_double, C.c_double,
                                       C.c_long, _ip, _lp, _ip, C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.c_void_p]
+        L.orc_mmmm_abs_sums.argtypes = [_dp, C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp, _ip, C.c_double, C.c_double,
+                                        C.c_long, _ip, _lp, _ip, C.c_int, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]
         L.orc_lj_make_table.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, _dp, _ip]
         L.orc_symmetry_make_m.argtypes = [_dp, _dp, _dp]
         L.orc_orthogonalize.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
@@ -224,6 +227,20 @@ def mmmm_energy(f, xyz, charges, ljtypes, ntypes, tableA, tableB, records, elect
                           records.rec_ptr, records.js, nthreads, C.byref(eel), C.byref(elj),
                           None if g is None else g.ctypes.data)
     return eel.value, elj.value
+
+
+def mmmm_abs_sums(f, xyz, charges, ljtypes, ntypes, tableA, tableB, records, electrostatic_scale=1.0, nthreads=0):
+    """(sum |e_el|, sum |e_lj|) of the pair terms of `records`: the scale of the rounding error of an FP64 sum
+    of the energies in any order."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    tidx = full_table(ntypes, tableA, tableB)
+    a1, a2 = C.c_double(), C.c_double()
+    lib().orc_mmmm_abs_sums(f, xyz.shape[0], xyz, np.ascontiguousarray(charges, np.float64),
+                            np.ascontiguousarray(ljtypes, np.int32), ntypes,
+                            np.ascontiguousarray(tableA, np.float64), np.ascontiguousarray(tableB, np.float64), tidx,
+                            electrostatic_scale, 1.0, records.rec_i.shape[0], records.rec_i, records.rec_ptr,
+                            records.js, nthreads, C.byref(a1), C.byref(a2))
+    return a1.value, a2.value
 
 
 def records_from_pairs(n, pairs):
@@ -481,6 +498,21 @@ class OracleABFS:
                                  escale * self.scale14, 1.0, 1, g)
             terms += [("MM/MM 1-4 Elect.", e1), ("MM/MM 1-4 LJ", e2)]
         return terms, g, rebuilt
+
+    def abs_sums(self, xyz):
+        """{label: sum of |pair term|} of the current lists at xyz (the rounding scale of each term)."""
+        s = self.s
+        out = {}
+        escale = 1.0 / self.dielectric
+        if self.records is not None and self.records.npairs > 0:
+            a1, a2 = mmmm_abs_sums(self.f, xyz, s.charges, s.ljtypes, s.ntypes, s.tableA, s.tableB, self.records,
+                                   escale, self.nthreads)
+            out["MM/MM Elect."], out["MM/MM LJ"] = a1, a2
+        if self.rec14.npairs > 0:
+            a1, a2 = mmmm_abs_sums(self.f, xyz, s.charges, s.ljtypes, s.ntypes, self.tA14, self.tB14, self.rec14,
+                                   escale * self.scale14, 1)
+            out["MM/MM 1-4 Elect."], out["MM/MM 1-4 LJ"] = a1, a2
+        return out
 
 
 # ----------------------------------------------------------------------------------------------------------------

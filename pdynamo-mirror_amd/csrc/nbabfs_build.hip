@@ -27,6 +27,12 @@ __global__ __launch_bounds__(256) void k_bbox_partial(const double *__restrict__
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         for (int d = 0; d < 3; d++) {
             double v = xyz[3 * (size_t)i + d];
+            // fmin/fmax skip NaN: a non-finite coordinate makes the box infinite
+            // instead, which the host rejects before any cell index is formed.
+            if (!isfinite(v)) {
+                mn[d] = -INFINITY;
+                mx[d] = INFINITY;
+            }
             mn[d] = fmin(mn[d], v);
             mx[d] = fmax(mx[d], v);
         }

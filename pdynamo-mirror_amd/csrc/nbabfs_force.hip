@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
     if (D.gacc)
         for (int k = blockIdx.x * 256 + threadIdx.x; k < NB_GACC_WORDS_PER_IMAGE * D.nimg; k += gridDim.x * 256)
             D.gacc[k] = 0;
-    bool up = false;
+    bool up = false, nonfinite = false;
     if (s < D.n_pad) {
         for (int k = 0; k < 6; k++) D.facc[(size_t)k * D.n_pad + s] = 0;
         const double *xyz = chk.xyz ? chk.xyz : D.xyz;
@@ -144,6 +144,7 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
             x = xyz[3 * (size_t)a];
             y = xyz[3 * (size_t)a + 1];
             z = xyz[3 * (size_t)a + 2];
+            nonfinite = !(isfinite(x) && isfinite(y) && isfinite(z));
             if (chk.ref && (!chk.free_flags || (chk.free_flags[a] & 1))) {
 #pragma clang fp contract(off)
                 const double dx = x - chk.ref[3 * (size_t)a];
@@ -163,6 +164,9 @@ __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ des
             D.styp[s] = t;
         }
     }
+    // A NaN or infinite coordinate fails no distance test (its pairs drop out
+    // silently), so it is reported here, where every atom is read.
+    raise_error(D.err, nonfinite, NB_ERR_COORD);
     if (chk.counter) {
         // One 64-bit atomic per block carries both its completion (high word)
         // and whether any of its atoms moved (low word), so the last block knows
@@ -218,6 +222,7 @@ __device__ __forceinline__ void pairs14_block(const SysDesc &D, const AbfsFactor
 #pragma clang fp contract(off)
     const int k = blk * 64 + (int)threadIdx.x;
     double e1 = 0.0, e2 = 0.0;
+    bool bad = false;
     if (k < D.n14) {
         const int i = D.p14[2 * k], j = D.p14[2 * k + 1];
         double xij = D.xyz[3 * (size_t)i] - D.xyz[3 * (size_t)j];
@@ -232,12 +237,12 @@ __device__ __forceinline__ void pairs14_block(const SysDesc &D, const AbfsFactor
             e2 = x2;
             const int si = D.atom_slot[i], sj = D.atom_slot[j];
             const double px = xij * (2.0 * dF), py = yij * (2.0 * dF), pz = zij * (2.0 * dF);
-            fx_atomic_add(D.facc, D.n_pad, si, 0, px);
-            fx_atomic_add(D.facc, D.n_pad, si, 1, py);
-            fx_atomic_add(D.facc, D.n_pad, si, 2, pz);
-            fx_atomic_add(D.facc, D.n_pad, sj, 0, -px);
-            fx_atomic_add(D.facc, D.n_pad, sj, 1, -py);
-            fx_atomic_add(D.facc, D.n_pad, sj, 2, -pz);
+            bad |= fx_atomic_add(D.facc, D.n_pad, si, 0, px);
+            bad |= fx_atomic_add(D.facc, D.n_pad, si, 1, py);
+            bad |= fx_atomic_add(D.facc, D.n_pad, si, 2, pz);
+            bad |= fx_atomic_add(D.facc, D.n_pad, sj, 0, -px);
+            bad |= fx_atomic_add(D.facc, D.n_pad, sj, 1, -py);
+            bad |= fx_atomic_add(D.facc, D.n_pad, sj, 2, -pz);
         }
     }
     e1 = wave_sum_f64(e1);
@@ -246,6 +251,7 @@ __device__ __forceinline__ void pairs14_block(const SysDesc &D, const AbfsFactor
         D.epart14[2 * blk] = e1;
         D.epart14[2 * blk + 1] = e2;
     }
+    raise_error(D.err, bad || !isfinite(e1) || !isfinite(e2), NB_ERR_RANGE);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -483,13 +489,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
             ig[k] = ((fr[0] + fr[8]) + (fr[16] + fr[24])) + ((fr[32] + fr[40]) + (fr[48] + fr[56]));
         }
         const int si = (I0 + b) * NB_CL + a;
+        bool bad = false;
 #pragma unroll
-        for (int k = 0; k < 3; k++) fx_atomic_add(facc, n_pad, si, k, IMG ? isc * (double)ig[k] : (double)ig[k]);
+        for (int k = 0; k < 3; k++) bad |= fx_atomic_add(facc, n_pad, si, k, IMG ? isc * (double)ig[k] : (double)ig[k]);
         if (!IMG) {
             // j-forces of the whole item.
             for (int k = lane; k < 3 * NB_CL * w.count; k += 64) {
                 const int el = k / (3 * NB_CL), r = k - el * 3 * NB_CL;
-                fx_atomic_add(facc, n_pad, sJ[el] * NB_CL + r / 3, r % 3, (double)sJF[k]);
+                bad |= fx_atomic_add(facc, n_pad, sJ[el] * NB_CL + r / 3, r % 3, (double)sJF[k]);
             }
         } else {
             // Image side: g'_j (scaled) is rotated back, g_j += R^T g'_j, and the
@@ -506,9 +513,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
                 const int sj = sJ[el] * NB_CL + jb;
                 const float *f = &sJF[el * 3 * NB_CL + 3 * jb];
                 const double gx = isc * (double)f[0], gy = isc * (double)f[1], gz = isc * (double)f[2];
-                fx_atomic_add(facc, n_pad, sj, 0, fma(sX[0], gx, fma(sX[3], gy, sX[6] * gz)));
-                fx_atomic_add(facc, n_pad, sj, 1, fma(sX[1], gx, fma(sX[4], gy, sX[7] * gz)));
-                fx_atomic_add(facc, n_pad, sj, 2, fma(sX[2], gx, fma(sX[5], gy, sX[8] * gz)));
+                bad |= fx_atomic_add(facc, n_pad, sj, 0, fma(sX[0], gx, fma(sX[3], gy, sX[6] * gz)));
+                bad |= fx_atomic_add(facc, n_pad, sj, 1, fma(sX[1], gx, fma(sX[4], gy, sX[7] * gz)));
+                bad |= fx_atomic_add(facc, n_pad, sj, 2, fma(sX[2], gx, fma(sX[5], gy, sX[8] * gz)));
                 G[0] += gx; G[1] += gy; G[2] += gz;
                 if (moments) {
                     const double4 xr = load_slot(xq, sj);
@@ -524,7 +531,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
             for (int k = 0; k < 12; k++) {
                 if (k < ncomp) {
                     const double v = wave_sum_f64(G[k]);
-                    if (lane == k) fx_add(&ga[2 * k], &ga[2 * k + 1], v);
+                    if (lane == k) bad |= fx_add(&ga[2 * k], &ga[2 * k + 1], v);
                 }
             }
         }
@@ -534,8 +541,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
             const int slot = it & (NB_ESLOTS - 1);
             const int kind = lane + (IMG ? 2 : 0);
             const double v = lane == 0 ? eel : elj;
-            fx_add(eacc_word(D->eacc, kind, 0, slot), eacc_word(D->eacc, kind, 1, slot), IMG ? isc * v : v);
+            bad |= fx_add(eacc_word(D->eacc, kind, 0, slot), eacc_word(D->eacc, kind, 1, slot), IMG ? isc * v : v);
         }
+        raise_error(D->err, bad, NB_ERR_RANGE);
     }
 }
 
@@ -651,6 +659,7 @@ __global__ __launch_bounds__(64) void k_force_image_fp64(const SysDesc *__restri
         ti[c] = nt * D.atype[i];
     }
     double e1 = 0.0, e2 = 0.0, G[12];
+    bool bad = false;
 #pragma unroll
     for (int k = 0; k < 12; k++) G[k] = 0.0;
     for (int e = w.first; e < w.first + w.count; e++) {
@@ -693,9 +702,9 @@ __global__ __launch_bounds__(64) void k_force_image_fp64(const SysDesc *__restri
         if (a == 0 && j >= 0) {
             const int sj = J * NB_CL + b;
             // Coordinates3_Rotate by R'^T.
-            fx_atomic_add(D.facc, D.n_pad, sj, 0, X.R[0] * gx + X.R[3] * gy + X.R[6] * gz);
-            fx_atomic_add(D.facc, D.n_pad, sj, 1, X.R[1] * gx + X.R[4] * gy + X.R[7] * gz);
-            fx_atomic_add(D.facc, D.n_pad, sj, 2, X.R[2] * gx + X.R[5] * gy + X.R[8] * gz);
+            bad |= fx_atomic_add(D.facc, D.n_pad, sj, 0, X.R[0] * gx + X.R[3] * gy + X.R[6] * gz);
+            bad |= fx_atomic_add(D.facc, D.n_pad, sj, 1, X.R[1] * gx + X.R[4] * gy + X.R[7] * gz);
+            bad |= fx_atomic_add(D.facc, D.n_pad, sj, 2, X.R[2] * gx + X.R[5] * gy + X.R[8] * gz);
             G[0] += gx;
             G[1] += gy;
             G[2] += gz;
@@ -715,9 +724,9 @@ __global__ __launch_bounds__(64) void k_force_image_fp64(const SysDesc *__restri
         const double vz = sum_lanes_xor(gi[c][2], 8, 16, 32);
         if (b == 0 && ia[c] >= 0) {
             const int si = (w.sc * NB_SCC + c) * NB_CL + a;
-            fx_atomic_add(D.facc, D.n_pad, si, 0, vx);
-            fx_atomic_add(D.facc, D.n_pad, si, 1, vy);
-            fx_atomic_add(D.facc, D.n_pad, si, 2, vz);
+            bad |= fx_atomic_add(D.facc, D.n_pad, si, 0, vx);
+            bad |= fx_atomic_add(D.facc, D.n_pad, si, 1, vy);
+            bad |= fx_atomic_add(D.facc, D.n_pad, si, 2, vz);
         }
     }
     long long *ga = D.gacc + (size_t)img * NB_GACC_WORDS_PER_IMAGE +
@@ -725,7 +734,7 @@ __global__ __launch_bounds__(64) void k_force_image_fp64(const SysDesc *__restri
 #pragma unroll
     for (int k = 0; k < 12; k++) {
         const double v = wave_sum_f64(G[k]);
-        if (lane == k && (moments || k < 3)) fx_add(&ga[2 * k], &ga[2 * k + 1], v);
+        if (lane == k && (moments || k < 3)) bad |= fx_add(&ga[2 * k], &ga[2 * k + 1], v);
     }
     e1 = wave_sum_f64(e1);
     e2 = wave_sum_f64(e2);
@@ -733,6 +742,7 @@ __global__ __launch_bounds__(64) void k_force_image_fp64(const SysDesc *__restri
         D.eimgpart[2 * (size_t)it] = e1;
         D.eimgpart[2 * (size_t)it + 1] = e2;
     }
+    raise_error(D.err, bad || !isfinite(e1) || !isfinite(e2), NB_ERR_RANGE);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -833,6 +843,14 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
         D.energies[5] = ilj;
         D.energies[6] = qlj;
         D.energies[7] = qlj14;
+        // The call's error bits travel with its energies; the word starts the next call clear.
+        int e = 0;
+        if (D.err) {
+            e = __hip_atomic_exchange(as_global(D.err), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // The plain path's energies are FP64 sums with no fixed-point stage.
+            if (plain && !(isfinite(el) && isfinite(lj) && isfinite(iel) && isfinite(ilj))) e |= NB_ERR_RANGE;
+        }
+        D.energies[NB_ESTATUS] = (double)e;
     }
 }
 
@@ -852,6 +870,7 @@ __global__ __launch_bounds__(256) void k_qclj(const SysDesc *__restrict__ descs,
     const int ti = D.ntypes * D.atype[i];
     const int si = D.atom_slot[i];
     double e[2] = {0.0, 0.0}, g[3] = {0.0, 0.0, 0.0};
+    bool bad = false;
     for (int list = 0; list < 2; list++) {
         const int *ptr = list ? D.qlj14_ptr : D.qlj_ptr;
         const int *idx = list ? D.qlj14_idx : D.qlj_idx;
@@ -873,11 +892,12 @@ __global__ __launch_bounds__(256) void k_qclj(const SysDesc *__restrict__ descs,
             g[1] += yij;
             g[2] += zij;
             const int sj = D.atom_slot[j];
-            fx_atomic_add(D.facc, D.n_pad, sj, 0, -xij);
-            fx_atomic_add(D.facc, D.n_pad, sj, 1, -yij);
-            fx_atomic_add(D.facc, D.n_pad, sj, 2, -zij);
+            bad |= fx_atomic_add(D.facc, D.n_pad, sj, 0, -xij);
+            bad |= fx_atomic_add(D.facc, D.n_pad, sj, 1, -yij);
+            bad |= fx_atomic_add(D.facc, D.n_pad, sj, 2, -zij);
         }
     }
+    raise_error(D.err, bad, NB_ERR_RANGE);
     // block_sum2's shared arrays are reused: a barrier after each sum.
     block_sum2<256>(e[0], e[1]);
     __syncthreads();
@@ -886,13 +906,16 @@ __global__ __launch_bounds__(256) void k_qclj(const SysDesc *__restrict__ descs,
     __syncthreads();
     double zz = 0.0;
     block_sum2<256>(gz, zz);
+    bool bad_i = false;
     if (threadIdx.x == 0) {
         D.qljpart[2 * ((size_t)r * gridDim.y + c)] = e[0];
         D.qljpart[2 * ((size_t)r * gridDim.y + c) + 1] = e[1];
-        fx_atomic_add(D.facc, D.n_pad, si, 0, g[0]);
-        fx_atomic_add(D.facc, D.n_pad, si, 1, g[1]);
-        fx_atomic_add(D.facc, D.n_pad, si, 2, gz);
+        bad_i |= fx_atomic_add(D.facc, D.n_pad, si, 0, g[0]);
+        bad_i |= fx_atomic_add(D.facc, D.n_pad, si, 1, g[1]);
+        bad_i |= fx_atomic_add(D.facc, D.n_pad, si, 2, gz);
+        bad_i |= !isfinite(e[0]) || !isfinite(e[1]);
     }
+    raise_error(D.err, bad_i, NB_ERR_RANGE);
 }
 
 void launch_qclj(const SysDesc *d_desc, int nrows, const AbfsFactors &F, hipStream_t s)

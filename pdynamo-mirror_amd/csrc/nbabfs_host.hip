@@ -43,8 +43,19 @@ static int fail(int code, const char *fmt, ...)
         if (s_ != NBABFS_STATUS_CONTINUE) return s_;                                                   \
     } while (0)
 
-static long g_alloc_version = 0;
+// The error bits a call's finalize pass stored beside its energies, as a status.
+static int call_status(int bits, const char *what)
+{
+    if (bits == 0) return NBABFS_STATUS_CONTINUE;
+    return fail(NBABFS_STATUS_VALUE_ERROR, "%s: %s%s%s", what, (bits & NB_ERR_COORD) ? "a coordinate is NaN or infinite" : "",
+                (bits & NB_ERR_COORD) && (bits & NB_ERR_RANGE) ? "; " : "",
+                (bits & NB_ERR_RANGE) ? "a gradient or energy contribution is NaN or beyond +-2^66" : "");
+}
 
+// A device buffer that grows on demand.  The device descriptors and batch
+// layouts hold its address; they are compared field by field with what the
+// current buffers give before every launch (ensure_desc, desc_current), so a
+// reallocation is never launched on with a stale address.
 template <class T>
 struct DBuf {
     T *p = nullptr;
@@ -61,7 +72,6 @@ struct DBuf {
             return fail(NBABFS_STATUS_OUT_OF_MEMORY, "hipMalloc of %zu bytes failed", mm * sizeof(T));
         }
         cap = mm;
-        g_alloc_version++;
         return NBABFS_STATUS_CONTINUE;
     }
     int upload(const T *h, size_t m, hipStream_t s)
@@ -80,7 +90,6 @@ struct DBuf {
     {
         std::swap(p, o.p);
         std::swap(cap, o.cap);
-        g_alloc_version++;
     }
     ~DBuf() { release(); }
 };
@@ -359,11 +368,14 @@ struct nbabfs_state {
     DBuf<long long> facc;
     DBuf<double> epart14, eplain, gplain, energies, eimgpart;
     DBuf<long long> eacc, vacc;
+    DBuf<int> errw;                // error word of the per-call kernels (NB_ERR_* bits)
+    bool err_dirty = false;        // a prepare pass ran since the last finalize pass read errw
     DBuf<SysDesc> d_desc;
     DBuf<int> heads, qstarts;
     PinBuf<SysDesc> h_desc;
     PinBuf<double> h_energies, h_grad;
-    long desc_version = -1;
+    SysDesc desc_shadow;           // the descriptor last uploaded to d_desc (valid when desc_valid)
+    bool desc_valid = false;
     long items_version = 0;        // bumped whenever the work items are cut or ordered anew
     int grad_assign = 0;           // NBABFS_GRADIENT_ASSIGN: energy calls overwrite the gradient
     const double *last_xyz = nullptr;
@@ -516,6 +528,22 @@ static int timing_events(nbabfs_state *S, int count, hipEvent_t *out)
 // ------------------------------------------------------------------------------------------------
 // Grid parameters (host FP64, reference operation order).
 // ------------------------------------------------------------------------------------------------
+// The box of a list build: k_bbox_partial turns a NaN or infinite coordinate
+// into an infinite box.  A finite box may span at most 1e4 A per axis: the
+// search culls with FP32 cluster boxes in the frame of the box minimum,
+// padded by 1e-3 A, which covers FP32 rounding (half an ulp, <= 4.9e-4 A)
+// only for coordinates below 2^14 A in that frame.
+static int check_box(const double *bb)
+{
+    for (int d = 0; d < 3; d++) {
+        if (!std::isfinite(bb[d]) || !std::isfinite(bb[3 + d]))
+            return fail(NBABFS_STATUS_VALUE_ERROR, "a coordinate is NaN or infinite (list build)");
+        if (bb[3 + d] - bb[d] > 1.0e4)
+            return fail(NBABFS_STATUS_VALUE_ERROR, "coordinates span %.6g A in axis %d (at most 1e4 A)", bb[3 + d] - bb[d], d);
+    }
+    return NBABFS_STATUS_CONTINUE;
+}
+
 // PairListGenerator_DetermineMethod (pCore-1.9.0/extensions/csource/PairListGenerator.c:498-525).
 static bool determine_method(const nbabfs_options &o, int n, const double *bb)
 {
@@ -801,6 +829,23 @@ static int order_items(nbabfs_state *S)
 
 static int build_qcmm_lists(nbabfs_state *S, const double *d_xyz);
 
+// The error word collects the NB_ERR_* bits of the kernels since the last
+// finalize pass, which reports and clears them.  Before a prepare pass, bits
+// left by an abandoned one (an update with no energy call after it, the
+// autotuner's passes) are cleared, so that they are not reported with the
+// next call; on the usual update-then-energy sequence this costs nothing.
+static void errw_before_prepare(nbabfs_state *S)
+{
+    if (S->err_dirty) launch_fill_int(S->errw.p, 1, 0, S->stream);
+    S->err_dirty = true;
+}
+
+static void errw_clear(nbabfs_state *S)
+{
+    launch_fill_int(S->errw.p, 1, 0, S->stream);
+    S->err_dirty = false;
+}
+
 // ------------------------------------------------------------------------------------------------
 // List build (NBModelABFSState_GridInitialize + GenerateLists).
 // ------------------------------------------------------------------------------------------------
@@ -817,6 +862,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     S->slots.clear();
     S->nimages = 0;
     S->img_npairs = 0;
+    errw_clear(S);      // a rebuild discards the passes queued on the old list
     if (n <= 0) {
         S->n_pad = S->ncl = S->nsc = 0;
         S->listValid = true;
@@ -832,6 +878,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     HIPCHK(hipStreamSynchronize(st));
     double bb[6];
     std::memcpy(bb, S->h_bbox.p, sizeof(bb));
+    CHK(check_box(bb));
 
     // Grid vs direct is decided once, on the first build (NBModelABFSState.c:212).
     if (S->isNew) S->useGrid = determine_method(o, n, bb);
@@ -864,12 +911,15 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
         g.corg[d] = bb[d];
     }
     double cw = std::cbrt((double)NB_SC * vol / (double)n);
-    cw = std::min(std::max(cw, 3.0), 1.0e4);
+    cw = std::max(cw, 3.0);
+    long long maxkeys = 8LL * n + 4096;
+    // A sparse system spread over a large box (atoms far apart) gets wider
+    // columns, so that the column and key arrays stay O(n).
+    while (std::ceil(ext[0] / cw) * std::ceil(ext[1] / cw) > (double)maxkeys) cw *= 2.0;
     g.cw = cw;
     g.ncx = std::max(1, (int)std::ceil(ext[0] / cw));
     g.ncy = std::max(1, (int)std::ceil(ext[1] / cw));
     double dz = 0.125 * cw;
-    long long maxkeys = 8LL * n + 4096;
     if ((long long)g.ncx * g.ncy * (long long)std::ceil(ext[2] / dz + 1.0) > maxkeys)
         dz = ext[2] / std::max(1.0, (double)(maxkeys / ((long long)g.ncx * g.ncy)) - 1.0);
     if (!(dz > 0.0)) dz = 1.0;
@@ -959,7 +1009,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     S->nitems = ((volatile int *)S->tail.h)[1];
     S->nent_main = S->nent;
     S->nitems_main = S->nitems;
-    S->desc_version = -1;
+    S->desc_valid = false;
     S->slots.clear();
     S->nimages = 0;
     S->img_npairs = 0;
@@ -1006,21 +1056,34 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
 
 static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *d_grad);
 
-// Upload the state's descriptor when allocations or the coordinate/gradient
-// arrays changed.  The staging copy is reused only after the stream drained.
+// Upload the state's descriptor when its content changed (a buffer was
+// reallocated, the list's sizes or the coordinate/gradient arrays changed).
+// The staging copy is reused only after the stream drained.
 static int ensure_desc(nbabfs_state *S, const double *d_xyz, double *d_grad)
 {
     CHK(S->d_desc.ensure(1));
     CHK(S->h_desc.ensure(1));
-    if (S->desc_version == g_alloc_version && S->last_xyz == d_xyz && S->last_grad == d_grad)
-        return NBABFS_STATUS_CONTINUE;
-    HIPCHK(hipStreamSynchronize(S->stream));
-    fill_desc(S, S->h_desc.p[0], d_xyz, d_grad);
-    HIPCHK(hipMemcpyAsync(S->d_desc.p, S->h_desc.p, sizeof(SysDesc), hipMemcpyHostToDevice, S->stream));
-    S->desc_version = g_alloc_version;
+    SysDesc D;
+    fill_desc(S, D, d_xyz, d_grad);
     S->last_xyz = d_xyz;
     S->last_grad = d_grad;
+    if (S->desc_valid && std::memcmp(&D, &S->desc_shadow, sizeof(SysDesc)) == 0) return NBABFS_STATUS_CONTINUE;
+    S->desc_valid = false;
+    HIPCHK(hipStreamSynchronize(S->stream));
+    S->h_desc.p[0] = D;
+    HIPCHK(hipMemcpyAsync(S->d_desc.p, S->h_desc.p, sizeof(SysDesc), hipMemcpyHostToDevice, S->stream));
+    S->desc_shadow = D;
+    S->desc_valid = true;
     return NBABFS_STATUS_CONTINUE;
+}
+
+// The uploaded descriptor describes the state's current buffers and sizes.
+static bool desc_current(nbabfs_state *S)
+{
+    if (!S->desc_valid) return false;
+    SysDesc D;
+    fill_desc(S, D, S->last_xyz, S->last_grad);
+    return std::memcmp(&D, &S->desc_shadow, sizeof(SysDesc)) == 0;
 }
 
 static int ensure_call_buffers(nbabfs_state *S)
@@ -1076,6 +1139,7 @@ static int update(nbabfs_state *S, const double *d_xyz, int *rebuilt)
         const bool plain = S->opt.precision == NBABFS_PLAIN_FP64;
         PrepareCheck chk{d_xyz, S->xyz_ref.p, S->has_fixed ? S->freef.p : nullptr, buf * buf, S->upflag.d,
                          S->dflag.p, S->epoch, S->prepcount.p, S->prepdone.d};
+        errw_before_prepare(S);
         launch_prepare(S->d_desc.p, 1, S->n_pad, plain ? 0 : 1, S->heads.p, chk, st);
         HIPCHK(hipGetLastError());
         // The descriptor's coordinates must be these for the speculative pass.
@@ -1162,6 +1226,7 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.epart14 = S->epart14.p;
     D.eplain = S->eplain.p;
     D.energies = S->energies.p;
+    D.err = S->errw.p;
     for (int d = 0; d < 3; d++) D.origin[d] = S->grid.corg[d];
     D.escale = S->escale;
     D.escale14 = S->escale14;
@@ -1209,7 +1274,7 @@ static int launch_force(nbabfs_state *S, ForceGuard g)
 {
     hipStream_t st = S->stream;
     // The kernels write through the descriptor's pointers: it must describe the current buffers.
-    if (S->desc_version != g_alloc_version) return fail(NBABFS_STATUS_LOGIC_ERROR, "pair kernel launched with an out-of-date descriptor");
+    if (!desc_current(S)) return fail(NBABFS_STATUS_LOGIC_ERROR, "pair kernel launched with an out-of-date descriptor");
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if (S->opt.precision != NBABFS_PLAIN_FP64) {
         const int nm = S->nitems_main, ni = S->nitems - S->nitems_main;
@@ -1271,6 +1336,7 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
     }
     if (S->forced_xyz != d_xyz) {
         PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, 0, nullptr, nullptr};
+        errw_before_prepare(S);
         launch_prepare(S->d_desc.p, 1, S->n_pad, plain ? 0 : 1, S->heads.p, none, st);
         CHK(launch_force(S, ForceGuard{nullptr, 0}));
     }
@@ -1279,6 +1345,7 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
     if (plain) launch_pairs14(S->d_desc.p, 1, nblk(S->n14, NB_P14_BLOCK), S->F, st);
     if (S->has_qc) launch_qclj(S->d_desc.p, S->nlj, S->F, st);
     launch_finalize(S->d_desc.p, 1, S->n, plain ? 1 : 0, st);
+    S->err_dirty = false;
     HIPCHK(hipGetLastError());
     return NBABFS_STATUS_CONTINUE;
 }
@@ -1315,7 +1382,7 @@ static int recut_items(nbabfs_state *S, int per_item)
         launch_cut_items(S->icounts.p, S->ieoff.p, S->iioff.p, nb, S->nsc, ipi, 0, S->items.p, st, true);
     }
     HIPCHK(hipGetLastError());
-    S->desc_version = -1;   // the descriptor carries the image item count
+    S->desc_valid = false;   // the descriptor carries the image item count
     return order_items(S);
 }
 
@@ -1381,6 +1448,8 @@ static int build_images(nbabfs_state *S, const double *d_xyz)
         HIPCHK(hipMemcpyAsync(S->h_ibb.p, S->ibb.p, 6 * (size_t)nt * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
+        CHK(check_box(S->h_bbox.p));
+        for (int k = 0; k < nt; k++) CHK(check_box(S->h_ibb.p + 6 * k));
         double lower[3], upper[3];
         {
 #pragma clang fp contract(off)
@@ -1598,7 +1667,6 @@ static int upload_image_xf(nbabfs_state *S)
     HIPCHK(hipMemcpyAsync(S->imgxf.p, S->h_imgxf.p, ns * sizeof(ImageXf), hipMemcpyHostToDevice, S->stream));
     std::memcpy(S->Mxf, S->M, sizeof(S->Mxf));
     S->xf_valid = true;
-    g_alloc_version++;   // descriptors pick up imgxf / gacc
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -1642,6 +1710,7 @@ static int autotune(nbabfs_state *S, const double *d_xyz, int *best_epi, int *be
     CHK(ensure_call_buffers(S));
     CHK(ensure_desc(S, d_xyz, S->last_grad));
     PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, 0, nullptr, nullptr};
+    errw_before_prepare(S);
     launch_prepare(S->d_desc.p, 1, S->n_pad, 1, S->heads.p, none, st);
     hipEvent_t ev[2 * reps];
     for (int k = 0; k < 2 * reps; k++) HIPCHK(hipEventCreate(&ev[k]));
@@ -1762,6 +1831,14 @@ extern "C" int nbabfs_state_create(const nbabfs_options *opt, int natoms, const 
     if ((nexclusions > 0 && !exclusions) || (ninteractions14 > 0 && !interactions14) || nexclusions < 0 ||
         ninteractions14 < 0)
         return fail(NBABFS_STATUS_INVALID_ARGUMENT, "bad exclusion or 1-4 arrays");
+    // Non-finite charges or LJ parameters would reach the energies unnoticed (the pair
+    // kernels report non-finite coordinates and out-of-range sums, not parameters).
+    for (int i = 0; i < natoms; i++)
+        if (!std::isfinite(charges[i])) return fail(NBABFS_STATUS_VALUE_ERROR, "charge of atom %d is not finite", i);
+    for (size_t k = 0; k < (size_t)ntypes * (ntypes + 1) / 2; k++)
+        if (!std::isfinite(tableA[k]) || !std::isfinite(tableB[k]) || (tableA14 && !std::isfinite(tableA14[k])) ||
+            (tableB14 && !std::isfinite(tableB14[k])))
+            return fail(NBABFS_STATUS_VALUE_ERROR, "LJ table entry %zu is not finite", k);
 
     nbabfs_state *S = new nbabfs_state();
     S->opt = o;
@@ -1882,8 +1959,10 @@ extern "C" int nbabfs_state_create(const nbabfs_options *opt, int natoms, const 
         (rc = S->ljA14.upload(A14.data(), nt2, st)) != NBABFS_STATUS_CONTINUE ||
         (rc = S->ljB14.upload(B14.data(), nt2, st)) != NBABFS_STATUS_CONTINUE ||
         (rc = S->xptr.upload(xptr.data(), xptr.size(), st)) != NBABFS_STATUS_CONTINUE ||
-        (rc = S->xidx.upload(xidx.data(), xidx.size(), st)) != NBABFS_STATUS_CONTINUE)
+        (rc = S->xidx.upload(xidx.data(), xidx.size(), st)) != NBABFS_STATUS_CONTINUE ||
+        (rc = S->errw.ensure(1)) != NBABFS_STATUS_CONTINUE)
         return bail(rc);
+    launch_fill_int(S->errw.p, 1, 0, st);
     if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(NBABFS_STATUS_LOGIC_ERROR, "upload failed"));
     *out = S;
     return NBABFS_STATUS_CONTINUE;
@@ -1928,16 +2007,17 @@ extern "C" int nbabfs_mmmm_energy_async(nbabfs_state *S, const double *d_xyz, do
 extern "C" int nbabfs_energies(nbabfs_state *S, double *energies, int *present)
 {
     if (!S || !energies) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
-    CHK(S->h_energies.ensure(4));
+    CHK(S->h_energies.ensure(NB_NENERGIES));
     CHK(S->energies.ensure(NB_NENERGIES));
-    HIPCHK(hipMemcpyAsync(S->h_energies.p, S->energies.p, 4 * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    HIPCHK(hipMemcpyAsync(S->h_energies.p, S->energies.p, NB_NENERGIES * sizeof(double), hipMemcpyDeviceToHost,
+                          S->stream));
     HIPCHK(hipStreamSynchronize(S->stream));
     std::memcpy(energies, S->h_energies.p, 4 * sizeof(double));
     if (present) {
         present[0] = S->npairs > 0;
         present[1] = S->n14 > 0;
     }
-    return NBABFS_STATUS_CONTINUE;
+    return call_status((int)S->h_energies.p[NB_ESTATUS], "energy call");
 }
 
 extern "C" int nbabfs_energy_host(nbabfs_state *S, const double *xyz, double *grad_inout, double *energies,
@@ -1977,7 +2057,7 @@ extern "C" int nbabfs_set_gradient_mode(nbabfs_state *S, int mode)
         return fail(NBABFS_STATUS_INVALID_ARGUMENT, "unknown gradient mode %d", mode);
     HIPCHK(hipStreamSynchronize(S->stream));
     S->grad_assign = mode == NBABFS_GRADIENT_ASSIGN ? 1 : 0;
-    S->desc_version = -1;
+    S->desc_valid = false;
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -2113,9 +2193,9 @@ extern "C" int nbabfs_image_terms(nbabfs_state *S, double *energies, double *dEd
     if (!S) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "state is NULL");
     hipStream_t st = S->stream;
     const int ns = (int)S->slots.size();
-    CHK(S->h_energies.ensure(6));
+    CHK(S->h_energies.ensure(NB_NENERGIES));
     CHK(S->energies.ensure(NB_NENERGIES));
-    HIPCHK(hipMemcpyAsync(S->h_energies.p, S->energies.p, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(S->h_energies.p, S->energies.p, NB_NENERGIES * sizeof(double), hipMemcpyDeviceToHost, st));
     if (dEdM && ns > 0) {
         CHK(S->h_gacc.ensure((size_t)NB_GACC_WORDS_PER_IMAGE * ns));
         HIPCHK(hipMemcpyAsync(S->h_gacc.p, S->gacc.p, (size_t)NB_GACC_WORDS_PER_IMAGE * ns * sizeof(long long),
@@ -2127,6 +2207,7 @@ extern "C" int nbabfs_image_terms(nbabfs_state *S, double *energies, double *dEd
         energies[1] = S->h_energies.p[5];
     }
     if (present) *present = S->nimages > 0;
+    CHK(call_status((int)S->h_energies.p[NB_ESTATUS], "energy call"));
     if (!dEdM || ns == 0) return NBABFS_STATUS_CONTINUE;
     const double *o = S->grid.corg;
     for (int k = 0; k < ns; k++) {
@@ -2254,6 +2335,9 @@ extern "C" int nbabfs_memcpy_htod(nbabfs_state *S, void *dst, const void *src, s
         HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S->stream));
         HIPCHK(hipStreamSynchronize(S->stream));
     } else {
+        // The states' streams are non-blocking: a copy on the null stream does
+        // not wait for their kernels, so the device is drained first.
+        HIPCHK(hipDeviceSynchronize());
         HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     }
     return NBABFS_STATUS_CONTINUE;
@@ -2267,6 +2351,8 @@ extern "C" int nbabfs_memcpy_dtoh(nbabfs_state *S, void *dst, const void *src, s
         HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S->stream));
         HIPCHK(hipStreamSynchronize(S->stream));
     } else {
+        // As above: a stateless copy must not overtake work queued on the states' streams.
+        HIPCHK(hipDeviceSynchronize());
         HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     }
     return NBABFS_STATUS_CONTINUE;
@@ -2276,7 +2362,8 @@ extern "C" int nbabfs_virial(nbabfs_state *S, double *virial)
 {
     if (!S || !virial) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
     for (int k = 0; k < 9; k++) virial[k] = 0.0;
-    if (!S->vacc.p || S->n == 0 || !S->listValid || S->desc_version < 0) return NBABFS_STATUS_CONTINUE;
+    // Nothing to report unless the accumulators and the descriptor are those of the last energy call.
+    if (!S->vacc.p || S->n == 0 || !S->listValid || !desc_current(S)) return NBABFS_STATUS_CONTINUE;
     // From the last call's gradient, still in the fixed-point accumulators.
     launch_fill_int((int *)S->vacc.p, 2 * (size_t)NB_VACC_WORDS, 0, S->stream);
     launch_virial(S->d_desc.p, S->n, S->opt.precision == NBABFS_PLAIN_FP64 ? 1 : 0, S->stream);
@@ -2313,6 +2400,7 @@ extern "C" int nbabfs_zero_async(nbabfs_state *S, void *dst, size_t bytes)
     if (!bytes) return NBABFS_STATUS_CONTINUE;
     if (!dst || bytes % 4) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "zero fill needs a 4-byte multiple");
     CHK(nbabfs_init());
+    if (!S) HIPCHK(hipDeviceSynchronize());
     launch_fill_int((int *)dst, bytes / 4, 0, S ? S->stream : nullptr);
     HIPCHK(hipGetLastError());
     if (!S) HIPCHK(hipDeviceSynchronize());
@@ -2344,17 +2432,25 @@ struct nbabfs_batch {
     PinBuf<double> h_energies;
     DBuf<double> energies;
     int nitems = 0, max_npad = 0, max_n = 0, max_n14b = 0;
-    long layout_version = -1;
-    std::vector<const double *> last_xyz;
-    std::vector<double *> last_grad;
+    std::vector<SysDesc> shadow;   // the descriptors of the current layout (graph_valid implies they are current)
+    bool layout_valid = false;
     std::vector<long> last_items;
-    std::vector<int> last_assign;
     hipGraphExec_t gexec = nullptr;
     bool graph_valid = false;
     bool mixed = true;
+    long long n_captures = 0, n_launches = 0, n_layouts = 0;
+    // Timing of the batched pair launch: with timing on, the calls enqueue the
+    // passes directly (no graph) with the runtime's dispatch events around the
+    // pair kernel.
+    bool timing = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double force_ms = 0.0;
+    long long force_launches = 0;
     ~nbabfs_batch()
     {
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -2391,10 +2487,7 @@ extern "C" int nbabfs_batch_create(nbabfs_state *const *states, int nsystems, in
         delete B;
         return fail(NBABFS_STATUS_LOGIC_ERROR, "hipStreamCreate failed");
     }
-    B->last_xyz.assign(nsystems, nullptr);
-    B->last_grad.assign(nsystems, nullptr);
     B->last_items.assign(nsystems, -1);
-    B->last_assign.assign(nsystems, -1);
     *out = B;
     return NBABFS_STATUS_CONTINUE;
 }
@@ -2436,17 +2529,16 @@ static int batch_layout(nbabfs_batch *B, const double *const *d_xyz, double *con
             hipLaunchKernelGGL(k_remap_items, dim3(nblk(s->nitems, 256)), dim3(256), 0, st, s->items.p, s->nitems, k,
                                B->items.p + first);
         first += s->nitems;
-        B->last_xyz[k] = d_xyz[k];
-        B->last_grad[k] = d_grad ? d_grad[k] : nullptr;
         B->last_items[k] = s->items_version;
-        B->last_assign[k] = s->grad_assign;
     }
     HIPCHK(hipMemcpyAsync(B->d_desc.p, B->h_desc.p, ns * sizeof(SysDesc), hipMemcpyHostToDevice, st));
+    B->shadow.assign(B->h_desc.p, B->h_desc.p + ns);
     std::vector<int> qs(9);
     for (int q = 0; q <= 8; q++) qs[q] = (int)((long long)total * q / 8);
     CHK(B->qstarts.upload(qs.data(), 9, st));
     HIPCHK(hipStreamSynchronize(st));
-    B->layout_version = g_alloc_version;
+    B->layout_valid = true;
+    B->n_layouts++;
     if (B->gexec) {
         (void)hipGraphExecDestroy(B->gexec);
         B->gexec = nullptr;
@@ -2455,7 +2547,7 @@ static int batch_layout(nbabfs_batch *B, const double *const *d_xyz, double *con
     return NBABFS_STATUS_CONTINUE;
 }
 
-static int batch_enqueue(nbabfs_batch *B)
+static int batch_enqueue(nbabfs_batch *B, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr)
 {
     hipStream_t st = B->stream;
     const int ns = (int)B->st.size();
@@ -2471,12 +2563,12 @@ static int batch_enqueue(nbabfs_batch *B)
                 maxcount = std::max(maxcount, s->opt.entriesPerItem);
             }
             launch_force_tiled(B->d_desc.p, B->items.p, q, s0->F, n_persistent_blocks(s0->opt, B->nitems), maxnt,
-                               maxcount, ForceGuard{nullptr, 0}, st);
+                               maxcount, ForceGuard{nullptr, 0}, st, false, ev_start, ev_stop);
         }
     } else {
         for (int k = 0; k < ns; k++)
             launch_force_plain(B->d_desc.p, k, B->st[k]->n, B->st[k]->F, B->st[k]->rowptr.p, B->st[k]->cols.p,
-                               ForceGuard{nullptr, 0}, st);
+                               ForceGuard{nullptr, 0}, st, k == 0 ? ev_start : nullptr, k == ns - 1 ? ev_stop : nullptr);
     }
     launch_pairs14(B->d_desc.p, ns, B->max_n14b, s0->F, st);
     launch_finalize(B->d_desc.p, ns, B->max_n, B->mixed ? 0 : 1, st);
@@ -2534,13 +2626,30 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
         s->isNew = false;
         if (rebuilt) rebuilt[k] = need ? 1 : 0;
     }
-    bool relayout = any || B->layout_version != g_alloc_version;
-    for (int k = 0; k < ns && !relayout; k++)
-        relayout = B->last_xyz[k] != d_xyz[k] || B->last_grad[k] != (d_grad ? d_grad[k] : nullptr) ||
-                   B->last_items[k] != B->st[k]->items_version || B->last_assign[k] != B->st[k]->grad_assign;
+    // A new layout (and graph) when some list was rebuilt or re-cut, or when a
+    // system's descriptor (its buffers, sizes or arrays) is not the one laid out.
+    bool relayout = any || !B->layout_valid || (int)B->shadow.size() != ns;
+    for (int k = 0; k < ns && !relayout; k++) {
+        SysDesc D;
+        fill_desc(B->st[k], D, d_xyz[k], d_grad ? d_grad[k] : nullptr);
+        D.energies = B->energies.p + NB_NENERGIES * (size_t)k;
+        relayout = B->last_items[k] != B->st[k]->items_version || std::memcmp(&D, &B->shadow[k], sizeof(SysDesc)) != 0;
+    }
     if (relayout) CHK(batch_layout(B, d_xyz, d_grad));
+    // The batch's prepare and finalize passes report into each state's error
+    // word: clear what an abandoned single-state pass left there.
+    for (auto *s : B->st)
+        if (s->err_dirty) {
+            launch_fill_int(s->errw.p, 1, 0, st);
+            s->err_dirty = false;
+        }
 
-    if (B->use_graph) {
+    const bool timed = B->timing && (B->mixed ? B->nitems > 0 : true);
+    if (timed) {
+        for (auto &e : B->ev)
+            if (!e) HIPCHK(hipEventCreate(&e));
+        CHK(batch_enqueue(B, B->ev[0], B->ev[1]));
+    } else if (B->use_graph) {
         if (!B->graph_valid) {
             hipGraph_t graph = nullptr;
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -2552,14 +2661,54 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
             (void)hipGraphDestroy(graph);
             if (ie != hipSuccess) return fail(NBABFS_STATUS_LOGIC_ERROR, "graph instantiate failed: %s", hipGetErrorString(ie));
             B->graph_valid = true;
+            B->n_captures++;
         }
         HIPCHK(hipGraphLaunch(B->gexec, st));
     } else {
         CHK(batch_enqueue(B));
     }
+    B->n_launches++;
     HIPCHK(hipStreamSynchronize(st));
-    for (int k = 0; k < ns; k++)
+    if (timed) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, B->ev[0], B->ev[1]));
+        B->force_ms += ms;
+        B->force_launches++;
+    }
+    int bad = -1;
+    for (int k = 0; k < ns; k++) {
         std::memcpy(energies + 4 * (size_t)k, B->h_energies.p + NB_NENERGIES * (size_t)k, 4 * sizeof(double));
+        if (bad < 0 && B->h_energies.p[NB_NENERGIES * (size_t)k + NB_ESTATUS] != 0.0) bad = k;
+    }
+    if (bad >= 0) {
+        char what[64];
+        std::snprintf(what, sizeof(what), "batch system %d", bad);
+        return call_status((int)B->h_energies.p[NB_NENERGIES * (size_t)bad + NB_ESTATUS], what);
+    }
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_batch_info(nbabfs_batch *B, long long *graph_captures, long long *launches, long long *layouts,
+                                 double *force_kernel_ms, long long *timed_launches)
+{
+    if (!B) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "batch is NULL");
+    if (graph_captures) *graph_captures = B->n_captures;
+    if (launches) *launches = B->n_launches;
+    if (layouts) *layouts = B->n_layouts;
+    if (force_kernel_ms) *force_kernel_ms = B->force_launches ? B->force_ms / (double)B->force_launches : 0.0;
+    if (timed_launches) *timed_launches = B->force_launches;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_batch_set_timing(nbabfs_batch *B, int enable)
+{
+    if (!B) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "batch is NULL");
+    HIPCHK(hipStreamSynchronize(B->stream));
+    B->timing = enable != 0;
+    if (B->timing) {
+        B->force_ms = 0.0;
+        B->force_launches = 0;
+    }
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -2752,7 +2901,7 @@ extern "C" int nbabfs_set_qc_atoms(nbabfs_state *S, int nqc, const int *index, c
     for (auto &r : l14) S->n_lj14 += (long long)r.size();
     HIPCHK(hipStreamSynchronize(st));
     S->has_qc = true;
-    S->desc_version = -1;
+    S->desc_valid = false;
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -2896,7 +3045,7 @@ static int build_qcmm_lists(nbabfs_state *S, const double *d_xyz)
     }
     S->sumQCMMElPairs += (double)S->n_el;
     S->sumQCMMLJPairs += (double)S->n_lj;
-    S->desc_version = -1;
+    S->desc_valid = false;
     return NBABFS_STATUS_CONTINUE;
 }
 

@@ -119,6 +119,7 @@ struct SysDesc {
     const int *qlj_atom, *qlj_ptr, *qlj_idx, *qlj14_ptr, *qlj14_idx;
     double *qljpart;            // 2 per (row, chunk) (QC/MM LJ, QC/MM 1-4 LJ), summed in that order
     int grad_assign;            // 1: the finalize pass overwrites grad instead of adding to it
+    int *err;                   // error word (NB_ERR_* bits) of the current call; cleared by the finalize pass
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -145,28 +146,49 @@ __device__ __forceinline__ long long rint_i64(double x)
 
 // v = hi*2^16 + lo*2^-32 with hi = round(v/2^16) and the remainder |r| <= 2^15
 // (exact) stored as round(r*2^32).  |v| is clamped to 2^66, far beyond any
-// physical gradient or energy, which also maps NaN to a finite value.
-__device__ __forceinline__ void fx_split(double v, long long &hi, long long &lo)
+// physical gradient or energy, and NaN is mapped to a finite value, so the
+// integer sums stay defined; the return value says that v was outside
+// (-2^66, 2^66) or NaN, and the callers raise the system's error word
+// (NB_ERR_RANGE) so that the energy call reports it instead of a wrong value.
+__device__ __forceinline__ bool fx_split(double v, long long &hi, long long &lo)
 {
     const double magic = 6755399441055744.0;
+    const bool bad = !(fabs(v) < 0x1p66);
     v = fmin(fmax(v, -0x1p66), 0x1p66);
     const double t = v * (1.0 / NB_FX_HI_UNIT) + magic;
     hi = __double_as_longlong(t) - __double_as_longlong(magic);
     const double r = fma(-(t - magic), NB_FX_HI_UNIT, v);
     lo = rint_i64(r * NB_FX_LO_SCALE);
+    return bad;
 }
 
-__device__ __forceinline__ void fx_add(long long *lo_word, long long *hi_word, double v)
+__device__ __forceinline__ bool fx_add(long long *lo_word, long long *hi_word, double v)
 {
     long long hi, lo;
-    fx_split(v, hi, lo);
+    const bool bad = fx_split(v, hi, lo);
     if (lo != 0) __hip_atomic_fetch_add(as_global(lo_word), lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (hi != 0) __hip_atomic_fetch_add(as_global(hi_word), hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return bad;
 }
 
-__device__ __forceinline__ void fx_atomic_add(long long *facc, int n_pad, int slot, int comp, double v)
+__device__ __forceinline__ bool fx_atomic_add(long long *facc, int n_pad, int slot, int comp, double v)
 {
-    fx_add(&facc[(size_t)comp * n_pad + slot], &facc[(size_t)(comp + 3) * n_pad + slot], v);
+    return fx_add(&facc[(size_t)comp * n_pad + slot], &facc[(size_t)(comp + 3) * n_pad + slot], v);
+}
+
+// Error word of a system (SysDesc::err), raised by the per-call kernels and
+// reported by the energy calls (then cleared by the finalize pass):
+// NB_ERR_COORD -- a coordinate is NaN or infinite; NB_ERR_RANGE -- a
+// gradient or energy contribution was NaN or beyond the fixed-point range.
+#define NB_ERR_COORD 1
+#define NB_ERR_RANGE 2
+
+// One wave-level report: the first lane of a wave with any flagged lane ORs
+// the bits into *err (a vector atomic; every system owns one word).
+__device__ __forceinline__ void raise_error(int *err, bool flagged, int bits)
+{
+    if (err && __any(flagged) && (threadIdx.x & 63) == 0)
+        __hip_atomic_fetch_or(as_global(err), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ double fx_value(const long long *facc, int n_pad, int slot, int comp)
@@ -175,9 +197,10 @@ __device__ __forceinline__ double fx_value(const long long *facc, int n_pad, int
            (double)facc[(size_t)comp * n_pad + slot] * (1.0 / NB_FX_LO_SCALE);
 }
 
-// Energies of one system: MM/MM Elect., LJ, 1-4 Elect., 1-4 LJ, Image Elect., Image LJ, QC/MM LJ,
-// QC/MM 1-4 LJ.
-#define NB_NENERGIES 8
+// Outputs of one system: MM/MM Elect., LJ, 1-4 Elect., 1-4 LJ, Image Elect., Image LJ, QC/MM LJ,
+// QC/MM 1-4 LJ, then the call's error bits (NB_ERR_*, as a double) at NB_ESTATUS.
+#define NB_NENERGIES 9
+#define NB_ESTATUS 8
 
 // Energy accumulators: [kind][lo word, hi word][slot], kinds 0 = Coulomb and
 // 1 = LJ of the main list, 2 and 3 of the image lists; a work item adds into

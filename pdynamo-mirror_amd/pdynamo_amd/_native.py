@@ -28,7 +28,7 @@ EXPORTED = ["nbabfs_default_options", "nbabfs_init", "nbabfs_last_error", "nbabf
             "nbabfs_images", "nbabfs_export_image_pairs", "nbabfs_set_qc_atoms", "nbabfs_set_qcmm_spline",
             "nbabfs_qcmm_terms", "nbabfs_qcmm_potentials", "nbabfs_qcmm_gradients_host", "nbabfs_export_qcmm_pairs",
             "nbabfs_qcmm_sites", "nbabfs_set_gradient_mode", "nbabfs_qcmm_potentials_async",
-            "nbabfs_qcmm_gradients_async"]
+            "nbabfs_qcmm_gradients_async", "nbabfs_batch_info", "nbabfs_batch_set_timing"]
 
 
 class CLibraryError(Exception):
@@ -118,6 +118,9 @@ def load():
     L.nbabfs_set_gradient_mode.argtypes = [vp, C.c_int]
     L.nbabfs_qcmm_potentials_async.argtypes = [vp, vp]
     L.nbabfs_qcmm_gradients_async.argtypes = [vp, vp, vp]
+    L.nbabfs_batch_info.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+    L.nbabfs_batch_set_timing.argtypes = [vp, C.c_int]
     _lib = L
     return L
 
@@ -447,6 +450,22 @@ class NativeBatch:
         rebuilt = np.zeros(ns, np.int32)
         check(load().nbabfs_batch_energy(self.h, xa, ga, _ptr(e), _ptr(rebuilt)))
         return e.reshape(ns, 4), rebuilt.astype(bool)
+
+    def info(self):
+        """{"graph_captures", "launches", "layouts"} of the batch so far."""
+        c, l, r = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        check(load().nbabfs_batch_info(self.h, C.byref(c), C.byref(l), C.byref(r), None, None))
+        return {"graph_captures": c.value, "launches": l.value, "layouts": r.value}
+
+    def set_timing(self, enable):
+        """Time the batched pair launch with dispatch events (calls run without the graph meanwhile)."""
+        check(load().nbabfs_batch_set_timing(self.h, 1 if enable else 0))
+
+    def kernel_timing(self):
+        """(mean ms of the batched pair launch, timed launches) since timing was enabled."""
+        ms, k = C.c_double(), C.c_longlong()
+        check(load().nbabfs_batch_info(self.h, None, None, None, C.byref(ms), C.byref(k)))
+        return ms.value, k.value
 
     def close(self):
         if getattr(self, "h", None):

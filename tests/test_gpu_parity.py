@@ -1,12 +1,16 @@
 """Parity of the HIP path (through the C ABI) with the FP64 oracle.
 
-Tolerances: mixed precision -- relative total energy <= 1e-6 and relative RMS
-gradient <= 1e-5 (per term: |dE| <= 1e-6 * |E_total|); plain FP64 -- <= 1e-12.
-Pair sets are compared bit for bit as sorted canonical keys max*n+min.
+Tolerances (parity_checks.check): mixed precision -- every energy term within
+1e-6 of its own magnitude (|dE_k| <= 1e-6 * max(|E_k|, 1 kJ/mol)), the total
+likewise, relative RMS gradient <= 1e-5 over all atoms and over the atoms in no
+overlapping pair; plain FP64 -- the same with 1e-12.  Pair sets are compared
+bit for bit as sorted canonical keys max*n+min.
 """
 
 import numpy as np
 import pytest
+
+from parity_checks import check
 
 pytestmark = pytest.mark.gpu
 
@@ -26,16 +30,8 @@ def _run(s, xyz=None):
     return e, terms, s.configuration.gradients3.copy()
 
 
-def _check(terms, grad, rterms, rgrad, etol, gtol):
-    rterms = dict(rterms)
-    assert set(terms) == set(rterms)
-    etot = sum(rterms.values())
-    scale = max(abs(etot), 1.0)
-    assert abs(sum(terms.values()) - etot) <= etol * scale
-    for k in rterms:
-        assert abs(terms[k] - rterms[k]) <= etol * scale, (k, terms[k], rterms[k])
-    rel = np.linalg.norm(grad - rgrad) / max(np.linalg.norm(rgrad), 1e-300)
-    assert rel <= gtol, rel
+def _check(terms, grad, rterms, rgrad, etol, gtol, label="", scales=None):
+    check(terms, grad, rterms, rgrad, etol, gtol, label, scales)
 
 
 @pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("fp64", 1e-12, 1e-12)])
@@ -47,7 +43,7 @@ def test_water648(oracle_mod, precision, etol, gtol):
     o = oracle_mod.OracleABFS(mm)
     rterms, rgrad, _ = o.energy(mm.coordinates3)
     assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
-    _check(terms, grad, rterms, rgrad, etol, gtol)
+    _check(terms, grad, rterms, rgrad, etol, gtol, "", o.abs_sums(mm.coordinates3))
 
 
 @pytest.mark.parametrize("natoms,box,seed", [(300, 30.0, 1), (900, 45.0, 2), (2000, 60.0, 3), (7, 10.0, 4)])
@@ -60,7 +56,7 @@ def test_random_mixtures(oracle_mod, natoms, box, seed, precision, etol, gtol):
     o = oracle_mod.OracleABFS(mm)
     rterms, rgrad, _ = o.energy(mm.coordinates3)
     assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
-    _check(terms, grad, rterms, rgrad, etol, gtol)
+    _check(terms, grad, rterms, rgrad, etol, gtol, "", o.abs_sums(mm.coordinates3))
 
 
 def _shaped(shape):
@@ -100,7 +96,7 @@ def test_extreme_shapes(oracle_mod, shape, precision, etol, gtol):
     _, terms, grad = _run(s)
     rterms, rgrad, _ = o.energy(mm.coordinates3)
     assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
-    _check(terms, grad, rterms, rgrad, etol, gtol)
+    _check(terms, grad, rterms, rgrad, etol, gtol, "", o.abs_sums(mm.coordinates3))
 
 
 def test_options_14_dielectric(oracle_mod):
@@ -124,7 +120,55 @@ def test_options_14_dielectric(oracle_mod):
                               scale14=0.5, tableA14=a14, tableB14=b14)
     rterms, rgrad, _ = o.energy(mm.coordinates3)
     assert "MM/MM 1-4 Elect." in terms
-    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)
+    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5, "", o.abs_sums(mm.coordinates3))
+
+
+def _water_with_14(nwater, box, seed):
+    """Waters on a jittered lattice (no overlapping atoms: the Coulomb and 1-4 terms are not small beside the
+    LJ term) plus intermolecular "1-4" pairs between atoms 2.5-5 A apart, excluded from the main list as a
+    force field's 1-4 pairs are, and a 1-4 LJ table of its own (H types with non-zero 1-4 parameters)."""
+    from pdynamo_amd import synthetic
+    mm = synthetic.water_lattice(nwater, box, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    x = mm.coordinates3
+    i = rng.integers(0, mm.natoms, 40 * nwater)
+    j = rng.integers(0, mm.natoms, 40 * nwater)
+    d = np.linalg.norm(x[i] - x[j], axis=1)
+    keep = (i // 3 != j // 3) & (d > 2.5) & (d < 5.0)
+    p14 = np.unique(np.stack([np.maximum(i, j), np.minimum(i, j)], 1)[keep], axis=0)[: nwater // 2]
+    mm.interactions14 = p14.astype(np.int32)
+    mm.exclusions = np.concatenate([mm.exclusions, mm.interactions14]).astype(np.int32)
+    a14 = mm.tableA * 0.5
+    b14 = mm.tableB * 0.75
+    a14[1:] = [2.0e3, 1.5e2]            # (H,O) and (H,H) 1-4 rows only
+    b14[1:] = [6.0e1, 8.0e0]
+    return mm, a14, b14
+
+
+@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("fp64", 1e-12, 1e-11)])
+def test_options_14_dielectric_water(oracle_mod, precision, etol, gtol):
+    """Dielectric, electrostaticScale14, a 1-4 LJ table of its own and non-default cutoffs on an overlap-free
+    water lattice, so that the Coulomb, 1-4 Coulomb and 1-4 LJ terms are each checked against their own
+    magnitude (NBModelABFS.c:228-301).  The plain path's gradient bound is 1e-11 here: its 1-4 gradients go
+    through the fixed-point sums, which round each contribution by up to 2^-33 kJ/mol/A (1.2e-10), a few
+    1e-12 of this system's gradients (38 kJ/mol/A RMS per atom at dielectric 2.5)."""
+    from pdynamo_amd import LJParameterContainer_FromTableCoefficients, MMAtomContainer, NBModelABFS, System
+    mm, a14, b14 = _water_with_14(1000, 31.0, 41)
+    lj = LJParameterContainer_FromTableCoefficients(mm.ntypes, mm.tableA, mm.tableB)
+    lj14 = LJParameterContainer_FromTableCoefficients(mm.ntypes, a14, b14)
+    s = System.FromMMParameters(mm.coordinates3, MMAtomContainer(mm.charges, mm.ljtypes), lj, mm.exclusions,
+                                mm.interactions14, lj14, electrostaticScale14=0.5)
+    s.DefineNBModel(NBModelABFS(dielectric=2.5, electrostaticScale14=0.5, innerCutoff=7.0, outerCutoff=11.0,
+                                listCutoff=12.5, dampingCutoff=0.75, precision=precision))
+    _, terms, grad = _run(s)
+    o = oracle_mod.OracleABFS(mm, damping=0.75, inner=7.0, outer=11.0, list_cutoff=12.5, dielectric=2.5,
+                              scale14=0.5, tableA14=a14, tableB14=b14)
+    rterms, rgrad, _ = o.energy(mm.coordinates3)
+    r = dict(rterms)
+    assert all(abs(r[k]) > 1.0 for k in ("MM/MM Elect.", "MM/MM 1-4 Elect.", "MM/MM 1-4 LJ")), r
+    assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
+    _check(terms, grad, rterms, rgrad, etol, gtol, "water1000 options " + precision,
+           o.abs_sums(mm.coordinates3))
 
 
 def test_rebuild_sequence(oracle_mod):
@@ -139,7 +183,7 @@ def test_rebuild_sequence(oracle_mod):
         _, terms, grad = _run(s, xyz)
         rterms, rgrad, rb = o.energy(xyz)
         flags.append((s.configuration.nbState.lastRebuilt, rb))
-        _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)
+        _check(terms, grad, rterms, rgrad, 1e-6, 1e-5, "", o.abs_sums(xyz))
         assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
     assert all(a == b for a, b in flags), flags
     assert 2 <= sum(b for _, b in flags) < len(flags)
@@ -169,7 +213,7 @@ def test_cell_size_invariance(oracle_mod, factor):
     o = oracle_mod.OracleABFS(mm, cell_factor=factor)
     rterms, rgrad, _ = o.energy(mm.coordinates3)
     assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
-    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)
+    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5, "", o.abs_sums(mm.coordinates3))
 
 
 def test_solute30000(oracle_mod):
@@ -181,7 +225,7 @@ def test_solute30000(oracle_mod):
     o = oracle_mod.OracleABFS(mm, nthreads=8)
     rterms, rgrad, _ = o.energy(mm.coordinates3)
     assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
-    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)
+    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5, "", o.abs_sums(mm.coordinates3))
 
 
 def test_batch_matches_single():
@@ -380,7 +424,7 @@ def test_autotune_keeps_the_list_and_parity(oracle_mod):
     assert np.array_equal(s.configuration.nbState.Pairs(), pairs0)
     o = oracle_mod.OracleABFS(mm)
     rterms, rgrad, _ = o.energy(mm.coordinates3)
-    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5)
+    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5, "", o.abs_sums(mm.coordinates3))
 
 
 @pytest.mark.parametrize("precision,tol", [("mixed", 2e-5), ("fp64", 1e-7)])
@@ -483,4 +527,4 @@ def test_randomized_configurations(oracle_mod, seed):
         print("seed", seed, "n", n, precision, flush=True)
         _, terms, grad = _run(s)
         assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
-        _check(terms, grad, rterms, rgrad, etol, gtol)
+        _check(terms, grad, rterms, rgrad, etol, gtol, "", o.abs_sums(mm.coordinates3))

@@ -451,3 +451,31 @@ def test_zero_extent_grid_is_invalid_argument(oracle_mod):
             else:
                 r.energy(mm.coordinates3)
                 assert np.array_equal(r.canonical(), o.records.canonical())
+
+
+def test_abs_sums_brute_force(oracle_mod):
+    """The pair-magnitude sums behind the FP64 reordering tolerance of the GPU checks: sum |e_el| and sum |e_lj|
+    over the list equal a pure-Python loop over the pairs with the oracle's own pair terms, and bound the
+    totals (|E| <= sum |e|, equal for the 1-4 LJ of a same-sign set)."""
+    from pdynamo_amd import synthetic
+    mm = synthetic.random_small_system(60, 15.0, 3)
+    mm.interactions14 = np.array([[5, 1], [9, 2], [30, 4]], np.int32)
+    o = oracle_mod.OracleABFS(mm, scale14=0.5)
+    terms, _, _ = o.energy(mm.coordinates3)
+    s = o.abs_sums(mm.coordinates3)
+    f = oracle_mod.factors()
+    tidx = oracle_mod.full_table(mm.ntypes, mm.tableA, mm.tableB)
+    for (i, j), scale, keys in (((o.records.pairs()), 1.0, ("MM/MM Elect.", "MM/MM LJ")),
+                                ((o.rec14.pairs()), 0.5, ("MM/MM 1-4 Elect.", "MM/MM 1-4 LJ"))):
+        a1 = a2 = 0.0
+        for a, b in zip(i, j):
+            r2 = float(((mm.coordinates3[a] - mm.coordinates3[b]) ** 2).sum())
+            q = scale * oracle_mod.coulomb_factor() * mm.charges[a] * mm.charges[b]
+            k = tidx[mm.ljtypes[a] * mm.ntypes + mm.ljtypes[b]]
+            r = oracle_mod.pair_terms(f, r2, q, mm.tableA[k], mm.tableB[k])
+            if r:
+                a1 += abs(r[0])
+                a2 += abs(r[1])
+        assert s[keys[0]] == pytest.approx(a1, rel=1e-13) and s[keys[1]] == pytest.approx(a2, rel=1e-13)
+        t = dict(terms)
+        assert abs(t[keys[0]]) <= s[keys[0]] * (1 + 1e-14) and abs(t[keys[1]]) <= s[keys[1]] * (1 + 1e-14)
