@@ -8,8 +8,6 @@ pBabel-1.9.0/pBabel/CHARMMPSFFileReader.py, CHARMMParameterFileReader.py and
 XYZFileReader.py; the bonded MM terms are outside this build.
 """
 
-import itertools
-
 import numpy as np
 
 from .Parameters import LJParameterContainer_FromEpsilonSigma, MMAtomContainer, SelfPairList
@@ -33,7 +31,8 @@ _PSFIntegersPerRecord = {"Bonds": 2, "Angles": 3, "Dihedrals": 4, "Impropers": 4
 
 
 class CHARMMParameterFileReader:
-    """The NONBONDED section of a CHARMM parameter file (CHARMMParameterFileReader.py:76-160,315-340)."""
+    """The NONBONDED section of a CHARMM parameter file (the behaviour of CHARMMParameterFileReader.py:76-160,
+    315-345)."""
 
     def __init__(self, path):
         self.path = path
@@ -73,6 +72,39 @@ class CHARMMParameterFileReader:
                 return name, (tokens[1] if len(tokens) > 1 else "")
         return None, line
 
+    @staticmethod
+    def _triple(tokens, first):
+        """The three numbers (ignored, emin, rmin/2) of a record starting at tokens[first], or None."""
+        if len(tokens) < first + 3:
+            return None
+        try:
+            return tuple(float(v) for v in tokens[first:first + 3])
+        except ValueError:
+            return None
+
+    def _store_nonbond(self, tokens, conv):
+        """One NONBONDED record, "type ignored emin rmin/2 [ignored emin14 rmin14/2]" (kcal/mol, A): the type's
+        Lennard-Jones epsilon |emin| (in kJ/mol when converting) and sigma = 2 rmin/2 of the AMBER form, for the
+        normal and, when given, the 1-4 interaction (CHARMMParameterFileReader.py:315-345).  Types with a
+        wildcard character are kept apart; the first record of a type wins; a non-zero first column or a
+        positive emin is not a Lennard-Jones record of this form and is skipped with a warning."""
+        key = tokens[0].upper()
+        wild = any(c in key for c in _NonBondWildCards)
+        parts = [("nonbond", self._triple(tokens, 1), self.nonbondwilds if wild else self.nonbonds)]
+        if len(tokens) > 4:
+            parts.append(("1-4 nonbond", self._triple(tokens, 4), self.nonbond14wilds if wild else self.nonbond14s))
+        for what, values, table in parts:
+            if values is None:
+                self.warnings.append("Malformed %s record: %s." % (what, " ".join(tokens)))
+                break
+            ignored, emin, rhalf = values
+            if key in table:
+                self.warnings.append("Repeated %s record for type %s (the first is kept)." % (what, key))
+            elif ignored != 0.0 or emin > 0.0:
+                self.warnings.append("The %s record of type %s is not of the epsilon/Rmin form." % (what, key))
+            else:
+                table[key] = (abs(emin) * conv, 2.0 * rhalf)
+
     def Parse(self, convertUnits=True):
         if self.QPARSED:
             return self
@@ -87,29 +119,8 @@ class CHARMMParameterFileReader:
                 if section == "End":
                     break
                 current = section
-        for datum in body:
-            try:
-                t = datum[0].upper()
-                wild = any(w in t for w in _NonBondWildCards)
-                increment = 0
-                for normal, wilds, tag in ((self.nonbonds, self.nonbondwilds, "nonbond"),
-                                           (self.nonbond14s, self.nonbond14wilds, "1-4 nonbond")):
-                    target = wilds if wild else normal
-                    if t in target:
-                        self.warnings.append("Duplicate %s for %s." % (tag, t))
-                    else:
-                        p = float(datum[increment + 1])
-                        e = float(datum[increment + 2]) * conv
-                        r = float(datum[increment + 3]) * 2.0
-                        if p != 0.0 or e > 0.0:
-                            self.warnings.append("Unable to process non-epsilon/sigma %s for %s." % (tag, t))
-                        else:
-                            target[t] = (abs(e), r)
-                    if len(datum) <= 4:
-                        break
-                    increment += 3
-            except (IndexError, ValueError):
-                self.warnings.append("Unable to process nonbond: " + " ".join(datum) + ".")
+        for tokens in body:
+            self._store_nonbond(tokens, conv)
         self.QPARSED = True
         return self
 
@@ -176,44 +187,53 @@ class CHARMMPSFFileReader:
         index = {t: i for i, t in enumerate(self.uniqueAtomTypes)}
         return MMAtomContainer(np.array([q for _, q in self.atoms]), np.array([index[t] for t in types], np.int32))
 
+    def _explicit_exclusions(self, natoms):
+        """The PSF's NNB section as (i > j) pairs.  Atom i takes the first INB[i] entries of the excluded-atom
+        list, the reading pBabel's PSF reader gives the INB array (CHARMMPSFFileReader.py:367-372)."""
+        pairs = set()
+        if self.counters.get("Exclusions", 0) > 0:
+            for i, end in enumerate(self.exclusionsI[:natoms]):
+                for j in self.exclusionsJ[:end]:
+                    if j != i:
+                        pairs.add((max(i, j), min(i, j)))
+        return pairs
+
     def ToExclusionPairLists(self):
-        """Exclusions (1-2, 1-3, explicit, and every 1-4 candidate) and the 1-4 list, from the bonds as in CHARMM
-        (CHARMMPSFFileReader.py:335-396).  The explicit-exclusion loop keeps the reference's reading of the
-        INB array (each atom takes the J entries up to its INB value, counted from the start)."""
+        """Exclusions and 1-4 pairs from the bond graph, by CHARMM's rule (the behaviour of
+        CHARMMPSFFileReader.py:334-396): two atoms joined by a chain of at most three bonds are excluded from
+        the main list, and so are the PSF's explicit exclusions; the pairs whose shortest chain has exactly
+        three bonds interact through the 1-4 list, unless explicitly excluded.  Shortest chains come from a
+        breadth-first search of depth three from every atom."""
         natoms = self.counters.get("Atoms", 0)
         nbonds = self.counters.get("Bonds", 0)
-        i12 = set()
+        neighbours = [set() for _ in range(natoms)]
         for n in range(nbonds):
             i, j = self.bonds[2 * n] - 1, self.bonds[2 * n + 1] - 1
-            i12.add((max(i, j), min(i, j)))
-        connections = [[] for _ in range(natoms)]
-        for i, j in i12:
-            connections[i].append(j)
-            connections[j].append(i)
-        for c in connections:
-            c.sort()
-        i13 = set()
-        for j in range(natoms):
-            jb = connections[j]
-            for i in range(1, len(jb)):
-                for k in range(i):
-                    i13.add((jb[i], jb[k]))
-        i123 = i12 | i13
-        if self.counters.get("Exclusions", 0) > 0:
-            for i in range(natoms):
-                for j in self.exclusionsJ[0:self.exclusionsI[i]]:
-                    i123.add((max(i, j), min(i, j)))
-        i14p = set()
-        for j, k in i12:
-            for i in connections[j]:
-                if i != k:
-                    for l in connections[k]:
-                        if l != i and l != j:
-                            i14p.add((max(i, l), min(i, l)))
-        i1234 = i123 | i14p
-        i14 = i14p - i123
-        exclusions = SelfPairList(list(itertools.chain(*sorted(i1234)))) if i1234 else None
-        interactions14 = SelfPairList(list(itertools.chain(*sorted(i14)))) if i14 else None
+            if i != j:
+                neighbours[i].add(j)
+                neighbours[j].add(i)
+        near, third = set(), set()
+        for a in range(natoms):
+            seen = {a}
+            shell = [a]
+            for depth in (1, 2, 3):
+                nxt = []
+                for u in shell:
+                    for v in neighbours[u]:
+                        if v not in seen:
+                            seen.add(v)
+                            nxt.append(v)
+                shell = nxt
+                for b in shell:
+                    if b < a:                      # each pair once, from its larger atom
+                        near.add((a, b))
+                        if depth == 3:
+                            third.add((a, b))
+        explicit = self._explicit_exclusions(natoms)
+        excluded = near | explicit
+        onefour = third - explicit
+        exclusions = SelfPairList([k for pair in sorted(excluded) for k in pair]) if excluded else None
+        interactions14 = SelfPairList([k for pair in sorted(onefour) for k in pair]) if onefour else None
         return exclusions, interactions14
 
     def ToLJParameterContainers(self, parameters):

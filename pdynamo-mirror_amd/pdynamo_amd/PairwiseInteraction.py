@@ -1,9 +1,10 @@
 """``PairwiseInteractionABFS``: the QC/MM and QC/QC electrostatic interactions of the ABFS model.
 
 Options, validation and defaults as pMolecule-1.9.0/extensions/pyrex/pMolecule.PairwiseInteraction.pyx:107-240;
-the electrostatic spline ordinates as its ``MakeElectrostaticSplines`` / ``Integrate`` (pyx:24-105,182-194)
-with the adaptive Simpson rule of pCore-1.9.0/pCore/NumericalIntegration.py:21-70.  Like the
-reference, the ordinates are made here (host code) and the cubic spline itself is built by the
+the electrostatic spline ordinates as its ``MakeElectrostaticSplines`` (pyx:24-105,182-194): the same
+function, switch and grid, with the switched-region integrals done by a panelled Gauss-Legendre rule
+(the reference uses an adaptive Simpson rule; both are converged to the FP64 rounding of the ordinates).
+Like the reference, the ordinates are made here (host code) and the cubic spline itself is built by the
 native library (``nbabfs_set_qcmm_spline``, CubicSpline_MakeFromReal1DArrays with zero end slopes).
 
 Only the Gaussian models need a spline on this path: NBModelABFS's QC/MM interaction is
@@ -11,7 +12,6 @@ Only the Gaussian models need a spline on this path: NBModelABFS's QC/MM interac
 The MM/MM interaction uses the analytic form.
 """
 
-import heapq
 import math
 
 import numpy as np
@@ -57,81 +57,70 @@ def _switch(r, r_on, r_off):
     return (r_off ** 2 - r ** 2) ** 2 * (r_off ** 2 + 2.0 * r ** 2 - 3.0 * r_on ** 2) / (r_off ** 2 - r_on ** 2) ** 3
 
 
-def AdaptiveSimpsonsRule(F, a, b, tolerance, maximumIterations):
-    """Integral of F over [a, b]: intervals refined largest-error first from a heap until each
-    meets its share of the tolerance or the iterations run out.  Returns (integral, state)."""
-    width = math.fabs(a - b)
-    per_length = tolerance / width
+# Gauss-Legendre rule of 10 points on [-1, 1] (Golub-Welsch nodes and weights, exact for polynomials of
+# degree 19).  The integrand of the switched region is analytic, and each spline interval (1/50 A by
+# default) is split into two panels, so the rule's error is far below the FP64 rounding of the sums.
+_GL_NODES, _GL_WEIGHTS = np.polynomial.legendre.leggauss(10)
+_GL_PANELS = 2
+
+
+def _gauss_legendre(f, a, b):
+    """Integral of f over [a, b] (a <= b) by the panelled Gauss-Legendre rule."""
     total = 0.0
-    heap = []
-    count = 0
-    mid = 0.5 * (a + b)
-    fa, fm, fb = F(a), F(mid), F(b)
-    heapq.heappush(heap, (0.0, width, a, mid, b, fa, fm, fb, (fa + 4.0 * fm + fb) * width / 6.0))
-    while count < maximumIterations and heap:
-        _, w, x0, x2, x4, f0, f2, f4, whole = heapq.heappop(heap)
-        x1 = 0.5 * (x0 + x2)
-        x3 = 0.5 * (x2 + x4)
-        f1, f3 = F(x1), F(x3)
-        left = (f0 + 4.0 * f1 + f2) * w / 12.0
-        right = (f2 + 4.0 * f3 + f4) * w / 12.0
-        excess = left + right - whole
-        err = math.fabs(excess)
-        if err < 15.0 * w * per_length:
-            total += left + right + excess / 15.0
-        else:
-            w *= 0.5
-            heapq.heappush(heap, (-err, w, x0, x1, x2, f0, f1, f2, left))
-            heapq.heappush(heap, (-err, w, x2, x3, x4, f2, f3, f4, right))
-        count += 1
-    state = {"Iterations": count, "Is Converged": not heap}
-    if heap:
-        worst = 0.0
-        for (e, w, _, _, _, _, _, _, whole) in heap:
-            total += whole
-            worst = max(worst, math.fabs(e) * width / (30.0 * w))
-        state["Error"] = worst
-    if a > b:
-        total *= -1.0
-    return total, state
+    width = (b - a) / _GL_PANELS
+    for k in range(_GL_PANELS):
+        lo = a + k * width
+        half = 0.5 * width
+        mid = lo + half
+        total += half * math.fsum(w * f(mid + half * t) for t, w in zip(_GL_NODES, _GL_WEIGHTS))
+    return total
 
 
-def Integrate(F, G, rDamp, rOn, rOff, epsilon, maximumIterations, pointDensity):
-    """Spline ordinates of F on [0, rOff] (pyx:57-105): in the switching region minus the integral of
-    G S from r to rOff, below rOn F plus the offset that keeps it continuous, below rDamp a parabola.
-    Returns (x = r^2, y)."""
+def spline_abscissae(rOff, pointDensity):
+    """The spline's radii: ceil(density * rOff + 1) points, evenly spaced from 0, the last exactly rOff (the
+    layout of the reference's spline arrays, pMolecule.PairwiseInteraction.pyx:57-69)."""
     n = int(math.ceil(float(pointDensity) * rOff + 1.0))
-    r = np.zeros(n)
-    y = np.zeros(n)
     step = rOff / float(n - 1)
-    for i in range(1, n - 1):
-        r[i] = step * float(i)
+    r = step * np.arange(n, dtype=np.float64)
     r[-1] = rOff
+    return r
+
+
+def Integrate(F, G, rDamp, rOn, rOff, pointDensity):
+    """Spline ordinates (x = r^2, y) of a potential F with derivative G, force-switched between rOn and rOff
+    (the construction of pMolecule.PairwiseInteraction.pyx:57-105):
+
+    * rOn <= r <= rOff: y(r) = -int_r^rOff G(t) S(t) dt, S the force switch, so that y' = G S and y(rOff) = 0;
+    * rDamp <= r < rOn: y(r) = F(r) + c, with c making y continuous at rOn;
+    * r < rDamp: the parabola a r^2 + b whose value and slope match F + c at rDamp.
+
+    The switched-region integrals are summed per interval from rOff inward (each interval by the panelled
+    Gauss-Legendre rule, the running total in FP64)."""
+    r = spline_abscissae(rOff, pointDensity)
+    n = r.size
+    y = np.zeros(n)
 
     def GS(t):
         return G(t) * _switch(t, rOn, rOff)
 
-    acc = 0.0
-    top = n - 2
-    for i in range(n - 2, -1, -1):
-        lo, hi = r[i], r[i + 1]
-        if lo < rOn:
-            top = i
-            if hi > rOn:
-                acc -= AdaptiveSimpsonsRule(GS, rOn, hi, epsilon, maximumIterations)[0]
-            break
-        acc -= AdaptiveSimpsonsRule(GS, lo, hi, epsilon, maximumIterations)[0]
-        y[i] = acc
-    shift = acc - F(rOn)
-    if rDamp > 0:
-        f_d, g_d = F(rDamp) + shift, G(rDamp)
-        alpha, beta = 0.5 * g_d / rDamp, -0.5 * g_d * rDamp + f_d
+    # Interval k = [r_k, r_k+1] contributes its part above rOn.
+    above = [k for k in range(n - 1) if r[k + 1] > rOn]
+    tail = 0.0
+    for k in reversed(above):
+        tail -= _gauss_legendre(GS, max(r[k], rOn), r[k + 1])
+        if r[k] >= rOn:
+            y[k] = tail
+    shift = tail - F(rOn)          # tail is now -int_rOn^rOff G S
+    if rDamp > 0.0:
+        fd, gd = F(rDamp) + shift, G(rDamp)
+        a, b = 0.5 * gd / rDamp, fd - 0.5 * gd * rDamp
     else:
-        alpha = beta = 0.0
-    for i in range(top + 1):
-        t = r[i]
-        y[i] = alpha * t ** 2 + beta if t < rDamp else F(t) + shift
-    return r ** 2, y
+        a = b = 0.0
+    for i in range(n):
+        if r[i] >= rOn:
+            break
+        y[i] = a * r[i] * r[i] + b if r[i] < rDamp else F(r[i]) + shift
+    return r * r, y
 
 
 class PairwiseInteractionABFS:
@@ -200,7 +189,7 @@ class PairwiseInteractionABFS:
         else:
             raise NotImplementedError("the Delta/Delta spline is not used on this path (the analytic form is)")
         x, y = Integrate(_gaussian_potential(p), _gaussian_derivative(p), self.dampingCutoff, self.innerCutoff,
-                         self.outerCutoff, 1.0e-15, 1000, self.splinePointDensity)
+                         self.outerCutoff, self.splinePointDensity)
         if useAtomicUnits:
             y = y * (1.0 / UNITS_LENGTH_ANGSTROMS_TO_BOHRS)
         else:

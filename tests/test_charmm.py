@@ -103,3 +103,53 @@ def test_dhfr_fixture_is_the_readers_output(dhfr):
     assert np.array_equal(nb["exclusions"].pairs, dhfr.mm.exclusions)
     assert np.array_equal(nb["interactions14"].pairs, dhfr.mm.interactions14)
     assert np.array_equal(CHARMM.XYZFile_ToCoordinates3(os.path.join(d, "dhfr.xyz")), dhfr.mm.coordinates3)
+
+
+def test_exclusion_rule_on_rings_matches_path_enumeration():
+    """The shortest-chain rule (excluded within three bonds, 1-4 at exactly three) against a direct enumeration
+    of bond chains: 1-2 bonds, 1-3 pairs sharing a neighbour, 1-4 candidates the ends of every chain of three
+    distinct atoms' bonds; exclusions = all of them, 1-4 pairs = candidates that are neither 1-2 nor 1-3.  Random
+    graphs with 3-, 4- and 5-membered rings, where a pair can be both a 1-3 and the end of a three-bond chain."""
+    from pdynamo_amd import CHARMM
+    rng = np.random.default_rng(8)
+    for trial in range(30):
+        n = int(rng.integers(6, 40))
+        bonds = {(max(i, i + 1), min(i, i + 1)) for i in range(n - 1)}           # a chain ...
+        for _ in range(int(rng.integers(1, 8))):                                # ... closed into small rings
+            i = int(rng.integers(0, n))
+            j = i + int(rng.integers(2, 5))
+            if j < n:
+                bonds.add((j, i))
+        r = CHARMM.CHARMMPSFFileReader.__new__(CHARMM.CHARMMPSFFileReader)
+        r.counters = {"Atoms": n, "Bonds": len(bonds)}
+        r.bonds = [k + 1 for b in sorted(bonds) for k in b]
+        r.exclusionsI, r.exclusionsJ = [], []
+        exc, i14 = r.ToExclusionPairLists()
+        nb = {a: set() for a in range(n)}
+        for i, j in bonds:
+            nb[i].add(j)
+            nb[j].add(i)
+        key = lambda a, b: (max(a, b), min(a, b))
+        i12 = {key(i, j) for i, j in bonds}
+        i13 = {key(a, b) for c in range(n) for a in nb[c] for b in nb[c] if a != b}
+        i14p = {key(a, d) for (b, c) in i12 for a in nb[b] for d in nb[c] if a != c and d != b and a != d}
+        i14p |= {key(a, d) for (c, b) in i12 for a in nb[b] for d in nb[c] if a != c and d != b and a != d}
+        assert {tuple(p) for p in exc.pairs.tolist()} == i12 | i13 | i14p, trial
+        got14 = set() if i14 is None else {tuple(p) for p in i14.pairs.tolist()}
+        assert got14 == i14p - i12 - i13, trial
+
+
+def test_explicit_exclusions_as_the_reference_reads_them():
+    """The NNB section: atom i takes the first INB[i] entries of the excluded-atom list (the reading of the
+    reference's PSF reader); explicit exclusions join the exclusions and leave the 1-4 list."""
+    from pdynamo_amd import CHARMM
+    r = CHARMM.CHARMMPSFFileReader.__new__(CHARMM.CHARMMPSFFileReader)
+    r.counters = {"Atoms": 6, "Bonds": 5, "Exclusions": 2}
+    r.bonds = [2, 1, 3, 2, 4, 3, 5, 4, 6, 5]              # chain 0-1-2-3-4-5 (1-based in the PSF)
+    r.exclusionsJ = [3, 5]                                # 0-based excluded atoms
+    r.exclusionsI = [2, 2, 2, 2, 2, 2]                    # INB per atom: each takes J[0:2]
+    exc, i14 = r.ToExclusionPairLists()
+    within3 = {(i, j) for i in range(6) for j in range(i) if i - j <= 3}
+    assert {tuple(p) for p in exc.pairs.tolist()} == within3 | {(5, 0), (5, 1)}
+    # 1-4 candidates (3, 0), (4, 1), (5, 2): (3, 0) and (5, 2) are explicitly excluded.
+    assert {tuple(p) for p in i14.pairs.tolist()} == {(4, 1)}
