@@ -30,7 +30,8 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6     # MI355X published FP64 vector peak
-DTYPE = {"mixed": "f64 pair core + f32 rsqrt seed/force vectors + int64 fixed-point sums", "fp64": "f64"}
+DTYPE = {"mixed": "f64 pair core + f32 rsqrt seed/force vectors + int64 fixed-point sums",
+         "tiled_fp64": "f64 pair core and force vectors + int64 fixed-point sums", "fp64": "f64"}
 
 
 def algorithmic_flops(xyz, keys, n):
@@ -306,19 +307,22 @@ def run_batch(args, systems, wdesc, workload):
             st.close()
         return out
 
-    other = "fp64" if args.precision == "mixed" else "mixed"
     cmp = {}
     if not args.no_compare:
-        ms_other_batch, kms_other, _, st_other, b_other = timed_batch(other, max(5, args.steps // 5), 2)
-        b_other.close()
-        for st in st_other:
-            st.close()
+        # The same batch in the other arithmetic paths, then the systems as sequential single-state calls.
+        for other in ("tiled_fp64", "fp64", "mixed"):
+            if other == args.precision:
+                continue
+            ms_o, kms_o, _, st_o, b_o = timed_batch(other, max(5, args.steps // 5), 2)
+            b_o.close()
+            for st in st_o:
+                st.close()
+            cmp.update({"batched_%s_ms" % other: ms_o, "pair_kernel_ms_%s" % other: kms_o,
+                        "speedup_vs_batched_%s" % other: ms_o / ms})
         ms_seq_plain = sequential("fp64")
         ms_seq_mixed = sequential("mixed")
-        cmp = {"batched_%s_ms" % other: ms_other_batch, "pair_kernel_ms_%s" % other: kms_other,
-               "sequential_plain_fp64_ms": ms_seq_plain, "sequential_mixed_ms": ms_seq_mixed,
-               "speedup_vs_sequential_plain_fp64": ms_seq_plain / ms,
-               "speedup_vs_batched_%s" % other: ms_other_batch / ms}
+        cmp.update({"sequential_plain_fp64_ms": ms_seq_plain, "sequential_mixed_ms": ms_seq_mixed,
+                    "speedup_vs_sequential_plain_fp64": ms_seq_plain / ms})
     result = {
         "metric": "pair interactions/s (reference list pairs d<=13.5 A per E+grad second)",
         "value": npairs / (ms * 1e-3), "unit": "pair interactions/s", "n_gpus": 1, "steps": args.steps,
@@ -342,7 +346,7 @@ def run_batch(args, systems, wdesc, workload):
             tpath = os.path.join(ROOT, "profiles", "traffic_k_force_tiled_%s.md" % workload)
         result["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                               "frac": achieved / peak, "traffic": traffic_from_summary(tpath),
-                              "kernel": "k_force_tiled" if args.precision == "mixed" else "k_force_plain",
+                              "kernel": "k_force_plain" if args.precision == "fp64" else "k_force_tiled",
                               "kernel_ms": kms, "launches": min(args.steps, 20), "flops_per_launch": flops,
                               "pairs_d_le_8": n_plain, "pairs_8_12": n_switch}
     batch.close()
@@ -443,7 +447,7 @@ def main():
     ap.add_argument("--workload", default="images16",
                     help="images16 (default: 16 perturbed images of the 30,000-atom system, one batch), droplets64, "
                          "solute30000, water200001, water648, dhfr, dhfr_periodic, qcmm_solvated")
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "tiled_fp64", "fp64"])
     ap.add_argument("--entries-per-item", type=int, default=0)
     ap.add_argument("--waves-per-cu", type=int, default=-1)
     ap.add_argument("--image-entries-per-item", type=int, default=0)
@@ -588,7 +592,7 @@ def main():
         result["roofline"] = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                               "frac": (achieved / peak) if achieved else None,
                               "traffic": traffic_from_summary(tpath),
-                              "kernel": "k_force_tiled" if args.precision == "mixed" else "k_force_plain",
+                              "kernel": "k_force_plain" if args.precision == "fp64" else "k_force_tiled",
                               "kernel_ms": kms, "launches": int(stats["forceKernelLaunches"]),
                               "flops_per_launch": flops, "pairs_d_le_8": n_plain, "pairs_8_12": n_switch}
     if not args.no_cpu_baseline:

@@ -43,6 +43,11 @@ extern "C" {
                                  vectors, 64-bit fixed-point sums, FP64 totals      */
 #define NBABFS_PLAIN_FP64 1   /* one thread per atom over a full CSR list, FP64 in
                                  the reference expression order (divide, sqrt)      */
+#define NBABFS_TILED_FP64 2   /* the tiled kernel's list, tiles and fixed-point sums
+                                 with FP64 pair arithmetic in the reference
+                                 expression order (divide, sqrt) and FP64 force
+                                 vectors: the mixed path without its FP32 parts
+                                 (no image lists)                                    */
 
 /* Upper bound of nbabfs_options.entriesPerItem (per-item j-force staging in LDS). */
 #define NBABFS_MAX_ENTRIES_PER_ITEM 128
@@ -63,7 +68,7 @@ typedef struct {
     double cutoffCellSizeFactor;   /* 0.5: cellSize = factor * listCutoff */
     int    minimumCellExtent;      /* 2    */
     int    minimumPoints;          /* 500  */
-    int    precision;              /* NBABFS_MIXED or NBABFS_PLAIN_FP64 */
+    int    precision;              /* NBABFS_MIXED, NBABFS_PLAIN_FP64 or NBABFS_TILED_FP64 */
     int    entriesPerItem;         /* cluster-pair entries per work item (0 = 16, at most
                                       NBABFS_MAX_ENTRIES_PER_ITEM) */
     int    wavesPerCU;             /* persistent-kernel waves per CU; 0 = one workgroup per work item */

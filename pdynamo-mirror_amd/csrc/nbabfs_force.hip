@@ -24,6 +24,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <type_traits>
 
 static inline int nblk(size_t m, int b) { return (int)((m + b - 1) / b); }
 
@@ -77,7 +78,7 @@ __device__ __forceinline__ bool pair_fp64(const AbfsFactors &F, double r2, doubl
 // A, -B, 0, B*bShift6 - A*aShift12; switched: the expanded squares
 // A*aK12*(s^6-aF6)^2 - B*bK6*(s^3-bF3)^2), so the region choice is a row
 // index instead of two evaluated branches.  The Coulomb switch keeps the
-// reference polynomial with the region selected per lane.  The damped region
+// reference polynomials, blended by a 0/1 region factor.  The damped region
 // (r < dampingCutoff) is a wave-uniform rare path on the raw A, B.
 __device__ __forceinline__ void pair_rows(const AbfsFactors &F, double r2, double qq, double4 C, bool sw,
                                           unsigned long long mact, const GAS double *__restrict__ ljA,
@@ -90,27 +91,33 @@ __device__ __forceinline__ void pair_rows(const AbfsFactors &F, double r2, doubl
     const double s2 = s * s;
     const double s3 = s2 * s;
     const double s6 = s3 * s3;
-    const double P = sw ? fma(-r2, fma(r2, fma(F.d, r2, F.c), F.b), F.a) : 1.0;
-    const double Q = sw ? fma(r2, fma(r2, fma(5.0 * F.d, r2, 3.0 * F.c), F.b), F.a) : 1.0;
-    const double K = sw ? F.qShift2 : F.qShift1;
-    double e1 = qq * fma(s, P, K);
-    // d1, d2: twice dE/d(r^2) (the factor 2 of the gradient folded into the
-    // constants; scaling by 2 is exact, so fs = d1 + d2 equals 2 (dF1 + dF2)).
-    double d1 = (-qq) * s3 * Q;
-    // (1/r) dE/dr of the polynomial = -3 s^5 (s^3 (4 C12 s^6 + 2 C6) + C3).
-    double e2 = fma(fma(C.x, s6, C.y), s6, fma(C.z, s3, C.w));
-    const double inner = fma(fma(4.0 * C.x, s6, 2.0 * C.y), s3, C.z);
-    double d2 = (-3.0 * s2) * s3 * inner;
+    // Coulomb in both undamped regions as one expression, m = 1 in the switched region and 0 in the plain
+    // one (the reference's E = qq (s P + K) with P = 1, K = qShift1 below the inner cutoff):
+    //   E / qq = s + qShift1 + m (s (P - 1) + qShift2 - qShift1),   P - 1 = (a - 1) - r2 (b + r2 (c + d r2)),
+    //   qq Q   = qq + qq m (Q - 1),                                 Q - 1 = (a - 1) + r2 (b + r2 (3c + 5d r2)).
+    // A product with m in {0, 1} is exact, so each region evaluates its own reference form.
+    const double m = sw ? 1.0 : 0.0;
+    const double pm1 = fma(-r2, fma(r2, fma(F.d, r2, F.c), F.b), F.a - 1.0);
+    const double qm1 = fma(r2, fma(r2, fma(5.0 * F.d, r2, 3.0 * F.c), F.b), F.a - 1.0);
+    double e1 = qq * fma(m, fma(s, pm1, F.qShift2 - F.qShift1), s + F.qShift1);
+    const double qQ = fma(qq * m, qm1, qq);
+    // Lennard-Jones row E = C12 s^12 + C6 s^6 + C3 s^3 + C0 (t1 = C12 s^6 + C6), and
+    // inner = s^3 (4 C12 s^6 + 2 C6) + C3 = 2 s^3 (C12 s^6 + t1) + C3.
+    const double t1 = fma(C.x, s6, C.y);
+    double e2 = fma(t1, s6, fma(C.z, s3, C.w));
+    const double inner = fma(2.0 * s3, fma(C.x, s6, t1), C.z);
+    // Twice dE/d(r^2) of both terms (the gradient's factor 2 folded in; exact):
+    //   Coulomb -qq s^3 Q, Lennard-Jones -3 s^5 inner, so fs = -s^3 (qq Q + 3 s^2 inner).
+    double f = -s3 * fma(3.0 * s2, inner, qQ);
     if (__builtin_amdgcn_ballot_w64(r2 < F.r2Damp) & mact) {
         if (r2 < F.r2Damp) {
             const double A = ljA[tij], B = ljB[tij];
             e1 = qq * fma(-F.qAlpha, r2, F.qF0);
-            d1 = (-qq * F.qAlpha) * 2.0;
             e2 = A * fma(-F.aAlpha, r2, F.aF0) - B * fma(-F.bAlpha, r2, F.bF0);
-            d2 = (-A * F.aAlpha + B * F.bAlpha) * 2.0;
+            f = (-qq * F.qAlpha) * 2.0 + (-A * F.aAlpha + B * F.bAlpha) * 2.0;
         }
     }
-    fs = d1 + d2;
+    fs = f;
     eel = e1;
     elj = e2;
 }
@@ -297,24 +304,35 @@ __device__ __forceinline__ float sum8(float v)
     return v;
 }
 
+// FP64 version (full-FP64 tiled kernel): a butterfly over the same 8 lanes;
+// every lane adds the same operands in the same tree, so all end bit-equal.
+__device__ __forceinline__ double sum8(double v)
+{
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    return v;
+}
+
 // LDS of the tiled kernel (dynamic): the item's i-cluster records and slot
 // types, the LJ row table of its system when it fits (ROWS_LDS), and the
 // per-entry j-force sums with their j-clusters, flushed to the fixed-point
 // accumulators once at the end of the item.  The i-force transpose at the end
 // of an item reuses the space of the records and the row table.
 #define NB_LDS_I_BYTES (NB_SC * 32 + NB_SC * 4)
-#define NB_LDS_F_BYTES (NB_SCC * 3 * NB_SC * 4)
+#define NB_LDS_F_BYTES(acc) (NB_SCC * 3 * NB_SC * (acc))
 #define NB_LDS_ROW_LIMIT 16384
 
-static inline size_t tiled_lds_head(int ntypes, bool rows_lds)
+// acc: bytes of a force accumulator (4: FP32, the mixed kernel; 8: FP64, the full-FP64 kernel).
+static inline size_t tiled_lds_head(int ntypes, bool rows_lds, int acc = 4)
 {
     size_t h = NB_LDS_I_BYTES + (rows_lds ? (size_t)2 * ntypes * ntypes * 32 : 0);
-    return h > NB_LDS_F_BYTES ? h : NB_LDS_F_BYTES;
+    return h > (size_t)NB_LDS_F_BYTES(acc) ? h : (size_t)NB_LDS_F_BYTES(acc);
 }
 
-static inline size_t tiled_lds_bytes(int ntypes, bool rows_lds, int max_count)
+static inline size_t tiled_lds_bytes(int ntypes, bool rows_lds, int max_count, int acc = 4)
 {
-    return tiled_lds_head(ntypes, rows_lds) + (size_t)max_count * (3 * NB_CL * 4 + 4);
+    return tiled_lds_head(ntypes, rows_lds, acc) + (size_t)max_count * (3 * NB_CL * acc + 4);
 }
 
 // Four waves per SIMD: the register allocator fits the kernel in 128 VGPRs
@@ -322,8 +340,11 @@ static inline size_t tiled_lds_bytes(int ntypes, bool rows_lds, int max_count)
 // IMG: the launch over the image lists' work items (the main list's items run
 // in an instance without any image code); it runs at three waves per SIMD,
 // whose register budget holds the transformation without spilling.
-template <bool ROWS_LDS, bool IMG>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4))) void k_force_tiled(const SysDesc *__restrict__ descs,
+// F64: the full-FP64 instance (the NBABFS_TILED_FP64 comparison path): the same list, tiles, staging and
+// fixed-point sums, with the reference's FP64 pair arithmetic (pair_fp64: divide and square root, no FP32
+// seed, raw A and B per type pair) and FP64 force vectors; three waves per SIMD for its larger accumulators.
+template <bool ROWS_LDS, bool IMG, bool F64 = false>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((IMG || F64) ? 3 : 4))) void k_force_tiled(const SysDesc *__restrict__ descs,
                                                     const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F,
                                                     int lds_head, ForceGuard guard, int nforce)
 {
@@ -340,10 +361,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
     const int bl = 8 * a + b;
     extern __shared__ __align__(32) unsigned char lds[];
     __shared__ __align__(16) double sX[12];     // image items: R then T of the item's image
+    using acc_t = typename std::conditional<F64, double, float>::type;
     double4 *sI = (double4 *)lds;
     int *sT = (int *)(lds + NB_SC * 32);
     double4 *sR = (double4 *)(lds + NB_LDS_I_BYTES);
-    float *sF = (float *)lds;
+    acc_t *sF = (acc_t *)lds;
     int q = blockIdx.x & 7;
     int tries = 0;
     // heads == null: one work item per workgroup (the hardware dispatcher
@@ -379,7 +401,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
         const int n_pad = D->n_pad;
         const int nt = D->ntypes;
         const int I0 = w.sc * NB_SCC;
-        float *sJF = (float *)(lds + lds_head);           // [entry][3 * NB_CL]
+        acc_t *sJF = (acc_t *)(lds + lds_head);           // [entry][3 * NB_CL]
         int *sJ = (int *)(sJF + 3 * NB_CL * w.count);      // [entry]
         // The item's image (-1: main list); all of its entries share it.
         int img = -1;
@@ -399,9 +421,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
         sT[lane] = 64 * nt * ty[I0 * NB_CL + lane];   // byte offset of the type's first row pair
         __syncthreads();
 
-        float fx[NB_SCC], fy[NB_SCC], fz[NB_SCC];
+        acc_t fx[NB_SCC], fy[NB_SCC], fz[NB_SCC];
 #pragma unroll
-        for (int c = 0; c < NB_SCC; c++) fx[c] = fy[c] = fz[c] = 0.0f;
+        for (int c = 0; c < NB_SCC; c++) fx[c] = fy[c] = fz[c] = (acc_t)0;
         double eel = 0.0, elj = 0.0;
         const int e_last = w.first + w.count - 1;
         // Pipeline: while entry e is evaluated, entry e+1's j-cluster record
@@ -425,7 +447,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
             const int lb = lb0;
             // Image items: the j-atom's image position (its load was issued one entry earlier).
             const double4 xjt = IMG ? image_slot(xj, sX) : xj;
-            float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+            acc_t gx = (acc_t)0, gy = (acc_t)0, gz = (acc_t)0;
 #pragma unroll
             for (int c = 0; c < NB_SCC; c++) {
                 if (cb & (1 << c)) {
@@ -439,20 +461,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
                     // Ballots of the two compares (lane masks as they stand), combined in scalar registers.
                     const unsigned long long mact = __builtin_amdgcn_ballot_w64(inl) & __builtin_amdgcn_ballot_w64(inr);
                     if (mact != 0ull) {
-                        const bool sw = r2 > F.r2On;
-                        const int off = roff + (sw ? 32 : 0);
-                        const double4 C = ROWS_LDS ? *(const double4 *)((const unsigned char *)sR + off)
-                                                   : load_slot(rows, off >> 5);
-                        double fs, x1, x2;
-                        pair_rows(F, r2, xi.w * xjt.w, C, sw, mact, ljA, ljB, roff >> 6, fs, x1, x2);
-                        // The sums take the active lanes only (the others may hold any value).
-                        if (act) {
-                            const float f = (float)fs;
-                            const float px = f * (float)dx, py = f * (float)dy, pz = f * (float)dz;
-                            fx[c] += px; fy[c] += py; fz[c] += pz;
-                            gx -= px; gy -= py; gz -= pz;
-                            eel += x1;
-                            elj += x2;
+                        if (F64) {
+                            // PairwiseInteraction.h:72-119 in FP64 with the reference's operations.
+                            if (act) {
+                                const int tij = roff >> 6;
+                                double x1, x2, dF;
+                                pair_fp64(F, r2, xi.w * xjt.w, ljA[tij], ljB[tij], x1, x2, dF);
+                                const double f = 2.0 * dF;
+                                const double px = f * dx, py = f * dy, pz = f * dz;
+                                fx[c] += px; fy[c] += py; fz[c] += pz;
+                                gx -= px; gy -= py; gz -= pz;
+                                eel += x1;
+                                elj += x2;
+                            }
+                        } else {
+                            const bool sw = r2 > F.r2On;
+                            const int off = roff + (sw ? 32 : 0);
+                            const double4 C = ROWS_LDS ? *(const double4 *)((const unsigned char *)sR + off)
+                                                       : load_slot(rows, off >> 5);
+                            double fs, x1, x2;
+                            pair_rows(F, r2, xi.w * xjt.w, C, sw, mact, ljA, ljB, roff >> 6, fs, x1, x2);
+                            // The sums take the active lanes only (the others may hold any value).
+                            if (act) {
+                                const float f = (float)fs;
+                                const float px = f * (float)dx, py = f * (float)dy, pz = f * (float)dz;
+                                fx[c] += px; fy[c] += py; fz[c] += pz;
+                                gx -= px; gy -= py; gz -= pz;
+                                eel += x1;
+                                elj += x2;
+                            }
                         }
                     }
                 }
@@ -482,10 +519,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
         }
         __syncthreads();
         // Partials of (c, a) sit at lanes 8b' + a, b' = 0..7 (stride 8).
-        float ig[3];
+        acc_t ig[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const float *fr = &sF[(3 * b + k) * NB_SC + a];
+            const acc_t *fr = &sF[(3 * b + k) * NB_SC + a];
             ig[k] = ((fr[0] + fr[8]) + (fr[16] + fr[24])) + ((fr[32] + fr[40]) + (fr[48] + fr[56]));
         }
         const int si = (I0 + b) * NB_CL + a;
@@ -511,7 +548,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(IMG ? 3 : 4)
             for (int k = lane; k < NB_CL * w.count; k += 64) {
                 const int el = k >> 3, jb = k & 7;
                 const int sj = sJ[el] * NB_CL + jb;
-                const float *f = &sJF[el * 3 * NB_CL + 3 * jb];
+                const acc_t *f = &sJF[el * 3 * NB_CL + 3 * jb];
                 const double gx = isc * (double)f[0], gy = isc * (double)f[1], gz = isc * (double)f[2];
                 bad |= fx_atomic_add(facc, n_pad, sj, 0, fma(sX[0], gx, fma(sX[3], gy, sX[6] * gz)));
                 bad |= fx_atomic_add(facc, n_pad, sj, 1, fma(sX[1], gx, fma(sX[4], gy, sX[7] * gz)));
@@ -568,13 +605,15 @@ __device__ __forceinline__ void block_sum2(double &x, double &y)
     y = sy[0];
 }
 
-__global__ __launch_bounds__(256) void k_force_plain(const SysDesc *__restrict__ descs, int sys, AbfsFactors F,
-                                                     const int *__restrict__ rowptr, const int *__restrict__ cols,
-                                                     ForceGuard guard)
+__global__ __launch_bounds__(256) void k_force_plain(const SysDesc *__restrict__ descs, AbfsFactors F, ForceGuard guard)
 {
 #pragma clang fp contract(off)
     if (guard.flag && *(const volatile int *)guard.flag == guard.epoch) return;
-    const SysDesc &D = descs[sys];
+    // blockIdx.y = system: one launch covers every system of a batch.
+    const SysDesc &D = descs[blockIdx.y];
+    if ((int)blockIdx.x >= D.n_plain_blocks) return;
+    const int *__restrict__ rowptr = D.rowptr;
+    const int *__restrict__ cols = D.cols;
     const int i = blockIdx.x * 256 + threadIdx.x;
     double e1 = 0.0, e2 = 0.0;
     if (i < D.n) {
@@ -936,14 +975,22 @@ void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, in
 
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
                         int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images,
-                        hipEvent_t ev_start, hipEvent_t ev_stop, int n14_blocks)
+                        hipEvent_t ev_start, hipEvent_t ev_stop, int n14_blocks, bool full_fp64)
 {
     if (images) n14_blocks = 0;
     if (nblocks + n14_blocks <= 0) return;
+    const dim3 grid(nblocks + n14_blocks), block(64);
+    if (full_fp64) {
+        // No row table: the FP64 pair terms read the raw A and B of the type pair.
+        const int head = (int)tiled_lds_head(max_ntypes, false, 8);
+        const size_t bytes = tiled_lds_bytes(max_ntypes, false, max_count, 8);
+        hipExtLaunchKernelGGL((k_force_tiled<false, false, true>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc,
+                              items, q, F, head, g, nblocks);
+        return;
+    }
     const bool rows_lds = tiled_lds_bytes(max_ntypes, true, max_count) <= NB_LDS_ROW_LIMIT;
     const int head = (int)tiled_lds_head(max_ntypes, rows_lds);
     const size_t bytes = tiled_lds_bytes(max_ntypes, rows_lds, max_count);
-    const dim3 grid(nblocks + n14_blocks), block(64);
     if (images) {
         if (rows_lds)
             hipExtLaunchKernelGGL((k_force_tiled<true, true>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items, q,
@@ -961,11 +1008,12 @@ void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParam
     }
 }
 
-void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
-                        const int *cols, ForceGuard g, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop)
+void launch_force_plain(const SysDesc *d_desc, int nsys, int max_n, const AbfsFactors &F, ForceGuard g, hipStream_t s,
+                        hipEvent_t ev_start, hipEvent_t ev_stop)
 {
-    hipExtLaunchKernelGGL(k_force_plain, dim3(max(nblk(n, 256), 1)), dim3(256), 0, s, ev_start, ev_stop, 0, d_desc, sys, F,
-                          rowptr, cols, g);
+    if (nsys <= 0) return;
+    hipExtLaunchKernelGGL(k_force_plain, dim3(max(nblk(max_n, 256), 1), nsys), dim3(256), 0, s, ev_start, ev_stop, 0,
+                          d_desc, F, g);
 }
 
 void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int nitems, const AbfsFactors &F,

@@ -1227,6 +1227,8 @@ static void fill_desc(nbabfs_state *S, SysDesc &D, const double *d_xyz, double *
     D.eplain = S->eplain.p;
     D.energies = S->energies.p;
     D.err = S->errw.p;
+    D.rowptr = S->csrValid ? S->rowptr.p : nullptr;
+    D.cols = S->csrValid ? S->cols.p : nullptr;
     for (int d = 0; d < 3; d++) D.origin[d] = S->grid.corg[d];
     D.escale = S->escale;
     D.escale14 = S->escale14;
@@ -1300,11 +1302,13 @@ static int launch_force(nbabfs_state *S, ForceGuard g)
         if (nm > 0) {
             QueueParams q{S->opt.wavesPerCU > 0 ? S->heads.p : nullptr, S->qstarts.p};
             launch_force_tiled(S->d_desc.p, S->items.p, q, S->F, n_persistent_blocks(S->opt, nm), S->ntypes,
-                               S->opt.entriesPerItem, g, st, false, ev[0], ev[1], nblk(S->n14, NB_P14_BLOCK));
+                               S->opt.entriesPerItem, g, st, false, ev[0], ev[1], nblk(S->n14, NB_P14_BLOCK),
+                               S->opt.precision == NBABFS_TILED_FP64);
         } else if (S->n14 > 0) {
             QueueParams q{nullptr, S->qstarts.p};
             launch_force_tiled(S->d_desc.p, S->items.p, q, S->F, 0, S->ntypes, S->opt.entriesPerItem, g, st, false,
-                               ni > 0 ? nullptr : ev[0], ni > 0 ? nullptr : ev[1], nblk(S->n14, NB_P14_BLOCK));
+                               ni > 0 ? nullptr : ev[0], ni > 0 ? nullptr : ev[1], nblk(S->n14, NB_P14_BLOCK),
+                               S->opt.precision == NBABFS_TILED_FP64);
         }
         if (sti != st) {
             HIPCHK(hipEventRecord(S->ev_join, sti));
@@ -1313,7 +1317,7 @@ static int launch_force(nbabfs_state *S, ForceGuard g)
     } else {
         const int nm = S->nitems_main, ni = S->nitems - S->nitems_main;
         if (S->timing) CHK(timing_events(S, 2, ev));
-        launch_force_plain(S->d_desc.p, 0, S->n, S->F, S->rowptr.p, S->cols.p, g, st, ev[0], ni > 0 ? nullptr : ev[1]);
+        launch_force_plain(S->d_desc.p, 1, S->n, S->F, g, st, ev[0], ni > 0 ? nullptr : ev[1]);
         if (ni > 0) launch_force_image_fp64(S->d_desc.p, S->items.p + nm, ni, S->F, g, st, nullptr, ev[1]);
     }
     HIPCHK(hipGetLastError());
@@ -1809,6 +1813,8 @@ extern "C" int nbabfs_state_create(const nbabfs_options *opt, int natoms, const 
     if (o.entriesPerItem > NBABFS_MAX_ENTRIES_PER_ITEM)
         return fail(NBABFS_STATUS_VALUE_ERROR, "entriesPerItem %d exceeds %d", o.entriesPerItem,
                     NBABFS_MAX_ENTRIES_PER_ITEM);
+    if (o.precision != NBABFS_MIXED && o.precision != NBABFS_PLAIN_FP64 && o.precision != NBABFS_TILED_FP64)
+        return fail(NBABFS_STATUS_VALUE_ERROR, "unknown precision %d", o.precision);
     if (o.wavesPerCU < 0) o.wavesPerCU = 0;
     if (o.imageEntriesPerItem <= 0) o.imageEntriesPerItem = 16;
     if (o.imageEntriesPerItem > NBABFS_MAX_ENTRIES_PER_ITEM)
@@ -2110,6 +2116,8 @@ extern "C" int nbabfs_set_transformations(nbabfs_state *S, int ntransformations,
     if (!S->isNew) return fail(NBABFS_STATUS_LOGIC_ERROR, "transformations must be set before the first update");
     if (S->has_qc && ntransformations > 0)
         return fail(NBABFS_STATUS_VALUE_ERROR, "QC/MM image lists are not supported: the state has QC atoms");
+    if (S->opt.precision == NBABFS_TILED_FP64 && ntransformations > 0)
+        return fail(NBABFS_STATUS_INVALID_ARGUMENT, "image lists are evaluated by the mixed and plain FP64 paths only");
     if (ntransformations < 0 || (ntransformations > 0 && (!rotations || !translations)) || imageExpandFactor < 0)
         return fail(NBABFS_STATUS_INVALID_ARGUMENT, "invalid transformations");
     S->has_sym = ntransformations > 0;
@@ -2437,7 +2445,8 @@ struct nbabfs_batch {
     std::vector<long> last_items;
     hipGraphExec_t gexec = nullptr;
     bool graph_valid = false;
-    bool mixed = true;
+    bool mixed = true;             // the tiled kernels run the pair passes (precision mixed or tiled FP64)
+    int precision = NBABFS_MIXED;
     long long n_captures = 0, n_launches = 0, n_layouts = 0;
     // Timing of the batched pair launch: with timing on, the calls enqueue the
     // passes directly (no graph) with the runtime's dispatch events around the
@@ -2476,9 +2485,10 @@ extern "C" int nbabfs_batch_create(nbabfs_state *const *states, int nsystems, in
         }
         B->st.push_back(states[k]);
     }
-    B->mixed = states[0]->opt.precision == NBABFS_MIXED;
+    B->precision = states[0]->opt.precision;
+    B->mixed = B->precision != NBABFS_PLAIN_FP64;     // the tiled kernels (mixed or full FP64)
     for (auto *s : B->st)
-        if ((s->opt.precision == NBABFS_MIXED) != B->mixed) {
+        if (s->opt.precision != B->precision) {
             delete B;
             return fail(NBABFS_STATUS_INVALID_ARGUMENT, "all states of a batch must share one precision");
         }
@@ -2563,12 +2573,11 @@ static int batch_enqueue(nbabfs_batch *B, hipEvent_t ev_start = nullptr, hipEven
                 maxcount = std::max(maxcount, s->opt.entriesPerItem);
             }
             launch_force_tiled(B->d_desc.p, B->items.p, q, s0->F, n_persistent_blocks(s0->opt, B->nitems), maxnt,
-                               maxcount, ForceGuard{nullptr, 0}, st, false, ev_start, ev_stop);
+                               maxcount, ForceGuard{nullptr, 0}, st, false, ev_start, ev_stop, 0,
+                               B->precision == NBABFS_TILED_FP64);
         }
     } else {
-        for (int k = 0; k < ns; k++)
-            launch_force_plain(B->d_desc.p, k, B->st[k]->n, B->st[k]->F, B->st[k]->rowptr.p, B->st[k]->cols.p,
-                               ForceGuard{nullptr, 0}, st, k == 0 ? ev_start : nullptr, k == ns - 1 ? ev_stop : nullptr);
+        launch_force_plain(B->d_desc.p, ns, B->max_n, s0->F, ForceGuard{nullptr, 0}, st, ev_start, ev_stop);
     }
     launch_pairs14(B->d_desc.p, ns, B->max_n14b, s0->F, st);
     launch_finalize(B->d_desc.p, ns, B->max_n, B->mixed ? 0 : 1, st);

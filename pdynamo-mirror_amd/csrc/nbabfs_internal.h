@@ -118,6 +118,7 @@ struct SysDesc {
     int n_qlj;
     const int *qlj_atom, *qlj_ptr, *qlj_idx, *qlj14_ptr, *qlj14_idx;
     double *qljpart;            // 2 per (row, chunk) (QC/MM LJ, QC/MM 1-4 LJ), summed in that order
+    const int *rowptr, *cols;   // plain path: full CSR neighbour rows (n + 1, 2 npairs)
     int grad_assign;            // 1: the finalize pass overwrites grad instead of adding to it
     int *err;                   // error word (NB_ERR_* bits) of the current call; cleared by the finalize pass
 };

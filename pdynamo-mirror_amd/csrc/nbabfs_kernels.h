@@ -128,14 +128,16 @@ void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, in
 // images: the launch over image-list items (one workgroup per item).  The pair
 // launchers take optional timing events (start, stop: recorded by the runtime
 // at the dispatch's own begin and end; null = not timed).
+// full_fp64: the NBABFS_TILED_FP64 instance (FP64 pair arithmetic and force vectors; not for image items).
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
                         int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images = false,
-                        hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr, int n14_blocks = 0);
+                        hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr, int n14_blocks = 0,
+                        bool full_fp64 = false);
 // The 1-4 list in blocks of NB_P14_BLOCK pairs (k_pairs14, or extra workgroups of a single-state pair launch).
 #define NB_P14_BLOCK 64
-void launch_force_plain(const SysDesc *d_desc, int sys, int n, const AbfsFactors &F, const int *rowptr,
-                        const int *cols, ForceGuard g, hipStream_t s, hipEvent_t ev_start = nullptr,
-                        hipEvent_t ev_stop = nullptr);
+// The plain FP64 CSR pass of nsys systems in one launch (blockIdx.y = system; rows from the descriptors).
+void launch_force_plain(const SysDesc *d_desc, int nsys, int max_n, const AbfsFactors &F, ForceGuard g, hipStream_t s,
+                        hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 // Plain FP64 image pass over the image lists' work items.
 void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int nitems, const AbfsFactors &F,
                              ForceGuard g, hipStream_t s, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);

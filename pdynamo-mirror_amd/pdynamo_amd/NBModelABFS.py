@@ -189,7 +189,7 @@ class NBModelABFSState:
 class NBModelABFS:
     """Define an ABFS NB model."""
 
-    _PRECISIONS = {"mixed": _native.MIXED, "fp64": _native.PLAIN_FP64}
+    _PRECISIONS = {"mixed": _native.MIXED, "fp64": _native.PLAIN_FP64, "tiled_fp64": _native.TILED_FP64}
 
     def __init__(self, **options):
         self.label = "ABFS"
@@ -238,7 +238,7 @@ class NBModelABFS:
         if "precision" in keywordArguments:
             p = keywordArguments.pop("precision")
             if p not in self._PRECISIONS:
-                raise ValueError("Unrecognized precision: {!r} (use 'mixed' or 'fp64').".format(p))
+                raise ValueError("Unrecognized precision: {!r} (use 'mixed', 'fp64' or 'tiled_fp64').".format(p))
             self.precision = p
         if "entriesPerItem" in keywordArguments:
             v = int(keywordArguments.pop("entriesPerItem"))

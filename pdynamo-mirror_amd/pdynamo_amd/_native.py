@@ -14,7 +14,7 @@ from .build import LIB
 
 STATUS_CONTINUE = 16
 GRADIENT_ADD, GRADIENT_ASSIGN = 0, 1           # nbabfs_set_gradient_mode
-MIXED, PLAIN_FP64 = 0, 1
+MIXED, PLAIN_FP64, TILED_FP64 = 0, 1, 2
 QCMM_MM, QCMM_RC, QCMM_RD = 0, 1, 2
 
 # Every symbol include/nbabfs.h declares.

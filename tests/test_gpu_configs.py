@@ -9,8 +9,10 @@
 
 Per call and system: the rebuild decision equals the oracle's, the pair set is bit for bit the oracle's
 (sorted canonical keys max*n+min) on every call that built a list, every energy term is within 1e-6 (mixed) /
-1e-12 (plain FP64) of its own magnitude, or within the FP64 reordering bound 4u sum|e_pair| of the oracle's
-pair terms where that is larger, and the relative RMS gradient within 1e-5 / 1e-12 (parity_checks.check).
+1e-10 (tiled full FP64: the fixed-point sums round each work item's energy by up to 2^-33 kJ/mol) / 1e-12
+(plain FP64) of its own magnitude, or within the FP64 reordering bound 4u sum|e_pair| of the oracle's pair
+terms where that is larger, and the relative RMS gradient within 1e-5 / 1e-10 / 1e-12 (parity_checks.check).
+All three GPU paths run on every configuration.
 """
 
 import os
@@ -23,7 +25,8 @@ from parity_checks import check
 pytestmark = pytest.mark.gpu
 
 NTHREADS = max(1, min(16, len(os.sched_getaffinity(0))))
-TOL = {"mixed": (1e-6, 1e-5), "fp64": (1e-12, 1e-12)}
+TOL = {"mixed": (1e-6, 1e-5), "tiled_fp64": (1e-10, 1e-10), "fp64": (1e-12, 1e-12)}
+PRECISIONS = ("mixed", "tiled_fp64", "fp64")
 LABELS = ("MM/MM Elect.", "MM/MM LJ", "MM/MM 1-4 Elect.", "MM/MM 1-4 LJ")
 
 
@@ -50,7 +53,7 @@ def test_water200001_moves(oracle_mod):
     mm = synthetic.config2()
     seq = [mm.coordinates3] + synthetic.displacement_sequence(mm, 10, 0.05, seed=7)
     o = oracle_mod.OracleABFS(mm, nthreads=NTHREADS)
-    states = {p: _state(mm, p) for p in ("mixed", "fp64")}
+    states = {p: _state(mm, p) for p in PRECISIONS}
     g = _native.DeviceArray(mm.coordinates3.shape)
     rebuilds = 0
     for call, xyz in enumerate(seq):
@@ -85,7 +88,7 @@ class _Batches:
         self.systems = systems
         self.offs = np.concatenate([[0], np.cumsum([m.natoms for m in systems])]).astype(np.int64)
         self.states, self.batches, self.X, self.G = {}, {}, {}, {}
-        for precision in ("mixed", "fp64"):
+        for precision in PRECISIONS:
             sts = [_state(m, precision) for m in systems]
             for s in sts:
                 s.set_gradient_mode(_native.GRADIENT_ASSIGN)     # the batch writes every gradient row
@@ -118,7 +121,7 @@ def _batch_step(B, oracles, xyzs, name):
         rterms, rgrad, rb = o.energy(xyz)
         refs.append((rterms, rgrad, rb, o.records.canonical() if rb else None, o.abs_sums(xyz)))
     flags = None
-    for precision in ("mixed", "fp64"):
+    for precision in PRECISIONS:
         e, rebuilt, grads = B.call(precision, xyzs)
         for k, (mm, (rterms, rgrad, rb, keys, scales)) in enumerate(zip(B.systems, refs)):
             label = "%s system %d (%d atoms) %s" % (name, k, mm.natoms, precision)
@@ -143,7 +146,7 @@ def test_droplets64_batch(oracle_mod):
     try:
         assert _batch_step(B, oracles, xyzs, "droplets64").all()
         assert not _batch_step(B, oracles, xyzs, "droplets64 repeat").any()
-        for precision in ("mixed", "fp64"):
+        for precision in PRECISIONS:
             info = B.batches[precision].info()
             assert info["graph_captures"] == 1 and info["launches"] == 2, info
     finally:
@@ -162,7 +165,7 @@ def test_images16_batch_partial_rebuild(oracle_mod):
     B = _Batches(systems)
     try:
         assert _batch_step(B, oracles, [m.coordinates3 for m in systems], "images16").all()
-        for precision in ("mixed", "fp64"):
+        for precision in PRECISIONS:
             info = B.batches[precision].info()
             assert info["graph_captures"] == 1 and info["launches"] == 1, info
         rebuilt = _batch_step(B, oracles, moved, "images16 moved")
@@ -194,7 +197,7 @@ def test_uneven_batch_partial_rebuild(oracle_mod):
         rebuilt = _batch_step(B, oracles, moved, "uneven5 moved")
         assert set(np.flatnonzero(rebuilt)) == movers, np.flatnonzero(rebuilt)
         assert not _batch_step(B, oracles, moved, "uneven5 repeat").any()
-        for precision in ("mixed", "fp64"):
+        for precision in PRECISIONS:
             info = B.batches[precision].info()
             assert info == {"graph_captures": 2, "launches": 3, "layouts": 2}, info
     finally:

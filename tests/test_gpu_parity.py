@@ -3,7 +3,7 @@
 Tolerances (parity_checks.check): mixed precision -- every energy term within
 1e-6 of its own magnitude (|dE_k| <= 1e-6 * max(|E_k|, 1 kJ/mol)), the total
 likewise, relative RMS gradient <= 1e-5 over all atoms and over the atoms in no
-overlapping pair; plain FP64 -- the same with 1e-12.  Pair sets are compared
+overlapping pair; tiled full FP64 -- 1e-10; plain FP64 -- the same with 1e-12.  Pair sets are compared
 bit for bit as sorted canonical keys max*n+min.
 """
 
@@ -13,6 +13,10 @@ import pytest
 from parity_checks import check
 
 pytestmark = pytest.mark.gpu
+
+# The three GPU paths and their bounds: mixed (FP64 pair core with FP32 parts), the tiled kernel in full FP64
+# (fixed-point sums round each work item's energy by up to 2^-33 kJ/mol), the plain FP64 CSR kernel.
+PATHS = [("mixed", 1e-6, 1e-5), ("tiled_fp64", 1e-10, 1e-10), ("fp64", 1e-12, 1e-12)]
 
 
 def _system(mm, **opts):
@@ -34,7 +38,7 @@ def _check(terms, grad, rterms, rgrad, etol, gtol, label="", scales=None):
     check(terms, grad, rterms, rgrad, etol, gtol, label, scales)
 
 
-@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("fp64", 1e-12, 1e-12)])
+@pytest.mark.parametrize("precision,etol,gtol", PATHS)
 def test_water648(oracle_mod, precision, etol, gtol):
     from pdynamo_amd import synthetic
     mm = synthetic.config0()
@@ -47,7 +51,7 @@ def test_water648(oracle_mod, precision, etol, gtol):
 
 
 @pytest.mark.parametrize("natoms,box,seed", [(300, 30.0, 1), (900, 45.0, 2), (2000, 60.0, 3), (7, 10.0, 4)])
-@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("fp64", 1e-12, 1e-12)])
+@pytest.mark.parametrize("precision,etol,gtol", PATHS)
 def test_random_mixtures(oracle_mod, natoms, box, seed, precision, etol, gtol):
     from pdynamo_amd import synthetic
     mm = synthetic.random_small_system(natoms, box, seed)
@@ -77,7 +81,7 @@ def _shaped(shape):
 
 
 @pytest.mark.parametrize("shape", ["slab", "rod", "thin_plane", "line"])
-@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("fp64", 1e-12, 1e-12)])
+@pytest.mark.parametrize("precision,etol,gtol", PATHS)
 def test_extreme_shapes(oracle_mod, shape, precision, etol, gtol):
     """Flat and thin systems: super-cluster columns much wider than the system in some direction,
     columns padded to whole super-clusters; same pair set and terms as the oracle.  An exact line of
@@ -145,7 +149,8 @@ def _water_with_14(nwater, box, seed):
     return mm, a14, b14
 
 
-@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("fp64", 1e-12, 1e-11)])
+@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("tiled_fp64", 1e-10, 1e-10),
+                                                 ("fp64", 1e-12, 1e-11)])
 def test_options_14_dielectric_water(oracle_mod, precision, etol, gtol):
     """Dielectric, electrostaticScale14, a 1-4 LJ table of its own and non-default cutoffs on an overlap-free
     water lattice, so that the Coulomb, 1-4 Coulomb and 1-4 LJ terms are each checked against their own
@@ -216,16 +221,17 @@ def test_cell_size_invariance(oracle_mod, factor):
     _check(terms, grad, rterms, rgrad, 1e-6, 1e-5, "", o.abs_sums(mm.coordinates3))
 
 
-def test_solute30000(oracle_mod):
+@pytest.mark.parametrize("precision,etol,gtol", PATHS)
+def test_solute30000(oracle_mod, precision, etol, gtol):
     """The 30,000-atom solute-in-water system (damped-region pairs included)."""
     from pdynamo_amd import synthetic
     mm = synthetic.config1()
-    s = _system(mm)
+    s = _system(mm, precision=precision)
     _, terms, grad = _run(s)
     o = oracle_mod.OracleABFS(mm, nthreads=8)
     rterms, rgrad, _ = o.energy(mm.coordinates3)
     assert np.array_equal(s.configuration.nbState.Pairs(), o.records.canonical())
-    _check(terms, grad, rterms, rgrad, 1e-6, 1e-5, "", o.abs_sums(mm.coordinates3))
+    _check(terms, grad, rterms, rgrad, etol, gtol, "", o.abs_sums(mm.coordinates3))
 
 
 def test_batch_matches_single():
@@ -292,7 +298,7 @@ def test_kernel_timing_passes(precision, periodic):
     st.close()
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("precision", ["mixed", "tiled_fp64", "fp64"])
 def test_gradient_assign_mode(precision):
     """nbabfs_set_gradient_mode(ASSIGN): every gradient row is overwritten with the model's
     gradient, bit for bit what ADD gives on a zeroed array, for the device and host entry points
@@ -347,7 +353,7 @@ def test_gradient_assign_mode(precision):
     batch.close()
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("precision", ["mixed", "tiled_fp64", "fp64"])
 def test_call_orders_match(precision):
     """Update and energy calls in other orders give the same bits as update-then-energy.
 
@@ -453,11 +459,12 @@ def test_virial_matches_strain_derivative(oracle_mod, precision, tol):
         assert abs(an - fd) <= tol * max(abs(fd), np.abs(W).max()), (S, an, fd)
 
 
-@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("fp64", None, 1e-12)])
+@pytest.mark.parametrize("precision,etol,gtol", [("mixed", 1e-6, 1e-5), ("tiled_fp64", None, 1e-10),
+                                                 ("fp64", None, 1e-12)])
 def test_dhfr(oracle_mod, dhfr, precision, etol, gtol):
     """The reference's DHFR benchmark system (CHARMM22, 23,558 atoms, 35 LJ types so the LJ rows are read from
     global memory): the published pair counts exactly; the published four terms to their 4 printed decimals
-    (plain FP64) or to 1e-6 of the total (mixed); gradients against the oracle."""
+    (both FP64 paths) or to 1e-6 of the total (mixed); gradients against the oracle."""
     s = dhfr.system(precision=precision)
     _, terms, grad = _run(s)
     st = s.configuration.nbState.Statistics()
@@ -519,7 +526,7 @@ def test_randomized_configurations(oracle_mod, seed):
     o = oracle_mod.OracleABFS(mm, damping=damp, inner=inner, outer=outer, list_cutoff=lst, dielectric=diel,
                               scale14=sc14)
     rterms, rgrad, _ = o.energy(mm.coordinates3)
-    for precision, etol, gtol in (("mixed", 1e-6, 1e-5), ("fp64", 1e-12, 1e-12)):
+    for precision, etol, gtol in PATHS:
         s = System.FromMMParameters(mm.coordinates3, MMAtomContainer(mm.charges, mm.ljtypes), lj, mm.exclusions,
                                     mm.interactions14)
         s.DefineNBModel(NBModelABFS(dielectric=diel, electrostaticScale14=sc14, innerCutoff=inner, outerCutoff=outer,
