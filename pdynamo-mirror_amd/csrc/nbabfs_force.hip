@@ -224,10 +224,10 @@ __device__ __forceinline__ double4 image_slot(double4 v, const volatile double *
 // accumulators, the block's two energies (wave sums) into epart14[blk].  Run
 // by k_pairs14, or by extra workgroups of the main pair launch.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ void pairs14_block(const SysDesc &D, const AbfsFactors &F, int blk)
+__device__ __forceinline__ void pairs14_block(const SysDesc &D, const AbfsFactors &F, int blk, int lane)
 {
 #pragma clang fp contract(off)
-    const int k = blk * 64 + (int)threadIdx.x;
+    const int k = blk * 64 + lane;
     double e1 = 0.0, e2 = 0.0;
     bool bad = false;
     if (k < D.n14) {
@@ -254,7 +254,7 @@ __device__ __forceinline__ void pairs14_block(const SysDesc &D, const AbfsFactor
     }
     e1 = wave_sum_f64(e1);
     e2 = wave_sum_f64(e2);
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
         D.epart14[2 * blk] = e1;
         D.epart14[2 * blk + 1] = e2;
     }
@@ -314,25 +314,39 @@ __device__ __forceinline__ double sum8(double v)
     return v;
 }
 
-// LDS of the tiled kernel (dynamic): the item's i-cluster records and slot
-// types, the LJ row table of its system when it fits (ROWS_LDS), and the
-// per-entry j-force sums with their j-clusters, flushed to the fixed-point
-// accumulators once at the end of the item.  The i-force transpose at the end
-// of an item reuses the space of the records and the row table.
+// LDS of the tiled kernel (dynamic).  A workgroup holds WPG wavefronts, each
+// evaluating one work item.  At offset 0 the LJ row table of the group's
+// system when it fits (ROWS_LDS: staged once per group), then one region per
+// wave: the item's i-cluster records and slot types, then the per-entry
+// j-force sums with their j-clusters, flushed to the fixed-point accumulators
+// once at the end of the item; the i-force transpose at the end of the item
+// reuses the start of the region once the j-forces are flushed.
 #define NB_LDS_I_BYTES (NB_SC * 32 + NB_SC * 4)
 #define NB_LDS_F_BYTES(acc) (NB_SCC * 3 * NB_SC * (acc))
 #define NB_LDS_ROW_LIMIT 16384
+#define NB_LDS_GROUP_LIMIT (160 * 1024 / 4)
+
+static inline size_t tiled_lds_rows(int ntypes, bool rows_lds) { return rows_lds ? (size_t)2 * ntypes * ntypes * 32 : 0; }
 
 // acc: bytes of a force accumulator (4: FP32, the mixed kernel; 8: FP64, the full-FP64 kernel).
-static inline size_t tiled_lds_head(int ntypes, bool rows_lds, int acc = 4)
+static inline size_t tiled_lds_wave(int max_count, int acc)
 {
-    size_t h = NB_LDS_I_BYTES + (rows_lds ? (size_t)2 * ntypes * ntypes * 32 : 0);
-    return h > (size_t)NB_LDS_F_BYTES(acc) ? h : (size_t)NB_LDS_F_BYTES(acc);
+    const size_t items = NB_LDS_I_BYTES + (size_t)max_count * (3 * NB_CL * acc + 4);
+    return std::max(items, (size_t)NB_LDS_F_BYTES(acc));
 }
 
-static inline size_t tiled_lds_bytes(int ntypes, bool rows_lds, int max_count, int acc = 4)
+// Wave-scope synchronisation of a wave's own LDS region (workgroups of several
+// independent waves); a workgroup barrier when the workgroup is one wave.
+template <int WPG>
+__device__ __forceinline__ void wave_region_sync()
 {
-    return tiled_lds_head(ntypes, rows_lds, acc) + (size_t)max_count * (3 * NB_CL * acc + 4);
+    if (WPG == 1) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 }
 
 // Four waves per SIMD: the register allocator fits the kernel in 128 VGPRs
@@ -343,40 +357,60 @@ static inline size_t tiled_lds_bytes(int ntypes, bool rows_lds, int max_count, i
 // F64: the full-FP64 instance (the NBABFS_TILED_FP64 comparison path): the same list, tiles, staging and
 // fixed-point sums, with the reference's FP64 pair arithmetic (pair_fp64: divide and square root, no FP32
 // seed, raw A and B per type pair) and FP64 force vectors; three waves per SIMD for its larger accumulators.
-template <bool ROWS_LDS, bool IMG, bool F64 = false>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((IMG || F64) ? 3 : 4))) void k_force_tiled(const SysDesc *__restrict__ descs,
-                                                    const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F,
-                                                    int lds_head, ForceGuard guard, int nforce)
+// WPG: wavefronts per workgroup.  With 4 (the one-item-per-wave launches of
+// the main list), the row table is staged once for four items and a group's
+// LDS stays small enough for four waves per SIMD; the four items of a group
+// belong to one system (batches pad each system's items to a multiple of 4
+// with empty ones).  Image items and the persistent queues run one wave per
+// group.
+template <bool ROWS_LDS, bool IMG, bool F64, int WPG>
+__global__ __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu((IMG || F64) ? 3 : 4))) void k_force_tiled(
+    const SysDesc *__restrict__ descs, const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F, int rows_bytes,
+    int wave_bytes, ForceGuard guard, int nitems, int nforce_groups)
 {
     if (guard.flag && *(const volatile int *)guard.flag == guard.epoch) return;
-    // Workgroups past the pair blocks evaluate the 1-4 list of system 0 (single-state launches).
-    if (!IMG && (int)blockIdx.x >= nforce) {
-        pairs14_block(descs[0], F, (int)blockIdx.x - nforce);
+    const int lane = threadIdx.x & 63;
+    const int wave = WPG == 1 ? 0 : (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // Groups past the pair blocks evaluate the 1-4 list of system 0 (single-state launches), one block per wave.
+    if (!IMG && (int)blockIdx.x >= nforce_groups) {
+        const int blk = ((int)blockIdx.x - nforce_groups) * WPG + wave;
+        if (blk < descs[0].n14_blocks) pairs14_block(descs[0], F, blk, lane);
         return;
     }
     // Lane l = 8b + a: a is the i-atom within an i-cluster, b the j-atom of
     // the entry's j-cluster.  The entry masks are stored per l' = 8a + b.
-    const int lane = threadIdx.x;
     const int a = lane & 7, b = lane >> 3;
     const int bl = 8 * a + b;
     extern __shared__ __align__(32) unsigned char lds[];
     __shared__ __align__(16) double sX[12];     // image items: R then T of the item's image
     using acc_t = typename std::conditional<F64, double, float>::type;
-    double4 *sI = (double4 *)lds;
-    int *sT = (int *)(lds + NB_SC * 32);
-    double4 *sR = (double4 *)(lds + NB_LDS_I_BYTES);
-    acc_t *sF = (acc_t *)lds;
+    double4 *sR = (double4 *)lds;
+    unsigned char *region = lds + rows_bytes + (size_t)wave * wave_bytes;
+    double4 *sI = (double4 *)region;
+    int *sT = (int *)(region + NB_SC * 32);
+    acc_t *sF = (acc_t *)region;
+    // The group's row table, staged once when the items are one per wave.
+    int rows_sys = -1;
+    if (ROWS_LDS && qp.heads == nullptr && WPG > 1) {
+        rows_sys = items[(int)blockIdx.x * WPG].sys;
+        const SysDesc *D0 = &descs[rows_sys];
+        const GAS double *__restrict__ rows0 = as_global(D0->ljrow);
+        const int nr = 2 * D0->ntypes * D0->ntypes;
+        for (int k = threadIdx.x; k < nr; k += 64 * WPG) sR[k] = load_slot(rows0, k);
+        __syncthreads();
+    }
     int q = blockIdx.x & 7;
     int tries = 0;
-    // heads == null: one work item per workgroup (the hardware dispatcher
+    // heads == null: one work item per wave (the hardware dispatcher
     // balances the load); otherwise a persistent loop over per-XCD queues whose
-    // counters sit on separate 128-byte lines.
+    // counters sit on separate 128-byte lines (one wave per group).
     bool done = false;
     while (!done && tries < 8) {
         int it;
         if (qp.heads == nullptr) {
-            it = blockIdx.x;
+            it = (int)blockIdx.x * WPG + wave;
             done = true;
+            if (it >= nitems) break;
         } else {
             int local = 0;
             if (lane == 0) local = atomicAdd(&qp.heads[32 * q], 1);
@@ -390,6 +424,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((IMG || F64)
             it = qs + local;
         }
         const WorkItem w = items[it];
+        // Empty items pad a batch's systems to whole groups; an item of another system than the staged rows
+        // would be a layout error (reported, never evaluated with the wrong rows).
+        if (w.count <= 0) continue;
+        if (rows_sys >= 0 && w.sys != rows_sys) {
+            raise_error(descs[rows_sys].err, true, NB_ERR_RANGE);
+            continue;
+        }
         const SysDesc *D = &descs[IMG ? (w.sys & 0xffff) : w.sys];
         const GAS double *__restrict__ xq = (const GAS double *)D->xyzq;
         const GAS int *__restrict__ ty = as_global(D->styp);
@@ -401,7 +442,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((IMG || F64)
         const int n_pad = D->n_pad;
         const int nt = D->ntypes;
         const int I0 = w.sc * NB_SCC;
-        acc_t *sJF = (acc_t *)(lds + lds_head);           // [entry][3 * NB_CL]
+        acc_t *sJF = (acc_t *)(region + NB_LDS_I_BYTES);   // [entry][3 * NB_CL]
         int *sJ = (int *)(sJF + 3 * NB_CL * w.count);      // [entry]
         // The item's image (-1: main list); all of its entries share it.
         int img = -1;
@@ -413,13 +454,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((IMG || F64)
             isc = uniform_f64(X[12]);
         }
 
-        __syncthreads();
+        wave_region_sync<WPG>();
         if (IMG && lane < 12) sX[lane] = X[lane];
-        if (ROWS_LDS)
+        if (ROWS_LDS && rows_sys < 0)
             for (int k = lane; k < 2 * nt * nt; k += 64) sR[k] = load_slot(rows, k);
         sI[lane] = load_slot(xq, I0 * NB_CL + lane);
         sT[lane] = 64 * nt * ty[I0 * NB_CL + lane];   // byte offset of the type's first row pair
-        __syncthreads();
+        wave_region_sync<WPG>();
 
         acc_t fx[NB_SCC], fy[NB_SCC], fz[NB_SCC];
 #pragma unroll
@@ -508,27 +549,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((IMG || F64)
             xj = xjn;
             tj = tjn;
         }
-        // i-forces: transpose through LDS so that lane 8b+a sums atom a of
-        // i-cluster c = b over the eight lanes that held its partials.
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < NB_SCC; c++) {
-            sF[(3 * c + 0) * NB_SC + lane] = fx[c];
-            sF[(3 * c + 1) * NB_SC + lane] = fy[c];
-            sF[(3 * c + 2) * NB_SC + lane] = fz[c];
-        }
-        __syncthreads();
-        // Partials of (c, a) sit at lanes 8b' + a, b' = 0..7 (stride 8).
-        acc_t ig[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const acc_t *fr = &sF[(3 * b + k) * NB_SC + a];
-            ig[k] = ((fr[0] + fr[8]) + (fr[16] + fr[24])) + ((fr[32] + fr[40]) + (fr[48] + fr[56]));
-        }
-        const int si = (I0 + b) * NB_CL + a;
+        // The entries' j-force sums (written by other lanes) are complete.
+        wave_region_sync<WPG>();
         bool bad = false;
-#pragma unroll
-        for (int k = 0; k < 3; k++) bad |= fx_atomic_add(facc, n_pad, si, k, IMG ? isc * (double)ig[k] : (double)ig[k]);
         if (!IMG) {
             // j-forces of the whole item.
             for (int k = lane; k < 3 * NB_CL * w.count; k += 64) {
@@ -572,6 +595,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu((IMG || F64)
                 }
             }
         }
+        // i-forces: transpose through LDS (over the records and the flushed
+        // j-force sums) so that lane 8b+a sums atom a of i-cluster c = b over
+        // the eight lanes that held its partials.
+        wave_region_sync<WPG>();
+#pragma unroll
+        for (int c = 0; c < NB_SCC; c++) {
+            sF[(3 * c + 0) * NB_SC + lane] = fx[c];
+            sF[(3 * c + 1) * NB_SC + lane] = fy[c];
+            sF[(3 * c + 2) * NB_SC + lane] = fz[c];
+        }
+        wave_region_sync<WPG>();
+        // Partials of (c, a) sit at lanes 8b' + a, b' = 0..7 (stride 8).
+        acc_t ig[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const acc_t *fr = &sF[(3 * b + k) * NB_SC + a];
+            ig[k] = ((fr[0] + fr[8]) + (fr[16] + fr[24])) + ((fr[32] + fr[40]) + (fr[48] + fr[56]));
+        }
+        const int si = (I0 + b) * NB_CL + a;
+#pragma unroll
+        for (int k = 0; k < 3; k++) bad |= fx_atomic_add(facc, n_pad, si, k, IMG ? isc * (double)ig[k] : (double)ig[k]);
         eel = wave_sum_f64(eel);
         elj = wave_sum_f64(elj);
         if (lane < 2) {
@@ -792,7 +836,7 @@ __global__ __launch_bounds__(64) void k_pairs14(const SysDesc *__restrict__ desc
 {
     const SysDesc &D = descs[blockIdx.y];
     if ((int)blockIdx.x >= D.n14_blocks) return;
-    pairs14_block(D, F, blockIdx.x);
+    pairs14_block(D, F, blockIdx.x, threadIdx.x);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -973,39 +1017,56 @@ void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, in
                        chk);
 }
 
+template <bool ROWS_LDS, bool IMG, bool F64, int WPG>
+static void launch_tiled_instance(dim3 grid, size_t bytes, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
+                                  const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
+                                  int rows_bytes, int wave_bytes, ForceGuard g, int nitems, int nforce_groups)
+{
+    hipExtLaunchKernelGGL((k_force_tiled<ROWS_LDS, IMG, F64, WPG>), grid, dim3(64 * WPG), bytes, s, ev_start, ev_stop, 0,
+                          d_desc, items, q, F, rows_bytes, wave_bytes, g, nitems, nforce_groups);
+}
+
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
                         int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images,
                         hipEvent_t ev_start, hipEvent_t ev_stop, int n14_blocks, bool full_fp64)
 {
     if (images) n14_blocks = 0;
     if (nblocks + n14_blocks <= 0) return;
-    const dim3 grid(nblocks + n14_blocks), block(64);
+    // One item per wave, four waves per group, unless the items come from the persistent queues or the images.
+    const int wpg = (images || q.heads != nullptr) ? 1 : NB_TILED_WPG;
+    const int ngroups = nblk(nblocks, wpg);
+    const dim3 grid(ngroups + nblk(n14_blocks, wpg));
+    const int acc = full_fp64 ? 8 : 4;
+    const int wave_bytes = (int)tiled_lds_wave(max_count, acc);
+    // The row table goes to LDS when the group's LDS still lets four groups of four waves (or ten single waves) share a CU.
+    const size_t limit = wpg == 1 ? NB_LDS_ROW_LIMIT : NB_LDS_GROUP_LIMIT;
+    const bool rows_lds = !full_fp64 && tiled_lds_rows(max_ntypes, true) + (size_t)wpg * wave_bytes <= limit;
+    const int rows_bytes = (int)tiled_lds_rows(max_ntypes, rows_lds);
+    const size_t bytes = rows_bytes + (size_t)wpg * wave_bytes;
+#define NB_TILED_ARGS grid, bytes, s, ev_start, ev_stop, d_desc, items, q, F, rows_bytes, wave_bytes, g, nblocks, ngroups
     if (full_fp64) {
         // No row table: the FP64 pair terms read the raw A and B of the type pair.
-        const int head = (int)tiled_lds_head(max_ntypes, false, 8);
-        const size_t bytes = tiled_lds_bytes(max_ntypes, false, max_count, 8);
-        hipExtLaunchKernelGGL((k_force_tiled<false, false, true>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc,
-                              items, q, F, head, g, nblocks);
-        return;
-    }
-    const bool rows_lds = tiled_lds_bytes(max_ntypes, true, max_count) <= NB_LDS_ROW_LIMIT;
-    const int head = (int)tiled_lds_head(max_ntypes, rows_lds);
-    const size_t bytes = tiled_lds_bytes(max_ntypes, rows_lds, max_count);
-    if (images) {
-        if (rows_lds)
-            hipExtLaunchKernelGGL((k_force_tiled<true, true>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items, q,
-                                  F, head, g, nblocks);
+        if (wpg == 1)
+            launch_tiled_instance<false, false, true, 1>(NB_TILED_ARGS);
         else
-            hipExtLaunchKernelGGL((k_force_tiled<false, true>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items, q,
-                                  F, head, g, nblocks);
+            launch_tiled_instance<false, false, true, NB_TILED_WPG>(NB_TILED_ARGS);
+    } else if (images) {
+        if (rows_lds)
+            launch_tiled_instance<true, true, false, 1>(NB_TILED_ARGS);
+        else
+            launch_tiled_instance<false, true, false, 1>(NB_TILED_ARGS);
+    } else if (wpg == 1) {
+        if (rows_lds)
+            launch_tiled_instance<true, false, false, 1>(NB_TILED_ARGS);
+        else
+            launch_tiled_instance<false, false, false, 1>(NB_TILED_ARGS);
     } else {
         if (rows_lds)
-            hipExtLaunchKernelGGL((k_force_tiled<true, false>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items, q,
-                                  F, head, g, nblocks);
+            launch_tiled_instance<true, false, false, NB_TILED_WPG>(NB_TILED_ARGS);
         else
-            hipExtLaunchKernelGGL((k_force_tiled<false, false>), grid, block, bytes, s, ev_start, ev_stop, 0, d_desc, items,
-                                  q, F, head, g, nblocks);
+            launch_tiled_instance<false, false, false, NB_TILED_WPG>(NB_TILED_ARGS);
     }
+#undef NB_TILED_ARGS
 }
 
 void launch_force_plain(const SysDesc *d_desc, int nsys, int max_n, const AbfsFactors &F, ForceGuard g, hipStream_t s,

@@ -250,13 +250,17 @@ extern "C" void nbabfs_default_options(nbabfs_options *o)
     o->imageEntriesPerItem = 16;
 }
 
-__global__ void k_remap_items(const WorkItem *__restrict__ src, int count, int sys, WorkItem *__restrict__ dst)
+// A system's items in a batch, followed by empty items up to `padded` (whole workgroups of the pair launch).
+__global__ void k_remap_items(const WorkItem *__restrict__ src, int count, int padded, int sys,
+                              WorkItem *__restrict__ dst)
 {
     int k = blockIdx.x * 256 + threadIdx.x;
     if (k < count) {
         WorkItem w = src[k];
         w.sys = sys;
         dst[k] = w;
+    } else if (k < padded) {
+        dst[k] = WorkItem{sys, 0, 0, 0};
     }
 }
 
@@ -2517,7 +2521,7 @@ static int batch_layout(nbabfs_batch *B, const double *const *d_xyz, double *con
     int total = 0;
     B->max_npad = B->max_n = B->max_n14b = 0;
     for (auto *s : B->st) {
-        total += s->nitems;
+        total += nblk(s->nitems, NB_TILED_WPG) * NB_TILED_WPG;
         B->max_npad = std::max(B->max_npad, s->n_pad);
         B->max_n = std::max(B->max_n, s->n);
         B->max_n14b = std::max(B->max_n14b, nblk(s->n14, NB_P14_BLOCK));
@@ -2535,10 +2539,11 @@ static int batch_layout(nbabfs_batch *B, const double *const *d_xyz, double *con
         SysDesc &D = B->h_desc.p[k];
         fill_desc(s, D, d_xyz[k], d_grad ? d_grad[k] : nullptr);
         D.energies = B->energies.p + NB_NENERGIES * (size_t)k;
-        if (s->nitems > 0)
-            hipLaunchKernelGGL(k_remap_items, dim3(nblk(s->nitems, 256)), dim3(256), 0, st, s->items.p, s->nitems, k,
-                               B->items.p + first);
-        first += s->nitems;
+        const int padded = nblk(s->nitems, NB_TILED_WPG) * NB_TILED_WPG;
+        if (padded > 0)
+            hipLaunchKernelGGL(k_remap_items, dim3(nblk(padded, 256)), dim3(256), 0, st, s->items.p, s->nitems, padded,
+                               k, B->items.p + first);
+        first += padded;
         B->last_items[k] = s->items_version;
     }
     HIPCHK(hipMemcpyAsync(B->d_desc.p, B->h_desc.p, ns * sizeof(SysDesc), hipMemcpyHostToDevice, st));

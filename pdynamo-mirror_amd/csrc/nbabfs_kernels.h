@@ -125,6 +125,7 @@ struct ForceGuard {
 };
 void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, int *heads, const PrepareCheck &chk,
                     hipStream_t s);
+// nblocks: work items (one per wave) or, with q.heads, the persistent waves.
 // images: the launch over image-list items (one workgroup per item).  The pair
 // launchers take optional timing events (start, stop: recorded by the runtime
 // at the dispatch's own begin and end; null = not timed).
@@ -133,8 +134,11 @@ void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParam
                         int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images = false,
                         hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr, int n14_blocks = 0,
                         bool full_fp64 = false);
-// The 1-4 list in blocks of NB_P14_BLOCK pairs (k_pairs14, or extra workgroups of a single-state pair launch).
+// The 1-4 list in blocks of NB_P14_BLOCK pairs (k_pairs14, or extra waves of a single-state pair launch).
 #define NB_P14_BLOCK 64
+// Waves per workgroup of the one-item-per-wave pair launches; a batch pads each system's items to a multiple of it
+// (items of one workgroup share a system and its LJ row table).
+#define NB_TILED_WPG 4
 // The plain FP64 CSR pass of nsys systems in one launch (blockIdx.y = system; rows from the descriptors).
 void launch_force_plain(const SysDesc *d_desc, int nsys, int max_n, const AbfsFactors &F, ForceGuard g, hipStream_t s,
                         hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
