@@ -301,6 +301,41 @@ __global__ __launch_bounds__(256) void k_cluster_geom(const double *__restrict__
     centre[c] = make_float4(ce[0], ce[1], ce[2], 0.0f);
 }
 
+// Slot-ordered search records of the self-list search: the atom's FP64
+// position with its index and flag bits (NB_SF_*), and its reference cell.
+__global__ __launch_bounds__(256) void k_slot_records(const double *__restrict__ xyz, const int *__restrict__ slot_atom,
+                                                      int n_pad, const int4 *__restrict__ cellv,
+                                                      const unsigned char *__restrict__ free_flags, int use_grid,
+                                                      double4 *__restrict__ srec, int4 *__restrict__ scell)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= n_pad) return;
+    const int a = slot_atom[s];
+    double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+    int4 c = make_int4(0, 0, 0, 0);
+    int fl = NB_SF_EMPTY;
+    if (a >= 0) {
+        p.x = xyz[3 * (size_t)a];
+        p.y = xyz[3 * (size_t)a + 1];
+        p.z = xyz[3 * (size_t)a + 2];
+        c = cellv[a];
+        fl = free_flags ? (free_flags[a] & (NB_SF_FREE | NB_SF_QC)) : NB_SF_FREE;
+        if (use_grid && c.w == 0) fl |= NB_SF_OFFGRID;
+    }
+    p.w = __hiloint2double(fl, a);
+    srec[s] = p;
+    scell[s] = c;
+}
+
+// The reference's squared distance, in its operation order
+// (PairListGenerator.c:131-140: no contraction into fused multiply-adds).
+__device__ __forceinline__ double ref_dist2(double xi, double yi, double zi, double xj, double yj, double zj)
+{
+#pragma clang fp contract(off)
+    const double xij = xi - xj, yij = yi - yj, zij = zi - zj;
+    return ((xij * xij) + (yij * yij)) + (zij * zij);
+}
+
 __device__ __forceinline__ float box_gap2(float4 alo, float4 ahi, float4 blo, float4 bhi)
 {
     float gx = fmaxf(0.0f, fmaxf(alo.x - bhi.x, blo.x - ahi.x));
@@ -311,21 +346,26 @@ __device__ __forceinline__ float box_gap2(float4 alo, float4 ahi, float4 blo, fl
 
 // ------------------------------------------------------------------------------------------------
 // Cluster-pair search: one workgroup of NB_SEARCH_WAVES wavefronts per
-// super-cluster.  The super-cluster's atoms (FP64 position, grid cell, free
-// flag, exclusion list) and the grid stencil are staged in LDS.  The candidate
-// j-clusters of all x-y columns within reach are scanned as one flattened
-// range and appended to an LDS list in scan order; the waves then share out
-// the candidates round-robin and evaluate each with the reference pair rule
-// (the j-atom's data in registers, fetched one candidate ahead), writing the
-// result into the candidate's own scratch slot.  k_compact keeps the slots
-// with any listed pair, in candidate order, so the entries of each
-// super-cluster come out in a deterministic order.
+// super-cluster.  The super-cluster's atoms and the grid stencil are staged
+// in LDS.  The candidate j-clusters of all x-y columns within reach are
+// scanned as one flattened range and appended to an LDS list in scan order;
+// the waves then share out the candidates round-robin and evaluate each with
+// the reference pair rule (the j-atom's record in registers, fetched one
+// candidate ahead), writing the result into the candidate's own scratch slot.
+// k_compact keeps the slots with any listed pair, in candidate order, so the
+// entries of each super-cluster come out in a deterministic order.
 //
 // The reference pair predicate for atoms i, j (PairListGenerator.c:103-143,
 // 1020-1150): both atoms in the MM selection, not excluded, OR selection (free atoms), and either the grid
 // rule (both on grid; stencil flag 2 = whole cell pair within the cutoff,
 // 1 = FP64 distance test, 0 = out of range) or, for the direct method, the
-// FP64 distance test alone.
+// FP64 distance test alone.  The self-list evaluation computes every part of
+// it without branches (a tile's lanes all take the same path); only the
+// exclusion scan, needed when j falls inside i's excluded index range, is a
+// branch.
+//
+// STEN (self lists): 0 direct method, 1 grid with the stencil in LDS, 2 grid
+// with the stencil read from global memory (more than NB_STENCIL_LDS cells).
 // ------------------------------------------------------------------------------------------------
 #define NB_SEARCH_WAVES 8
 #define NB_SEARCH_THREADS (64 * NB_SEARCH_WAVES)
@@ -340,10 +380,9 @@ struct JAtom {
     int atom, free_;
 };
 
-template <bool CROSS>
-__device__ __forceinline__ JAtom load_jatom(int J, int b, const double *__restrict__ xyz,
-                                            const int4 *__restrict__ cellv, const int *__restrict__ slot_atom,
-                                            const unsigned char *__restrict__ free_flags)
+__device__ __forceinline__ JAtom load_jatom_cross(int J, int b, const double *__restrict__ xyz,
+                                                  const int *__restrict__ slot_atom,
+                                                  const unsigned char *__restrict__ free_flags)
 {
     JAtom r;
     r.atom = slot_atom[J * NB_CL + b];
@@ -354,10 +393,137 @@ __device__ __forceinline__ JAtom load_jatom(int J, int b, const double *__restri
         r.x = xyz[3 * (size_t)r.atom];
         r.y = xyz[3 * (size_t)r.atom + 1];
         r.z = xyz[3 * (size_t)r.atom + 2];
-        if (!CROSS) r.cell = cellv[r.atom];
         if (free_flags) r.free_ = free_flags[r.atom];
     }
     return r;
+}
+
+// Write candidate k's entry (scratch slot base + k); returns 1 if it lists a pair.
+__device__ __forceinline__ int search_store(Entry *__restrict__ scratch, size_t sbase, int slot, int scap, int lane,
+                                            int Jc, int sc, int img, int lbits, int cbits)
+{
+    if (slot < scap) {
+        Entry *en = &scratch[sbase + slot];
+        en->bits[lane] = (unsigned char)lbits;
+        if (lane == 0) {
+            en->J = Jc;
+            en->sc = sc;
+            en->cbits = cbits;
+            en->img = img;
+        }
+    }
+    return cbits ? 1 : 0;
+}
+
+// Image lists: the candidates sCand[0..npend) of one wave (k = w, w + NB_SEARCH_WAVES, ...).
+__device__ __forceinline__ int search_eval_cross(const SearchParams &P, int npend, int w, int lane, int I0, int sc,
+                                                 int img, int base, int scap, size_t sbase, const int *sCand,
+                                                 const unsigned char *sCm, const int *sIatom, const int *sIfree,
+                                                 const int *sIon, const double *sIx, const double *sIy,
+                                                 const double *sIz, const double *__restrict__ jxyz,
+                                                 const int *__restrict__ slot_atom, Entry *__restrict__ scratch)
+{
+    const int a = lane >> 3, b = lane & 7;
+    int nvalid = 0;
+    JAtom jn;
+    if (w < npend) jn = load_jatom_cross(sCand[w], b, jxyz, slot_atom, P.free_flags);
+    for (int k = w; k < npend; k += NB_SEARCH_WAVES) {
+        const JAtom jd = jn;
+        const int Jc = sCand[k];
+        const int cmc = sCm[k];
+        if (k + NB_SEARCH_WAVES < npend) jn = load_jatom_cross(sCand[k + NB_SEARCH_WAVES], b, jxyz, slot_atom, P.free_flags);
+        int lbits = 0, cbits = 0;
+        for (int c = 0; c < NB_SCC; c++) {
+            if (!(cmc & (1 << c))) continue;
+            const int li = c * NB_CL + a;
+            bool ok = (sIatom[li] >= 0) && (jd.atom >= 0);
+            if (ok && P.free_flags) {
+                // Atom flags: bit 0 free (the OR selection), bit 1 outside the MM
+                // selection (pure QC atoms, NBModelABFSState.c:339-341).
+                const int fl = sIfree[li] | jd.free_;
+                ok = (fl & 1) && !(fl & 2);
+            }
+            if (ok && P.use_grid) ok = sIon[li] != 0;
+            if (ok) ok = ref_dist2(sIx[li], sIy[li], sIz[li], jd.x, jd.y, jd.z) <= P.cutoff2;
+            if (ok) lbits |= 1 << c;
+            if (__ballot(ok)) cbits |= 1 << c;
+        }
+        nvalid += search_store(scratch, sbase, base + k, scap, lane, Jc, sc, img, lbits, cbits);
+    }
+    return nvalid;
+}
+
+// Self lists: the candidates sCand[0..npend) of one wave.  Lane 8a + b
+// evaluates i-atom (c, a) of each listed tile c against j-atom b; lanes with
+// b < a hold the lower triangle of the self tile (J = I0 + c).  Every part of
+// the rule is computed without branches; only the exclusion scan (j inside
+// i's excluded index range) branches.
+template <int STEN>
+__device__ __forceinline__ int search_eval_self(const SearchParams &P, int npend, int w, int lane, int I0, int sc,
+                                                int base, int scap, size_t sbase, const int *sCand,
+                                                const unsigned char *sCm, const double4 *sIp, const int4 *sIc,
+                                                const int2 *sIxr, const int *sXoff, const int *sXcnt,
+                                                const int *sXpool, const unsigned char *sSten,
+                                                Entry *__restrict__ scratch)
+{
+    const int a = lane >> 3, b = lane & 7;
+    const unsigned wst = 2 * P.u + 1, nst = wst * wst * wst;
+    int nvalid = 0;
+    double4 jpn = make_double4(0.0, 0.0, 0.0, 0.0);
+    int4 jcn = make_int4(0, 0, 0, 0);
+    if (w < npend) {
+        const int Jn = sCand[w];
+        jpn = P.srec[Jn * NB_CL + b];
+        if (STEN) jcn = P.scell[Jn * NB_CL + b];
+    }
+    for (int k = w; k < npend; k += NB_SEARCH_WAVES) {
+        const double4 jp = jpn;
+        const int4 jc = jcn;
+        const int Jc = __builtin_amdgcn_readfirstlane(sCand[k]);
+        const int cmc = __builtin_amdgcn_readfirstlane(sCm[k]);
+        if (k + NB_SEARCH_WAVES < npend) {
+            const int Jn = sCand[k + NB_SEARCH_WAVES];
+            jpn = P.srec[Jn * NB_CL + b];
+            if (STEN) jcn = P.scell[Jn * NB_CL + b];
+        }
+        const int jatom = __double2loint(jp.w), jfl = __double2hiint(jp.w);
+        int lbits = 0, cbits = 0;
+#pragma unroll
+        for (int c = 0; c < NB_SCC; c++) {
+            if (!(cmc & (1 << c))) continue;
+            const int li = c * NB_CL + a;
+            const double4 ip = sIp[li];
+            const int4 ic = sIc[li];
+            const int2 xr = sIxr[li];
+            // Both atoms real, in the MM selection and (grid) on the grid, one of them free.
+            bool ok = ((ic.w | jfl) & (NB_SF_FREE | NB_SF_QC | NB_SF_OFFGRID | NB_SF_EMPTY)) == NB_SF_FREE;
+            ok = ok & ((Jc < I0 + c) | (b < a));
+            const bool near = ref_dist2(ip.x, ip.y, ip.z, jp.x, jp.y, jp.z) <= P.cutoff2;
+            if (STEN) {
+                // Cell offset + u per axis (sIc holds the cell - u), in [0, 2u] within the stencil.
+                const unsigned ux = (unsigned)(jc.x - ic.x), uy = (unsigned)(jc.y - ic.y), uz = (unsigned)(jc.z - ic.z);
+                const bool inb = max(max(ux, uy), uz) < wst;
+                const unsigned si = min((ux * wst + uy) * wst + uz, nst - 1u);
+                const int flag = STEN == 1 ? (int)sSten[si] : (int)P.stencil[si];
+                ok = ok & inb & ((flag == 2) | ((flag == 1) & near));
+            } else {
+                ok = ok & near;
+            }
+            const bool inx = ok & (jatom >= xr.x) & (jatom <= xr.y);
+            if (__ballot(inx)) {
+                if (inx) {
+                    const int xo = sXoff[li], xn = sXcnt[li];
+                    const int *xl = xo >= 0 ? sXpool + xo : P.xidx + (-1 - xo);
+                    for (int x = 0; x < xn; x++)
+                        if (xl[x] == jatom) ok = false;
+                }
+            }
+            lbits |= ok ? (1 << c) : 0;
+            if (__ballot(ok)) cbits |= 1 << c;
+        }
+        nvalid += search_store(scratch, sbase, base + k, scap, lane, Jc, sc, -1, lbits, cbits);
+    }
+    return nvalid;
 }
 
 //
@@ -368,9 +534,8 @@ __device__ __forceinline__ JAtom load_jatom(int J, int b, const double *__restri
 // no exclusions, every (i, j) pair including an atom with its own image, the
 // OR selection, i on the reference grid (grid method), and the FP64 distance
 // test against the image coordinates GenerateImageLists produced.
-template <bool CROSS>
+template <bool CROSS, int STEN>
 __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, const double *__restrict__ xyz,
-                                                              const int4 *__restrict__ cellv,
                                                               const int *__restrict__ slot_atom,
                                                               const float4 *__restrict__ blo,
                                                               const float4 *__restrict__ bhi,
@@ -390,11 +555,16 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
     const int a = lane >> 3, b = lane & 7;
     const int I0 = sc * NB_SCC;
     __shared__ float4 sIlo[NB_SCC], sIhi[NB_SCC];
-    __shared__ double sIx[NB_SC], sIy[NB_SC], sIz[NB_SC];
-    __shared__ int4 sIcell[NB_SC];
-    __shared__ int sIatom[NB_SC], sIfree[NB_SC], sXoff[NB_SC], sXcnt[NB_SC], sXlo[NB_SC], sXhi[NB_SC];
-    __shared__ int sXpool[NB_XPOOL];
-    __shared__ unsigned char sSten[NB_STENCIL_LDS];
+    // Cross lists: the i-atoms' positions, on-grid and free flags.
+    __shared__ double sIx[CROSS ? NB_SC : 1], sIy[CROSS ? NB_SC : 1], sIz[CROSS ? NB_SC : 1];
+    __shared__ int sIatom[CROSS ? NB_SC : 1], sIfree[CROSS ? NB_SC : 1], sIon[CROSS ? NB_SC : 1];
+    // Self lists: the i-atoms' records, their cells minus u with the flag bits, exclusion index ranges.
+    __shared__ double4 sIp[CROSS ? 1 : NB_SC];
+    __shared__ int4 sIc[CROSS ? 1 : NB_SC];
+    __shared__ int2 sIxr[CROSS ? 1 : NB_SC];
+    __shared__ int sXoff[CROSS ? 1 : NB_SC], sXcnt[CROSS ? 1 : NB_SC];
+    __shared__ int sXpool[CROSS ? 1 : NB_XPOOL];
+    __shared__ unsigned char sSten[STEN == 1 ? NB_STENCIL_LDS : 1];
     __shared__ int sCand[NB_CAND];
     __shared__ unsigned char sCm[NB_CAND];
     __shared__ int sColStart[NB_MAXCOL], sColPre[NB_MAXCOL + 1];
@@ -406,43 +576,48 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
             sIlo[lane] = blo[I0 + lane];
             sIhi[lane] = bhi[I0 + lane];
         }
-        const int ia = slot_atom[I0 * NB_CL + lane];
-        int xs = 0, xc = 0;
-        sIatom[lane] = ia;
-        if (ia >= 0) {
-            sIx[lane] = xyz[3 * (size_t)ia];
-            sIy[lane] = xyz[3 * (size_t)ia + 1];
-            sIz[lane] = xyz[3 * (size_t)ia + 2];
-            if (CROSS) sIcell[lane] = make_int4(0, 0, 0, P.ongrid ? (int)P.ongrid[ia] : 1);
-            else sIcell[lane] = cellv[ia];
-            sIfree[lane] = P.free_flags ? P.free_flags[ia] : 1;
-            if (!CROSS) {
+        if (CROSS) {
+            const int ia = slot_atom[I0 * NB_CL + lane];
+            sIatom[lane] = ia;
+            if (ia >= 0) {
+                sIx[lane] = xyz[3 * (size_t)ia];
+                sIy[lane] = xyz[3 * (size_t)ia + 1];
+                sIz[lane] = xyz[3 * (size_t)ia + 2];
+                sIon[lane] = P.ongrid ? (int)P.ongrid[ia] : 1;
+                sIfree[lane] = P.free_flags ? P.free_flags[ia] : 1;
+            }
+        } else {
+            const double4 ip = P.srec[I0 * NB_CL + lane];
+            const int4 ic = P.scell[I0 * NB_CL + lane];
+            const int ia = __double2loint(ip.w);
+            sIp[lane] = ip;
+            sIc[lane] = make_int4(ic.x - P.u, ic.y - P.u, ic.z - P.u, __double2hiint(ip.w));
+            int xs = 0, xc = 0;
+            if (ia >= 0) {
                 xs = P.xptr[ia];
                 xc = P.xptr[ia + 1] - xs;
             }
-        }
-        int pre = xc;   // inclusive scan of exclusion counts over the wave
+            int pre = xc;   // inclusive scan of exclusion counts over the wave
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int v = __shfl_up(pre, o, 64);
-            if (lane >= o) pre += v;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(pre, o, 64);
+                if (lane >= o) pre += v;
+            }
+            const int off = pre - xc;
+            if (off + xc <= NB_XPOOL) {
+                for (int k = 0; k < xc; k++) sXpool[off + k] = P.xidx[xs + k];
+                sXoff[lane] = off;
+            } else {
+                sXoff[lane] = -1 - xs;      // list read from global memory
+            }
+            sXcnt[lane] = xc;
+            // Rows are ascending: j outside [first, last] is not excluded.
+            sIxr[lane] = xc > 0 ? make_int2(P.xidx[xs], P.xidx[xs + xc - 1]) : make_int2(0, -1);
         }
-        const int off = pre - xc;
-        if (off + xc <= NB_XPOOL) {
-            for (int k = 0; k < xc; k++) sXpool[off + k] = P.xidx[xs + k];
-            sXoff[lane] = off;
-        } else {
-            sXoff[lane] = -1 - xs;      // list read from global memory
-        }
-        sXcnt[lane] = xc;
-        // Rows are ascending: j outside [first, last] is not excluded.
-        sXlo[lane] = xc > 0 ? P.xidx[xs] : 0;
-        sXhi[lane] = xc > 0 ? P.xidx[xs + xc - 1] : -1;
     }
     const int wst = 2 * P.u + 1;
-    const int nst = (P.use_grid && !CROSS) ? wst * wst * wst : 0;
-    const bool sten_lds = nst <= NB_STENCIL_LDS;
-    if (sten_lds)
+    const int nst = wst * wst * wst;
+    if (STEN == 1)
         for (int k = tid; k < nst; k += NB_SEARCH_THREADS) sSten[k] = P.stencil[k];
     __syncthreads();
 
@@ -476,72 +651,12 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
         // Evaluate the listed candidates sCand[0..npend): this wave takes
         // k = w, w + NB_SEARCH_WAVES, ...
         auto evaluate = [&](int npend) {
-            JAtom jn;
-            if (w < npend) jn = load_jatom<CROSS>(sCand[w], b, jxyz, cellv, slot_atom, P.free_flags);
-            for (int k = w; k < npend; k += NB_SEARCH_WAVES) {
-                const JAtom jd = jn;
-                const int Jc = sCand[k];
-                const int cmc = sCm[k];
-                if (k + NB_SEARCH_WAVES < npend)
-                    jn = load_jatom<CROSS>(sCand[k + NB_SEARCH_WAVES], b, jxyz, cellv, slot_atom, P.free_flags);
-                int lbits = 0, cbits = 0;
-                for (int c = 0; c < NB_SCC; c++) {
-                    if (!(cmc & (1 << c))) continue;
-                    const int li = c * NB_CL + a;
-                    const int i = sIatom[li];
-                    bool ok = (i >= 0) && (jd.atom >= 0) && (CROSS || (Jc < I0 + c) || (b < a));
-                    if (ok && P.free_flags) {
-                        // Atom flags: bit 0 free (the OR selection), bit 1 outside the MM
-                        // selection (pure QC atoms, NBModelABFSState.c:339-341).
-                        const int fl = sIfree[li] | jd.free_;
-                        ok = (fl & 1) && !(fl & 2);
-                    }
-                    int flag = 1;
-                    if (CROSS) {
-                        if (ok && P.use_grid) ok = sIcell[li].w != 0;
-                    } else if (ok && P.use_grid) {
-                        const int4 ci = sIcell[li];
-                        ok = (ci.w & jd.cell.w) != 0;
-                        const int dx = jd.cell.x - ci.x, dy = jd.cell.y - ci.y, dz = jd.cell.z - ci.z;
-                        ok = ok && dx >= -P.u && dx <= P.u && dy >= -P.u && dy <= P.u && dz >= -P.u && dz <= P.u;
-                        if (ok) {
-                            const int si = ((dx + P.u) * wst + (dy + P.u)) * wst + (dz + P.u);
-                            flag = sten_lds ? sSten[si] : P.stencil[si];
-                            ok = flag != 0;
-                        }
-                    }
-                    if (ok && flag == 1) {
-#pragma clang fp contract(off)
-                        const double xij = sIx[li] - jd.x, yij = sIy[li] - jd.y, zij = sIz[li] - jd.z;
-                        const double r2 = ((xij * xij) + (yij * yij)) + (zij * zij);
-                        ok = r2 <= P.cutoff2;
-                    }
-                    if (!CROSS && ok && jd.atom >= sXlo[li] && jd.atom <= sXhi[li]) {
-                        const int xo = sXoff[li], xn = sXcnt[li];
-                        if (xo >= 0) {
-                            for (int x = 0; x < xn; x++)
-                                if (sXpool[xo + x] == jd.atom) ok = false;
-                        } else {
-                            for (int x = 0; x < xn; x++)
-                                if (P.xidx[-1 - xo + x] == jd.atom) ok = false;
-                        }
-                    }
-                    if (ok) lbits |= 1 << c;
-                    if (__ballot(ok)) cbits |= 1 << c;
-                }
-                const int slot = base + k;
-                if (slot < scap) {
-                    Entry *en = &scratch[sbase + slot];
-                    en->bits[lane] = (unsigned char)lbits;
-                    if (lane == 0) {
-                        en->J = Jc;
-                        en->sc = sc;
-                        en->cbits = cbits;
-                        en->img = img;
-                    }
-                }
-                if (cbits) nvalid++;
-            }
+            if (CROSS)
+                nvalid += search_eval_cross(P, npend, w, lane, I0, sc, img, base, scap, sbase, sCand, sCm, sIatom,
+                                            sIfree, sIon, sIx, sIy, sIz, jxyz, slot_atom, scratch);
+            else
+                nvalid += search_eval_self<STEN>(P, npend, w, lane, I0, sc, base, scap, sbase, sCand, sCm, sIp, sIc,
+                                                 sIxr, sXoff, sXcnt, sXpool, sSten, scratch);
         };
 
         int npend = 0;
@@ -1046,22 +1161,39 @@ void launch_cluster_geom(const double *xyz, const int *slot_atom, int ncl, const
                        origin[1], origin[2], blo, bhi, centre);
 }
 
-void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4 *cellv, const int *slot_atom,
-                   const float4 *blo, const float4 *bhi, const int2 *colcl, Entry *scratch, int *ncand, int *counts,
-                   int *overflow, hipStream_t s)
+void launch_slot_records(const double *xyz, const int *slot_atom, int n_pad, const int4 *cellv,
+                         const unsigned char *free_flags, int use_grid, double4 *srec, int4 *scell, hipStream_t s)
 {
-    if (nsc > 0)
-        hipLaunchKernelGGL(k_search<false>, dim3(nsc), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, cellv, slot_atom, blo,
-                           bhi, colcl, scratch, ncand, counts, overflow);
+    if (n_pad > 0)
+        hipLaunchKernelGGL(k_slot_records, dim3(nblk(n_pad, 256)), dim3(256), 0, s, xyz, slot_atom, n_pad, cellv,
+                           free_flags, use_grid, srec, scell);
 }
 
-void launch_search_images(const SearchParams &P, int nsc, int nimg, const double *xyz, const int4 *cellv,
-                          const int *slot_atom, const float4 *blo, const float4 *bhi, Entry *scratch, int *ncand,
-                          int *counts, int *overflow, hipStream_t s)
+void launch_search(const SearchParams &P, int nsc, const double *xyz, const int *slot_atom, const float4 *blo,
+                   const float4 *bhi, const int2 *colcl, Entry *scratch, int *ncand, int *counts, int *overflow,
+                   hipStream_t s)
+{
+    if (nsc <= 0) return;
+    const int wst = 2 * P.u + 1;
+    const int sten = !P.use_grid ? 0 : (wst * wst * wst <= NB_STENCIL_LDS ? 1 : 2);
+    if (sten == 0)
+        hipLaunchKernelGGL((k_search<false, 0>), dim3(nsc), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, slot_atom, blo, bhi,
+                           colcl, scratch, ncand, counts, overflow);
+    else if (sten == 1)
+        hipLaunchKernelGGL((k_search<false, 1>), dim3(nsc), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, slot_atom, blo, bhi,
+                           colcl, scratch, ncand, counts, overflow);
+    else
+        hipLaunchKernelGGL((k_search<false, 2>), dim3(nsc), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, slot_atom, blo, bhi,
+                           colcl, scratch, ncand, counts, overflow);
+}
+
+void launch_search_images(const SearchParams &P, int nsc, int nimg, const double *xyz, const int *slot_atom,
+                          const float4 *blo, const float4 *bhi, Entry *scratch, int *ncand, int *counts, int *overflow,
+                          hipStream_t s)
 {
     if (nsc > 0 && nimg > 0)
-        hipLaunchKernelGGL(k_search<true>, dim3(nsc, nimg), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, cellv,
-                           slot_atom, blo, bhi, (const int2 *)nullptr, scratch, ncand, counts, overflow);
+        hipLaunchKernelGGL((k_search<true, 0>), dim3(nsc, nimg), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, slot_atom, blo,
+                           bhi, (const int2 *)nullptr, scratch, ncand, counts, overflow);
 }
 
 void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int nblocks, int nsc, int per_item,

@@ -361,6 +361,8 @@ struct nbabfs_state {
     DBuf<int> slot_atom, atom_slot;
     DBuf<int4> cellv;
     DBuf<float4> blo, bhi, centre;
+    DBuf<double4> srec;      // slot-ordered search records (self-list search)
+    DBuf<int4> scell;
     DBuf<Entry> entries, scratch;
     DBuf<WorkItem> items, items2;
     DBuf<int> icost, ihist, icur;
@@ -967,9 +969,14 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     CHK(S->bhi.ensure(S->ncl));
     CHK(S->centre.ensure(S->ncl));
     launch_cluster_geom(d_xyz, S->slot_atom.p, S->ncl, g.corg, S->blo.p, S->bhi.p, S->centre.p, st);
+    CHK(S->srec.ensure(S->n_pad));
+    CHK(S->scell.ensure(S->n_pad));
+    launch_slot_records(d_xyz, S->slot_atom.p, S->n_pad, S->cellv.p, S->has_flags ? S->freef.p : nullptr, g.use_grid,
+                        S->srec.p, S->scell.p, st);
 
     // Cluster-pair search with the exact reference predicate.
     SearchParams P;
+    std::memset(&P, 0, sizeof(P));
     P.cutoff2 = o.listCutoff * o.listCutoff;
     P.stencil = S->stencil.p;
     P.u = S->u;
@@ -981,6 +988,8 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     P.cw = (float)cw;
     P.ncx = g.ncx;
     P.ncy = g.ncy;
+    P.srec = S->srec.p;
+    P.scell = S->scell.p;
     CHK(S->counts.ensure(S->nsc));
     CHK(S->ncand.ensure(S->nsc));
     // Search, then the entry and item offsets; the totals and the capacity
@@ -995,8 +1004,8 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
         CHK(S->scratch.ensure((size_t)S->nsc * S->cap));
         CHK(S->flag.ensure(1));
         launch_fill_int(S->flag.p, 1, 0, st);
-        launch_search(P, S->nsc, d_xyz, S->cellv.p, S->slot_atom.p, S->blo.p, S->bhi.p, S->colcl.p, S->scratch.p,
-                      S->ncand.p, S->counts.p, S->flag.p, st);
+        launch_search(P, S->nsc, d_xyz, S->slot_atom.p, S->blo.p, S->bhi.p, S->colcl.p, S->scratch.p, S->ncand.p,
+                      S->counts.p, S->flag.p, st);
         launch_scan(S->counts.p, S->eoff.p, S->nsc, S->scantmp.p, st);
         launch_items_per_sc(S->counts.p, S->nsc, per_item, S->nits.p, st);
         launch_scan(S->nits.p, S->ioff.p, S->nsc, S->scantmp.p, st);
@@ -1594,8 +1603,8 @@ static int build_images(nbabfs_state *S, const double *d_xyz)
             CHK(S->iscratch.ensure(std::max(total_cand, 1)));
             P.soff = S->isoff.p;
             P.scap = S->ibound.p;
-            launch_search_images(P, nsc, ns, d_xyz, S->cellv.p, S->slot_atom.p, S->blo.p, S->bhi.p, S->iscratch.p,
-                                 S->incand.p, S->icounts.p, S->flag.p, st);
+            launch_search_images(P, nsc, ns, d_xyz, S->slot_atom.p, S->blo.p, S->bhi.p, S->iscratch.p, S->incand.p,
+                                 S->icounts.p, S->flag.p, st);
             launch_scan(S->icounts.p, S->ieoff.p, nb, S->scantmp.p, st);
             const int per_item = image_per_item(o);
             launch_items_per_sc(S->icounts.p, nb, per_item, S->inits.p, st);

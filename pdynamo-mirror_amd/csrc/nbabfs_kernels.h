@@ -37,7 +37,16 @@ struct SearchParams {
     const int *soff;                      // scratch offset and capacity of each (image, super-cluster)
     const int *scap;
     int n, ncl;
+    // Self lists: slot-ordered search records (k_slot_records).
+    const double4 *srec;
+    const int4 *scell;
 };
+
+// Search record flag bits (the high word of srec[s].w; the low word is the atom, -1 for an empty slot).
+#define NB_SF_FREE 1        // free atom (every atom when no atom is fixed)
+#define NB_SF_QC 2          // outside the MM selection (pure QC atom)
+#define NB_SF_OFFGRID 4     // grid method: the atom lies outside the reference grid
+#define NB_SF_EMPTY 8       // padding slot
 
 // One step of a transformation's (a, b, c) sequence in GenerateImageLists:
 // the coordinates are displaced by d, stored as image `out` when that image is
@@ -61,9 +70,13 @@ void launch_slots(const int *sorted, const int *key, int n, int nzb, const int *
                   hipStream_t s);
 void launch_cluster_geom(const double *xyz, const int *slot_atom, int ncl, const double *origin, float4 *blo,
                          float4 *bhi, float4 *centre, hipStream_t s);
-void launch_search(const SearchParams &P, int nsc, const double *xyz, const int4 *cellv, const int *slot_atom,
-                   const float4 *blo, const float4 *bhi, const int2 *colcl, Entry *scratch, int *ncand, int *counts,
-                   int *overflow, hipStream_t s);
+// Slot-ordered search records: {x, y, z, atom | flags << 32} and the atom's reference cell.
+void launch_slot_records(const double *xyz, const int *slot_atom, int n_pad, const int4 *cellv,
+                         const unsigned char *free_flags, int use_grid, double4 *srec, int4 *scell, hipStream_t s);
+// Self-list search (the records of P.srec / P.scell must describe the current coordinates and cells).
+void launch_search(const SearchParams &P, int nsc, const double *xyz, const int *slot_atom, const float4 *blo,
+                   const float4 *bhi, const int2 *colcl, Entry *scratch, int *ncand, int *counts, int *overflow,
+                   hipStream_t s);
 void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int nblocks, int nsc, int per_item,
                       int sys, WorkItem *items, hipStream_t s, bool images = false);
 // Image lists: R'x + T' per transformation (12 doubles each), the stepped image
@@ -72,9 +85,9 @@ void launch_image_transform(const double *xyz, int n, const double *rt, int ntra
 void launch_image_coords(const double *tcoords, int n, int ntrans, const int *step_off, const ImageStep *steps,
                          double *out, hipStream_t s);
 void launch_ongrid(const double *xyz, int n, const BuildGrid &g, unsigned char *flags, hipStream_t s);
-void launch_search_images(const SearchParams &P, int nsc, int nimg, const double *xyz, const int4 *cellv,
-                          const int *slot_atom, const float4 *blo, const float4 *bhi, Entry *scratch, int *ncand,
-                          int *counts, int *overflow, hipStream_t s);
+void launch_search_images(const SearchParams &P, int nsc, int nimg, const double *xyz, const int *slot_atom,
+                          const float4 *blo, const float4 *bhi, Entry *scratch, int *ncand, int *counts, int *overflow,
+                          hipStream_t s);
 // Upper bound of each (image, super-cluster) block's candidates (its scratch capacity).
 void launch_image_bounds(const SearchParams &P, int nsc, int nimg, const float4 *blo, const float4 *bhi, int *bound,
                          hipStream_t s);
