@@ -144,24 +144,28 @@ __global__ __launch_bounds__(256) void k_scan_add(int *__restrict__ out, int m, 
 // slot order (and everything downstream) is deterministic.
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_scatter(const int *__restrict__ key, int n, int *__restrict__ cursor,
-                                                 int *__restrict__ sorted)
+                                                 int *__restrict__ unsorted)
 {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     int pos = atomicAdd(&cursor[key[i]], 1);
-    sorted[pos] = i;
+    unsorted[pos] = i;
 }
 
-__global__ __launch_bounds__(256) void k_bucket_sort(const int *__restrict__ off, int nkeys, int n, int *__restrict__ sorted)
+// Deterministic bucket order: atom i goes to its bucket's start plus the
+// number of the bucket's atoms with a smaller index (ascending atom order,
+// whatever order the scatter's atomics produced).
+__global__ __launch_bounds__(256) void k_bucket_rank(const int *__restrict__ key, const int *__restrict__ off,
+                                                     int nkeys, int n, const int *__restrict__ unsorted,
+                                                     int *__restrict__ sorted)
 {
-    int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= nkeys) return;
-    int b = off[k], e = (k + 1 < nkeys) ? off[k + 1] : n;
-    for (int p = b + 1; p < e; p++) {
-        int v = sorted[p], q = p - 1;
-        while (q >= b && sorted[q] > v) { sorted[q + 1] = sorted[q]; q--; }
-        sorted[q + 1] = v;
-    }
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int k = key[i];
+    const int b = off[k], e = (k + 1 < nkeys) ? off[k + 1] : n;
+    int r = 0;
+    for (int p = b; p < e; p++) r += unsorted[p] < i ? 1 : 0;
+    sorted[b + r] = i;
 }
 
 // Column sizes padded to whole super-clusters.
@@ -203,52 +207,60 @@ __global__ __launch_bounds__(256) void k_slots(const int *__restrict__ sorted, c
 // and cut at 16, each quarter ranked by z and cut at 8; full clusters come
 // first.  Ties are broken by atom index, so the order is deterministic.  The
 // pair set does not depend on it (every pair is tested atom by atom).
+// Order of the split: coordinate, then atom index; empty slots (atom -1, key
+// +infinity) after every atom.
 __device__ __forceinline__ bool before(double u, int a, double v, int b)
 {
-    return u < v || (u == v && a < b);
+    const unsigned ua = (unsigned)a, ub = (unsigned)b;   // -1 (empty) compares last
+    return u < v || (u == v && ua < ub);
 }
 
+// One bitonic sort of the wave's lanes in blocks of n (a power of two, up to
+// 64), ascending in every block: lane l ends up holding the element of rank
+// l % n of its block.  An element is its key and the lane it came from.
+__device__ __forceinline__ void wave_bitonic(double &key, int &src, int &atom, int n)
+{
+    const int l = threadIdx.x & 63;
+    for (int size = 2; size <= n; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const int pl = l ^ stride;
+            const double pk = __hiloint2double(__shfl(__double2hiint(key), pl, 64), __shfl(__double2loint(key), pl, 64));
+            const int ps = __shfl(src, pl, 64);
+            const int pa = __shfl(atom, pl, 64);
+            const bool up = size == n || (l & size) == 0;
+            const bool keep_min = ((l & stride) == 0) == up;
+            const bool pless = before(pk, pa, key, atom);
+            if (pless == keep_min) {
+                key = pk;
+                src = ps;
+                atom = pa;
+            }
+        }
+    }
+}
+
+// 2 x 2 x 2 split of each super-cluster: the 64 atoms are sorted by x, each
+// half of 32 by y, each quarter of 16 by z, and lane l's atom after the three
+// sorts takes slot l (cluster l / 8).  Empty slots sort last at every level, so
+// the atoms of a part-filled super-cluster fill its first slots of each part.
 __global__ __launch_bounds__(64) void k_sc_split(const double *__restrict__ xyz, int *__restrict__ slot_atom,
                                                  int *__restrict__ atom_slot)
 {
-    __shared__ double sx[NB_SC], sy[NB_SC], sz[NB_SC];
-    __shared__ int sa[NB_SC], sg[NB_SC], sOut[NB_SC];
     const int l = threadIdx.x;
     const size_t base = (size_t)blockIdx.x * NB_SC;
-    const int a = slot_atom[base + l];
-    const bool valid = a >= 0;
-    sa[l] = a;
-    sx[l] = valid ? xyz[3 * (size_t)a] : 0.0;
-    sy[l] = valid ? xyz[3 * (size_t)a + 1] : 0.0;
-    sz[l] = valid ? xyz[3 * (size_t)a + 2] : 0.0;
-    sg[l] = valid ? 0 : -1;
-    sOut[l] = -1;
-    __syncthreads();
-    int pos = 0;
-    // Level 0 ranks by x, level 1 by y within the x half, level 2 by z within the quarter.
-    for (int lev = 0; lev < 3; lev++) {
-        const double *sc = lev == 0 ? sx : (lev == 1 ? sy : sz);
-        const double u = sc[l];
-        const int g = sg[l];
-        int rank = 0, size = 0;
-        for (int k = 0; k < NB_SC; k++) {
-            if (sg[k] != g) continue;
-            size++;
-            if (before(sc[k], sa[k], u, a)) rank++;
-        }
-        const int cut = min(32 >> lev, size);
-        const int hi = rank >= cut ? 1 : 0;
-        pos += hi * (32 >> lev);
-        __syncthreads();
-        if (valid) sg[l] = 2 * g + hi;
-        __syncthreads();
-        if (lev == 2) pos += rank - (hi ? cut : 0);
-    }
-    if (valid && pos >= 0 && pos < NB_SC) sOut[pos] = a;
-    __syncthreads();
-    const int o = sOut[l];
-    slot_atom[base + l] = o;
-    if (o >= 0) atom_slot[o] = (int)(base + l);
+    const int a0 = slot_atom[base + l];
+    double c[3];
+    for (int d = 0; d < 3; d++) c[d] = a0 >= 0 ? xyz[3 * (size_t)a0 + d] : __builtin_inf();
+    int src = l, atom = a0;
+    double key = c[0];
+    wave_bitonic(key, src, atom, 64);
+    // Next key: y (then z) of the element each lane now holds.
+    key = __hiloint2double(__shfl(__double2hiint(c[1]), src, 64), __shfl(__double2loint(c[1]), src, 64));
+    wave_bitonic(key, src, atom, 32);
+    key = __hiloint2double(__shfl(__double2hiint(c[2]), src, 64), __shfl(__double2loint(c[2]), src, 64));
+    wave_bitonic(key, src, atom, 16);
+    slot_atom[base + l] = atom;
+    if (atom >= 0) atom_slot[atom] = (int)(base + l);
 }
 
 // Per-column cluster range [first, last) for the search.
@@ -750,38 +762,63 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
 // candidate order), count their pairs into *npairs (or per block), and cut
 // them into work items.  Block b compacts search block b, whose
 // super-cluster is b % nsc.
-__global__ __launch_bounds__(64) void k_compact(const Entry *__restrict__ scratch, int cap, const int *__restrict__ soff,
-                                                const int *__restrict__ ncand, const int *__restrict__ counts,
-                                                const int *__restrict__ eoff, const int *__restrict__ ioff, int nsc,
-                                                int per_item, int sys, Entry *__restrict__ entries,
-                                                WorkItem *__restrict__ items, unsigned long long *__restrict__ npairs,
-                                                unsigned long long *__restrict__ npairs_block)
+#define NB_COMPACT_THREADS 256
+__global__ __launch_bounds__(NB_COMPACT_THREADS) void k_compact(const Entry *__restrict__ scratch, int cap,
+                                                                const int *__restrict__ soff,
+                                                                const int *__restrict__ ncand,
+                                                                const int *__restrict__ counts,
+                                                                const int *__restrict__ eoff,
+                                                                const int *__restrict__ ioff, int nsc, int per_item,
+                                                                int sys, Entry *__restrict__ entries,
+                                                                WorkItem *__restrict__ items,
+                                                                unsigned long long *__restrict__ npairs_block)
 {
-    const int sc = blockIdx.x, lane = threadIdx.x;
+    const int sc = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int nc = soff ? ncand[sc] : min(ncand[sc], cap), cnt = counts[sc], e0 = eoff[sc], i0 = ioff[sc];
     const Entry *__restrict__ src = scratch + (soff ? (size_t)soff[sc] : (size_t)sc * cap);
+    __shared__ int sW[NB_COMPACT_THREADS / 64];
+    __shared__ unsigned long long sP[NB_COMPACT_THREADS / 64];
     int pos = 0;
     unsigned long long pc = 0;
-    for (int k0 = 0; k0 < nc; k0 += 64) {
-        const int k = k0 + lane;
+    // Chunks of NB_COMPACT_THREADS candidates: the valid ones keep their order.
+    for (int k0 = 0; k0 < nc; k0 += NB_COMPACT_THREADS) {
+        const int k = k0 + tid;
         const bool valid = k < nc && src[k].cbits != 0;
         const unsigned long long m = __ballot(valid);
-        if (valid) {
-            const Entry en = src[k];
-            entries[e0 + pos + __popcll(m & ((1ull << lane) - 1ull))] = en;
-            const unsigned long long *wds = (const unsigned long long *)en.bits;
-            for (int q = 0; q < NB_SC / 8; q++) pc += __popcll(wds[q]);
+        if (lane == 0) sW[w] = __popcll(m);
+        __syncthreads();
+        int before = 0, total = 0;
+        for (int v = 0; v < NB_COMPACT_THREADS / 64; v++) {
+            before += v < w ? sW[v] : 0;
+            total += sW[v];
         }
-        pos += __popcll(m);
+        if (valid) {
+            // 80-byte entry as five 16-byte words.
+            const int4 *s4 = (const int4 *)&src[k];
+            int4 *d4 = (int4 *)&entries[e0 + pos + before + __popcll(m & ((1ull << lane) - 1ull))];
+            int4 q[5];
+#pragma unroll
+            for (int t = 0; t < 5; t++) q[t] = s4[t];
+#pragma unroll
+            for (int t = 0; t < 5; t++) d4[t] = q[t];
+            // The lane masks are words 1..4.
+#pragma unroll
+            for (int t = 1; t < 5; t++)
+                pc += (unsigned long long)(__popc(q[t].x) + __popc(q[t].y) + __popc(q[t].z) + __popc(q[t].w));
+        }
+        pos += total;
+        __syncthreads();
     }
     for (int o = 32; o >= 1; o >>= 1) pc += __shfl_xor(pc, o, 64);
-    if (npairs_block) {
-        if (lane == 0) npairs_block[sc] = pc;
-    } else if (lane == 0 && pc) {
-        atomicAdd(npairs, pc);
+    if (lane == 0) sP[w] = pc;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long t = 0;
+        for (int v = 0; v < NB_COMPACT_THREADS / 64; v++) t += sP[v];
+        npairs_block[sc] = t;
     }
     const int nit = (cnt + per_item - 1) / per_item;
-    for (int k = lane; k < nit; k += 64) {
+    for (int k = tid; k < nit; k += NB_COMPACT_THREADS) {
         WorkItem wi;
         wi.sys = sys;
         if (soff) wi.sys |= (sc / nsc + 1) << 16;   // image lists: image of the block, plus one
@@ -799,14 +836,18 @@ __global__ __launch_bounds__(256) void k_item_cost(const WorkItem *__restrict__ 
                                                    const Entry *__restrict__ entries, int maxcost,
                                                    int *__restrict__ cost, int *__restrict__ hist)
 {
-    const int k = blockIdx.x * 256 + threadIdx.x;
+    // One wavefront per item: its lanes read the entries' i-cluster masks.
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (k >= nitems) return;
     const WorkItem w = items[k];
     int c = 0;
-    for (int e = w.first; e < w.first + w.count; e++) c += __popc(entries[e].cbits & 0xff);
+    for (int e = w.first + lane; e < w.first + w.count; e += 64) c += __popc(entries[e].cbits & 0xff);
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
     c = min(c, maxcost);
-    cost[k] = c;
-    atomicAdd(&hist[maxcost - c], 1);
+    if (lane == 0) {
+        cost[k] = c;
+        if (hist) atomicAdd(&hist[maxcost - c], 1);
+    }
 }
 
 // Rank r in the longest-first order goes to position r, or, for the
@@ -817,6 +858,77 @@ __device__ __forceinline__ int queue_start(int nitems, int q)
     int s = 0;
     for (int v = 0; v < q; v++) s += (nitems - v + 7) / 8;
     return s;
+}
+
+// Longest-first order in one workgroup: histogram of the costs, its scan and
+// the scatter all in LDS.  Most items share a few costs, so the histogram is
+// kept in NB_ORDER_COPIES copies (wave w adds into copy w % NB_ORDER_COPIES)
+// to spread the atomics on a hot bin; the scan runs over (bin, copy) in that
+// order.  Items of equal cost keep no particular order.
+#define NB_ORDER_THREADS 1024
+#define NB_ORDER_BINS 2048
+#define NB_ORDER_COPIES 4
+#define NB_ORDER_UNROLL 8
+__global__ __launch_bounds__(NB_ORDER_THREADS) void k_item_order(const WorkItem *__restrict__ items, int nitems,
+                                                                 const int *__restrict__ cost, int maxcost,
+                                                                 int interleave, WorkItem *__restrict__ out)
+{
+    __shared__ int sH[NB_ORDER_BINS * NB_ORDER_COPIES];    // [bin][copy]
+    __shared__ int sW[NB_ORDER_THREADS / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, cp = w % NB_ORDER_COPIES;
+    for (int b = t; b < NB_ORDER_BINS * NB_ORDER_COPIES; b += NB_ORDER_THREADS) sH[b] = 0;
+    __syncthreads();
+    // The loads of a pass are issued together (indices clamped, so none is conditional).
+    for (int k0 = t; k0 < nitems; k0 += NB_ORDER_UNROLL * NB_ORDER_THREADS) {
+        int cc[NB_ORDER_UNROLL];
+#pragma unroll
+        for (int u = 0; u < NB_ORDER_UNROLL; u++) cc[u] = cost[min(k0 + u * NB_ORDER_THREADS, nitems - 1)];
+#pragma unroll
+        for (int u = 0; u < NB_ORDER_UNROLL; u++)
+            if (k0 + u * NB_ORDER_THREADS < nitems) atomicAdd(&sH[(maxcost - cc[u]) * NB_ORDER_COPIES + cp], 1);
+    }
+    __syncthreads();
+    // Exclusive scan over (bin, copy): eight consecutive counters per thread.
+    constexpr int PER = NB_ORDER_BINS * NB_ORDER_COPIES / NB_ORDER_THREADS;
+    int h[PER], sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        h[q] = sH[PER * t + q];
+        sum += h[q];
+    }
+    int incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) sW[w] = incl;
+    __syncthreads();
+    int ex = incl - sum;
+    for (int v = 0; v < w; v++) ex += sW[v];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        sH[PER * t + q] = ex;
+        ex += h[q];
+    }
+    __syncthreads();
+    for (int k0 = t; k0 < nitems; k0 += NB_ORDER_UNROLL * NB_ORDER_THREADS) {
+        int cc[NB_ORDER_UNROLL];
+        int4 wi[NB_ORDER_UNROLL];
+#pragma unroll
+        for (int u = 0; u < NB_ORDER_UNROLL; u++) {
+            const int k = min(k0 + u * NB_ORDER_THREADS, nitems - 1);
+            cc[u] = cost[k];
+            wi[u] = ((const int4 *)items)[k];
+        }
+        int r[NB_ORDER_UNROLL];
+#pragma unroll
+        for (int u = 0; u < NB_ORDER_UNROLL; u++)
+            r[u] = k0 + u * NB_ORDER_THREADS < nitems ? atomicAdd(&sH[(maxcost - cc[u]) * NB_ORDER_COPIES + cp], 1) : -1;
+#pragma unroll
+        for (int u = 0; u < NB_ORDER_UNROLL; u++)
+            if (r[u] >= 0) ((int4 *)out)[interleave ? queue_start(nitems, r[u] & 7) + (r[u] >> 3) : r[u]] = wi[u];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_item_scatter(const WorkItem *__restrict__ items, int nitems,
@@ -846,6 +958,66 @@ __global__ __launch_bounds__(256) void k_cut_items(const int *__restrict__ count
         wi.first = e0 + k * per_item;
         wi.count = min(per_item, cnt - k * per_item);
         items[i0 + k] = wi;
+    }
+}
+
+// Entry and item offsets of the super-clusters from their entry counts, and
+// the totals (entries, items, search capacity needed) into out[0..2], in one
+// workgroup (each thread scans a contiguous chunk).
+#define NB_OFFS_THREADS 1024
+#define NB_OFFS_MAX (64 * NB_OFFS_THREADS)
+__global__ __launch_bounds__(NB_OFFS_THREADS) void k_list_offsets(const int *__restrict__ counts, int m, int per_item,
+                                                                  int *__restrict__ eoff, int *__restrict__ nits,
+                                                                  int *__restrict__ ioff, const int *__restrict__ need,
+                                                                  int *__restrict__ out)
+{
+    __shared__ int sE[NB_OFFS_THREADS / 64], sI[NB_OFFS_THREADS / 64];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int chunk = (m + NB_OFFS_THREADS - 1) / NB_OFFS_THREADS;
+    const int lo = min(t * chunk, m), hi = min(lo + chunk, m);
+    int se = 0, si = 0;
+    for (int k = lo; k < hi; k++) {
+        const int c = counts[k];
+        se += c;
+        si += (c + per_item - 1) / per_item;
+    }
+    int ie = se, ii = si;   // inclusive scans over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(ie, o, 64), u = __shfl_up(ii, o, 64);
+        if (lane >= o) {
+            ie += v;
+            ii += u;
+        }
+    }
+    if (lane == 63) {
+        sE[w] = ie;
+        sI[w] = ii;
+    }
+    __syncthreads();
+    int be = 0, bi = 0, te = 0, ti = 0;
+    for (int v = 0; v < NB_OFFS_THREADS / 64; v++) {
+        if (v < w) {
+            be += sE[v];
+            bi += sI[v];
+        }
+        te += sE[v];
+        ti += sI[v];
+    }
+    int e = be + ie - se, i = bi + ii - si;
+    for (int k = lo; k < hi; k++) {
+        const int c = counts[k], ni = (c + per_item - 1) / per_item;
+        eoff[k] = e;
+        ioff[k] = i;
+        nits[k] = ni;
+        e += c;
+        i += ni;
+    }
+    if (t == 0) {
+        volatile int *o = out;
+        o[0] = te;
+        o[1] = ti;
+        o[2] = *need;
     }
 }
 
@@ -1133,12 +1305,12 @@ void launch_fill_int(int *p, size_t m, int v, hipStream_t s)
     if (m > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblk(m, 256)), dim3(256), 0, s, p, m, v);
 }
 
-void launch_sort_slots(const int *key, int n, int nkeys, int ncol, int nzb, int *off, int *cursor, int *sorted,
-                       int *padded, hipStream_t s)
+void launch_sort_slots(const int *key, int n, int nkeys, int ncol, int nzb, int *off, int *cursor, int *unsorted,
+                       int *sorted, int *padded, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, key, n, cursor, sorted);
-    hipLaunchKernelGGL(k_bucket_sort, dim3(nblk(nkeys, 256)), dim3(256), 0, s, off, nkeys, n, sorted);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, key, n, cursor, unsorted);
+    hipLaunchKernelGGL(k_bucket_rank, dim3(nblk(n, 256)), dim3(256), 0, s, key, off, nkeys, n, unsorted, sorted);
     hipLaunchKernelGGL(k_colpad, dim3(nblk(ncol, 256)), dim3(256), 0, s, off, ncol, nzb, n, padded);
 }
 
@@ -1235,12 +1407,28 @@ void launch_order_items(const WorkItem *items, int nitems, const Entry *entries,
                         int *hist, int *cursor, int *scantmp, WorkItem *out, hipStream_t s)
 {
     if (nitems <= 0) return;
+    if (maxcost + 1 <= NB_ORDER_BINS) {
+        hipLaunchKernelGGL(k_item_cost, dim3(nblk(nitems, 4)), dim3(256), 0, s, items, nitems, entries, maxcost, cost,
+                           (int *)nullptr);
+        hipLaunchKernelGGL(k_item_order, dim3(1), dim3(NB_ORDER_THREADS), 0, s, items, nitems, cost, maxcost,
+                           interleave, out);
+        return;
+    }
     launch_fill_int(hist, (size_t)maxcost + 1, 0, s);
-    hipLaunchKernelGGL(k_item_cost, dim3(nblk(nitems, 256)), dim3(256), 0, s, items, nitems, entries, maxcost, cost,
+    hipLaunchKernelGGL(k_item_cost, dim3(nblk(nitems, 4)), dim3(256), 0, s, items, nitems, entries, maxcost, cost,
                        hist);
     launch_scan(hist, cursor, maxcost + 1, scantmp, s);
     hipLaunchKernelGGL(k_item_scatter, dim3(nblk(nitems, 256)), dim3(256), 0, s, items, nitems, cost, maxcost,
                        interleave, cursor, out);
+}
+
+bool launch_list_offsets(const int *counts, int nsc, int per_item, int *eoff, int *nits, int *ioff, const int *need,
+                         int *out, hipStream_t s)
+{
+    if (nsc <= 0 || nsc > NB_OFFS_MAX) return false;
+    hipLaunchKernelGGL(k_list_offsets, dim3(1), dim3(NB_OFFS_THREADS), 0, s, counts, nsc, per_item, eoff, nits, ioff,
+                       need, out);
+    return true;
 }
 
 void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, hipStream_t s)
@@ -1248,13 +1436,27 @@ void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, 
     hipLaunchKernelGGL(k_items_per_sc, dim3(nblk(nsc, 256)), dim3(256), 0, s, counts, nsc, per_item, nitems);
 }
 
+// Sum of the per-block pair counts into *out (one workgroup).
+__global__ __launch_bounds__(256) void k_sum_u64(const unsigned long long *__restrict__ v, int m,
+                                                 unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long sP[4];
+    unsigned long long t = 0;
+    for (int k = threadIdx.x; k < m; k += 256) t += v[k];
+    for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
+    if ((threadIdx.x & 63) == 0) sP[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (sP[0] + sP[1]) + (sP[2] + sP[3]);
+}
+
 void launch_compact(const Entry *scratch, int cap, const int *soff, const int *ncand, const int *counts,
                     const int *eoff, const int *ioff, int nblocks, int nsc, int per_item, int sys, Entry *entries,
                     WorkItem *items, unsigned long long *npairs, unsigned long long *npairs_block, hipStream_t s)
 {
-    if (nblocks > 0)
-        hipLaunchKernelGGL(k_compact, dim3(nblocks), dim3(64), 0, s, scratch, cap, soff, ncand, counts, eoff, ioff,
-                           nsc, per_item, sys, entries, items, npairs, npairs_block);
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL(k_compact, dim3(nblocks), dim3(NB_COMPACT_THREADS), 0, s, scratch, cap, soff, ncand, counts,
+                       eoff, ioff, nsc, per_item, sys, entries, items, npairs_block);
+    if (npairs) hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(256), 0, s, npairs_block, nblocks, npairs);
 }
 
 void launch_image_bounds(const SearchParams &P, int nsc, int nimg, const float4 *blo, const float4 *bhi, int *bound,

@@ -351,6 +351,13 @@ struct nbabfs_state {
     // Update state (NBModelABFSState.isNew / useGridSearch).
     bool isNew = true, useGrid = false, listValid = false, csrValid = false;
     long long calls = 0, updates = 0, npairs = 0;
+    // The pair count of the last build reaches h_np without a synchronisation; it is read (and added to the
+    // statistics when np_stat) at the next synchronisation that needs it.
+    PinBuf<unsigned long long> h_np;
+    bool np_pending = false, np_stat = false;
+    std::vector<unsigned char> stencil_h;   // stencil flags last uploaded
+    PinBuf<int> h_qs;                       // staging of the queue starts
+    hipEvent_t ev_qs = nullptr, ev_desc = nullptr;   // completion of the last staging copies
     double sumPairs = 0.0;
 
     // List.
@@ -361,6 +368,8 @@ struct nbabfs_state {
     DBuf<int> slot_atom, atom_slot;
     DBuf<int4> cellv;
     DBuf<float4> blo, bhi, centre;
+    DBuf<unsigned long long> npblock;   // pairs per super-cluster of the last build
+    DBuf<int> unsorted;                 // bucket contents in scatter order
     DBuf<double4> srec;      // slot-ordered search records (self-list search)
     DBuf<int4> scell;
     DBuf<Entry> entries, scratch;
@@ -476,6 +485,8 @@ struct nbabfs_state {
         for (auto e : ev) (void)hipEventDestroy(e);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_qs) (void)hipEventDestroy(ev_qs);
+        if (ev_desc) (void)hipEventDestroy(ev_desc);
         if (stream2) (void)hipStreamDestroy(stream2);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
@@ -806,6 +817,40 @@ static void inverse_derivative(const double *m, int i, int j, double *o)
 // the eight per-XCD queues), then the queue bounds.  The main list's items
 // [0, nitems_main) and the image lists' items after them are ordered apart:
 // they run in two launches.
+// The pair count of the last build (synchronises the stream when it is still pending).
+static int resolve_npairs(nbabfs_state *S)
+{
+    if (!S->np_pending) return NBABFS_STATUS_CONTINUE;
+    HIPCHK(hipStreamSynchronize(S->stream));
+    S->npairs = (long long)S->h_np.p[0];
+    S->np_pending = false;
+    if (S->np_stat) S->sumPairs += (double)S->npairs;
+    S->np_stat = false;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// Counts an update in the statistics (its pairs once the count is known).
+static void count_update(nbabfs_state *S)
+{
+    S->updates++;
+    if (S->np_pending) S->np_stat = true;
+    else S->sumPairs += (double)S->npairs;
+}
+
+// Wait until the staging copy recorded by *ev has completed (an event, not the whole stream).
+static int staging_free(hipEvent_t ev)
+{
+    if (ev) HIPCHK(hipEventSynchronize(ev));
+    return NBABFS_STATUS_CONTINUE;
+}
+
+static int staging_done(hipEvent_t *ev, hipStream_t st)
+{
+    if (!*ev) HIPCHK(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(*ev, st));
+    return NBABFS_STATUS_CONTINUE;
+}
+
 static int order_items(nbabfs_state *S)
 {
     hipStream_t st = S->stream;
@@ -829,7 +874,13 @@ static int order_items(nbabfs_state *S)
     S->items_version++;
     qs[0] = 0;
     for (int q = 0; q < 8; q++) qs[q + 1] = interleave ? qs[q] + (n - q + 7) / 8 * (n > q) : (int)((long long)n * (q + 1) / 8);
-    CHK(S->qstarts.upload(qs.data(), 9, st));
+    // Pinned staging: the copy is asynchronous; the staging is rewritten only after it completed.
+    CHK(S->h_qs.ensure(9));
+    CHK(S->qstarts.ensure(9));
+    CHK(staging_free(S->ev_qs));
+    std::memcpy(S->h_qs.p, qs.data(), 9 * sizeof(int));
+    HIPCHK(hipMemcpyAsync(S->qstarts.p, S->h_qs.p, 9 * sizeof(int), hipMemcpyHostToDevice, st));
+    CHK(staging_done(&S->ev_qs, st));
     return NBABFS_STATUS_CONTINUE;
 }
 
@@ -860,6 +911,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     hipStream_t st = S->stream;
     const int n = S->n;
     const nbabfs_options &o = S->opt;
+    CHK(resolve_npairs(S));
     S->listValid = false;
     S->csrValid = false;
     S->npairs = 0;
@@ -906,7 +958,11 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
         S->u = 0;
         flags.assign(1, 1);
     }
-    CHK(S->stencil.upload(flags.data(), flags.size(), st));
+    if (flags != S->stencil_h || !S->stencil.p) {
+        CHK(S->stencil.upload(flags.data(), flags.size(), st));
+        HIPCHK(hipStreamSynchronize(st));     // the copy reads pageable memory
+        S->stencil_h = flags;
+    }
 
     // Super-cluster columns: width so that 64 consecutive atoms of a column
     // (one super-cluster, split into 2 x 2 x 2 clusters) fill a rough cube.
@@ -951,7 +1007,9 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     launch_bin(d_xyz, n, g, S->cellv.p, S->key.p, S->count.p, st);
     launch_scan(S->count.p, S->off.p, nkeys, S->scantmp.p, st);
     HIPCHK(hipMemcpyAsync(S->cursor.p, S->off.p, nkeys * sizeof(int), hipMemcpyDeviceToDevice, st));
-    launch_sort_slots(S->key.p, n, nkeys, ncol, g.nzb, S->off.p, S->cursor.p, S->sorted.p, S->padded.p, st);
+    CHK(S->unsorted.ensure(n));
+    launch_sort_slots(S->key.p, n, nkeys, ncol, g.nzb, S->off.p, S->cursor.p, S->unsorted.p, S->sorted.p, S->padded.p,
+                      st);
     launch_scan(S->padded.p, S->colstart.p, ncol, S->scantmp.p, st);
     HIPCHK(hipMemcpyAsync(S->h_int.p, S->colstart.p + (ncol - 1), sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(S->h_int.p + 1, S->padded.p + (ncol - 1), sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1006,11 +1064,14 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
         launch_fill_int(S->flag.p, 1, 0, st);
         launch_search(P, S->nsc, d_xyz, S->slot_atom.p, S->blo.p, S->bhi.p, S->colcl.p, S->scratch.p, S->ncand.p,
                       S->counts.p, S->flag.p, st);
-        launch_scan(S->counts.p, S->eoff.p, S->nsc, S->scantmp.p, st);
-        launch_items_per_sc(S->counts.p, S->nsc, per_item, S->nits.p, st);
-        launch_scan(S->nits.p, S->ioff.p, S->nsc, S->scantmp.p, st);
-        hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, S->eoff.p, S->counts.p, S->ioff.p, S->nits.p,
-                           S->nsc, S->flag.p, S->tail.d);
+        if (!launch_list_offsets(S->counts.p, S->nsc, per_item, S->eoff.p, S->nits.p, S->ioff.p, S->flag.p, S->tail.d,
+                                 st)) {
+            launch_scan(S->counts.p, S->eoff.p, S->nsc, S->scantmp.p, st);
+            launch_items_per_sc(S->counts.p, S->nsc, per_item, S->nits.p, st);
+            launch_scan(S->nits.p, S->ioff.p, S->nsc, S->scantmp.p, st);
+            hipLaunchKernelGGL(k_list_totals, dim3(1), dim3(64), 0, st, S->eoff.p, S->counts.p, S->ioff.p, S->nits.p,
+                               S->nsc, S->flag.p, S->tail.d);
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
         const int need = ((volatile int *)S->tail.h)[2];
@@ -1029,13 +1090,12 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     CHK(S->entries.ensure(S->nent));
     CHK(S->items.ensure(S->nitems));
     CHK(S->u64.ensure(2));
-    HIPCHK(hipMemsetAsync(S->u64.p, 0, 2 * sizeof(unsigned long long), st));
+    CHK(S->npblock.ensure(S->nsc));
     launch_compact(S->scratch.p, S->cap, nullptr, S->ncand.p, S->counts.p, S->eoff.p, S->ioff.p, S->nsc, S->nsc,
-                   per_item, 0, S->entries.p, S->items.p, S->u64.p, nullptr, st);
-    unsigned long long np = 0;
-    HIPCHK(hipMemcpyAsync(&np, S->u64.p, sizeof(np), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    S->npairs = (long long)np;
+                   per_item, 0, S->entries.p, S->items.p, S->u64.p, S->npblock.p, st);
+    CHK(S->h_np.ensure(1));
+    HIPCHK(hipMemcpyAsync(S->h_np.p, S->u64.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    S->np_pending = true;
 
     // Per-call buffers.
     CHK(S->xyzq.ensure(S->n_pad));
@@ -1049,6 +1109,7 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
 
     // Plain path: full CSR rows (both directions).
     if (o.precision == NBABFS_PLAIN_FP64) {
+        CHK(resolve_npairs(S));
         CHK(S->rowptr.ensure(n + 1));
         CHK(S->cols.ensure(std::max<size_t>(2 * (size_t)S->npairs, 1)));
         CHK(S->gplain.ensure(3 * (size_t)n));
@@ -1082,9 +1143,10 @@ static int ensure_desc(nbabfs_state *S, const double *d_xyz, double *d_grad)
     S->last_grad = d_grad;
     if (S->desc_valid && std::memcmp(&D, &S->desc_shadow, sizeof(SysDesc)) == 0) return NBABFS_STATUS_CONTINUE;
     S->desc_valid = false;
-    HIPCHK(hipStreamSynchronize(S->stream));
+    CHK(staging_free(S->ev_desc));
     S->h_desc.p[0] = D;
     HIPCHK(hipMemcpyAsync(S->d_desc.p, S->h_desc.p, sizeof(SysDesc), hipMemcpyHostToDevice, S->stream));
+    CHK(staging_done(&S->ev_desc, S->stream));
     S->desc_shadow = D;
     S->desc_valid = true;
     return NBABFS_STATUS_CONTINUE;
@@ -1202,8 +1264,7 @@ static int update(nbabfs_state *S, const double *d_xyz, int *rebuilt)
     }
     if (S->has_sym && (doUpdate || doImages)) CHK(build_images(S, d_xyz));
     if (doUpdate || doImages) {
-        S->updates++;
-        S->sumPairs += (double)S->npairs;
+        count_update(S);
         S->sumImages += (double)S->nimages;
         S->sumImagePairs += (double)S->img_npairs;
     }
@@ -2032,6 +2093,7 @@ extern "C" int nbabfs_energies(nbabfs_state *S, double *energies, int *present)
                           S->stream));
     HIPCHK(hipStreamSynchronize(S->stream));
     std::memcpy(energies, S->h_energies.p, 4 * sizeof(double));
+    CHK(resolve_npairs(S));
     if (present) {
         present[0] = S->npairs > 0;
         present[1] = S->n14 > 0;
@@ -2096,6 +2158,7 @@ extern "C" int nbabfs_stats_get(nbabfs_state *S, nbabfs_stats *out)
 {
     if (!S || !out) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
     CHK(collect_timing(S));
+    CHK(resolve_npairs(S));
     std::memset(out, 0, sizeof(*out));
     out->numberOfCalls = S->calls;
     out->numberOfUpdates = S->updates;
@@ -2310,6 +2373,7 @@ extern "C" long long nbabfs_export_pairs(nbabfs_state *S, int which, long long *
         return m;
     }
     if (!S->listValid) return -fail(NBABFS_STATUS_LOGIC_ERROR, "no pair list");
+    if (resolve_npairs(S) != NBABFS_STATUS_CONTINUE) return -NBABFS_STATUS_LOGIC_ERROR;
     if (!keys || S->npairs == 0) return S->npairs;
     hipStream_t st = S->stream;
     DBuf<long long> dk;
@@ -2642,8 +2706,7 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
                                       s->stream));
             HIPCHK(hipStreamSynchronize(s->stream));
             s->have_ref = true;
-            s->updates++;
-            s->sumPairs += (double)s->npairs;
+            count_update(s);
             any = true;
         }
         s->isNew = false;

@@ -51,6 +51,7 @@ struct Entry {              // 80 bytes
     int img;                // -1: main list; else the image whose copy of J is paired
     unsigned char bits[NB_SC];  // per lane l = 8a+b: bit c = pair (c,a)-(J,b) listed
 };
+static_assert(sizeof(Entry) == 80, "entries are copied as five 16-byte words");
 
 // Real-space transformation of one image for the pair kernel, in the cluster
 // frame: x' - origin = R (x - origin) + T.  scale multiplies the image's
@@ -78,6 +79,7 @@ struct WorkItem {           // 16 bytes
     int first;              // first entry
     int count;              // entries
 };
+static_assert(sizeof(WorkItem) == 16, "work items are moved as one 16-byte word");
 
 // Everything a per-call kernel needs to know about one system.
 struct SysDesc {

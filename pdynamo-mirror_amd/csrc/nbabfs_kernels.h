@@ -63,8 +63,8 @@ void launch_bin(const double *xyz, int n, const BuildGrid &g, int4 *cellv, int *
 size_t scan_tmp_size(int m);
 void launch_scan(const int *in, int *out, int m, int *tmp, hipStream_t s);
 void launch_fill_int(int *p, size_t m, int v, hipStream_t s);
-void launch_sort_slots(const int *key, int n, int nkeys, int ncol, int nzb, int *off, int *cursor, int *sorted,
-                       int *padded, hipStream_t s);
+void launch_sort_slots(const int *key, int n, int nkeys, int ncol, int nzb, int *off, int *cursor, int *unsorted,
+                       int *sorted, int *padded, hipStream_t s);
 void launch_slots(const int *sorted, const int *key, int n, int nzb, const int *off, const int *colstart,
                   const int *padded, int ncol, const double *xyz, int nsc, int *slot_atom, int *atom_slot, int2 *colcl,
                   hipStream_t s);
@@ -97,8 +97,12 @@ void launch_export_image_pairs(const Entry *entries, int nent, const int *slot_a
 void launch_order_items(const WorkItem *items, int nitems, const Entry *entries, int maxcost, int interleave, int *cost,
                         int *hist, int *cursor, int *scantmp, WorkItem *out, hipStream_t s);
 void launch_items_per_sc(const int *counts, int nsc, int per_item, int *nitems, hipStream_t s);
-// nblocks compaction blocks (super-cluster = block % nsc); pairs are added
-// into *npairs, or stored per block into npairs_block when that is given.
+// Entry and item offsets (exclusive scans of counts and of ceil(counts / per_item)) and the totals
+// {entries, items, *need} into out, in one launch; false (nothing launched) when nsc is beyond its single workgroup.
+bool launch_list_offsets(const int *counts, int nsc, int per_item, int *eoff, int *nits, int *ioff, const int *need,
+                         int *out, hipStream_t s);
+// nblocks compaction blocks (super-cluster = block % nsc); the pairs of each
+// block are stored into npairs_block and, when npairs is given, their sum into *npairs.
 // soff (or null: block * cap) gives each block's scratch offset.
 void launch_compact(const Entry *scratch, int cap, const int *soff, const int *ncand, const int *counts,
                     const int *eoff, const int *ioff, int nblocks, int nsc, int per_item, int sys, Entry *entries,
