@@ -175,6 +175,7 @@ def run_qcmm(args):
             st.qcmm_gradients(qcq, hg)
 
     def timed(qc):
+        settle(lambda: step(qc), args.settle)
         for _ in range(args.warmup):
             step(qc)
         _native.synchronize()
@@ -197,7 +198,7 @@ def run_qcmm(args):
         "value": n_list / (ms * 1e-3), "unit": "pair interactions/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 (QC/MM) + the MM/MM path's precision", "data": "synthetic (seeded generator, pdynamo_amd.synthetic)",
-        "config": {"workload": "QC/MM chain + 40 QC waters in 7,000 waters (RC coupling, 2 link atoms)",
+        "config": {"clock_settle_s": args.settle, "workload": "QC/MM chain + 40 QC waters in 7,000 waters (RC coupling, 2 link atoms)",
                    "atoms": mm.natoms, "qc_atoms": nqc, "precision": args.precision, "list_pairs": n_list,
                    "mmmm_pairs": int(s["npairs"]), "qcmm_el_pairs": int(s["qcmmElPairs"]),
                    "qcmm_lj_pairs": int(s["qcmmLJPairs"]), "ms_mm_part_only": ms_mm,
@@ -228,6 +229,22 @@ def run_qcmm(args):
                                       "sample": "%d QC/MM calls (Update, MMMMEnergy, QCMMEnergyLJ, QCMMPotentials, "
                                                 "QCMMGradients) after the first list build, %.1f ms each" % (calls, dt * 1e3)}
     print(json.dumps(result))
+
+
+def settle(fn, seconds):
+    """Untimed calls for `seconds` of wall time before the warmup steps.  The GPU's clocks ramp up under load:
+    back-to-back pair launches get about a fifth faster over their first ~30 ms, so a handful of warmup steps
+    alone would time the ramp rather than the kernel."""
+    from pdynamo_amd import _native
+    n = 0
+    t_end = time.perf_counter() + seconds
+    while time.perf_counter() < t_end:
+        fn()
+        n += 1
+        if n % 16 == 0:
+            _native.synchronize()
+    _native.synchronize()
+    return n
 
 
 def make_batch(name):
@@ -265,6 +282,7 @@ def run_batch(args, systems, wdesc, workload):
     def timed_batch(precision, steps, warmup, kernel_pass=True):
         states = states_for(precision)
         batch = _native.NativeBatch(states, use_graph=True)
+        settle(lambda: batch.energy(xp, gp), args.settle)
         for _ in range(warmup):
             batch.energy(xp, gp)
         _native.synchronize()
@@ -328,7 +346,7 @@ def run_batch(args, systems, wdesc, workload):
         "value": npairs / (ms * 1e-3), "unit": "pair interactions/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": DTYPE[args.precision], "data": "synthetic (seeded generator, pdynamo_amd.synthetic)",
-        "config": {"workload": wdesc, "systems": len(systems), "atoms": int(offs[-1]), "list_pairs": npairs,
+        "config": {"clock_settle_s": args.settle, "workload": wdesc, "systems": len(systems), "atoms": int(offs[-1]), "list_pairs": npairs,
                    "min_atoms": int(min(sizes)), "max_atoms": int(max(sizes)), "precision": args.precision,
                    "work_items": int(sum(s_["workItems"] for s_ in stats)),
                    "cluster_entries": int(sum(s_["entries"] for s_ in stats)),
@@ -444,6 +462,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--settle", type=float, default=0.5,
+                    help="seconds of untimed calls before the warmup steps (the GPU's clocks ramp up under load)")
     ap.add_argument("--workload", default="images16",
                     help="images16 (default: 16 perturbed images of the 30,000-atom system, one batch), droplets64, "
                          "solute30000, water200001, water648, dhfr, dhfr_periodic, qcmm_solvated")
@@ -527,6 +547,11 @@ def main():
         rebuilds[0] += st.update(x.ptr)
         st.energy_async(x.ptr, g.ptr)
 
+    # Clock settling with the first coordinates (the list stays valid for them), then the warmup steps.
+    def settle_step():
+        st.update(xs[0].ptr)
+        st.energy_async(xs[0].ptr, g.ptr)
+    settle(settle_step, args.settle)
     for _ in range(args.warmup):
         step()
     rebuilds[0] = 0
@@ -569,7 +594,7 @@ def main():
         "vs_baseline": None,
         "dtype": DTYPE[args.precision],
         "data": "synthetic (seeded generator, pdynamo_amd.synthetic)",
-        "config": {"workload": wdesc, "atoms": mm.natoms, "list_pairs": n_list, "precision": args.precision,
+        "config": {"clock_settle_s": args.settle, "workload": wdesc, "atoms": mm.natoms, "list_pairs": n_list, "precision": args.precision,
                    "list_cutoff": 13.5, "cutoffs": [0.5, 8.0, 12.0],
                    "energies_kJ_mol": [float(v) for v in e], "work_items": int(stats["workItems"]),
                    "cluster_entries": int(stats["entries"]), "moves_sigma_A": args.moves,

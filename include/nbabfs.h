@@ -311,13 +311,18 @@ int nbabfs_synchronize(nbabfs_state *state);
 int nbabfs_batch_create(nbabfs_state *const *states, int nsystems, int use_graph, nbabfs_batch **out);
 void nbabfs_batch_destroy(nbabfs_batch *batch);
 /* d_xyz[k], d_grad[k] device pointers per system (d_grad entries may be
- * NULL); energies (4*nsystems) and rebuilt (nsystems, may be NULL) on host. */
+ * NULL); energies (4*nsystems) and rebuilt (nsystems, may be NULL) on host.
+ * The update test of every system and the batched passes are queued together
+ * (the passes do nothing when the test asks for a rebuild, and run again after
+ * it), so a call with no rebuild waits for the GPU once. */
 int nbabfs_batch_energy(nbabfs_batch *batch, const double *const *d_xyz, double *const *d_grad, double *energies,
                         int *rebuilt);
 /* Bookkeeping of a batch (no counterpart in the reference): hipGraph captures,
- * batched launches and re-layouts (concatenations of the systems' work items,
- * after a rebuild of any system or a change of its arrays) so far, and the
- * mean duration (ms) of the batched pair launch over the timed launches.
+ * batched passes whose results a call returned (a pass queued behind the
+ * update test and abandoned because some system had to be rebuilt is not
+ * counted) and re-layouts (concatenations of the systems' work items, after a
+ * rebuild of any system or a change of its arrays) so far, and the mean
+ * duration (ms) of the batched pair launch over the timed launches.
  * Outputs may be NULL. */
 int nbabfs_batch_info(nbabfs_batch *batch, long long *graph_captures, long long *launches, long long *layouts,
                       double *force_kernel_ms, long long *timed_launches);

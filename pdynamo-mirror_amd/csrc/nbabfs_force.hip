@@ -130,6 +130,14 @@ __device__ __forceinline__ void pair_rows(const AbfsFactors &F, double r2, doubl
 // free atom moved more than half the buffer since the list was built) and
 // stores chk.epoch into *chk.flag when it holds.
 // ------------------------------------------------------------------------------------------------
+// A guarded launch does nothing when its update test asked for a rebuild.
+__device__ __forceinline__ bool guard_skip(const ForceGuard &g)
+{
+    if (!g.flag) return false;
+    const int epoch = g.epoch_ptr ? *(const volatile int *)g.epoch_ptr : g.epoch;
+    return *(const volatile int *)g.flag == epoch;
+}
+
 __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ descs, int tiled, int *__restrict__ heads,
                                                  PrepareCheck chk)
 {
@@ -368,7 +376,7 @@ __global__ __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu((IMG |
     const SysDesc *__restrict__ descs, const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F, int rows_bytes,
     int wave_bytes, ForceGuard guard, int nitems, int nforce_groups)
 {
-    if (guard.flag && *(const volatile int *)guard.flag == guard.epoch) return;
+    if (guard_skip(guard)) return;
     const int lane = threadIdx.x & 63;
     const int wave = WPG == 1 ? 0 : (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // Groups past the pair blocks evaluate the 1-4 list of system 0 (single-state launches), one block per wave.
@@ -652,7 +660,7 @@ __device__ __forceinline__ void block_sum2(double &x, double &y)
 __global__ __launch_bounds__(256) void k_force_plain(const SysDesc *__restrict__ descs, AbfsFactors F, ForceGuard guard)
 {
 #pragma clang fp contract(off)
-    if (guard.flag && *(const volatile int *)guard.flag == guard.epoch) return;
+    if (guard_skip(guard)) return;
     // blockIdx.y = system: one launch covers every system of a batch.
     const SysDesc &D = descs[blockIdx.y];
     if ((int)blockIdx.x >= D.n_plain_blocks) return;
@@ -715,7 +723,7 @@ __global__ __launch_bounds__(64) void k_force_image_fp64(const SysDesc *__restri
                                                          ForceGuard guard)
 {
 #pragma clang fp contract(off)
-    if (guard.flag && *(const volatile int *)guard.flag == guard.epoch) return;
+    if (guard_skip(guard)) return;
     const int it = blockIdx.x;
     const WorkItem w = items[it];
     const SysDesc &D = descs[w.sys & 0xffff];
@@ -832,10 +840,10 @@ __global__ __launch_bounds__(64) void k_force_image_fp64(const SysDesc *__restri
 // 1-4 list (NBModelABFS_MMMMEnergy's second pass, NBModelABFS.c:276-294): FP64
 // with scale14 and the 1-4 LJ table; gradients into the fixed-point arrays.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_pairs14(const SysDesc *__restrict__ descs, AbfsFactors F)
+__global__ __launch_bounds__(64) void k_pairs14(const SysDesc *__restrict__ descs, AbfsFactors F, ForceGuard guard)
 {
     const SysDesc &D = descs[blockIdx.y];
-    if ((int)blockIdx.x >= D.n14_blocks) return;
+    if ((int)blockIdx.x >= D.n14_blocks || guard_skip(guard)) return;
     pairs14_block(D, F, blockIdx.x, threadIdx.x);
 }
 
@@ -857,8 +865,9 @@ __device__ void sum_range2(const double *__restrict__ p, int m, double &x, doubl
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ descs, int plain)
+__global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ descs, int plain, ForceGuard guard)
 {
+    if (guard_skip(guard)) return;
     const SysDesc &D = descs[blockIdx.y];
     // The last block of each system sums its energies while the others add
     // the gradients (the two are independent, so neither waits for the other).
@@ -1084,16 +1093,17 @@ void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int n
         hipExtLaunchKernelGGL(k_force_image_fp64, dim3(nitems), dim3(64), 0, s, ev_start, ev_stop, 0, d_desc, items, F, g);
 }
 
-void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s)
+void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s,
+                    ForceGuard g)
 {
     if (nsys > 0 && max_n14_blocks > 0)
-        hipLaunchKernelGGL(k_pairs14, dim3(max_n14_blocks, nsys), dim3(64), 0, s, d_desc, F);
+        hipLaunchKernelGGL(k_pairs14, dim3(max_n14_blocks, nsys), dim3(64), 0, s, d_desc, F, g);
 }
 
-void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s)
+void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s, ForceGuard g)
 {
     if (nsys <= 0) return;
-    hipLaunchKernelGGL(k_finalize, dim3(nblk(max_n, 256) + 1, nsys), dim3(256), 0, s, d_desc, plain);
+    hipLaunchKernelGGL(k_finalize, dim3(nblk(max_n, 256) + 1, nsys), dim3(256), 0, s, d_desc, plain, g);
 }
 
 // ------------------------------------------------------------------------------------------------

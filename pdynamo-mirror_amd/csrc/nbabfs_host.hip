@@ -285,10 +285,29 @@ struct CheckDesc {
     double bufsq;
 };
 
-__global__ __launch_bounds__(256) void k_check_batch(const CheckDesc *__restrict__ cd, int *__restrict__ flags, int epoch)
+// Update test of a batch, queued in front of its passes (and captured with
+// them).  guard (device words): k_batch_epoch advances guard[1], the epoch of
+// this test, and clears guard[0]; the test sets guard[0] to the epoch when
+// some system must be rebuilt, and the passes behind it then do nothing.  The
+// host reads the epoch (flags[nsys]) and the systems' flags (flags[k] == epoch:
+// rebuild) from mapped memory after the call's one synchronisation.
+__global__ void k_batch_epoch(int *__restrict__ guard, int *__restrict__ flags, int nsys)
+{
+    if (threadIdx.x == 0) {
+        const int e = guard[1];
+        const int n = (e <= 0 || e == 0x7fffffff) ? 1 : e + 1;
+        guard[1] = n;
+        guard[0] = 0;
+        *(volatile int *)&flags[nsys] = n;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_check_batch(const CheckDesc *__restrict__ cd, int *__restrict__ flags,
+                                                     int *__restrict__ guard)
 {
 #pragma clang fp contract(off)
     const CheckDesc &C = cd[blockIdx.y];
+    const int epoch = guard[1];
     int i = blockIdx.x * 256 + threadIdx.x;
     bool up = false;
     if (C.ref && i < C.n && (!C.free_flags || (C.free_flags[i] & 1))) {
@@ -297,7 +316,10 @@ __global__ __launch_bounds__(256) void k_check_batch(const CheckDesc *__restrict
         double dz = C.xyz[3 * (size_t)i + 2] - C.ref[3 * (size_t)i + 2];
         up = dx * dx + dy * dy + dz * dz > C.bufsq;
     }
-    if (__any(up) && threadIdx.x % 64 == 0) *(volatile int *)&flags[blockIdx.y] = epoch;
+    if (__any(up) && threadIdx.x % 64 == 0) {
+        *(volatile int *)&flags[blockIdx.y] = epoch;
+        *(volatile int *)&guard[0] = epoch;
+    }
 }
 
 // Largest displacement of a free atom since the last list build (CheckForUpdate's maximumDisplacement).
@@ -2510,8 +2532,10 @@ struct nbabfs_batch {
     PinBuf<SysDesc> h_desc;
     DBuf<WorkItem> items;
     DBuf<int> heads, qstarts;
-    MappedInts flags;
-    int epoch = 0;
+    MappedInts flags;               // [k]: epoch when system k must be rebuilt; [nsys]: epoch of the last test
+    // Guard words of the batch's passes: [0] the epoch when some system must be rebuilt, [1] the test's epoch.
+    DBuf<int> gguard;
+    std::vector<CheckDesc> check_shadow;   // the test's descriptors on the device
     DBuf<CheckDesc> d_check;
     PinBuf<CheckDesc> h_check;
     PinBuf<double> h_energies;
@@ -2528,7 +2552,7 @@ struct nbabfs_batch {
     // Timing of the batched pair launch: with timing on, the calls enqueue the
     // passes directly (no graph) with the runtime's dispatch events around the
     // pair kernel.
-    bool timing = false;
+    bool timing = false, timed_pass = false;
     hipEvent_t ev[2] = {nullptr, nullptr};
     double force_ms = 0.0;
     long long force_launches = 0;
@@ -2640,6 +2664,11 @@ static int batch_enqueue(nbabfs_batch *B, hipEvent_t ev_start = nullptr, hipEven
     hipStream_t st = B->stream;
     const int ns = (int)B->st.size();
     nbabfs_state *s0 = B->st[0];
+    const ForceGuard g{B->gguard.p, 0, B->gguard.p + 1};
+    hipLaunchKernelGGL(k_batch_epoch, dim3(1), dim3(64), 0, st, B->gguard.p, B->flags.d, ns);
+    if (B->max_n > 0)
+        hipLaunchKernelGGL(k_check_batch, dim3(nblk(B->max_n, 256), ns), dim3(256), 0, st, B->d_check.p, B->flags.d,
+                           B->gguard.p);
     PrepareCheck none{nullptr, nullptr, nullptr, 0.0, nullptr, nullptr, 0, nullptr, nullptr};
     launch_prepare(B->d_desc.p, ns, B->max_npad, B->mixed ? 1 : 0, B->heads.p, none, st);
     if (B->mixed) {
@@ -2651,70 +2680,24 @@ static int batch_enqueue(nbabfs_batch *B, hipEvent_t ev_start = nullptr, hipEven
                 maxcount = std::max(maxcount, s->opt.entriesPerItem);
             }
             launch_force_tiled(B->d_desc.p, B->items.p, q, s0->F, n_persistent_blocks(s0->opt, B->nitems), maxnt,
-                               maxcount, ForceGuard{nullptr, 0}, st, false, ev_start, ev_stop, 0,
-                               B->precision == NBABFS_TILED_FP64);
+                               maxcount, g, st, false, ev_start, ev_stop, 0, B->precision == NBABFS_TILED_FP64);
         }
     } else {
-        launch_force_plain(B->d_desc.p, ns, B->max_n, s0->F, ForceGuard{nullptr, 0}, st, ev_start, ev_stop);
+        launch_force_plain(B->d_desc.p, ns, B->max_n, s0->F, g, st, ev_start, ev_stop);
     }
-    launch_pairs14(B->d_desc.p, ns, B->max_n14b, s0->F, st);
-    launch_finalize(B->d_desc.p, ns, B->max_n, B->mixed ? 0 : 1, st);
+    launch_pairs14(B->d_desc.p, ns, B->max_n14b, s0->F, st, g);
+    launch_finalize(B->d_desc.p, ns, B->max_n, B->mixed ? 0 : 1, st, g);
     HIPCHK(hipMemcpyAsync(B->h_energies.p, B->energies.p, NB_NENERGIES * (size_t)ns * sizeof(double),
                           hipMemcpyDeviceToHost, st));
     return NBABFS_STATUS_CONTINUE;
 }
 
-extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, double *const *d_grad,
-                                   double *energies, int *rebuilt)
+// A new layout (and graph) when some list was rebuilt or re-cut, or when a
+// system's descriptor (its buffers, sizes or arrays) is not the one laid out.
+static int batch_relayout_if_needed(nbabfs_batch *B, const double *const *d_xyz, double *const *d_grad)
 {
-    if (!B || !d_xyz || !energies) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
     const int ns = (int)B->st.size();
-    hipStream_t st = B->stream;
-    for (auto *s : B->st)
-        if (s->F.r2Off != B->st[0]->F.r2Off || s->F.r2On != B->st[0]->F.r2On || s->F.r2Damp != B->st[0]->F.r2Damp)
-            return fail(NBABFS_STATUS_INVALID_ARGUMENT, "all states of a batch must share the cutoffs");
-    // Rebuild decisions for all systems with one launch and one synchronisation.
-    CHK(B->d_check.ensure(ns));
-    CHK(B->h_check.ensure(ns));
-    CHK(B->flags.ensure(ns));
-    CHK(B->h_energies.ensure(NB_NENERGIES * (size_t)ns));
-    int maxn = 0;
-    for (int k = 0; k < ns; k++) {
-        nbabfs_state *s = B->st[k];
-        const double buf = 0.5 * (s->opt.listCutoff - s->opt.outerCutoff);
-        B->h_check.p[k] = CheckDesc{d_xyz[k], (s->isNew || !s->have_ref) ? nullptr : s->xyz_ref.p,
-                                    s->has_fixed ? s->freef.p : nullptr, s->n, buf * buf};
-        maxn = std::max(maxn, s->n);
-    }
-    HIPCHK(hipMemcpyAsync(B->d_check.p, B->h_check.p, ns * sizeof(CheckDesc), hipMemcpyHostToDevice, st));
-    B->epoch = next_epoch(B->epoch);
-    if (maxn > 0)
-        hipLaunchKernelGGL(k_check_batch, dim3(nblk(maxn, 256), ns), dim3(256), 0, st, B->d_check.p, B->flags.d, B->epoch);
-    HIPCHK(hipStreamSynchronize(st));
-    bool any = false;
-    for (int k = 0; k < ns; k++) {
-        nbabfs_state *s = B->st[k];
-        s->forced_xyz = nullptr;
-        const bool need = s->isNew || !s->have_ref || !s->listValid || ((volatile int *)B->flags.h)[k] == B->epoch;
-        s->calls++;
-        if (need) {
-            HIPCHK(hipStreamSynchronize(s->stream));
-            CHK(build_list(s, d_xyz[k]));
-            CHK(s->xyz_ref.ensure(3 * (size_t)std::max(s->n, 1)));
-            if (s->n)
-                HIPCHK(hipMemcpyAsync(s->xyz_ref.p, d_xyz[k], 3 * (size_t)s->n * sizeof(double), hipMemcpyDeviceToDevice,
-                                      s->stream));
-            HIPCHK(hipStreamSynchronize(s->stream));
-            s->have_ref = true;
-            count_update(s);
-            any = true;
-        }
-        s->isNew = false;
-        if (rebuilt) rebuilt[k] = need ? 1 : 0;
-    }
-    // A new layout (and graph) when some list was rebuilt or re-cut, or when a
-    // system's descriptor (its buffers, sizes or arrays) is not the one laid out.
-    bool relayout = any || !B->layout_valid || (int)B->shadow.size() != ns;
+    bool relayout = !B->layout_valid || (int)B->shadow.size() != ns;
     for (int k = 0; k < ns && !relayout; k++) {
         SysDesc D;
         fill_desc(B->st[k], D, d_xyz[k], d_grad ? d_grad[k] : nullptr);
@@ -2726,11 +2709,18 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
     // word: clear what an abandoned single-state pass left there.
     for (auto *s : B->st)
         if (s->err_dirty) {
-            launch_fill_int(s->errw.p, 1, 0, st);
+            launch_fill_int(s->errw.p, 1, 0, B->stream);
             s->err_dirty = false;
         }
+    return NBABFS_STATUS_CONTINUE;
+}
 
+// The batch's passes (graph replay, or enqueued directly when timed or without graphs).
+static int batch_launch(nbabfs_batch *B)
+{
+    hipStream_t st = B->stream;
     const bool timed = B->timing && (B->mixed ? B->nitems > 0 : true);
+    B->timed_pass = timed;
     if (timed) {
         for (auto &e : B->ev)
             if (!e) HIPCHK(hipEventCreate(&e));
@@ -2753,14 +2743,103 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
     } else {
         CHK(batch_enqueue(B));
     }
-    B->n_launches++;
-    HIPCHK(hipStreamSynchronize(st));
-    if (timed) {
-        float ms = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms, B->ev[0], B->ev[1]));
-        B->force_ms += ms;
-        B->force_launches++;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// The timed pair launch of the last completed pass (the stream has synchronised).
+static int batch_collect_timing(nbabfs_batch *B)
+{
+    if (!B->timed_pass) return NBABFS_STATUS_CONTINUE;
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, B->ev[0], B->ev[1]));
+    B->force_ms += ms;
+    B->force_launches++;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// Rebuilds system k's list (NBModelABFS_Update's rebuild branch).
+static int batch_rebuild(nbabfs_batch *B, int k, const double *d_xyz)
+{
+    nbabfs_state *s = B->st[k];
+    HIPCHK(hipStreamSynchronize(s->stream));
+    CHK(build_list(s, d_xyz));
+    CHK(s->xyz_ref.ensure(3 * (size_t)std::max(s->n, 1)));
+    if (s->n)
+        HIPCHK(hipMemcpyAsync(s->xyz_ref.p, d_xyz, 3 * (size_t)s->n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->have_ref = true;
+    count_update(s);
+    return NBABFS_STATUS_CONTINUE;
+}
+
+// The update test's descriptors (uploaded when they changed: new coordinate arrays or reference copies).
+static int batch_check_descs(nbabfs_batch *B, const double *const *d_xyz)
+{
+    const int ns = (int)B->st.size();
+    std::vector<CheckDesc> cd(ns);
+    for (int k = 0; k < ns; k++) {
+        nbabfs_state *s = B->st[k];
+        const double buf = 0.5 * (s->opt.listCutoff - s->opt.outerCutoff);
+        cd[k] = CheckDesc{d_xyz[k], s->have_ref ? s->xyz_ref.p : nullptr, s->has_fixed ? s->freef.p : nullptr, s->n,
+                          buf * buf};
     }
+    if ((int)B->check_shadow.size() == ns && std::memcmp(cd.data(), B->check_shadow.data(), ns * sizeof(CheckDesc)) == 0)
+        return NBABFS_STATUS_CONTINUE;
+    // The staging copy of the last upload has completed: every call ends with a synchronisation.
+    std::memcpy(B->h_check.p, cd.data(), ns * sizeof(CheckDesc));
+    HIPCHK(hipMemcpyAsync(B->d_check.p, B->h_check.p, ns * sizeof(CheckDesc), hipMemcpyHostToDevice, B->stream));
+    B->check_shadow = cd;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, double *const *d_grad,
+                                   double *energies, int *rebuilt)
+{
+    if (!B || !d_xyz || !energies) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
+    const int ns = (int)B->st.size();
+    hipStream_t st = B->stream;
+    for (auto *s : B->st)
+        if (s->F.r2Off != B->st[0]->F.r2Off || s->F.r2On != B->st[0]->F.r2On || s->F.r2Damp != B->st[0]->F.r2Damp)
+            return fail(NBABFS_STATUS_INVALID_ARGUMENT, "all states of a batch must share the cutoffs");
+    CHK(B->d_check.ensure(ns));
+    CHK(B->h_check.ensure(ns));
+    CHK(B->flags.ensure(ns + 1));
+    CHK(B->h_energies.ensure(NB_NENERGIES * (size_t)ns));
+    if (!B->gguard.p) {
+        CHK(B->gguard.ensure(2));
+        launch_fill_int(B->gguard.p, 2, 0, st);
+    }
+    // Rebuilds known before any test (first call, no list) come first.
+    for (int k = 0; k < ns; k++) {
+        nbabfs_state *s = B->st[k];
+        s->forced_xyz = nullptr;
+        s->calls++;
+        const bool need = s->isNew || !s->have_ref || !s->listValid;
+        if (need) CHK(batch_rebuild(B, k, d_xyz[k]));
+        s->isNew = false;
+        if (rebuilt) rebuilt[k] = need ? 1 : 0;
+    }
+    // The update test of every system and the batched passes behind it, guarded by it: when no system moved
+    // too far (the usual case) the call waits for the GPU once; otherwise the passes do nothing, the lists that
+    // must be rebuilt are rebuilt, and the test and passes run again.
+    for (int pass = 0; pass < 2; pass++) {
+        CHK(batch_check_descs(B, d_xyz));
+        CHK(batch_relayout_if_needed(B, d_xyz, d_grad));
+        CHK(batch_launch(B));
+        HIPCHK(hipStreamSynchronize(st));
+        const int epoch = ((volatile int *)B->flags.h)[ns];
+        bool any = false;
+        for (int k = 0; k < ns; k++) {
+            if (((volatile int *)B->flags.h)[k] != epoch) continue;
+            if (pass == 1) return fail(NBABFS_STATUS_LOGIC_ERROR, "batch system %d moved beyond its new list", k);
+            CHK(batch_rebuild(B, k, d_xyz[k]));
+            if (rebuilt) rebuilt[k] = 1;
+            any = true;
+        }
+        if (!any) break;
+    }
+    B->n_launches++;
+    CHK(batch_collect_timing(B));
     int bad = -1;
     for (int k = 0; k < ns; k++) {
         std::memcpy(energies + 4 * (size_t)k, B->h_energies.p + NB_NENERGIES * (size_t)k, 4 * sizeof(double));

@@ -136,9 +136,12 @@ struct PrepareCheck {
 
 // A force launch queued before the update decision is known: every
 // workgroup returns at once when *flag == epoch (the list must be rebuilt).
+// With epoch_ptr, the epoch is read from device memory (*epoch_ptr), so a
+// captured graph compares against the epoch of the check that ran before it.
 struct ForceGuard {
     const int *flag;
     int epoch;
+    const int *epoch_ptr;
 };
 void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, int *heads, const PrepareCheck &chk,
                     hipStream_t s);
@@ -162,8 +165,10 @@ void launch_force_plain(const SysDesc *d_desc, int nsys, int max_n, const AbfsFa
 // Plain FP64 image pass over the image lists' work items.
 void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int nitems, const AbfsFactors &F,
                              ForceGuard g, hipStream_t s, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
-void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s);
-void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s);
+void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s,
+                    ForceGuard g = ForceGuard{nullptr, 0, nullptr});
+void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s,
+                     ForceGuard g = ForceGuard{nullptr, 0, nullptr});
 // Virial of the single system's last call (NB_VACC_WORDS in D.vacc, zeroed by the caller).
 void launch_virial(const SysDesc *d_desc, int n, int plain, hipStream_t s);
 
