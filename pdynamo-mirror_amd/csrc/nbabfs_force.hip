@@ -871,14 +871,18 @@ __global__ __launch_bounds__(256) void k_finalize(const SysDesc *__restrict__ de
     const SysDesc &D = descs[blockIdx.y];
     // The last block of each system sums its energies while the others add
     // the gradients (the two are independent, so neither waits for the other).
+    // One thread per slot: the fixed-point words are read in slot order
+    // (coalesced) and each atom's row is written through slot_atom.
     if (blockIdx.x + 1 != gridDim.x) {
-        const int i = blockIdx.x * 256 + threadIdx.x;
-        if (i < D.n && D.grad != nullptr) {
-            const int s = D.atom_slot[i];
-            for (int k = 0; k < 3; k++) {
-                double g = fx_value(D.facc, D.n_pad, s, k);
-                if (plain) g += D.gplain[3 * (size_t)i + k];
-                D.grad[3 * (size_t)i + k] = D.grad_assign ? g : D.grad[3 * (size_t)i + k] + g;
+        const int s = blockIdx.x * 256 + threadIdx.x;
+        if (s < D.n_pad && D.grad != nullptr) {
+            const int i = D.slot_atom[s];
+            if (i >= 0) {
+                for (int k = 0; k < 3; k++) {
+                    double g = fx_value(D.facc, D.n_pad, s, k);
+                    if (plain) g += D.gplain[3 * (size_t)i + k];
+                    D.grad[3 * (size_t)i + k] = D.grad_assign ? g : D.grad[3 * (size_t)i + k] + g;
+                }
             }
         }
         return;
@@ -1100,10 +1104,10 @@ void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const A
         hipLaunchKernelGGL(k_pairs14, dim3(max_n14_blocks, nsys), dim3(64), 0, s, d_desc, F, g);
 }
 
-void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s, ForceGuard g)
+void launch_finalize(const SysDesc *d_desc, int nsys, int max_npad, int plain, hipStream_t s, ForceGuard g)
 {
     if (nsys <= 0) return;
-    hipLaunchKernelGGL(k_finalize, dim3(nblk(max_n, 256) + 1, nsys), dim3(256), 0, s, d_desc, plain, g);
+    hipLaunchKernelGGL(k_finalize, dim3(nblk(max_npad, 256) + 1, nsys), dim3(256), 0, s, d_desc, plain, g);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -1444,7 +1444,7 @@ static int mmmm_async(nbabfs_state *S, const double *d_xyz, double *d_grad)
     // The tiled pair launch evaluates the 1-4 list in its extra workgroups.
     if (plain) launch_pairs14(S->d_desc.p, 1, nblk(S->n14, NB_P14_BLOCK), S->F, st);
     if (S->has_qc) launch_qclj(S->d_desc.p, S->nlj, S->F, st);
-    launch_finalize(S->d_desc.p, 1, S->n, plain ? 1 : 0, st);
+    launch_finalize(S->d_desc.p, 1, S->n_pad, plain ? 1 : 0, st);
     S->err_dirty = false;
     HIPCHK(hipGetLastError());
     return NBABFS_STATUS_CONTINUE;
@@ -2686,7 +2686,7 @@ static int batch_enqueue(nbabfs_batch *B, hipEvent_t ev_start = nullptr, hipEven
         launch_force_plain(B->d_desc.p, ns, B->max_n, s0->F, g, st, ev_start, ev_stop);
     }
     launch_pairs14(B->d_desc.p, ns, B->max_n14b, s0->F, st, g);
-    launch_finalize(B->d_desc.p, ns, B->max_n, B->mixed ? 0 : 1, st, g);
+    launch_finalize(B->d_desc.p, ns, B->max_npad, B->mixed ? 0 : 1, st, g);
     HIPCHK(hipMemcpyAsync(B->h_energies.p, B->energies.p, NB_NENERGIES * (size_t)ns * sizeof(double),
                           hipMemcpyDeviceToHost, st));
     return NBABFS_STATUS_CONTINUE;

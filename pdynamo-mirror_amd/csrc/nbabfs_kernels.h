@@ -167,7 +167,8 @@ void launch_force_image_fp64(const SysDesc *d_desc, const WorkItem *items, int n
                              ForceGuard g, hipStream_t s, hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 void launch_pairs14(const SysDesc *d_desc, int nsys, int max_n14_blocks, const AbfsFactors &F, hipStream_t s,
                     ForceGuard g = ForceGuard{nullptr, 0, nullptr});
-void launch_finalize(const SysDesc *d_desc, int nsys, int max_n, int plain, hipStream_t s,
+// max_npad: the largest padded slot count of the systems (one thread per slot).
+void launch_finalize(const SysDesc *d_desc, int nsys, int max_npad, int plain, hipStream_t s,
                      ForceGuard g = ForceGuard{nullptr, 0, nullptr});
 // Virial of the single system's last call (NB_VACC_WORDS in D.vacc, zeroed by the caller).
 void launch_virial(const SysDesc *d_desc, int n, int plain, hipStream_t s);
