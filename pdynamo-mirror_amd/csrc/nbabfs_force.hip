@@ -506,9 +506,10 @@ __global__ __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu((IMG |
                     const double dx = xi.x - xjt.x, dy = xi.y - xjt.y, dz = xi.z - xjt.z;
                     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
                     const bool inr = r2 <= F.r2Off;
-                    const bool act = inl && inr;
-                    // Ballots of the two compares (lane masks as they stand), combined in scalar registers.
+                    // Ballots of the two compares (lane masks as they stand), combined in scalar registers; the
+                    // active lanes are that mask again (an execution mask, no per-lane recomputation).
                     const unsigned long long mact = __builtin_amdgcn_ballot_w64(inl) & __builtin_amdgcn_ballot_w64(inr);
+                    const bool act = __builtin_amdgcn_inverse_ballot_w64(mact);
                     if (mact != 0ull) {
                         if (F64) {
                             // PairwiseInteraction.h:72-119 in FP64 with the reference's operations.

@@ -442,10 +442,21 @@ class NativeBatch:
         check(L.nbabfs_batch_create(arr, len(self.states), 1 if use_graph else 0, C.byref(h)))
         self.h = h.value
 
+    def _pointer_array(self, ptrs, slot):
+        # The ctypes array of the last call is reused while the pointers are the same.
+        key = tuple(ptrs)
+        cached = self._arrays.get(slot)
+        if cached is None or cached[0] != key:
+            cached = (key, (C.c_void_p * len(key))(*key))
+            self._arrays[slot] = cached
+        return cached[1]
+
     def energy(self, d_xyz_ptrs, d_grad_ptrs=None):
         ns = len(self.states)
-        xa = (C.c_void_p * ns)(*d_xyz_ptrs)
-        ga = None if d_grad_ptrs is None else (C.c_void_p * ns)(*d_grad_ptrs)
+        if not hasattr(self, "_arrays"):
+            self._arrays = {}
+        xa = self._pointer_array(d_xyz_ptrs, 0)
+        ga = None if d_grad_ptrs is None else self._pointer_array(d_grad_ptrs, 1)
         e = np.zeros(4 * ns)
         rebuilt = np.zeros(ns, np.int32)
         check(load().nbabfs_batch_energy(self.h, xa, ga, _ptr(e), _ptr(rebuilt)))
