@@ -462,30 +462,35 @@ __global__ __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu((IMG |
             isc = uniform_f64(X[12]);
         }
 
-        wave_region_sync<WPG>();
+        // The item's independent loads go out together: its super-cluster's records and types and its first two
+        // entries' headers and mask bytes; then the first j-cluster's records, which need the first header.
+        // Pipeline of the entry loop: while entry e is evaluated, entry e+1's j-cluster record and entry e+2's
+        // header and mask byte are in flight.  The indices are clamped, so every iteration issues the same loads
+        // and no atomic is pending inside the loop.
+        const int e_last = w.first + w.count - 1;
+        const double4 irec = load_slot(xq, I0 * NB_CL + lane);
+        const int ityp = ty[I0 * NB_CL + lane];
+        int2 jc0 = entry_jc(eb, w.first);
+        int lb0 = entry_bits(eb, w.first, bl);
+        int2 jc1 = entry_jc(eb, min(w.first + 1, e_last));
+        int lb1 = entry_bits(eb, min(w.first + 1, e_last), bl);
+
+        // A persistent wave's region still holds its previous item's data until every lane is past it.
+        if (qp.heads != nullptr) wave_region_sync<WPG>();
         if (IMG && lane < 12) sX[lane] = X[lane];
         if (ROWS_LDS && rows_sys < 0)
             for (int k = lane; k < 2 * nt * nt; k += 64) sR[k] = load_slot(rows, k);
-        sI[lane] = load_slot(xq, I0 * NB_CL + lane);
-        sT[lane] = 64 * nt * ty[I0 * NB_CL + lane];   // byte offset of the type's first row pair
+        sI[lane] = irec;
+        sT[lane] = 64 * nt * ityp;   // byte offset of the type's first row pair
+        int J = __builtin_amdgcn_readfirstlane(jc0.x);
+        double4 xj = load_slot(xq, J * NB_CL + b);
+        int tj = 64 * ty[J * NB_CL + b];
         wave_region_sync<WPG>();
 
         acc_t fx[NB_SCC], fy[NB_SCC], fz[NB_SCC];
 #pragma unroll
         for (int c = 0; c < NB_SCC; c++) fx[c] = fy[c] = fz[c] = (acc_t)0;
         double eel = 0.0, elj = 0.0;
-        const int e_last = w.first + w.count - 1;
-        // Pipeline: while entry e is evaluated, entry e+1's j-cluster record
-        // and entry e+2's header and mask byte are in flight.  The indices are
-        // clamped, so every iteration issues the same loads and no atomic is
-        // pending inside the loop.
-        int2 jc0 = entry_jc(eb, w.first);
-        int lb0 = entry_bits(eb, w.first, bl);
-        int2 jc1 = entry_jc(eb, min(w.first + 1, e_last));
-        int lb1 = entry_bits(eb, min(w.first + 1, e_last), bl);
-        int J = __builtin_amdgcn_readfirstlane(jc0.x);
-        double4 xj = load_slot(xq, J * NB_CL + b);
-        int tj = 64 * ty[J * NB_CL + b];
         for (int e = w.first; e <= e_last; e++) {
             const int Jn = __builtin_amdgcn_readfirstlane(jc1.x);
             const double4 xjn = load_slot(xq, Jn * NB_CL + b);
