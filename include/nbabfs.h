@@ -73,6 +73,9 @@ typedef struct {
                                       NBABFS_MAX_ENTRIES_PER_ITEM) */
     int    wavesPerCU;             /* persistent-kernel waves per CU; 0 = one workgroup per work item */
     int    imageEntriesPerItem;    /* entries per work item of the image lists (0 = 16) */
+    int    useGridByCell;          /* 1 (the ABFS default): grid search cell by cell
+                                      (PairListGenerator.c:1020-1099); 0: point by cell
+                                      (PairListGenerator.c:1155-1218) for the MM/MM list */
 } nbabfs_options;
 
 typedef struct {

@@ -221,6 +221,65 @@ static void save_record(PairSink *s, int i, int n, const int *ind)
     s->rec_ptr[s->nrec] = s->npairs;
 }
 
+/* Point/cell self list (pCore-1.9.0/extensions/csource/PairListGenerator.c:1155-1218): atom i, ascending from 1,
+ * is the point.  The cells searched are the stencil offsets around i's cell that lie on the grid and pass the
+ * point/cell range test of RegularGrid_FindCellsWithinRangeOfPoint (RegularGrid.c:305-336): the smallest squared
+ * distance from the point to the cell below cutoff^2; a cell whose largest squared distance is at most cutoff^2 is
+ * taken whole, otherwise each atom by the distance test.  Atoms j of a cell come in ascending order and the scan of
+ * a cell stops at j >= i.  Distances are RegularGrid_DistancesSquaredBetweenCellAndPoint's (RegularGrid.c:139-180,
+ * non-periodic branch): per axis the cell [n h + lower, n h + lower + h] against the coordinate, summed in axis
+ * order.  A point off the grid has no cell (RegularGrid_CellIDToIndices, RegularGrid.c:69-85, returns without
+ * writing): its search is centred on the cell indices last decoded, those of the latest earlier point on the grid,
+ * or, before any, the lower corner (-u, -u, -u) where the search-range construction (RegularGrid.c:526-575) leaves
+ * the same work array. */
+static void point_cell_records(PairSink *s, int n, const double *xyz, const int *xptr, const int *xidx, double c2,
+                               double cellSize, double cutoff, const int *bins, const double *lower, const int *cell,
+                               const long *first, const long *total, const int *points, int noff, const int *off,
+                               unsigned char *incl, int *ind)
+{
+    const long stride0 = (long)bins[1] * bins[2], stride1 = bins[2];
+    const int u = (int)ceil(cutoff / cellSize);
+    int pc[3] = {-u, -u, -u};
+    for (int i = 1; i < n; i++) {
+        if (cell[i] >= 0) {
+            long c = cell[i];
+            pc[0] = (int)(c / stride0);
+            pc[1] = (int)((c % stride0) / stride1);
+            pc[2] = (int)(c % stride1);
+        }
+        int xf = xptr ? xptr[i] : 0, xl = xptr ? xptr[i + 1] : 0, m = 0;
+        for (int x = xf; x < xl; x++) { int j = xidx[x]; if (j >= i) break; incl[j] = 0; }
+        for (int k = 0; k < noff; k++) {
+            int idx[3] = {pc[0] + off[3 * k], pc[1] + off[3 * k + 1], pc[2] + off[3 * k + 2]};
+            if (idx[0] < 0 || idx[0] >= bins[0] || idx[1] < 0 || idx[1] >= bins[1] || idx[2] < 0 || idx[2] >= bins[2])
+                continue;
+            double maxR2 = 0.0, minR2 = 0.0;
+            for (int d = 0; d < 3; d++) {
+                double lo = (double)idx[d] * cellSize + lower[d], hi = lo + cellSize, x = xyz[3 * i + d];
+                double mx, mn;
+                if (x > hi) { mx = x - lo; mn = x - hi; }
+                else if (x < lo) { mx = hi - x; mn = lo - x; }
+                else { mx = (x - lo) > (hi - x) ? (x - lo) : (hi - x); mn = 0.0; }
+                maxR2 += mx * mx;
+                minR2 += mn * mn;
+            }
+            if (!(minR2 < c2)) continue;
+            const int all = maxR2 <= c2;
+            const long cid = idx[0] * stride0 + idx[1] * stride1 + idx[2];
+            for (long p = first[cid], pe = first[cid] + total[cid]; p < pe; p++) {
+                int j = points[p];
+                if (j >= i) break;
+                if (!incl[j]) continue;
+                if (all) { ind[m++] = j; continue; }
+                double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1], dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+                if ((dx * dx) + (dy * dy) + (dz * dz) <= c2) ind[m++] = j;
+            }
+        }
+        save_record(s, i, m, ind);
+        for (int x = xf; x < xl; x++) { int j = xidx[x]; if (j >= i) break; incl[j] = 1; }
+    }
+}
+
 long orc_self_pair_list(int n, const double *xyz, const int *xptr, const int *xidx,
                         double cutoff, double cellSize, int method,
                         long cap_rec, long cap_pairs, int *rec_i, long *rec_ptr, int *js,
@@ -248,7 +307,7 @@ long orc_self_pair_list(int n, const double *xyz, const int *xptr, const int *xi
             for (int x = xf; x < xl; x++) { int j = xidx[x]; if (j >= i) break; incl[j] = 1; }
         }
     } else {
-        /* Cell/cell (PairListGenerator.c:1020-1099). */
+        /* Cell/cell (PairListGenerator.c:1020-1099); method 2, point/cell, shares the grid and occupancy. */
         int bins[3];
         double lower[3];
         int *cell = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
@@ -277,7 +336,10 @@ long orc_self_pair_list(int n, const double *xyz, const int *xptr, const int *xi
         long *ncell_id = (long *)malloc(sizeof(long) * (size_t)noff);
         int *ncell_all = (int *)malloc(sizeof(int) * (size_t)noff);
         long stride0 = (long)bins[1] * bins[2], stride1 = bins[2];
-        for (long c1 = 0; c1 < ncells; c1++) {
+        if (method == 2)
+            point_cell_records(&s, n, xyz, xptr, xidx, c2, cellSize, cutoff, bins, lower, cell, first, total, points,
+                               noff, off, incl, ind);
+        else for (long c1 = 0; c1 < ncells; c1++) {
             long p1s = first[c1], p1t = total[c1];
             int ci[3] = {(int)(c1 / stride0), (int)((c1 % stride0) / stride1), (int)(c1 % stride1)};
             int nn = 0;

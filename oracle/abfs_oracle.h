@@ -54,8 +54,8 @@ long orc_make_grid(int n, const double *xyz, double cellSize, int *bins, double 
  * the capacity of offsets/flags in entries. */
 int orc_search_range(double cellSize, double cutoff, int cap, int *offsets, int *flags);
 
-/* Self pair list (pCx/csource/PairListGenerator.c:946-1150).  method: 1 grid
- * cell/cell, 0 direct.  Exclusions as symmetric CSR sorted ascending per row.
+/* Self pair list (pCx/csource/PairListGenerator.c:946-1218).  method: 1 grid
+ * cell/cell, 2 grid point/cell, 0 direct.  Exclusions as symmetric CSR sorted ascending per row.
  * Output records in reference order: rec_i[r], rec_ptr[r..r+1] into js.
  * Returns npairs, or -1 if capacities are too small (need_* report sizes), or
  * ORC_INVALID_ARGUMENT for a cell grid with no cells (a zero extent). */

@@ -395,8 +395,10 @@ class OracleABFS:
 
     def __init__(self, system, damping=0.5, inner=8.0, outer=12.0, list_cutoff=13.5, dielectric=1.0,
                  scale14=1.0, cell_factor=0.5, min_extent=2, min_points=500, tableA14=None, tableB14=None,
-                 fixed=None, nthreads=0, transformations=None, check_for_inverses=True, image_expand_factor=0):
+                 fixed=None, nthreads=0, transformations=None, check_for_inverses=True, image_expand_factor=0,
+                 use_grid_by_cell=True):
         self.s = system
+        self.use_grid_by_cell = use_grid_by_cell
         self.transformations = transformations
         self.check_for_inverses = check_for_inverses
         self.image_expand_factor = image_expand_factor
@@ -430,8 +432,9 @@ class OracleABFS:
                                                  self.min_points)
             if self.fixed is not None:
                 raise NotImplementedError("fixed atoms are checked against the reference driver instead")
+            method = (1 if self.use_grid_by_cell else 2) if self.use_grid else 0
             self.records = self_pair_list(xyz, self.xptr, self.xidx, self.list_cutoff,
-                                          self.cell_factor * self.list_cutoff, 1 if self.use_grid else 0)
+                                          self.cell_factor * self.list_cutoff, method)
             self.ref = np.array(xyz, dtype=np.float64, copy=True)
             self.updates += 1
         return rebuild
@@ -554,8 +557,10 @@ class ReferenceABFS:
 
     def __init__(self, system, damping=0.5, inner=8.0, outer=12.0, list_cutoff=13.5, dielectric=1.0,
                  scale14=1.0, cell_factor=0.5, min_extent=2, min_points=500, tableA14=None, tableB14=None,
-                 nthreads=1, transformations=None, check_for_inverses=True, image_expand_factor=0):
-        """transformations: None, or (rotations (k,3,3), translations (k,3)) in fractional form."""
+                 nthreads=1, transformations=None, check_for_inverses=True, image_expand_factor=0,
+                 use_grid_by_cell=True):
+        """transformations: None, or (rotations (k,3,3), translations (k,3)) in fractional form.
+        use_grid_by_cell: the generator's useGridByCell (cell/cell or point/cell grid search)."""
         L = ref_lib()
         s = system
         self.n = s.natoms
@@ -566,7 +571,7 @@ class ReferenceABFS:
         tB14 = None if tableB14 is None else np.ascontiguousarray(tableB14, np.float64)
         self._keep += [tA14, tB14]
         opts = np.array([damping, inner, outer, list_cutoff, dielectric, scale14])
-        gen = np.array([cell_factor, min_extent, min_points], dtype=np.float64)
+        gen = np.array([cell_factor, min_extent, min_points, 1.0 if use_grid_by_cell else 0.0], dtype=np.float64)
         args = [self.n, np.ascontiguousarray(s.charges, np.float64), np.ascontiguousarray(s.ljtypes, np.int32),
                 s.ntypes, np.ascontiguousarray(s.tableA, np.float64), np.ascontiguousarray(s.tableB, np.float64),
                 None if tA14 is None else tA14.ctypes.data, None if tB14 is None else tB14.ctypes.data,

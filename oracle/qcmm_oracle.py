@@ -508,7 +508,7 @@ class ReferenceQCMM:
 
     def __init__(self, system, qc, coupling, spline_x, spline_y, fixed=None, damping=0.5, inner=8.0, outer=12.0,
                  list_cutoff=13.5, dielectric=1.0, scale14=1.0, cell_factor=0.5, min_extent=2, min_points=500,
-                 nthreads=1):
+                 nthreads=1, use_grid_by_cell=True):
         L = _ref()
         s = system
         self.n, self.nqc = s.natoms, qc.nqc
@@ -523,7 +523,7 @@ class ReferenceQCMM:
                 np.ascontiguousarray(qc.mmp if qc.mmp.size else np.zeros(1), np.int32)]
         self._keep = [ex, p14, fx, self.sx, self.sy] + arrs
         opts = np.array([damping, inner, outer, list_cutoff, dielectric, scale14])
-        gen = np.array([cell_factor, min_extent, min_points], dtype=np.float64)
+        gen = np.array([cell_factor, min_extent, min_points, 1.0 if use_grid_by_cell else 0.0], dtype=np.float64)
         self.h = L.ref_create_qc(self.n, np.ascontiguousarray(s.charges, np.float64),
                                  np.ascontiguousarray(s.ljtypes, np.int32), s.ntypes,
                                  np.ascontiguousarray(s.tableA, np.float64), np.ascontiguousarray(s.tableB, np.float64),

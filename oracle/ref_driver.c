@@ -76,8 +76,8 @@ static LJParameterContainer *make_lj(int ntypes, const double *tA, const double 
 }
 
 /* opts: damping, inner, outer, list, dielectric, scale14.
- * gen: cutoffCellSizeFactor, minimumCellExtent, minimumPoints (the ABFS
- * default generator of pMolecule.NBModelABFS.pyx:59-71 otherwise). */
+ * gen: cutoffCellSizeFactor, minimumCellExtent, minimumPoints, useGridByCell
+ * (the ABFS default generator of pMolecule.NBModelABFS.pyx:59-71 otherwise). */
 static Transformation3Container *make_transformations(int ntrans, const double *rot, const double *trans)
 {
     Transformation3Container *tc = Transformation3Container_Allocate(ntrans);
@@ -123,7 +123,7 @@ static void *create_common(int n, const double *charges, const int *ljtypes, int
     h->gen->minimumCellSize = 3.0;
     h->gen->minimumExtentFactor = 1.5;
     h->gen->sortIndices = False;
-    h->gen->useGridByCell = True;
+    h->gen->useGridByCell = gen[3] != 0.0 ? True : False;
     h->gen->cellSize = h->gen->cutoffCellSizeFactor * h->gen->cutoff;
     h->pw = PairwiseInteractionABFS_Allocate();
     h->pw->dampingCutoff = opts[0];

@@ -315,10 +315,20 @@ __global__ __launch_bounds__(256) void k_cluster_geom(const double *__restrict__
 
 // Slot-ordered search records of the self-list search: the atom's FP64
 // position with its index and flag bits (NB_SF_*), and its reference cell.
+//
+// Point/cell search (pointcell): an atom off the grid still searches as the
+// point of its pairs with lower-index atoms.  The reference takes its search
+// cell from the cell indices it decoded last (RegularGrid_CellIDToIndices,
+// pCore-1.9.0/extensions/csource/RegularGrid.c:69-85, leaves them unchanged
+// for a point with no cell): those of the latest earlier atom (index 1 up, in
+// the MM selection) on the grid, else the corner (-u, -u, -u) at which the
+// search-range construction leaves the same array (RegularGrid.c:526-575).
+// That cell is stored for such an atom; its OFFGRID flag stays set.
 __global__ __launch_bounds__(256) void k_slot_records(const double *__restrict__ xyz, const int *__restrict__ slot_atom,
                                                       int n_pad, const int4 *__restrict__ cellv,
                                                       const unsigned char *__restrict__ free_flags, int use_grid,
-                                                      double4 *__restrict__ srec, int4 *__restrict__ scell)
+                                                      int pointcell, int u, double4 *__restrict__ srec,
+                                                      int4 *__restrict__ scell)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= n_pad) return;
@@ -332,7 +342,20 @@ __global__ __launch_bounds__(256) void k_slot_records(const double *__restrict__
         p.z = xyz[3 * (size_t)a + 2];
         c = cellv[a];
         fl = free_flags ? (free_flags[a] & (NB_SF_FREE | NB_SF_QC)) : NB_SF_FREE;
-        if (use_grid && c.w == 0) fl |= NB_SF_OFFGRID;
+        if (use_grid && c.w == 0) {
+            fl |= NB_SF_OFFGRID;
+            if (pointcell) {
+                int4 sc = make_int4(-u, -u, -u, 0);
+                for (int k = a - 1; k >= 1; k--) {
+                    const int4 ck = cellv[k];
+                    if (ck.w != 0 && !(free_flags && (free_flags[k] & NB_SF_QC))) {
+                        sc = make_int4(ck.x, ck.y, ck.z, 0);
+                        break;
+                    }
+                }
+                c = sc;
+            }
+        }
     }
     p.w = __hiloint2double(fl, a);
     srec[s] = p;
@@ -465,12 +488,47 @@ __device__ __forceinline__ int search_eval_cross(const SearchParams &P, int npen
     return nvalid;
 }
 
+// Squared smallest and largest distances between a point and grid cell n
+// (RegularGrid_DistancesSquaredBetweenCellAndPoint, pCore-1.9.0/extensions/
+// csource/RegularGrid.c:139-180, non-periodic branch, in its operation order:
+// the cell spans [n h + lower, n h + lower + h] on each axis; the per-axis
+// squares are summed in axis order).
+__device__ __forceinline__ void point_cell_r2(const SearchParams &P, double x, double y, double z, int nx, int ny,
+                                              int nz, double &minR2, double &maxR2)
+{
+#pragma clang fp contract(off)
+    const double pt[3] = {x, y, z};
+    const int nn[3] = {nx, ny, nz};
+    double mn2[3], mx2[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double lo = (double)nn[d] * P.gh + P.glower[d];
+        const double hi = lo + P.gh;
+        const double v = pt[d];
+        const double dl = v - lo, du = hi - v;
+        const double mx = v > hi ? dl : (v < lo ? du : (dl > du ? dl : du));
+        const double mn = v > hi ? v - hi : (v < lo ? lo - v : 0.0);
+        mn2[d] = mn * mn;
+        mx2[d] = mx * mx;
+    }
+    minR2 = (mn2[0] + mn2[1]) + mn2[2];
+    maxR2 = (mx2[0] + mx2[1]) + mx2[2];
+}
+
 // Self lists: the candidates sCand[0..npend) of one wave.  Lane 8a + b
 // evaluates i-atom (c, a) of each listed tile c against j-atom b; lanes with
 // b < a hold the lower triangle of the self tile (J = I0 + c).  Every part of
 // the rule is computed without branches; only the exclusion scan (j inside
 // i's excluded index range) branches.
-template <int STEN>
+//
+// PC (point/cell grid search, PairListGenerator.c:1155-1218): the atom with
+// the higher index is the point and the other atom's cell is searched.  The
+// pair is listed when that cell is in the stencil around the point's search
+// cell, the point/cell smallest squared distance is below cutoff^2, and either
+// the largest is at most cutoff^2 (the cell is taken whole) or the FP64
+// distance test passes.  The lower-index atom must be on the grid; the point
+// may be off it (its record then holds its search cell, see k_slot_records).
+template <int STEN, bool PC>
 __device__ __forceinline__ int search_eval_self(const SearchParams &P, int npend, int w, int lane, int I0, int sc,
                                                 int base, int scap, size_t sbase, const int *sCand,
                                                 const unsigned char *sCm, const double4 *sIp, const int4 *sIc,
@@ -508,10 +566,29 @@ __device__ __forceinline__ int search_eval_self(const SearchParams &P, int npend
             const int4 ic = sIc[li];
             const int2 xr = sIxr[li];
             // Both atoms real, in the MM selection and (grid) on the grid, one of them free.
-            bool ok = ((ic.w | jfl) & (NB_SF_FREE | NB_SF_QC | NB_SF_OFFGRID | NB_SF_EMPTY)) == NB_SF_FREE;
+            bool ok;
+            if (PC) {
+                const bool ipt = __double2loint(ip.w) > jatom;      // i is the point
+                ok = (((ic.w | jfl) & (NB_SF_FREE | NB_SF_QC | NB_SF_EMPTY)) == NB_SF_FREE) &
+                     !((ipt ? jfl : ic.w) & NB_SF_OFFGRID);
+            } else {
+                ok = ((ic.w | jfl) & (NB_SF_FREE | NB_SF_QC | NB_SF_OFFGRID | NB_SF_EMPTY)) == NB_SF_FREE;
+            }
             ok = ok & ((Jc < I0 + c) | (b < a));
             const bool near = ref_dist2(ip.x, ip.y, ip.z, jp.x, jp.y, jp.z) <= P.cutoff2;
-            if (STEN) {
+            if (PC) {
+                // The stencil is symmetric, so the offset between the two search cells decides membership in
+                // either direction.
+                const unsigned ux = (unsigned)(jc.x - ic.x), uy = (unsigned)(jc.y - ic.y), uz = (unsigned)(jc.z - ic.z);
+                const bool inb = max(max(ux, uy), uz) < wst;
+                const unsigned si = min((ux * wst + uy) * wst + uz, nst - 1u);
+                const int flag = STEN == 1 ? (int)sSten[si] : (int)P.stencil[si];
+                const bool ipt = __double2loint(ip.w) > jatom;
+                double minR2, maxR2;
+                point_cell_r2(P, ipt ? ip.x : jp.x, ipt ? ip.y : jp.y, ipt ? ip.z : jp.z, ipt ? jc.x : ic.x + P.u,
+                              ipt ? jc.y : ic.y + P.u, ipt ? jc.z : ic.z + P.u, minR2, maxR2);
+                ok = ok & inb & (flag != 0) & (minR2 < P.cutoff2) & ((maxR2 <= P.cutoff2) | near);
+            } else if (STEN) {
                 // Cell offset + u per axis (sIc holds the cell - u), in [0, 2u] within the stencil.
                 const unsigned ux = (unsigned)(jc.x - ic.x), uy = (unsigned)(jc.y - ic.y), uz = (unsigned)(jc.z - ic.z);
                 const bool inb = max(max(ux, uy), uz) < wst;
@@ -546,7 +623,7 @@ __device__ __forceinline__ int search_eval_self(const SearchParams &P, int npend
 // no exclusions, every (i, j) pair including an atom with its own image, the
 // OR selection, i on the reference grid (grid method), and the FP64 distance
 // test against the image coordinates GenerateImageLists produced.
-template <bool CROSS, int STEN>
+template <bool CROSS, int STEN, bool PC = false>
 __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, const double *__restrict__ xyz,
                                                               const int *__restrict__ slot_atom,
                                                               const float4 *__restrict__ blo,
@@ -667,7 +744,7 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
                 nvalid += search_eval_cross(P, npend, w, lane, I0, sc, img, base, scap, sbase, sCand, sCm, sIatom,
                                             sIfree, sIon, sIx, sIy, sIz, jxyz, slot_atom, scratch);
             else
-                nvalid += search_eval_self<STEN>(P, npend, w, lane, I0, sc, base, scap, sbase, sCand, sCm, sIp, sIc,
+                nvalid += search_eval_self<STEN, PC>(P, npend, w, lane, I0, sc, base, scap, sbase, sCand, sCm, sIp, sIc,
                                                  sIxr, sXoff, sXcnt, sXpool, sSten, scratch);
         };
 
@@ -1334,11 +1411,12 @@ void launch_cluster_geom(const double *xyz, const int *slot_atom, int ncl, const
 }
 
 void launch_slot_records(const double *xyz, const int *slot_atom, int n_pad, const int4 *cellv,
-                         const unsigned char *free_flags, int use_grid, double4 *srec, int4 *scell, hipStream_t s)
+                         const unsigned char *free_flags, int use_grid, int pointcell, int u, double4 *srec,
+                         int4 *scell, hipStream_t s)
 {
     if (n_pad > 0)
         hipLaunchKernelGGL(k_slot_records, dim3(nblk(n_pad, 256)), dim3(256), 0, s, xyz, slot_atom, n_pad, cellv,
-                           free_flags, use_grid, srec, scell);
+                           free_flags, use_grid, pointcell, u, srec, scell);
 }
 
 void launch_search(const SearchParams &P, int nsc, const double *xyz, const int *slot_atom, const float4 *blo,
@@ -1348,6 +1426,15 @@ void launch_search(const SearchParams &P, int nsc, const double *xyz, const int 
     if (nsc <= 0) return;
     const int wst = 2 * P.u + 1;
     const int sten = !P.use_grid ? 0 : (wst * wst * wst <= NB_STENCIL_LDS ? 1 : 2);
+    if (sten != 0 && P.pointcell) {
+        if (sten == 1)
+            hipLaunchKernelGGL((k_search<false, 1, true>), dim3(nsc), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, slot_atom,
+                               blo, bhi, colcl, scratch, ncand, counts, overflow);
+        else
+            hipLaunchKernelGGL((k_search<false, 2, true>), dim3(nsc), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, slot_atom,
+                               blo, bhi, colcl, scratch, ncand, counts, overflow);
+        return;
+    }
     if (sten == 0)
         hipLaunchKernelGGL((k_search<false, 0>), dim3(nsc), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, slot_atom, blo, bhi,
                            colcl, scratch, ncand, counts, overflow);

@@ -248,6 +248,7 @@ extern "C" void nbabfs_default_options(nbabfs_options *o)
     o->entriesPerItem = 16;
     o->wavesPerCU = 0;
     o->imageEntriesPerItem = 16;
+    o->useGridByCell = 1;
 }
 
 // A system's items in a batch, followed by empty items up to `padded` (whole workgroups of the pair launch).
@@ -1051,8 +1052,9 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     launch_cluster_geom(d_xyz, S->slot_atom.p, S->ncl, g.corg, S->blo.p, S->bhi.p, S->centre.p, st);
     CHK(S->srec.ensure(S->n_pad));
     CHK(S->scell.ensure(S->n_pad));
+    const int pointcell = (g.use_grid && !o.useGridByCell) ? 1 : 0;
     launch_slot_records(d_xyz, S->slot_atom.p, S->n_pad, S->cellv.p, S->has_flags ? S->freef.p : nullptr, g.use_grid,
-                        S->srec.p, S->scell.p, st);
+                        pointcell, S->u, S->srec.p, S->scell.p, st);
 
     // Cluster-pair search with the exact reference predicate.
     SearchParams P;
@@ -1070,6 +1072,9 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     P.ncy = g.ncy;
     P.srec = S->srec.p;
     P.scell = S->scell.p;
+    P.pointcell = pointcell;
+    for (int d = 0; d < 3; d++) P.glower[d] = g.lower[d];
+    P.gh = g.h;
     CHK(S->counts.ensure(S->nsc));
     CHK(S->ncand.ensure(S->nsc));
     // Search, then the entry and item offsets; the totals and the capacity

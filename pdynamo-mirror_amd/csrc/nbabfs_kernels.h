@@ -40,6 +40,10 @@ struct SearchParams {
     // Self lists: slot-ordered search records (k_slot_records).
     const double4 *srec;
     const int4 *scell;
+    // Point/cell grid search (useGridByCell false): the reference grid's lower corner and cell size.
+    int pointcell;
+    double glower[3];
+    double gh;
 };
 
 // Search record flag bits (the high word of srec[s].w; the low word is the atom, -1 for an empty slot).
@@ -72,7 +76,8 @@ void launch_cluster_geom(const double *xyz, const int *slot_atom, int ncl, const
                          float4 *bhi, float4 *centre, hipStream_t s);
 // Slot-ordered search records: {x, y, z, atom | flags << 32} and the atom's reference cell.
 void launch_slot_records(const double *xyz, const int *slot_atom, int n_pad, const int4 *cellv,
-                         const unsigned char *free_flags, int use_grid, double4 *srec, int4 *scell, hipStream_t s);
+                         const unsigned char *free_flags, int use_grid, int pointcell, int u, double4 *srec,
+                         int4 *scell, hipStream_t s);
 // Self-list search (the records of P.srec / P.scell must describe the current coordinates and cells).
 void launch_search(const SearchParams &P, int nsc, const double *xyz, const int *slot_atom, const float4 *blo,
                    const float4 *bhi, const int2 *colcl, Entry *scratch, int *ncand, int *counts, int *overflow,

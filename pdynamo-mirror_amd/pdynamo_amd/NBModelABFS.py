@@ -313,6 +313,7 @@ class NBModelABFS:
         o.cutoffCellSizeFactor = self.generator.cutoffCellSizeFactor
         o.minimumCellExtent = self.generator.minimumCellExtent
         o.minimumPoints = self.generator.minimumPoints
+        o.useGridByCell = 1 if self.generator.useGridByCell else 0
         o.precision = self._PRECISIONS[self.precision]
         o.entriesPerItem = self.entriesPerItem
         o.wavesPerCU = self.wavesPerCU

@@ -40,7 +40,7 @@ class Options(C.Structure):
                 ("listCutoff", C.c_double), ("dielectric", C.c_double), ("electrostaticScale14", C.c_double),
                 ("cutoffCellSizeFactor", C.c_double), ("minimumCellExtent", C.c_int), ("minimumPoints", C.c_int),
                 ("precision", C.c_int), ("entriesPerItem", C.c_int), ("wavesPerCU", C.c_int),
-                ("imageEntriesPerItem", C.c_int)]
+                ("imageEntriesPerItem", C.c_int), ("useGridByCell", C.c_int)]
 
 
 class Stats(C.Structure):

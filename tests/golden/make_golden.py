@@ -32,8 +32,9 @@ def run_case(name, mm, coords_seq, opts=None, lj14=None, grad_calls=None):
     kw = dict(damping=opts.get("damping", 0.5), inner=opts.get("inner", 8.0), outer=opts.get("outer", 12.0),
               list_cutoff=opts.get("list_cutoff", 13.5), dielectric=opts.get("dielectric", 1.0),
               scale14=opts.get("scale14", 1.0), cell_factor=opts.get("cell_factor", 0.5))
+    grid_by_cell = bool(opts.get("grid_by_cell", True))
     ref = O.ReferenceABFS(mm, tableA14=None if lj14 is None else lj14[0], tableB14=None if lj14 is None else lj14[1],
-                          nthreads=1, **kw)
+                          nthreads=1, use_grid_by_cell=grid_by_cell, **kw)
     energies, present, rebuilt, grads, npairs = [], [], [], [], []
     keys0 = None
     for k, xyz in enumerate(coords_seq):
@@ -58,11 +59,41 @@ def run_case(name, mm, coords_seq, opts=None, lj14=None, grad_calls=None):
                grads=np.array([grads[k] for k in (grad_calls or range(len(grads)))]),
                grad_calls=np.array(list(grad_calls or range(len(grads)))), npairs=np.array(npairs), use_grid=use_grid,
                keys_first=np.diff(np.concatenate([[0], keys0])).astype(np.uint32) if keys0.size else np.zeros(0, np.uint32),
-               keys_base=np.int64(keys0[0]) if keys0.size else np.int64(0))
+               keys_base=np.int64(keys0[0]) if keys0.size else np.int64(0), grid_by_cell=grid_by_cell)
     if lj14 is not None:
         out["tableA14"], out["tableB14"] = lj14
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print(name, mm.natoms, "pairs", npairs, "rebuilt", rebuilt, "grid", use_grid)
+
+
+def edge_system(seed):
+    """900 atoms in a 27 A box whose extent is exactly 4 cells of 6.75 A on every axis: the atoms placed on the
+    upper faces fall outside the reference grid (floor((x - lower)/h) = 4)."""
+    mm = S.random_small_system(900, 27.0, 100 + seed)
+    x = np.clip(mm.coordinates3, 0.5, 26.5)
+    rng = np.random.default_rng(seed)
+    k = rng.choice(900, 6, replace=False)
+    x[k[0]] = [0.0, 0.0, 0.0]
+    x[k[1]] = [27.0, 27.0, 27.0]
+    x[k[2]] = [27.0, 13.1, 5.2]
+    x[k[3]] = [3.3, 27.0, 20.0]
+    x[k[4]] = [9.9, 11.0, 27.0]
+    x[k[5]] = [27.0, 27.0, 1.0]
+    mm.coordinates3 = np.ascontiguousarray(x)
+    return mm
+
+
+def pointcell_cases():
+    """Point/cell generators (useGridByCell False), PairListGenerator.c:1155-1218."""
+    for seed in (0, 1):
+        e = edge_system(seed)
+        run_case("pointcell_edge900_s%d" % seed, e, [e.coordinates3], opts=dict(grid_by_cell=False))
+    wg = S.water_lattice(900, 28.0, seed=9)
+    for f in (0.25, 1.0):
+        run_case("pointcell_water2700_cell%.2f" % f, wg, [wg.coordinates3], opts=dict(cell_factor=f, grid_by_cell=False))
+    w = S.water_lattice(1000, 31.0, seed=7)
+    run_case("pointcell_water3000_moves", w, [w.coordinates3] + S.displacement_sequence(w, 4, 0.1, seed=3),
+             opts=dict(grid_by_cell=False), grad_calls=[0, 4])
 
 
 def main():
@@ -93,7 +124,11 @@ def main():
     # Large cells on a grid: same-cell pairs are distance-tested (offset 0 not fully in range).
     big = S.random_small_system(1500, 90.0, 13)
     run_case("random1500_cell2.75", big, [big.coordinates3], opts=dict(cell_factor=2.75))
+    pointcell_cases()
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["pointcell"]:
+        pointcell_cases()
+    else:
+        main()

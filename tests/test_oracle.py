@@ -30,6 +30,7 @@ def _keys(fx):
 
 class _Sys:
     def __init__(self, fx):
+        self.name = str(fx["name"]) if "name" in fx else "fixture"
         self.natoms = fx["charges"].shape[0]
         self.coordinates3 = fx["coords"][0]
         self.charges = fx["charges"]
@@ -55,6 +56,7 @@ ALL_FIXTURES = sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
                       if not p.endswith("dhfr_nb.npz") and not os.path.basename(p).startswith("qcmm_"))
 QCMM_FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "qcmm_*.npz")))
 REFERENCE_FIXTURES = [p for p in ALL_FIXTURES if not _has_key(p, "rotations")]
+POINTCELL_FIXTURES = [p for p in REFERENCE_FIXTURES if os.path.basename(p).startswith("pointcell_")]
 IMAGE_FIXTURES = [p for p in ALL_FIXTURES if _has_key(p, "rotations")]
 
 
@@ -65,7 +67,8 @@ def test_oracle_matches_reference_fixtures(oracle_mod, path):
     sysm = _Sys(fx)
     orc = oracle_mod.OracleABFS(sysm, damping=o[0], inner=o[1], outer=o[2], list_cutoff=o[3], dielectric=o[4],
                                 scale14=o[5], cell_factor=o[6], tableA14=fx.get("tableA14"),
-                                tableB14=fx.get("tableB14"), nthreads=2)
+                                tableB14=fx.get("tableB14"), nthreads=2,
+                                use_grid_by_cell=bool(fx.get("grid_by_cell", True)))
     gcalls = list(fx["grad_calls"])
     for k, xyz in enumerate(fx["coords"]):
         terms, g, rebuilt = orc.energy(xyz)
@@ -189,13 +192,38 @@ def _brute(xyz, excl, cutoff):
     return np.array(sorted(k for k in keys if k not in ex), dtype=np.int64)
 
 
-@pytest.mark.parametrize("natoms,box,method", [(400, 32.0, 0), (700, 48.0, 1), (900, 30.0, 1)])
+@pytest.mark.parametrize("natoms,box,method", [(400, 32.0, 0), (700, 48.0, 1), (900, 30.0, 1), (700, 48.0, 2),
+                                                (900, 30.0, 2)])
 def test_pair_list_is_brute_force_set(oracle_mod, natoms, box, method):
     from pdynamo_amd import synthetic
     mm = synthetic.random_small_system(natoms, box, 100 + natoms)
     xp, xi = oracle_mod.exclusion_csr(mm.natoms, mm.exclusions)
     rec = oracle_mod.self_pair_list(mm.coordinates3, xp, xi, 13.5, 6.75, method)
     assert np.array_equal(rec.canonical(), _brute(mm.coordinates3, mm.exclusions, 13.5))
+
+
+def test_point_cell_edge_atoms(oracle_mod):
+    """Point/cell lists with atoms on the upper faces of a box whose extent is a whole number of cells, so that
+    they fall off the reference grid.  A pair whose higher-index atom is on the grid is listed exactly when both
+    atoms are on the grid and within the cutoff (the brute-force set); an atom off the grid is never the lower atom
+    of a pair, but as the higher atom it keeps pairs, all within the cutoff, with atoms around the cell searched
+    before it.  The exact set is the reference library's (pointcell_edge900_*.npz)."""
+    fx = _golden("pointcell_edge900_s0")
+    xyz = fx["coords"][0]
+    n = xyz.shape[0]
+    bins, lower, cell = oracle_mod.make_grid(xyz, 6.75)
+    off = np.nonzero(cell < 0)[0]
+    assert list(bins) == [4, 4, 4] and off.size == 5
+    xp, xi = oracle_mod.exclusion_csr(n, fx["exclusions"])
+    keys = oracle_mod.self_pair_list(xyz, xp, xi, 13.5, 6.75, 2).canonical()
+    hi, lo = keys // n, keys % n
+    assert not np.isin(lo, off).any()
+    brute = _brute(xyz, fx["exclusions"], 13.5)
+    bhi, blo = brute // n, brute % n
+    on = ~np.isin(hi, off)
+    assert np.array_equal(keys[on], brute[~np.isin(bhi, off) & ~np.isin(blo, off)])
+    assert (~on).sum() > 0 and np.isin(keys[~on], brute).all()
+    assert np.array_equal(keys, _keys(fx))
 
 
 def test_grid_and_stencil_closed_form(oracle_mod):
