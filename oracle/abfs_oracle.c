@@ -722,13 +722,12 @@ static void translate_coordinates(int n, double *xyz, const double *d)
     for (int i = 0; i < n; i++) { xyz[3 * i] += d[0]; xyz[3 * i + 1] += d[1]; xyz[3 * i + 2] += d[2]; }
 }
 
-/* Cross pair list between xyz (i) and the image coordinates ixyz (j): every
- * pair with the FP64 squared distance (x, y, z order) <= cutoff^2 and i on
- * the primary grid when ongrid is given (MakeCrossPairListFromCoordinates3
- * CellCell / Direct, pCx/csource/PairListGenerator.c:611-860, without
- * exclusions or selections).  Records: i ascending, j ascending. */
-static long cross_pair_list(int n, const double *xyz, const double *ixyz, const int *ongrid, double cutoff,
-                            PairSink *s)
+/* Cross pair list between xyz (i) and the image coordinates ixyz (j), direct
+ * method (MakeCrossPairListFromCoordinates3Direct, pCore-1.9.0/extensions/
+ * csource/PairListGenerator.c:795-858, without exclusions or selections):
+ * every pair with the FP64 squared distance (x, y, z order) <= cutoff^2.
+ * Records: i ascending, j ascending. */
+static long cross_pair_list_direct(int n, const double *xyz, const double *ixyz, double cutoff, PairSink *s)
 {
     double c2 = cutoff * cutoff;
     int *ind = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
@@ -759,7 +758,6 @@ static long cross_pair_list(int n, const double *xyz, const double *ixyz, const 
         free(cur);
     }
     for (int i = 0; i < n; i++) {
-        if (ongrid && !ongrid[i]) continue;
         int m = 0, k0[3], k1[3];
         for (int d = 0; d < 3; d++) {
             double f = (xyz[3 * i + d] - lo[d]) / cutoff;
@@ -791,6 +789,134 @@ static long cross_pair_list(int n, const double *xyz, const double *ixyz, const 
     return s->npairs;
 }
 
+/* Point/cell squared distances between point x and cell idx of a grid with lower corner lower and cell size h
+ * (RegularGrid_DistancesSquaredBetweenCellAndPoint, RegularGrid.c:139-180, non-periodic branch). */
+static void point_cell_distances(const double *x, const int *idx, const double *lower, double h, double *minR2,
+                                 double *maxR2)
+{
+    double mxs = 0.0, mns = 0.0;
+    for (int d = 0; d < 3; d++) {
+        double lo = (double)idx[d] * h + lower[d], hi = lo + h, v = x[d], mx, mn;
+        if (v > hi) { mx = v - lo; mn = v - hi; }
+        else if (v < lo) { mx = hi - v; mn = lo - v; }
+        else { mx = (v - lo) > (hi - v) ? (v - lo) : (hi - v); mn = 0.0; }
+        mxs += mx * mx;
+        mns += mn * mn;
+    }
+    *minR2 = mns;
+    *maxR2 = mxs;
+}
+
+/* Cross pair list between xyz (i, on the primary grid: bins1, lower1, cell size h, cell1[i] or -1) and the image
+ * coordinates ixyz (j), grid method, without exclusions or selections
+ * (PairListGenerator_CrossPairListFromDoubleCoordinates3, PairListGenerator.c:414-445, with
+ * MakeCrossPairListFromCoordinates3CellCell / PointCell, :703-794 and :861-940).
+ *
+ * The image points get a grid conforming to the primary one (Coordinates3_MakeConformingGrid,
+ * Coordinates3.c:998-1072): its lower cell indices lowerN are those of the image points' box corner on the
+ * primary grid (floor, no regularisation), its bins reach the cell of the box's upper corner, and its lower corner
+ * is lower1 + lowerN h; each image point takes its cell on that grid or none (RegularGridOccupancy_Fill,
+ * RegularGridOccupancy.c:77-130).  The cells searched for a primary cell I are I + o - lowerN over the stencil
+ * offsets o, where on the image grid (RegularGrid.c:341-405).
+ *
+ * method 1 (cell/cell): i in the primary cells in cell order; a cell is taken whole where the stencil flag of o is
+ * 2 (fully within range), otherwise each j by the distance test.
+ * method 2 (point/cell): i ascending over all points; I is i's cell or, for a point off the grid, the cell indices
+ * decoded last (the latest earlier point on the grid, else (-u, -u, -u), as in the self list); a cell is searched
+ * when its smallest squared distance to x_i on the image grid is below cutoff^2 and taken whole when its largest
+ * is at most cutoff^2.  Records in the reference's order; j ascending within a cell. */
+static long cross_pair_list_grid(int n, const double *xyz, const double *ixyz, int method, double cutoff, double h,
+                                 const int *bins1, const double *lower1, const int *cell1, int noff, const int *off,
+                                 const int *flg, PairSink *s)
+{
+    double c2 = cutoff * cutoff;
+    double lo[3], ext[3], up[3], lower2[3];
+    int lowerN[3], bins2[3];
+    enclosing_box(n, ixyz, lo, ext);
+    for (int d = 0; d < 3; d++) {
+        up[d] = ext[d] + lo[d];
+        lowerN[d] = (int)floor((lo[d] - lower1[d]) / h);
+        int upperN = (int)floor((up[d] - lower1[d]) / h);
+        bins2[d] = upperN - lowerN[d] + 1;
+        lower2[d] = lower1[d] + (double)lowerN[d] * h;
+    }
+    long ncell2 = (long)bins2[0] * bins2[1] * bins2[2];
+    long st2[3] = {(long)bins2[1] * bins2[2], bins2[2], 1};
+    long *first = (long *)calloc((size_t)ncell2 + 1, sizeof(long));
+    long *total = (long *)calloc((size_t)ncell2 + 1, sizeof(long));
+    int *points = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *c2id = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *ind = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    for (int j = 0; j < n; j++) {
+        long c = 0;
+        int offg = 0;
+        for (int d = 0; d < 3; d++) {
+            int k = (int)floor((ixyz[3 * j + d] - lower2[d]) / h);
+            if (k < 0 || k >= bins2[d]) offg = 1;
+            c += st2[d] * k;
+        }
+        c2id[j] = offg ? -1 : (int)c;
+        if (!offg) total[c]++;
+    }
+    for (long c = 0, acc = 0; c < ncell2; c++) { first[c] = acc; acc += total[c]; }
+    for (int j = 0; j < n; j++) if (c2id[j] >= 0) points[first[c2id[j]]++] = j;
+    for (long c = 0; c < ncell2; c++) first[c] -= total[c];
+    const long st1[3] = {(long)bins1[1] * bins1[2], bins1[2], 1};
+    const int u = (int)ceil(cutoff / h);
+    int pc[3] = {-u, -u, -u};
+    /* Outer order: cell/cell walks the primary cells, point/cell the points. */
+    long ncell1 = (long)bins1[0] * bins1[1] * bins1[2];
+    int *order = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int norder = 0;
+    if (method == 1) {
+        long *cnt = (long *)calloc((size_t)ncell1 + 1, sizeof(long));
+        for (int i = 0; i < n; i++) if (cell1[i] >= 0) cnt[cell1[i] + 1]++;
+        for (long c = 0; c < ncell1; c++) cnt[c + 1] += cnt[c];
+        for (int i = 0; i < n; i++) if (cell1[i] >= 0) order[cnt[cell1[i]]++] = i;
+        norder = (int)cnt[ncell1 - 1];
+        free(cnt);
+    } else {
+        for (int i = 0; i < n; i++) order[i] = i;
+        norder = n;
+    }
+    for (int q = 0; q < norder; q++) {
+        int i = order[q], m = 0;
+        if (cell1[i] >= 0) {
+            long c = cell1[i];
+            pc[0] = (int)(c / st1[0]);
+            pc[1] = (int)((c % st1[0]) / st1[1]);
+            pc[2] = (int)(c % st1[1]);
+        }
+        for (int k = 0; k < noff; k++) {
+            int idx[3] = {pc[0] + off[3 * k] - lowerN[0], pc[1] + off[3 * k + 1] - lowerN[1],
+                          pc[2] + off[3 * k + 2] - lowerN[2]};
+            if (idx[0] < 0 || idx[0] >= bins2[0] || idx[1] < 0 || idx[1] >= bins2[1] || idx[2] < 0 ||
+                idx[2] >= bins2[2])
+                continue;
+            int all;
+            if (method == 1) {
+                all = flg[k] == 2;
+            } else {
+                double minR2, maxR2;
+                point_cell_distances(xyz + 3 * i, idx, lower2, h, &minR2, &maxR2);
+                if (!(minR2 < c2)) continue;
+                all = maxR2 <= c2;
+            }
+            const long cid = idx[0] * st2[0] + idx[1] * st2[1] + idx[2];
+            for (long p = first[cid], pe = first[cid] + total[cid]; p < pe; p++) {
+                int j = points[p];
+                if (all) { ind[m++] = j; continue; }
+                double dx = xyz[3 * i] - ixyz[3 * j], dy = xyz[3 * i + 1] - ixyz[3 * j + 1],
+                       dz = xyz[3 * i + 2] - ixyz[3 * j + 2];
+                if ((dx * dx) + (dy * dy) + (dz * dz) <= c2) ind[m++] = j;
+            }
+        }
+        save_record(s, i, m, ind);
+    }
+    free(order); free(first); free(total); free(points); free(c2id); free(ind);
+    return s->npairs;
+}
+
 /* GenerateImageLists for the MM/MM list of a pure MM system
  * (pMx/csource/NBModelABFS.c:753-1045): transformations t (fractional
  * rotation rot[9t..], translation tr[3t..]) with their inverses, box search
@@ -803,10 +929,11 @@ static long cross_pair_list(int n, const double *xyz, const double *ixyz, const 
  * pairs dropped as ImageList_CreateImage does, pMx/csource/ImageList.c:31-62):
  * abct[4k..] = a, b, c, t; scale[k]; records appended to the sink with
  * img_rec[k] = first record of image k (img_rec[nimg] = total records).
- * ongrid (n) may be NULL (direct method).  Returns the number of images, or
- * -1 on a capacity overflow; sizes[3] = images, records, pairs needed. */
-long orc_image_lists(int n, const double *xyz, const int *ongrid, int ntrans, const double *rot, const double *tr,
-                     const double *sp, double cutoff, int checkForInverses, int expandFactor,
+ * method: 0 direct, 1 grid cell/cell, 2 grid point/cell; the primary grid is
+ * made from xyz with cellSize.  Returns the number of images, or -1 on a
+ * capacity overflow; sizes[3] = images, records, pairs needed. */
+long orc_image_lists(int n, const double *xyz, int method, double cellSize, int ntrans, const double *rot,
+                     const double *tr, const double *sp, double cutoff, int checkForInverses, int expandFactor,
                      long cap_img, int *abct, double *scale, long *img_rec,
                      long cap_rec, long cap_pairs, int *rec_i, long *rec_ptr, int *js, long *sizes)
 {
@@ -817,6 +944,16 @@ long orc_image_lists(int n, const double *xyz, const int *ongrid, int ntrans, co
     PairSink s = {rec_i, rec_ptr, js, cap_rec, cap_pairs, 0, 0, 0};
     long nimg = 0;
     if (cap_rec > 0 && rec_ptr) rec_ptr[0] = 0;
+    int bins1[3] = {0, 0, 0}, noff = 0, *cell1 = NULL, *off = NULL, *flg = NULL;
+    double lower1[3] = {0.0, 0.0, 0.0};
+    if (method != 0 && n > 0) {
+        cell1 = (int *)malloc(sizeof(int) * (size_t)n);
+        orc_make_grid(n, xyz, cellSize, bins1, lower1, cell1);
+        noff = orc_search_range(cellSize, cutoff, 0, NULL, NULL);
+        off = (int *)malloc(sizeof(int) * 3 * (size_t)noff);
+        flg = (int *)malloc(sizeof(int) * (size_t)noff);
+        orc_search_range(cellSize, cutoff, noff, off, flg);
+    }
     orc_symmetry_make_m(sp, M, Minv);
     enclosing_box(n, xyz, lower, ext);
     for (int d = 0; d < 3; d++) { upper[d] = ext[d] + lower[d]; lower[d] += -cutoff; upper[d] += cutoff; }
@@ -866,7 +1003,11 @@ long orc_image_lists(int n, const double *xyz, const int *ongrid, int ntrans, co
                     if (ilower[0] <= upper[0] && ilower[1] <= upper[1] && ilower[2] <= upper[2] &&
                         iupper[0] >= lower[0] && iupper[1] >= lower[1] && iupper[2] >= lower[2]) {
                         long r0 = s.nrec, p0 = s.npairs;
-                        cross_pair_list(n, xyz, ixyz, ongrid, cutoff, &s);
+                        if (method == 0)
+                            cross_pair_list_direct(n, xyz, ixyz, cutoff, &s);
+                        else
+                            cross_pair_list_grid(n, xyz, ixyz, method, cutoff, cellSize, bins1, lower1, cell1, noff,
+                                                 off, flg, &s);
                         if (s.npairs > p0) {
                             if (nimg < cap_img) {
                                 abct[4 * nimg] = a; abct[4 * nimg + 1] = b; abct[4 * nimg + 2] = c;
@@ -885,6 +1026,9 @@ long orc_image_lists(int n, const double *xyz, const int *ongrid, int ntrans, co
     if (nimg <= cap_img && img_rec) img_rec[nimg] = s.nrec;
     free(inverses);
     free(ixyz);
+    free(cell1);
+    free(off);
+    free(flg);
     sizes[0] = nimg;
     sizes[1] = s.nrec;
     sizes[2] = s.npairs;

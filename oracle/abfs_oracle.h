@@ -102,8 +102,8 @@ void orc_symmetry_make_m(const double *sp, double *M, double *Minv);
 void orc_orthogonalize(const double *M, const double *Minv, const double *rot, const double *tr, double *Ro,
                        double *To);
 int orc_find_inverses(int ntrans, const double *rot, const double *tr, int *inverses);
-long orc_image_lists(int n, const double *xyz, const int *ongrid, int ntrans, const double *rot, const double *tr,
-                     const double *sp, double cutoff, int checkForInverses, int expandFactor,
+long orc_image_lists(int n, const double *xyz, int method, double cellSize, int ntrans, const double *rot,
+                     const double *tr, const double *sp, double cutoff, int checkForInverses, int expandFactor,
                      long cap_img, int *abct, double *scale, long *img_rec,
                      long cap_rec, long cap_pairs, int *rec_i, long *rec_ptr, int *js, long *sizes);
 void orc_image_energy(const double *f, int n, const double *xyz, const double *charge, const int *ljtype, int ntypes,

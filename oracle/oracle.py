@@ -67,7 +67,8 @@ def lib():
         L.orc_orthogonalize.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_find_inverses.argtypes = [C.c_int, _dp, _dp, _ip]
         L.orc_image_lists.restype = C.c_long
-        L.orc_image_lists.argtypes = [C.c_int, _dp, C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int,
+        L.orc_image_lists.argtypes = [C.c_int, _dp, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, C.c_double, C.c_int,
+                                      C.c_int,
                                       C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_void_p,
                                       C.c_void_p, C.c_void_p, _lp]
         L.orc_image_energy.argtypes = [_dp, C.c_int, _dp, _dp, _ip, C.c_int, _dp, _dp, _ip, C.c_double, _dp, _dp, _dp,
@@ -323,24 +324,18 @@ class ImageLists:
 
 
 def image_lists(xyz, transformations, sp, cutoff=13.5, use_grid=True, cell_size=6.75, check_for_inverses=True,
-                expand_factor=0):
+                expand_factor=0, use_grid_by_cell=True):
     """GenerateImageLists (MM/MM) on the restatement."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     n = xyz.shape[0]
     rot = np.ascontiguousarray(transformations[0], np.float64).reshape(-1)
     tr = np.ascontiguousarray(transformations[1], np.float64).reshape(-1)
-    ongrid = None
-    if use_grid:
-        bins = np.zeros(3, np.int32)
-        lower = np.zeros(3)
-        cell = np.zeros(max(n, 1), np.int32)
-        lib().orc_make_grid(n, xyz, cell_size, bins, lower, cell)
-        ongrid = np.ascontiguousarray((cell[:n] >= 0).astype(np.int32))
+    method = (1 if use_grid_by_cell else 2) if use_grid else 0
     sp = np.asarray(sp, np.float64)
     sizes = np.zeros(3, np.int64)
     L = lib()
-    args = [n, xyz, None if ongrid is None else ongrid.ctypes.data, tr.shape[0] // 3, rot, tr, sp, cutoff,
-            1 if check_for_inverses else 0, int(expand_factor)]
+    args = [n, xyz, method, cell_size, tr.shape[0] // 3, rot, tr, sp, cutoff, 1 if check_for_inverses else 0,
+            int(expand_factor)]
     L.orc_image_lists(*args, 0, None, None, None, 0, 0, None, None, None, sizes)
     ni, nr, npr = (int(v) for v in sizes)
     abct = np.zeros(4 * max(ni, 1), np.int32)
@@ -466,7 +461,7 @@ class OracleABFS:
         if rebuilt or self.check_for_image_update(sp):
             self.images = image_lists(xyz, self.transformations, sp, self.list_cutoff, bool(self.use_grid),
                                       self.cell_factor * self.list_cutoff, self.check_for_inverses,
-                                      self.image_expand_factor)
+                                      self.image_expand_factor, self.use_grid_by_cell)
             self.ref_sp = np.array(sp, np.float64)
             rebuilt = True
         s = self.s

@@ -450,15 +450,53 @@ __device__ __forceinline__ int search_store(Entry *__restrict__ scratch, size_t 
     return cbits ? 1 : 0;
 }
 
+// Squared smallest and largest distances between a point and grid cell n
+// (RegularGrid_DistancesSquaredBetweenCellAndPoint, pCore-1.9.0/extensions/
+// csource/RegularGrid.c:139-180, non-periodic branch, in its operation order:
+// the cell spans [n h + lower, n h + lower + h] on each axis; the per-axis
+// squares are summed in axis order).
+__device__ __forceinline__ void point_cell_r2(const double *lower, double h, double x, double y, double z, int nx,
+                                              int ny, int nz, double &minR2, double &maxR2)
+{
+#pragma clang fp contract(off)
+    const double pt[3] = {x, y, z};
+    const int nn[3] = {nx, ny, nz};
+    double mn2[3], mx2[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double lo = (double)nn[d] * h + lower[d];
+        const double hi = lo + h;
+        const double v = pt[d];
+        const double dl = v - lo, du = hi - v;
+        const double mx = v > hi ? dl : (v < lo ? du : (dl > du ? dl : du));
+        const double mn = v > hi ? v - hi : (v < lo ? lo - v : 0.0);
+        mn2[d] = mn * mn;
+        mx2[d] = mx * mx;
+    }
+    minR2 = (mn2[0] + mn2[1]) + mn2[2];
+    maxR2 = (mx2[0] + mx2[1]) + mx2[2];
+}
+
 // Image lists: the candidates sCand[0..npend) of one wave (k = w, w + NB_SEARCH_WAVES, ...).
+//
+// Grid method (PairListGenerator.c:703-794,861-940 over the conforming grid of
+// the image, RegularGrid.c:341-405): j must lie on the image's grid, in cell
+// k2; the offset o = k2 + lowN - I from i's primary search cell I must be in
+// the stencil.  Cell/cell (i on the primary grid): the pair is listed when the
+// stencil flag of o is 2 (the cell pair is fully within range) or the FP64
+// distance test passes.  Point/cell (any i): when the smallest squared distance
+// from x_i to cell k2 of the image grid is below cutoff^2 and either the
+// largest is at most cutoff^2 or the distance test passes.
+template <bool PC>
 __device__ __forceinline__ int search_eval_cross(const SearchParams &P, int npend, int w, int lane, int I0, int sc,
                                                  int img, int base, int scap, size_t sbase, const int *sCand,
                                                  const unsigned char *sCm, const int *sIatom, const int *sIfree,
-                                                 const int *sIon, const double *sIx, const double *sIy,
-                                                 const double *sIz, const double *__restrict__ jxyz,
+                                                 const int4 *sIcell, const double *sIx, const double *sIy,
+                                                 const double *sIz, const ImageGrid &G, const double *__restrict__ jxyz,
                                                  const int *__restrict__ slot_atom, Entry *__restrict__ scratch)
 {
     const int a = lane >> 3, b = lane & 7;
+    const int wst = 2 * P.u + 1;
     int nvalid = 0;
     JAtom jn;
     if (w < npend) jn = load_jatom_cross(sCand[w], b, jxyz, slot_atom, P.free_flags);
@@ -467,6 +505,18 @@ __device__ __forceinline__ int search_eval_cross(const SearchParams &P, int npen
         const int Jc = sCand[k];
         const int cmc = sCm[k];
         if (k + NB_SEARCH_WAVES < npend) jn = load_jatom_cross(sCand[k + NB_SEARCH_WAVES], b, jxyz, slot_atom, P.free_flags);
+        // j's cell on the image grid (RegularGrid_FindCellIDOfPoint, RegularGrid.c:240-251).
+        int jk[3] = {0, 0, 0};
+        bool jon = true;
+        if (P.use_grid) {
+#pragma clang fp contract(off)
+            const double jx[3] = {jd.x, jd.y, jd.z};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                jk[d] = (int)floor((jx[d] - G.lower[d]) / P.gh);
+                jon = jon && jk[d] >= 0 && jk[d] < G.bins[d];
+            }
+        }
         int lbits = 0, cbits = 0;
         for (int c = 0; c < NB_SCC; c++) {
             if (!(cmc & (1 << c))) continue;
@@ -478,41 +528,34 @@ __device__ __forceinline__ int search_eval_cross(const SearchParams &P, int npen
                 const int fl = sIfree[li] | jd.free_;
                 ok = (fl & 1) && !(fl & 2);
             }
-            if (ok && P.use_grid) ok = sIon[li] != 0;
-            if (ok) ok = ref_dist2(sIx[li], sIy[li], sIz[li], jd.x, jd.y, jd.z) <= P.cutoff2;
+            bool near = false;
+            if (ok) near = ref_dist2(sIx[li], sIy[li], sIz[li], jd.x, jd.y, jd.z) <= P.cutoff2;
+            if (ok && P.use_grid) {
+                const int4 ic = sIcell[li];
+                const unsigned ux = (unsigned)(jk[0] + G.lowN[0] - ic.x + P.u);
+                const unsigned uy = (unsigned)(jk[1] + G.lowN[1] - ic.y + P.u);
+                const unsigned uz = (unsigned)(jk[2] + G.lowN[2] - ic.z + P.u);
+                const bool inb = max(max(ux, uy), uz) < (unsigned)wst;
+                const int flag = inb ? (int)P.stencil[(ux * wst + uy) * wst + uz] : 0;
+                ok = jon && flag != 0;
+                if (PC) {
+                    if (ok) {
+                        double minR2, maxR2;
+                        point_cell_r2(G.lower, P.gh, sIx[li], sIy[li], sIz[li], jk[0], jk[1], jk[2], minR2, maxR2);
+                        ok = (minR2 < P.cutoff2) && (maxR2 <= P.cutoff2 || near);
+                    }
+                } else {
+                    ok = ok && ic.w != 0 && (flag == 2 || near);
+                }
+            } else {
+                ok = ok && near;
+            }
             if (ok) lbits |= 1 << c;
             if (__ballot(ok)) cbits |= 1 << c;
         }
         nvalid += search_store(scratch, sbase, base + k, scap, lane, Jc, sc, img, lbits, cbits);
     }
     return nvalid;
-}
-
-// Squared smallest and largest distances between a point and grid cell n
-// (RegularGrid_DistancesSquaredBetweenCellAndPoint, pCore-1.9.0/extensions/
-// csource/RegularGrid.c:139-180, non-periodic branch, in its operation order:
-// the cell spans [n h + lower, n h + lower + h] on each axis; the per-axis
-// squares are summed in axis order).
-__device__ __forceinline__ void point_cell_r2(const SearchParams &P, double x, double y, double z, int nx, int ny,
-                                              int nz, double &minR2, double &maxR2)
-{
-#pragma clang fp contract(off)
-    const double pt[3] = {x, y, z};
-    const int nn[3] = {nx, ny, nz};
-    double mn2[3], mx2[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const double lo = (double)nn[d] * P.gh + P.glower[d];
-        const double hi = lo + P.gh;
-        const double v = pt[d];
-        const double dl = v - lo, du = hi - v;
-        const double mx = v > hi ? dl : (v < lo ? du : (dl > du ? dl : du));
-        const double mn = v > hi ? v - hi : (v < lo ? lo - v : 0.0);
-        mn2[d] = mn * mn;
-        mx2[d] = mx * mx;
-    }
-    minR2 = (mn2[0] + mn2[1]) + mn2[2];
-    maxR2 = (mx2[0] + mx2[1]) + mx2[2];
 }
 
 // Self lists: the candidates sCand[0..npend) of one wave.  Lane 8a + b
@@ -585,7 +628,7 @@ __device__ __forceinline__ int search_eval_self(const SearchParams &P, int npend
                 const int flag = STEN == 1 ? (int)sSten[si] : (int)P.stencil[si];
                 const bool ipt = __double2loint(ip.w) > jatom;
                 double minR2, maxR2;
-                point_cell_r2(P, ipt ? ip.x : jp.x, ipt ? ip.y : jp.y, ipt ? ip.z : jp.z, ipt ? jc.x : ic.x + P.u,
+                point_cell_r2(P.glower, P.gh, ipt ? ip.x : jp.x, ipt ? ip.y : jp.y, ipt ? ip.z : jp.z, ipt ? jc.x : ic.x + P.u,
                               ipt ? jc.y : ic.y + P.u, ipt ? jc.z : ic.z + P.u, minR2, maxR2);
                 ok = ok & inb & (flag != 0) & (minR2 < P.cutoff2) & ((maxR2 <= P.cutoff2) | near);
             } else if (STEN) {
@@ -638,6 +681,8 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
     const double *__restrict__ jxyz = CROSS ? P.ixyz + (size_t)img * 3 * P.n : xyz;
     const float4 *__restrict__ jblo = CROSS ? P.iblo + (size_t)img * P.ncl : blo;
     const float4 *__restrict__ jbhi = CROSS ? P.ibhi + (size_t)img * P.ncl : bhi;
+    ImageGrid G = {};
+    if (CROSS && P.igrid) G = P.igrid[img];
     const size_t sbase = CROSS ? (size_t)P.soff[blk] : (size_t)blk * P.cap;
     const int scap = CROSS ? P.scap[blk] : P.cap;
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
@@ -646,7 +691,8 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
     __shared__ float4 sIlo[NB_SCC], sIhi[NB_SCC];
     // Cross lists: the i-atoms' positions, on-grid and free flags.
     __shared__ double sIx[CROSS ? NB_SC : 1], sIy[CROSS ? NB_SC : 1], sIz[CROSS ? NB_SC : 1];
-    __shared__ int sIatom[CROSS ? NB_SC : 1], sIfree[CROSS ? NB_SC : 1], sIon[CROSS ? NB_SC : 1];
+    __shared__ int sIatom[CROSS ? NB_SC : 1], sIfree[CROSS ? NB_SC : 1];
+    __shared__ int4 sIcell[CROSS ? NB_SC : 1];
     // Self lists: the i-atoms' records, their cells minus u with the flag bits, exclusion index ranges.
     __shared__ double4 sIp[CROSS ? 1 : NB_SC];
     __shared__ int4 sIc[CROSS ? 1 : NB_SC];
@@ -672,7 +718,7 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
                 sIx[lane] = xyz[3 * (size_t)ia];
                 sIy[lane] = xyz[3 * (size_t)ia + 1];
                 sIz[lane] = xyz[3 * (size_t)ia + 2];
-                sIon[lane] = P.ongrid ? (int)P.ongrid[ia] : 1;
+                sIcell[lane] = P.icell ? P.icell[ia] : make_int4(0, 0, 0, 1);
                 sIfree[lane] = P.free_flags ? P.free_flags[ia] : 1;
             }
         } else {
@@ -741,8 +787,8 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
         // k = w, w + NB_SEARCH_WAVES, ...
         auto evaluate = [&](int npend) {
             if (CROSS)
-                nvalid += search_eval_cross(P, npend, w, lane, I0, sc, img, base, scap, sbase, sCand, sCm, sIatom,
-                                            sIfree, sIon, sIx, sIy, sIz, jxyz, slot_atom, scratch);
+                nvalid += search_eval_cross<PC>(P, npend, w, lane, I0, sc, img, base, scap, sbase, sCand, sCm, sIatom,
+                                                sIfree, sIcell, sIx, sIy, sIz, G, jxyz, slot_atom, scratch);
             else
                 nvalid += search_eval_self<STEN, PC>(P, npend, w, lane, I0, sc, base, scap, sbase, sCand, sCm, sIp, sIc,
                                                  sIxr, sXoff, sXcnt, sXpool, sSten, scratch);
@@ -1234,21 +1280,91 @@ __global__ __launch_bounds__(256) void k_image_coords(const double *__restrict__
     }
 }
 
-// Whether each atom lies on the reference grid built from the current
-// coordinates (the cross list's i loop runs over grid cells only).
-__global__ __launch_bounds__(256) void k_ongrid(const double *__restrict__ xyz, int n, BuildGrid g,
-                                                unsigned char *__restrict__ flags)
+// Primary-grid cells of the i atoms of the image lists (the grid made from the
+// current coordinates, NBModelABFSState_GridInitialize): cell indices with
+// w = 1 on the grid, w = 0 off it.  Point/cell search: an atom off the grid
+// searches around the cell indices decoded last in the reference's point loop
+// (RegularGrid_CellIDToIndices, pCore-1.9.0/extensions/csource/RegularGrid.c:
+// 69-85, writes nothing for a point with no cell), those of the latest earlier
+// atom (index 0 up, in the MM selection) on the grid, else (-u, -u, -u).
+__global__ __launch_bounds__(256) void k_image_cells(const double *__restrict__ xyz, int n, BuildGrid g, int pointcell,
+                                                     int u, const unsigned char *__restrict__ free_flags,
+                                                     int4 *__restrict__ icell)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    int on = 1;
+    int k[3], on = 1;
     for (int d = 0; d < 3; d++) {
-        const double fr = (xyz[3 * (size_t)i + d] - g.lower[d]) / g.h;
-        const int k = (int)floor(fr);
-        if (k < 0 || k >= g.bins[d]) on = 0;
+        k[d] = (int)floor((xyz[3 * (size_t)i + d] - g.lower[d]) / g.h);
+        if (k[d] < 0 || k[d] >= g.bins[d]) on = 0;
     }
-    flags[i] = (unsigned char)on;
+    int4 c = make_int4(k[0], k[1], k[2], on);
+    if (!on) {
+        c = make_int4(-u, -u, -u, 0);
+        if (pointcell)
+            for (int q = i - 1; q >= 0; q--) {
+                if (free_flags && (free_flags[q] & NB_SF_QC)) continue;
+                int kq[3], onq = 1;
+                for (int d = 0; d < 3; d++) {
+                    kq[d] = (int)floor((xyz[3 * (size_t)q + d] - g.lower[d]) / g.h);
+                    if (kq[d] < 0 || kq[d] >= g.bins[d]) onq = 0;
+                }
+                if (onq) {
+                    c = make_int4(kq[0], kq[1], kq[2], 0);
+                    break;
+                }
+            }
+    }
+    icell[i] = c;
+}
+
+// The grid of each image's points conforming to the primary grid
+// (Coordinates3_MakeConformingGrid, pCore-1.9.0/extensions/csource/
+// Coordinates3.c:998-1072): the box of the image points (Coordinates3_
+// EnclosingOrthorhombicBox: minimum, and extent added back for the upper
+// corner), the primary-grid cell indices of both corners (floor, not
+// regularised), bins from the lower to the upper corner's cell, and the lower
+// corner lower + lowN h.  One workgroup per image.
+__global__ __launch_bounds__(256) void k_image_grid(const double *__restrict__ ixyz, int n, BuildGrid g,
+                                                    ImageGrid *__restrict__ igrid)
+{
+#pragma clang fp contract(off)
+    __shared__ double smn[3][256], smx[3][256];
+    const double *x = ixyz + (size_t)blockIdx.x * 3 * n;
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int i = threadIdx.x; i < n; i += 256)
+        for (int d = 0; d < 3; d++) {
+            const double v = x[3 * (size_t)i + d];
+            mn[d] = fmin(mn[d], v);
+            mx[d] = fmax(mx[d], v);
+        }
+    for (int d = 0; d < 3; d++) {
+        smn[d][threadIdx.x] = mn[d];
+        smx[d][threadIdx.x] = mx[d];
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int d = 0; d < 3; d++) {
+                smn[d][threadIdx.x] = fmin(smn[d][threadIdx.x], smn[d][threadIdx.x + o]);
+                smx[d][threadIdx.x] = fmax(smx[d][threadIdx.x], smx[d][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ImageGrid G;
+        for (int d = 0; d < 3; d++) {
+            const double lo = smn[d][0];
+            const double up = (smx[d][0] - lo) + lo;
+            const int lowN = (int)floor((lo - g.lower[d]) / g.h);
+            const int upN = (int)floor((up - g.lower[d]) / g.h);
+            G.lowN[d] = lowN;
+            G.bins[d] = upN - lowN + 1;
+            G.lower[d] = g.lower[d] + (double)lowN * g.h;
+        }
+        igrid[blockIdx.x] = G;
+    }
 }
 
 // Full (both directions) CSR neighbour rows for the plain FP64 path.
@@ -1450,7 +1566,11 @@ void launch_search_images(const SearchParams &P, int nsc, int nimg, const double
                           const float4 *blo, const float4 *bhi, Entry *scratch, int *ncand, int *counts, int *overflow,
                           hipStream_t s)
 {
-    if (nsc > 0 && nimg > 0)
+    if (nsc <= 0 || nimg <= 0) return;
+    if (P.use_grid && P.pointcell)
+        hipLaunchKernelGGL((k_search<true, 0, true>), dim3(nsc, nimg), dim3(NB_SEARCH_THREADS), 0, s, P, xyz,
+                           slot_atom, blo, bhi, (const int2 *)nullptr, scratch, ncand, counts, overflow);
+    else
         hipLaunchKernelGGL((k_search<true, 0>), dim3(nsc, nimg), dim3(NB_SEARCH_THREADS), 0, s, P, xyz, slot_atom, blo,
                            bhi, (const int2 *)nullptr, scratch, ncand, counts, overflow);
 }
@@ -1477,9 +1597,17 @@ void launch_image_coords(const double *tcoords, int n, int ntrans, const int *st
                            out);
 }
 
-void launch_ongrid(const double *xyz, int n, const BuildGrid &g, unsigned char *flags, hipStream_t s)
+void launch_image_cells(const double *xyz, int n, const BuildGrid &g, int pointcell, int u,
+                        const unsigned char *free_flags, int4 *icell, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_ongrid, dim3(nblk(n, 256)), dim3(256), 0, s, xyz, n, g, flags);
+    if (n > 0)
+        hipLaunchKernelGGL(k_image_cells, dim3(nblk(n, 256)), dim3(256), 0, s, xyz, n, g, pointcell, u, free_flags,
+                           icell);
+}
+
+void launch_image_grid(const double *ixyz, int n, int nimg, const BuildGrid &g, ImageGrid *igrid, hipStream_t s)
+{
+    if (n > 0 && nimg > 0) hipLaunchKernelGGL(k_image_grid, dim3(nimg), dim3(256), 0, s, ixyz, n, g, igrid);
 }
 
 void launch_export_image_pairs(const Entry *entries, int nent, const int *slot_atom, long long n, int img,

@@ -460,7 +460,8 @@ struct nbabfs_state {
     DBuf<double> itc, ixyz, rtbuf, ibb;
     DBuf<float4> iblo, ibhi, icen;
     DBuf<float> ibox;
-    DBuf<unsigned char> ongrid;
+    DBuf<int4> icell;            // image lists: primary-grid search cells of the i atoms
+    DBuf<ImageGrid> igrid;       // image lists: conforming grid of each image
     DBuf<int> istep_off, ibound, isoff, icounts, incand, ieoff, inits, iioff;
     DBuf<ImageStep> isteps;
     DBuf<Entry> iscratch, entries_tmp;
@@ -1646,18 +1647,26 @@ static int build_images(nbabfs_state *S, const double *d_xyz)
             // The primary clusters' boxes for the current coordinates: atoms may
             // have moved since the main list (and its boxes) was built.
             launch_cluster_geom(d_xyz, S->slot_atom.p, ncl, S->grid.corg, S->blo.p, S->bhi.p, S->centre.p, st);
-            // i atoms on the grid made from the current coordinates (NBModelABFSState_GridInitialize).
-            const unsigned char *og = nullptr;
+            // Grid method: the i atoms' cells on the grid made from the current coordinates
+            // (NBModelABFSState_GridInitialize) and each image's conforming grid.
+            SearchParams P;
+            std::memset(&P, 0, sizeof(P));
             if (S->useGrid) {
                 BuildGrid g2;
                 std::memset(&g2, 0, sizeof(g2));
                 reference_grid(S->h_bbox.p, o.cutoffCellSizeFactor * o.listCutoff, g2);
-                CHK(S->ongrid.ensure(n));
-                launch_ongrid(d_xyz, n, g2, S->ongrid.p, st);
-                og = S->ongrid.p;
+                P.pointcell = o.useGridByCell ? 0 : 1;
+                CHK(S->icell.ensure(n));
+                launch_image_cells(d_xyz, n, g2, P.pointcell, S->u, S->has_flags ? S->freef.p : nullptr, S->icell.p,
+                                   st);
+                CHK(S->igrid.ensure(ns));
+                launch_image_grid(S->ixyz.p, n, ns, g2, S->igrid.p, st);
+                P.icell = S->icell.p;
+                P.igrid = S->igrid.p;
+                P.stencil = S->stencil.p;
+                P.u = S->u;
+                P.gh = g2.h;
             }
-            SearchParams P;
-            std::memset(&P, 0, sizeof(P));
             P.cutoff2 = cutoff * cutoff;
             P.use_grid = S->useGrid ? 1 : 0;
             P.free_flags = S->has_flags ? S->freef.p : nullptr;
@@ -1666,7 +1675,6 @@ static int build_images(nbabfs_state *S, const double *d_xyz)
             P.iblo = S->iblo.p;
             P.ibhi = S->ibhi.p;
             P.ibox = S->ibox.p;
-            P.ongrid = og;
             P.n = n;
             P.ncl = ncl;
             // Scratch: each (image, super-cluster) block gets its candidate bound.

@@ -16,6 +16,15 @@ struct BuildGrid {
     int ncx, ncy, nzb;
 };
 
+// Grid conforming to the primary grid that the reference makes for one image's points
+// (Coordinates3_MakeConformingGrid, pCore-1.9.0/extensions/csource/Coordinates3.c:998-1072): lower corner,
+// lower cell indices on the primary grid and number of cells per axis.
+struct ImageGrid {
+    double lower[3];
+    int lowN[3];
+    int bins[3];
+};
+
 struct SearchParams {
     double cutoff2;                       // listCutoff^2 (FP64 test)
     const unsigned char *stencil;         // (2u+1)^3 flags
@@ -33,7 +42,10 @@ struct SearchParams {
     const double *ixyz;
     const float4 *iblo, *ibhi;
     const float *ibox;
-    const unsigned char *ongrid;          // i atoms on the reference grid (null: direct method)
+    // Grid method: each i atom's search cell on the primary grid with its on-grid flag in w (k_image_cells),
+    // and each image's conforming grid (k_image_grid).
+    const int4 *icell;
+    const ImageGrid *igrid;
     const int *soff;                      // scratch offset and capacity of each (image, super-cluster)
     const int *scap;
     int n, ncl;
@@ -89,7 +101,9 @@ void launch_cut_items(const int *counts, const int *eoff, const int *ioff, int n
 void launch_image_transform(const double *xyz, int n, const double *rt, int ntrans, double *out, hipStream_t s);
 void launch_image_coords(const double *tcoords, int n, int ntrans, const int *step_off, const ImageStep *steps,
                          double *out, hipStream_t s);
-void launch_ongrid(const double *xyz, int n, const BuildGrid &g, unsigned char *flags, hipStream_t s);
+void launch_image_cells(const double *xyz, int n, const BuildGrid &g, int pointcell, int u,
+                        const unsigned char *free_flags, int4 *icell, hipStream_t s);
+void launch_image_grid(const double *ixyz, int n, int nimg, const BuildGrid &g, ImageGrid *igrid, hipStream_t s);
 void launch_search_images(const SearchParams &P, int nsc, int nimg, const double *xyz, const int *slot_atom,
                           const float4 *blo, const float4 *bhi, Entry *scratch, int *ncand, int *counts, int *overflow,
                           hipStream_t s);

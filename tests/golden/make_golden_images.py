@@ -41,10 +41,11 @@ CELLS = {"P1": (14.0, 15.0, 16.0, 80.0, 95.0, 100.0), "P-1": (13.0, 14.0, 15.0, 
          "I4": (18.0, 18.0, 10.0, 90.0, 90.0, 90.0), "R3": (15.0, 15.0, 15.0, 100.0, 100.0, 100.0)}
 
 
-def run_case(name, mm, transformations, calls, check_for_inverses=True, expand_factor=0, cell_factor=0.5):
+def run_case(name, mm, transformations, calls, check_for_inverses=True, expand_factor=0, cell_factor=0.5,
+             grid_by_cell=True):
     """calls: list of (coordinates, crystal parameters)."""
     ref = O.ReferenceABFS(mm, nthreads=1, transformations=transformations, check_for_inverses=check_for_inverses,
-                          image_expand_factor=expand_factor, cell_factor=cell_factor)
+                          image_expand_factor=expand_factor, cell_factor=cell_factor, use_grid_by_cell=grid_by_cell)
     energies, present, rebuilt, grads, dEdM, images = [], [], [], [], [], []
     keys = None
     for k, (xyz, sp) in enumerate(calls):
@@ -78,7 +79,7 @@ def run_case(name, mm, transformations, calls, check_for_inverses=True, expand_f
                energies=np.array(energies), present=np.array(present), rebuilt=np.array(rebuilt),
                grads=np.array(grads), dEdM=np.array(dEdM), use_grid=use_grid,
                images=np.concatenate(images), images_per_call=np.array([x.shape[0] for x in images]),
-               image_key_counts=counts, image_key_deltas=deltas.astype(np.uint32))
+               image_key_counts=counts, image_key_deltas=deltas.astype(np.uint32), grid_by_cell=grid_by_cell)
     assert np.array_equal(np.cumsum(deltas.astype(np.int64)[:counts[0]]) if nimg0 else flat, keys[0] if nimg0 else flat)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print(name, mm.natoms, "images", [x.shape[0] for x in images], "pairs", [int(x[:, 5].sum()) for x in images],
@@ -107,7 +108,24 @@ def main():
     huge = tuple([37.5, 37.5, 37.5] + list(sp[3:]))
     run_case("waterbox4500_moves_cells", w, tr,
              [(x0, sp), (x1, sp), (x2, big), (x2, huge), (x3, huge), (x3, sp)])
+    edge_cases()
+
+
+def edge_cases():
+    """A P1 box of 28 A holding the 900-atom system whose extent is exactly 4 cells of 6.75 A (atoms on the upper
+    faces fall off the primary grid), with both grid searches: the image lists' conforming grids, whole cells and,
+    for point/cell, the searches of points off the grid."""
+    from make_golden import edge_system
+    e = edge_system(0)
+    tr = (np.eye(3)[None], np.zeros((1, 3)))
+    sp = (28.0, 28.0, 28.0, 90.0, 90.0, 90.0)
+    x1 = e.coordinates3 + np.random.default_rng(2).normal(0.0, 0.02, e.coordinates3.shape)
+    for gbc, tag in ((True, "cellcell"), (False, "pointcell")):
+        run_case("edge900_P1_" + tag, e, tr, [(e.coordinates3, sp), (x1, sp)], grid_by_cell=gbc)
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["edge"]:
+        edge_cases()
+    else:
+        main()

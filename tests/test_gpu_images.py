@@ -1,7 +1,9 @@
 """Parity of the periodic-image path of the HIP library (through the C ABI).
 
-Against the reference library's own outputs (tests/golden/crystal_*.npz and
-waterbox4500_moves_cells.npz, written by make_golden_images.py) and the FP64
+Against the reference library's own outputs (tests/golden/crystal_*.npz,
+waterbox4500_moves_cells.npz and edge900_P1_*.npz, written by
+make_golden_images.py; the last pair is a box whose extent is a whole number of
+cells, searched cell by cell and point by cell) and the FP64
 oracle: the image list (order, a b c, transformation, scale, pair count per
 image) and every image's pair set exactly, the rebuild decisions of
 CheckForUpdate + CheckForImageUpdate exactly, and within the mixed-precision
@@ -32,6 +34,7 @@ def _state(fx, precision="mixed"):
     o = _native.default_options()
     o.precision = _native.MIXED if precision == "mixed" else _native.PLAIN_FP64
     o.cutoffCellSizeFactor = float(fx["options"][2])
+    o.useGridByCell = 1 if bool(fx.get("grid_by_cell", True)) else 0
     st = _native.NativeState(o, fx["charges"], fx["ljtypes"], int(fx["ntypes"]), fx["tableA"], fx["tableB"],
                              exclusions=fx["exclusions"], interactions14=fx["interactions14"])
     st.set_transformations(fx["rotations"], fx["translations"], bool(fx["options"][0]), int(fx["options"][1]))

@@ -354,7 +354,8 @@ def test_oracle_matches_reference_image_fixtures(oracle_mod, path):
     o = fx["options"]
     tr = (fx["rotations"], fx["translations"])
     orc = oracle_mod.OracleABFS(sysm, cell_factor=o[2], transformations=tr, check_for_inverses=bool(o[0]),
-                                image_expand_factor=int(o[1]), nthreads=2)
+                                image_expand_factor=int(o[1]), nthreads=2,
+                                use_grid_by_cell=bool(fx.get("grid_by_cell", True)))
     off = 0
     for k, (xyz, sp) in enumerate(zip(fx["coords"], fx["cells"])):
         terms, g, dm, up = orc.energy_sym(xyz, sp)
