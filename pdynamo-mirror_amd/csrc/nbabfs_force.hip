@@ -134,8 +134,10 @@ __device__ __forceinline__ void pair_rows(const AbfsFactors &F, double r2, doubl
 __device__ __forceinline__ bool guard_skip(const ForceGuard &g)
 {
     if (!g.flag) return false;
-    const int epoch = g.epoch_ptr ? *(const volatile int *)g.epoch_ptr : g.epoch;
-    return *(const volatile int *)g.flag == epoch;
+    // Two relaxed loads that may be in flight together (volatile loads would be issued one after the other).
+    const int epoch = g.epoch_ptr ? __hip_atomic_load(g.epoch_ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : g.epoch;
+    return __hip_atomic_load(g.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
 }
 
 __global__ __launch_bounds__(256) void k_prepare(const SysDesc *__restrict__ descs, int tiled, int *__restrict__ heads,
@@ -374,15 +376,26 @@ __device__ __forceinline__ void wave_region_sync()
 template <bool ROWS_LDS, bool IMG, bool F64, int WPG>
 __global__ __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu((IMG || F64) ? 3 : 4))) void k_force_tiled(
     const SysDesc *__restrict__ descs, const WorkItem *__restrict__ items, QueueParams qp, AbfsFactors F, int rows_bytes,
-    int wave_bytes, ForceGuard guard, int nitems, int nforce_groups)
+    int wave_bytes, ForceGuard guard, int nitems, int nforce_groups, int sys1)
 {
-    if (guard_skip(guard)) return;
     const int lane = threadIdx.x & 63;
     const int wave = WPG == 1 ? 0 : (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // A group's start is a chain of dependent loads (its work item, the system's descriptor, the records the item
+    // names).  The loads that depend on nothing go out together with the guard's flag: the work item of a wave
+    // with one item, and (sys1 >= 0: every item belongs to system sys1) the descriptor and its row table.
+    WorkItem w0 = {0, 0, 0, 0};
+    const bool one_item = qp.heads == nullptr && (IMG || (int)blockIdx.x < nforce_groups);
+    if (one_item && nitems > 0) w0 = items[min((int)blockIdx.x * WPG + wave, nitems - 1)];
+    // guard_skip with both loads issued here unconditionally (a null pointer reads the descriptor instead) and
+    // their values compared only where the answer is needed.
+    const int *gfp = guard.flag ? guard.flag : (const int *)descs;
+    const int *gep = guard.epoch_ptr ? guard.epoch_ptr : gfp;
+    const int gflag = __hip_atomic_load(gfp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int gepoch = __hip_atomic_load(gep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // Groups past the pair blocks evaluate the 1-4 list of system 0 (single-state launches), one block per wave.
     if (!IMG && (int)blockIdx.x >= nforce_groups) {
         const int blk = ((int)blockIdx.x - nforce_groups) * WPG + wave;
-        if (blk < descs[0].n14_blocks) pairs14_block(descs[0], F, blk, lane);
+        if (!guard_skip(guard) && blk < descs[0].n14_blocks) pairs14_block(descs[0], F, blk, lane);
         return;
     }
     // Lane l = 8b + a: a is the i-atom within an i-cluster, b the j-atom of
@@ -400,13 +413,15 @@ __global__ __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu((IMG |
     // The group's row table, staged once when the items are one per wave.
     int rows_sys = -1;
     if (ROWS_LDS && qp.heads == nullptr && WPG > 1) {
-        rows_sys = items[(int)blockIdx.x * WPG].sys;
+        rows_sys = sys1 >= 0 ? sys1 : items[(int)blockIdx.x * WPG].sys;
         const SysDesc *D0 = &descs[rows_sys];
         const GAS double *__restrict__ rows0 = as_global(D0->ljrow);
         const int nr = 2 * D0->ntypes * D0->ntypes;
         for (int k = threadIdx.x; k < nr; k += 64 * WPG) sR[k] = load_slot(rows0, k);
         __syncthreads();
     }
+    // The guard's answer is needed only now (the loads above are reads; the flag is uniform over the launch).
+    if (guard.flag != nullptr && gflag == (guard.epoch_ptr ? gepoch : guard.epoch)) return;
     int q = blockIdx.x & 7;
     int tries = 0;
     // heads == null: one work item per wave (the hardware dispatcher
@@ -431,7 +446,7 @@ __global__ __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu((IMG |
             }
             it = qs + local;
         }
-        const WorkItem w = items[it];
+        const WorkItem w = qp.heads == nullptr ? w0 : items[it];
         // Empty items pad a batch's systems to whole groups; an item of another system than the staged rows
         // would be a layout error (reported, never evaluated with the wrong rows).
         if (w.count <= 0) continue;
@@ -439,7 +454,7 @@ __global__ __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu((IMG |
             raise_error(descs[rows_sys].err, true, NB_ERR_RANGE);
             continue;
         }
-        const SysDesc *D = &descs[IMG ? (w.sys & 0xffff) : w.sys];
+        const SysDesc *D = &descs[sys1 >= 0 ? sys1 : (IMG ? (w.sys & 0xffff) : w.sys)];
         const GAS double *__restrict__ xq = (const GAS double *)D->xyzq;
         const GAS int *__restrict__ ty = as_global(D->styp);
         const GAS double *__restrict__ ljA = as_global(D->ljA);
@@ -1039,15 +1054,16 @@ void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, in
 template <bool ROWS_LDS, bool IMG, bool F64, int WPG>
 static void launch_tiled_instance(dim3 grid, size_t bytes, hipStream_t s, hipEvent_t ev_start, hipEvent_t ev_stop,
                                   const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
-                                  int rows_bytes, int wave_bytes, ForceGuard g, int nitems, int nforce_groups)
+                                  int rows_bytes, int wave_bytes, ForceGuard g, int nitems, int nforce_groups,
+                                  int sys1)
 {
     hipExtLaunchKernelGGL((k_force_tiled<ROWS_LDS, IMG, F64, WPG>), grid, dim3(64 * WPG), bytes, s, ev_start, ev_stop, 0,
-                          d_desc, items, q, F, rows_bytes, wave_bytes, g, nitems, nforce_groups);
+                          d_desc, items, q, F, rows_bytes, wave_bytes, g, nitems, nforce_groups, sys1);
 }
 
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
                         int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images,
-                        hipEvent_t ev_start, hipEvent_t ev_stop, int n14_blocks, bool full_fp64)
+                        hipEvent_t ev_start, hipEvent_t ev_stop, int n14_blocks, bool full_fp64, int sys1)
 {
     if (images) n14_blocks = 0;
     if (nblocks + n14_blocks <= 0) return;
@@ -1062,7 +1078,8 @@ void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParam
     const bool rows_lds = !full_fp64 && tiled_lds_rows(max_ntypes, true) + (size_t)wpg * wave_bytes <= limit;
     const int rows_bytes = (int)tiled_lds_rows(max_ntypes, rows_lds);
     const size_t bytes = rows_bytes + (size_t)wpg * wave_bytes;
-#define NB_TILED_ARGS grid, bytes, s, ev_start, ev_stop, d_desc, items, q, F, rows_bytes, wave_bytes, g, nblocks, ngroups
+#define NB_TILED_ARGS grid, bytes, s, ev_start, ev_stop, d_desc, items, q, F, rows_bytes, wave_bytes, g, nblocks, ngroups, \
+                      sys1
     if (full_fp64) {
         // No row table: the FP64 pair terms read the raw A and B of the type pair.
         if (wpg == 1)

@@ -1399,18 +1399,18 @@ static int launch_force(nbabfs_state *S, ForceGuard g)
             QueueParams q{nullptr, S->qstarts.p};
             const int o = nm > 0 ? 2 : 0;
             launch_force_tiled(S->d_desc.p, S->items.p + nm, q, S->F, ni, S->ntypes, image_per_item(S->opt), g, sti,
-                               true, ev[o], ev[o + 1]);
+                               true, ev[o], ev[o + 1], 0, false, 0);
         }
         if (nm > 0) {
             QueueParams q{S->opt.wavesPerCU > 0 ? S->heads.p : nullptr, S->qstarts.p};
             launch_force_tiled(S->d_desc.p, S->items.p, q, S->F, n_persistent_blocks(S->opt, nm), S->ntypes,
                                S->opt.entriesPerItem, g, st, false, ev[0], ev[1], nblk(S->n14, NB_P14_BLOCK),
-                               S->opt.precision == NBABFS_TILED_FP64);
+                               S->opt.precision == NBABFS_TILED_FP64, 0);
         } else if (S->n14 > 0) {
             QueueParams q{nullptr, S->qstarts.p};
             launch_force_tiled(S->d_desc.p, S->items.p, q, S->F, 0, S->ntypes, S->opt.entriesPerItem, g, st, false,
                                ni > 0 ? nullptr : ev[0], ni > 0 ? nullptr : ev[1], nblk(S->n14, NB_P14_BLOCK),
-                               S->opt.precision == NBABFS_TILED_FP64);
+                               S->opt.precision == NBABFS_TILED_FP64, 0);
         }
         if (sti != st) {
             HIPCHK(hipEventRecord(S->ev_join, sti));

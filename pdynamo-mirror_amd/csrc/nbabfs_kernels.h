@@ -169,10 +169,12 @@ void launch_prepare(const SysDesc *d_desc, int nsys, int max_npad, int tiled, in
 // launchers take optional timing events (start, stop: recorded by the runtime
 // at the dispatch's own begin and end; null = not timed).
 // full_fp64: the NBABFS_TILED_FP64 instance (FP64 pair arithmetic and force vectors; not for image items).
+// sys1 >= 0: every item belongs to system sys1 (a single-state launch), so the kernel reads its descriptor without
+// waiting for the items.
 void launch_force_tiled(const SysDesc *d_desc, const WorkItem *items, QueueParams q, const AbfsFactors &F,
                         int nblocks, int max_ntypes, int max_count, ForceGuard g, hipStream_t s, bool images = false,
                         hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr, int n14_blocks = 0,
-                        bool full_fp64 = false);
+                        bool full_fp64 = false, int sys1 = -1);
 // The 1-4 list in blocks of NB_P14_BLOCK pairs (k_pairs14, or extra waves of a single-state pair launch).
 #define NB_P14_BLOCK 64
 // Waves per workgroup of the one-item-per-wave pair launches; a batch pads each system's items to a multiple of it
