@@ -768,7 +768,11 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
         const float reach2 = reach * reach;
         int cx0 = 0, cy0 = 0, nyc = 1, ncol = 0;
         const int Jmax = I0 + NB_SCC - 1;
+        // z-bins that can hold an atom within reach of the super-cluster (one bin of margin for the FP32 box).
+        int zb_lo = 0, zb_hi = 0;
         if (!CROSS) {
+            zb_lo = min(max(0, (int)floorf((lo.z - reach) / P.dz) - 1), P.nzb - 1);
+            zb_hi = min(max(0, (int)floorf((hi.z + reach) / P.dz) + 1), P.nzb - 1);
             cx0 = (int)floorf((lo.x - reach) / P.cw);
             cy0 = (int)floorf((lo.y - reach) / P.cw);
             int cx1 = (int)floorf((hi.x + reach) / P.cw), cy1 = (int)floorf((hi.y + reach) / P.cw);
@@ -805,9 +809,19 @@ __global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, co
                     int len = 0;
                     if (k < nc) {
                         const int col = c0 + k;
-                        const int2 r = colcl[(cx0 + col / nyc) * P.ncy + cy0 + col % nyc];
-                        len = max(0, min(r.y, Jmax + 1) - r.x);
-                        sColStart[k] = r.x;
+                        const int ci = (cx0 + col / nyc) * P.ncy + cy0 + col % nyc;
+                        const int2 r = colcl[ci];
+                        // The column's atoms are in z-bin order from its first slot: the super-clusters holding
+                        // atoms of bins zb_lo..zb_hi (slots plo..phi of the column) are the only ones in reach.
+                        const int base = P.zoff[ci * P.nzb];
+                        const int plo = P.zoff[ci * P.nzb + zb_lo] - base;
+                        const int phi = (zb_hi + 1 < P.nzb ? P.zoff[ci * P.nzb + zb_hi + 1]
+                                                            : (ci + 1 < P.ncol ? P.zoff[(ci + 1) * P.nzb] : P.n)) -
+                                        base;
+                        const int js = max(r.x, r.x + (plo / NB_SC) * NB_SCC);
+                        const int je = min(min(r.y, Jmax + 1), r.x + (phi + NB_SC - 1) / NB_SC * NB_SCC);
+                        len = max(0, je - js);
+                        sColStart[k] = js;
                     }
                     int incl = len;
 #pragma unroll

@@ -1073,6 +1073,11 @@ static int build_list(nbabfs_state *S, const double *d_xyz)
     P.ncy = g.ncy;
     P.srec = S->srec.p;
     P.scell = S->scell.p;
+    P.zoff = S->off.p;
+    P.dz = (float)g.dz;
+    P.nzb = g.nzb;
+    P.ncol = ncol;
+    P.n = n;
     P.pointcell = pointcell;
     for (int d = 0; d < 3; d++) P.glower[d] = g.lower[d];
     P.gh = g.h;

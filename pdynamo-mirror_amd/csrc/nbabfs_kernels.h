@@ -52,6 +52,11 @@ struct SearchParams {
     // Self lists: slot-ordered search records (k_slot_records).
     const double4 *srec;
     const int4 *scell;
+    // Self lists: the counting sort's offsets per (column, z-bin) key, the z-bin height and count, so that each
+    // column is scanned only over the super-clusters whose z-bins come within reach.
+    const int *zoff;
+    float dz;
+    int nzb, ncol;
     // Point/cell grid search (useGridByCell false): the reference grid's lower corner and cell size.
     int pointcell;
     double glower[3];
