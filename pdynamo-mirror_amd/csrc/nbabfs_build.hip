@@ -667,7 +667,7 @@ __device__ __forceinline__ int search_eval_self(const SearchParams &P, int npend
 // OR selection, i on the reference grid (grid method), and the FP64 distance
 // test against the image coordinates GenerateImageLists produced.
 template <bool CROSS, int STEN, bool PC = false>
-__global__ __launch_bounds__(NB_SEARCH_THREADS) void k_search(SearchParams P, const double *__restrict__ xyz,
+__global__ __launch_bounds__(NB_SEARCH_THREADS) __attribute__((amdgpu_waves_per_eu((CROSS || PC) ? 1 : 8))) void k_search(SearchParams P, const double *__restrict__ xyz,
                                                               const int *__restrict__ slot_atom,
                                                               const float4 *__restrict__ blo,
                                                               const float4 *__restrict__ bhi,
