@@ -235,12 +235,13 @@ static void save_record(PairSink *s, int i, int n, const int *ind)
 static void point_cell_records(PairSink *s, int n, const double *xyz, const int *xptr, const int *xidx, double c2,
                                double cellSize, double cutoff, const int *bins, const double *lower, const int *cell,
                                const long *first, const long *total, const int *points, int noff, const int *off,
-                               unsigned char *incl, int *ind)
+                               const unsigned char *qand, unsigned char *incl, int *ind)
 {
     const long stride0 = (long)bins[1] * bins[2], stride1 = bins[2];
     const int u = (int)ceil(cutoff / cellSize);
     int pc[3] = {-u, -u, -u};
     for (int i = 1; i < n; i++) {
+        if (qand && !qand[i]) continue;      /* outside the AND selection: no search, no cell decoded */
         if (cell[i] >= 0) {
             long c = cell[i];
             pc[0] = (int)(c / stride0);
@@ -269,7 +270,7 @@ static void point_cell_records(PairSink *s, int n, const double *xyz, const int 
             for (long p = first[cid], pe = first[cid] + total[cid]; p < pe; p++) {
                 int j = points[p];
                 if (j >= i) break;
-                if (!incl[j]) continue;
+                if (!incl[j] || (qand && !qand[j])) continue;
                 if (all) { ind[m++] = j; continue; }
                 double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1], dz = xyz[3 * i + 2] - xyz[3 * j + 2];
                 if ((dx * dx) + (dy * dy) + (dz * dz) <= c2) ind[m++] = j;
@@ -281,7 +282,7 @@ static void point_cell_records(PairSink *s, int n, const double *xyz, const int 
 }
 
 long orc_self_pair_list(int n, const double *xyz, const int *xptr, const int *xidx,
-                        double cutoff, double cellSize, int method,
+                        double cutoff, double cellSize, int method, const unsigned char *qand,
                         long cap_rec, long cap_pairs, int *rec_i, long *rec_ptr, int *js,
                         long *nrec_out)
 {
@@ -338,7 +339,7 @@ long orc_self_pair_list(int n, const double *xyz, const int *xptr, const int *xi
         long stride0 = (long)bins[1] * bins[2], stride1 = bins[2];
         if (method == 2)
             point_cell_records(&s, n, xyz, xptr, xidx, c2, cellSize, cutoff, bins, lower, cell, first, total, points,
-                               noff, off, incl, ind);
+                               noff, off, qand, incl, ind);
         else for (long c1 = 0; c1 < ncells; c1++) {
             long p1s = first[c1], p1t = total[c1];
             int ci[3] = {(int)(c1 / stride0), (int)((c1 % stride0) / stride1), (int)(c1 % stride1)};

@@ -55,13 +55,16 @@ long orc_make_grid(int n, const double *xyz, double cellSize, int *bins, double 
 int orc_search_range(double cellSize, double cutoff, int cap, int *offsets, int *flags);
 
 /* Self pair list (pCx/csource/PairListGenerator.c:946-1218).  method: 1 grid
- * cell/cell, 2 grid point/cell, 0 direct.  Exclusions as symmetric CSR sorted ascending per row.
+ * cell/cell, 2 grid point/cell, 0 direct.  qand (n, or NULL for every atom):
+ * the AND selection of the point/cell search, whose points outside it neither
+ * search nor decode a cell (the other methods list every atom; callers filter
+ * their rows).  Exclusions as symmetric CSR sorted ascending per row.
  * Output records in reference order: rec_i[r], rec_ptr[r..r+1] into js.
  * Returns npairs, or -1 if capacities are too small (need_* report sizes), or
  * ORC_INVALID_ARGUMENT for a cell grid with no cells (a zero extent). */
 #define ORC_INVALID_ARGUMENT (-(1L << 62))
 long orc_self_pair_list(int n, const double *xyz, const int *xptr, const int *xidx,
-                        double cutoff, double cellSize, int method,
+                        double cutoff, double cellSize, int method, const unsigned char *qand,
                         long cap_rec, long cap_pairs, int *rec_i, long *rec_ptr, int *js,
                         long *nrec_out);
 

@@ -52,7 +52,7 @@ def lib():
         L.orc_make_grid.restype = C.c_long
         L.orc_search_range.argtypes = [C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
         L.orc_self_pair_list.argtypes = [C.c_int, _dp, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int,
-                                         C.c_long, C.c_long, _ip, _lp, _ip, C.POINTER(C.c_long)]
+                                         C.c_void_p, C.c_long, C.c_long, _ip, _lp, _ip, C.POINTER(C.c_long)]
         L.orc_self_pair_list.restype = C.c_long
         L.orc_check_for_update.argtypes = [C.c_int, _dp, _dp, C.c_void_p, C.c_double, C.c_double,
                                            C.POINTER(C.c_double)]
@@ -170,9 +170,11 @@ class PairRecords:
         return np.sort(np.maximum(i, j) * self.n + np.minimum(i, j))
 
 
-def self_pair_list(xyz, xptr, xidx, cutoff=13.5, cell_size=6.75, method=1):
+def self_pair_list(xyz, xptr, xidx, cutoff=13.5, cell_size=6.75, method=1, qand=None):
+    """method: 0 direct, 1 grid cell/cell, 2 grid point/cell; qand: the point/cell AND selection (bool per atom)."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     n = xyz.shape[0]
+    qa = None if qand is None else np.ascontiguousarray(qand, dtype=np.uint8)
     cap_r, cap_p = max(n, 1), max(16 * n, 1024)
     while True:
         rec_i = np.zeros(cap_r, np.int32)
@@ -181,7 +183,8 @@ def self_pair_list(xyz, xptr, xidx, cutoff=13.5, cell_size=6.75, method=1):
         nrec = C.c_long()
         npairs = lib().orc_self_pair_list(n, xyz, None if xptr is None else xptr.ctypes.data,
                                           None if xidx is None else xidx.ctypes.data,
-                                          cutoff, cell_size, int(method), cap_r, cap_p, rec_i, rec_ptr, js,
+                                          cutoff, cell_size, int(method), None if qa is None else qa.ctypes.data,
+                                          cap_r, cap_p, rec_i, rec_ptr, js,
                                           C.byref(nrec))
         if npairs >= 0:
             r = nrec.value

@@ -295,10 +295,88 @@ def _within(xi, xj, cutoff):
     return r2 <= cutoff * cutoff
 
 
-class QCMMLists:
-    """The QC/MM lists of one update, as sorted arrays per row."""
+def _cells(x, lower, h, bins):
+    """RegularGrid_FindCellIDOfPoint as indices: floor((x - lower)/h) per axis and whether all are on the grid."""
+    k = np.floor((x - lower) / h).astype(np.int64)
+    return k, np.all((k >= 0) & (k < bins), axis=1)
 
-    def __init__(self, qc, xyz, exclusions, interactions14, coupling, free=None, cutoff=13.5, charges=None):
+
+def _point_cell_r2(p, k, lower, h):
+    """RegularGrid_DistancesSquaredBetweenCellAndPoint (non-periodic), point p against cells k (rows)."""
+    lo = k * h + lower
+    hi = lo + h
+    dl, du = p - lo, hi - p
+    mx = np.where(p > hi, dl, np.where(p < lo, du, np.maximum(dl, du)))
+    mn = np.where(p > hi, p - hi, np.where(p < lo, lo - p, 0.0))
+    return (mn[:, 0] * mn[:, 0] + mn[:, 1] * mn[:, 1]) + mn[:, 2] * mn[:, 2], \
+           (mx[:, 0] * mx[:, 0] + mx[:, 1] * mx[:, 1]) + mx[:, 2] * mx[:, 2]
+
+
+class CrossGrid:
+    """The grid search of MakeCrossPairListFromCoordinates3 (pCore-1.9.0/extensions/csource/PairListGenerator.c:
+    611-940): row points on grid 1 (made from rows_grid_xyz), column points on a grid conforming to it made from the
+    box of the columns in col_box (Coordinates3_MakeConformingGrid, Coordinates3.c:998-1072), or on grid 1 itself
+    (same=True, PairListGenerator_CrossPairListFromSingleCoordinates3, :450-493).  select(r, x_r, cols) is the
+    mask of the candidate columns the search lists for row r: the column on its grid, in a cell I + o - lowN over
+    the stencil offsets o around the row's search cell I, and then whole cells (cell/cell: stencil flag 2; point/
+    cell: largest point/cell distance) or the FP64 distance test; point/cell cells also need the smallest
+    point/cell distance below the cutoff.  Cell/cell rows must be on grid 1; point/cell rows off it search around
+    the cell of the latest earlier row on it ((-u, -u, -u) before any)."""
+
+    def __init__(self, point_cell, h, cutoff, grid1_xyz, rows_xyz, cols_xyz, col_box, same=False):
+        self.pc, self.h, self.c2 = point_cell, h, cutoff * cutoff
+        bins1, lower1, _ = O.make_grid(grid1_xyz, h)
+        self.u = int(np.ceil(cutoff / h))
+        off, flg = O.search_range(h, cutoff)
+        w = 2 * self.u + 1
+        self.sten = np.zeros((w, w, w), np.int32)
+        for (a, b, c), f in zip(off, flg):
+            self.sten[a + self.u, b + self.u, c + self.u] = f
+        if same:
+            self.low = np.zeros(3, np.int64)
+            bins2, lower2 = bins1, lower1
+        else:
+            lo = col_box.min(axis=0)
+            up = (col_box.max(axis=0) - lo) + lo
+            self.low = np.floor((lo - lower1) / h).astype(np.int64)
+            bins2 = np.floor((up - lower1) / h).astype(np.int64) - self.low + 1
+            lower2 = lower1 + self.low * h
+        self.lower2 = lower2
+        self.kc, self.onc = _cells(cols_xyz, lower2, h, bins2)
+        kr, onr = _cells(rows_xyz, lower1, h, bins1)
+        self.rows_on = onr
+        self.scell = kr.copy()
+        last = np.array([-self.u] * 3, np.int64)
+        for r in range(rows_xyz.shape[0]):
+            if onr[r]:
+                last = kr[r]
+            else:
+                self.scell[r] = last
+
+    def select(self, r, xr, cols_xyz, cand):
+        if not self.pc and not self.rows_on[r]:
+            return np.zeros(cand.size, bool)
+        o = self.kc[cand] + self.low - self.scell[r] + self.u
+        w = self.sten.shape[0]
+        inb = np.all((o >= 0) & (o < w), axis=1)
+        oc = np.clip(o, 0, w - 1)
+        flag = np.where(inb, self.sten[oc[:, 0], oc[:, 1], oc[:, 2]], 0)
+        d = xr[None, :] - cols_xyz[cand]
+        near = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2] <= self.c2
+        ok = self.onc[cand] & (flag != 0)
+        if self.pc:
+            mn, mx = _point_cell_r2(xr[None, :], self.kc[cand], self.lower2, self.h)
+            return ok & (mn < self.c2) & ((mx <= self.c2) | near)
+        return ok & ((flag == 2) | near)
+
+
+class QCMMLists:
+    """The QC/MM lists of one update, as sorted arrays per row.  grid: None (direct method) or (point_cell, h):
+    the grid search of both cross lists (CrossGrid; the electrostatic list on the grid of the QC coordinates when
+    there is more than one QC atom, NBModelABFSState_GridInitialize, pMolecule-1.9.0/extensions/csource/
+    NBModelABFSState.c:207-226, the LJ list on the grid of all atoms)."""
+
+    def __init__(self, qc, xyz, exclusions, interactions14, coupling, free=None, cutoff=13.5, charges=None, grid=None):
         n = qc.n
         self.free = np.ones(n, bool) if free is None else np.asarray(free, bool)
         excl = _csr_rows(n, exclusions)
@@ -306,6 +384,10 @@ class QCMMLists:
         qx = qc.coordinates(xyz)
         sx, self.site_q, self.sites = qc.mm_sites(xyz, charges if charges is not None else np.zeros(n), coupling)
         self.el, self.el14, self.lj, self.lj14 = [], [], [], []
+        gel = None
+        if grid is not None and qc.nqc > 1:
+            # Columns: all atoms with the MM selection's box (MM coupling) or the MM sites (RC, RD).
+            gel = CrossGrid(grid[0], grid[1], cutoff, qx, qx, sx, sx[qc.mm] if coupling == 0 else sx)
         for q in range(qc.nqc):
             i = qc.index[q]
             if coupling == 0:
@@ -313,18 +395,24 @@ class QCMMLists:
                 ok = qc.mm.copy()
                 ok[list(ex)] = False
                 cand = np.nonzero(ok)[0]
-                self.el.append(cand[_within(qx[q], sx[cand], cutoff)])
-                self.el14.append(np.array(sorted(j for j in p14[i] if qc.mm[j] and j != i), np.int64))
             else:
                 cand = np.arange(sx.shape[0])
-                self.el.append(cand[_within(qx[q], sx, cutoff)])
+            sel = _within(qx[q], sx[cand], cutoff) if gel is None else gel.select(q, qx[q], sx, cand)
+            self.el.append(cand[sel])
+            if coupling == 0:
+                self.el14.append(np.array(sorted(j for j in p14[i] if qc.mm[j] and j != i), np.int64))
+            else:
                 self.el14.append(np.zeros(0, np.int64))
         self.lj_rows = np.nonzero(qc.pure)[0]
-        for i in self.lj_rows:
+        glj = None
+        if grid is not None:
+            glj = CrossGrid(grid[0], grid[1], cutoff, xyz, xyz[self.lj_rows], xyz, None, same=True)
+        for r, i in enumerate(self.lj_rows):
             ok = qc.mm & (self.free | self.free[i])
             ok[list(excl[i])] = False
             cand = np.nonzero(ok)[0]
-            self.lj.append(cand[_within(xyz[i], xyz[cand], cutoff)])
+            sel = _within(xyz[i], xyz[cand], cutoff) if glj is None else glj.select(r, xyz[i], xyz, cand)
+            self.lj.append(cand[sel])
             self.lj14.append(np.array(sorted(j for j in p14[i] if qc.mm[j] and (self.free[i] or self.free[j])),
                                       np.int64))
         self.site_xyz = sx
@@ -392,8 +480,9 @@ class QCMMOracle:
     """One QC/MM system: the MM/MM part (no pure QC atoms), QC/MM LJ, potentials and gradients."""
 
     def __init__(self, system, qc, coupling, spline, fixed=None, damping=0.5, inner=8.0, outer=12.0,
-                 list_cutoff=13.5, dielectric=1.0, scale14=1.0, tableA14=None, tableB14=None):
+                 list_cutoff=13.5, dielectric=1.0, scale14=1.0, tableA14=None, tableB14=None, use_grid_by_cell=True):
         self.s, self.qc, self.coupling, self.spline = system, qc, coupling, spline
+        self.use_grid_by_cell = use_grid_by_cell
         self.free = None if fixed is None else ~np.asarray(fixed, bool)
         self.f = O.factors(damping, inner, outer)
         self.outer, self.list_cutoff, self.dielectric, self.scale14 = outer, list_cutoff, dielectric, scale14
@@ -402,13 +491,17 @@ class QCMMOracle:
         self.xptr, self.xidx = O.exclusion_csr(system.natoms, system.exclusions)
         self.ref = None
         self.lists = None
+        self.use_grid = None           # decided once, on the first build (NBModelABFSState.c:212)
 
     def mmmm_records(self, xyz):
         """Self list of the MM selection with the free-atom OR test (the full list filtered)."""
         s = self.s
         use_grid = O.determine_method(xyz, self.list_cutoff)
-        rec = O.self_pair_list(xyz, self.xptr, self.xidx, self.list_cutoff, 0.5 * self.list_cutoff,
-                               1 if use_grid else 0)
+        method = (1 if self.use_grid_by_cell else 2) if use_grid else 0
+        # The MM selection is the AND selection of GenerateLists (NBModelABFS.c:1080); the point/cell search
+        # applies it itself (its points outside it decode no cell), the other methods through the filter.
+        rec = O.self_pair_list(xyz, self.xptr, self.xidx, self.list_cutoff, 0.5 * self.list_cutoff, method,
+                               qand=self.qc.mm if method == 2 else None)
         return self._filter(rec)
 
     def _filter(self, rec):
@@ -434,8 +527,11 @@ class QCMMOracle:
             rebuilt, _ = O.check_for_update(xyz, self.ref, self.list_cutoff, self.outer, fixed)
         if rebuilt:
             self.rec = self.mmmm_records(xyz)
+            if self.use_grid is None:
+                self.use_grid = O.determine_method(xyz, self.list_cutoff)
+            grid = (not self.use_grid_by_cell, 0.5 * self.list_cutoff) if self.use_grid else None
             self.lists = QCMMLists(qc, xyz, s.exclusions, s.interactions14, self.coupling, self.free,
-                                   self.list_cutoff, s.charges)
+                                   self.list_cutoff, s.charges, grid)
             p14 = O.records_from_pairs(s.natoms, s.interactions14)
             self.rec14 = self._filter(p14)
             self.ref = xyz.copy()

@@ -460,6 +460,7 @@ struct nbabfs_state {
     DBuf<double> itc, ixyz, rtbuf, ibb;
     DBuf<float4> iblo, ibhi, icen;
     DBuf<float> ibox;
+    DBuf<QcmmGrid> qgrid;        // QC/MM cross lists: grids of the electrostatic and LJ lists
     DBuf<int4> icell;            // image lists: primary-grid search cells of the i atoms
     DBuf<ImageGrid> igrid;       // image lists: conforming grid of each image
     DBuf<int> istep_off, ibound, isoff, icounts, incand, ieoff, inits, iioff;
@@ -3179,11 +3180,25 @@ static int build_qcmm_lists(nbabfs_state *S, const double *d_xyz)
     CHK(S->qcnt.ensure(rows_max));
     CHK(S->scantmp.ensure(scan_tmp_size(rows_max)));
     CHK(S->h_int.ensure(2));
+    // Grid method: the cross lists search the grids of NBModelABFSState_GridInitialize (pMolecule-1.9.0/
+    // extensions/csource/NBModelABFSState.c:207-226): the electrostatic list the QC coordinates' grid when there is
+    // more than one QC atom, the LJ list the grid of all atoms (the main list's).
+    if (S->useGrid) {
+        CHK(S->qgrid.ensure(2));
+        launch_qcmm_grids(G, S->freef.p, S->grid.lower, S->grid.bins, S->grid.h, S->qgrid.p, st);
+    }
     for (int list = 0; list < 2; list++) {
         CrossListParams P{};
         P.G = G;
         P.cutoff2 = c2;
         P.flags = S->freef.p;
+        if (S->useGrid && (list == 1 || S->nqc > 1)) {
+            P.grid = S->qgrid.p + list;
+            P.stencil = S->stencil.p;
+            P.u = S->u;
+            P.h = S->grid.h;
+            P.pointcell = S->opt.useGridByCell ? 0 : 1;
+        }
         DBuf<int> &ptr = list ? S->lj_ptr : S->el_ptr;
         DBuf<int> &idx = list ? S->lj_idx : S->el_idx;
         if (list == 0) {
@@ -3211,7 +3226,15 @@ static int build_qcmm_lists(nbabfs_state *S, const double *d_xyz)
             launch_cross_list(P, 0, st);
             launch_scan(S->qcnt.p, ptr.p, P.nrows + 1, S->scantmp.p, st);
             HIPCHK(hipMemcpyAsync(S->h_int.p, ptr.p + P.nrows, sizeof(int), hipMemcpyDeviceToHost, st));
+            S->h_int.p[1] = 0;
+            if (P.grid)
+                HIPCHK(hipMemcpyAsync(S->h_int.p + 1, &P.grid->bad, sizeof(int), hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
+            // A grid axis with no cells has no occupancy (RegularGridOccupancy_Allocate rejects zero cells,
+            // pCore-1.9.0/extensions/csource/RegularGridOccupancy.c:29,54).
+            if (S->h_int.p[1])
+                return fail(NBABFS_STATUS_INVALID_ARGUMENT,
+                            "the QC atoms' cell grid has an axis with no cells (zero extent)");
             total = S->h_int.p[0];
         } else {
             launch_fill_int(ptr.p, 1, 0, st);

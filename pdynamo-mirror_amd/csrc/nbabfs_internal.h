@@ -232,3 +232,30 @@ __device__ __forceinline__ float wave_sum_f32(float v)
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+
+// Squared smallest and largest distances between a point and grid cell n
+// (RegularGrid_DistancesSquaredBetweenCellAndPoint, pCore-1.9.0/extensions/
+// csource/RegularGrid.c:139-180, non-periodic branch, in its operation order:
+// the cell spans [n h + lower, n h + lower + h] on each axis; the per-axis
+// squares are summed in axis order).
+__device__ __forceinline__ void point_cell_r2(const double *lower, double h, double x, double y, double z, int nx,
+                                              int ny, int nz, double &minR2, double &maxR2)
+{
+#pragma clang fp contract(off)
+    const double pt[3] = {x, y, z};
+    const int nn[3] = {nx, ny, nz};
+    double mn2[3], mx2[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double lo = (double)nn[d] * h + lower[d];
+        const double hi = lo + h;
+        const double v = pt[d];
+        const double dl = v - lo, du = hi - v;
+        const double mx = v > hi ? dl : (v < lo ? du : (dl > du ? dl : du));
+        const double mn = v > hi ? v - hi : (v < lo ? lo - v : 0.0);
+        mn2[d] = mn * mn;
+        mx2[d] = mx * mx;
+    }
+    minR2 = (mn2[0] + mn2[1]) + mn2[2];
+    maxR2 = (mx2[0] + mx2[1]) + mx2[2];
+}

@@ -213,6 +213,15 @@ struct QcmmGeom {
     int nsites;
 };
 
+// Grid search of one QC/MM cross list (MakeCrossPairListFromCoordinates3, pCore-1.9.0/extensions/csource/
+// PairListGenerator.c:611-940): the rows' grid (lower1, bins1) and the columns' grid conforming to it (lower2,
+// its lower cell indices lowN on the rows' grid, bins2), made on the device by k_qcmm_grids.
+struct QcmmGrid {
+    double lower1[3], lower2[3];
+    int bins1[3], bins2[3], lowN[3];
+    int bad;                            // the rows' grid has an axis with no cells
+};
+
 struct CrossListParams {
     QcmmGeom G;
     int rows_are_qc;                    // rows: QC atoms (link coordinates), else atoms row_atom[r]
@@ -222,6 +231,11 @@ struct CrossListParams {
     const unsigned char *flags;         // atom flags (bit 0 free, bit 1 pure QC) of atom columns, or null
     int or_test;                        // apply the free-atom OR test
     int col_atoms;                      // columns: atoms, else MM sites
+    // Grid method (grid non-null): stencil flags over (2u+1)^3 offsets, cell size, point/cell search.
+    const QcmmGrid *grid;
+    const unsigned char *stencil;
+    int u, pointcell;
+    double h;
     double cutoff2;
     int *count;
     const int *ptr;
@@ -238,6 +252,11 @@ struct QcmmRows {                       // main and 1-4 lists of the QC atoms (C
 };
 
 void launch_cross_list(const CrossListParams &P, int pass, hipStream_t s);
+// The grids of the two QC/MM cross lists into out[0] (electrostatic: the QC coordinates' grid and the grid of the MM
+// sites, or of the MM selection's atoms with MM coupling, conforming to it) and out[1] (LJ: the grid of all atoms,
+// lower/bins of the main list, for rows and columns).
+void launch_qcmm_grids(const QcmmGeom &G, const unsigned char *flags, const double *lower, const int *bins, double h,
+                       QcmmGrid *out, hipStream_t s);
 // chunks workgroups per QC atom; part: 2 per (QC atom, chunk), main and 1-4 sums.
 void launch_qcmm_potentials(const QcmmGeom &G, const SplineDev &S, const QcmmRows &L, const double *site_q,
                             double escale, double escale14, double r2off, int chunks, double *part, hipStream_t s);

@@ -18,6 +18,8 @@
 // pCore-1.9.0/extensions/csource/PairListGenerator.c:28-85,414-493), the link
 // atom rules of QCAtomContainer.c:415-432,530-559,583-630 and the cubic
 // spline of pCore-1.9.0/extensions/csource/CubicSpline.c:86-126.
+#include <cfloat>
+
 #include "nbabfs_kernels.h"
 
 // QCAtomContainer_GetAtomCoordinates3: a link atom sits on the bond from its QC
@@ -150,7 +152,31 @@ __device__ __forceinline__ void block_sum_n(double *v, int k, double *sh)
 // (bit 0) of the row and column atoms when or_test is set, not in the row's
 // sorted exclusion range, and (dx*dx + dy*dy + dz*dz) <= cutoff^2 with
 // d = x_row - x_column in FP64.
+//
+// Grid method (P.grid; PairListGenerator.c:703-794,861-940 with RegularGrid.c:
+// 341-405): the column must lie on the columns' grid, in cell kc, with the
+// offset kc + lowN - I from the row's search cell I in the stencil.  Cell/cell:
+// the row on the rows' grid, and the stencil flag 2 (the cell pair fully in
+// range) or the distance test.  Point/cell: the smallest squared distance from
+// the row point to cell kc below cutoff^2, then the largest at most cutoff^2
+// or the distance test; a row off its grid searches around the cell of the
+// latest earlier row on it, (-u, -u, -u) before any (RegularGrid_CellIDToIndices
+// writes nothing for a point with no cell; the rows are the AND selection:
+// the QC atoms in order, or the pure QC atoms by atom index).
 // ------------------------------------------------------------------------------------------------
+// Cell of point x on a grid (RegularGrid_FindCellIDOfPoint, RegularGrid.c:240-251): floor((x - lower)/h) per axis;
+// returns whether the point is on the grid.
+__device__ __forceinline__ bool grid_cell(const double x[3], const double *lower, double h, const int *bins, int k[3])
+{
+#pragma clang fp contract(off)
+    bool on = true;
+    for (int d = 0; d < 3; d++) {
+        k[d] = (int)floor((x[d] - lower[d]) / h);
+        on = on && k[d] >= 0 && k[d] < bins[d];
+    }
+    return on;
+}
+
 template <int PASS>
 __global__ __launch_bounds__(256) void k_cross_list(CrossListParams P)
 {
@@ -159,7 +185,7 @@ __global__ __launch_bounds__(256) void k_cross_list(CrossListParams P)
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const QcmmGeom &G = P.G;
     double xr[3];
-    int rowflag = 1, ex0 = 0, ex1 = 0;
+    int rowflag = 1, ex0 = 0, ex1 = 0, ra = -1;
     if (P.rows_are_qc) {
         qc_coord(G, r, xr);
         if (P.ex_ptr) {
@@ -168,6 +194,7 @@ __global__ __launch_bounds__(256) void k_cross_list(CrossListParams P)
         }
     } else {
         const int a = P.row_atom[r];
+        ra = a;
         xr[0] = G.xyz[3 * (size_t)a];
         xr[1] = G.xyz[3 * (size_t)a + 1];
         xr[2] = G.xyz[3 * (size_t)a + 2];
@@ -177,6 +204,40 @@ __global__ __launch_bounds__(256) void k_cross_list(CrossListParams P)
             ex1 = P.ex_ptr[a + 1];
         }
     }
+    // Grid method: the row's search cell and whether the row is on its grid (every thread, the same answer).
+    const QcmmGrid *Gd = P.grid;
+    int rc[3] = {0, 0, 0};
+    bool ron = true;
+    if (Gd) {
+        ron = grid_cell(xr, Gd->lower1, P.h, Gd->bins1, rc);
+        if (!ron && P.pointcell) {
+            rc[0] = rc[1] = rc[2] = -P.u;
+            if (P.rows_are_qc) {
+                for (int q = r - 1; q >= 0; q--) {
+                    double xq[3];
+                    int kq[3];
+                    qc_coord(G, q, xq);
+                    if (grid_cell(xq, Gd->lower1, P.h, Gd->bins1, kq)) {
+                        rc[0] = kq[0]; rc[1] = kq[1]; rc[2] = kq[2];
+                        break;
+                    }
+                }
+            } else {
+                int best = -1;
+                for (int q = 0; q < P.nrows; q++) {
+                    const int a = P.row_atom[q];
+                    if (a >= ra || a <= best) continue;
+                    const double xq[3] = {G.xyz[3 * (size_t)a], G.xyz[3 * (size_t)a + 1], G.xyz[3 * (size_t)a + 2]};
+                    int kq[3];
+                    if (grid_cell(xq, Gd->lower1, P.h, Gd->bins1, kq)) {
+                        best = a;
+                        rc[0] = kq[0]; rc[1] = kq[1]; rc[2] = kq[2];
+                    }
+                }
+            }
+        }
+    }
+    const int wst = 2 * P.u + 1;
     const int ncol = P.col_atoms ? G.n : G.nsites;
     __shared__ int sW[4];
     __shared__ int sBase;
@@ -194,7 +255,25 @@ __global__ __launch_bounds__(256) void k_cross_list(CrossListParams P)
             double xc[3];
             site_coord(G, j, P.col_atoms != 0, xc);
             const double dx = xr[0] - xc[0], dy = xr[1] - xc[1], dz = xr[2] - xc[2];
-            ok = (dx * dx + dy * dy + dz * dz) <= P.cutoff2;
+            const bool near = (dx * dx + dy * dy + dz * dz) <= P.cutoff2;
+            if (Gd) {
+                int kc[3];
+                const bool con = grid_cell(xc, Gd->lower2, P.h, Gd->bins2, kc);
+                const int ox = kc[0] + Gd->lowN[0] - rc[0] + P.u, oy = kc[1] + Gd->lowN[1] - rc[1] + P.u,
+                          oz = kc[2] + Gd->lowN[2] - rc[2] + P.u;
+                const bool inb = ox >= 0 && ox < wst && oy >= 0 && oy < wst && oz >= 0 && oz < wst;
+                const int flag = inb ? (int)P.stencil[(ox * wst + oy) * wst + oz] : 0;
+                if (P.pointcell) {
+                    double minR2 = 0.0, maxR2 = 0.0;
+                    if (con && flag != 0)
+                        point_cell_r2(Gd->lower2, P.h, xr[0], xr[1], xr[2], kc[0], kc[1], kc[2], minR2, maxR2);
+                    ok = con && flag != 0 && minR2 < P.cutoff2 && (maxR2 <= P.cutoff2 || near);
+                } else {
+                    ok = ron && con && flag != 0 && (flag == 2 || near);
+                }
+            } else {
+                ok = near;
+            }
         }
         if (ok && ex1 > ex0) ok = !in_sorted(P.ex_idx, ex0, ex1, j);
         const unsigned long long m = __ballot(ok);
@@ -388,6 +467,106 @@ __global__ __launch_bounds__(256) void k_qcmm_coords(QcmmGeom G, double *qcxyz, 
 // ------------------------------------------------------------------------------------------------
 // Launchers.
 // ------------------------------------------------------------------------------------------------
+// The two grids of the QC/MM cross lists (one workgroup).  Electrostatic list: the QC coordinates' grid
+// (Coordinates3_MakeGrid, pCore-1.9.0/extensions/csource/Coordinates3.c:1111-1158, no selection) for the rows and,
+// for the columns, the grid conforming to it over the box of the MM selection's atoms (MM coupling: the atoms are
+// the columns, the box leaves out the pure QC atoms) or of the MM sites (Coordinates3_MakeConformingGrid,
+// Coordinates3.c:998-1072).  LJ list: the main list's grid for both.
+__global__ __launch_bounds__(256) void k_qcmm_grids(QcmmGeom G, const unsigned char *__restrict__ flags, double lx,
+                                                    double ly, double lz, int bx, int by, int bz, double h,
+                                                    QcmmGrid *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    __shared__ double smn[3][256], smx[3][256];
+    const int t = threadIdx.x;
+    auto reduce = [&](double *mn, double *mx) {
+        for (int d = 0; d < 3; d++) {
+            smn[d][t] = mn[d];
+            smx[d][t] = mx[d];
+        }
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o)
+                for (int d = 0; d < 3; d++) {
+                    smn[d][t] = fmin(smn[d][t], smn[d][t + o]);
+                    smx[d][t] = fmax(smx[d][t], smx[d][t + o]);
+                }
+            __syncthreads();
+        }
+        for (int d = 0; d < 3; d++) {
+            mn[d] = smn[d][0];
+            mx[d] = smx[d][0];
+        }
+        __syncthreads();
+    };
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int q = t; q < G.nqc; q += 256) {
+        double x[3];
+        qc_coord(G, q, x);
+        for (int d = 0; d < 3; d++) {
+            mn[d] = fmin(mn[d], x[d]);
+            mx[d] = fmax(mx[d], x[d]);
+        }
+    }
+    reduce(mn, mx);
+    QcmmGrid E;
+    E.bad = 0;
+    for (int d = 0; d < 3; d++) {
+        const double extent = mx[d] - mn[d];
+        const double upper = mn[d] + extent;
+        const double mid = 0.5 * (mn[d] + upper);
+        const double cells = ceil(extent / h);
+        E.bins1[d] = (int)cells;
+        E.lower1[d] = mid - 0.5 * (cells * h);
+        if (E.bins1[d] <= 0) E.bad = 1;
+    }
+    const bool atoms = G.site_a == nullptr;
+    const int ncol = atoms ? G.n : G.nsites;
+    for (int d = 0; d < 3; d++) {
+        mn[d] = DBL_MAX;
+        mx[d] = -DBL_MAX;
+    }
+    for (int j = t; j < ncol; j += 256) {
+        if (atoms && flags && (flags[j] & 2)) continue;
+        double x[3];
+        site_coord(G, j, atoms, x);
+        for (int d = 0; d < 3; d++) {
+            mn[d] = fmin(mn[d], x[d]);
+            mx[d] = fmax(mx[d], x[d]);
+        }
+    }
+    reduce(mn, mx);
+    for (int d = 0; d < 3; d++) {
+        const double lo = mn[d];
+        const double up = (mx[d] - lo) + lo;
+        const int lowN = (int)floor((lo - E.lower1[d]) / h);
+        const int upN = (int)floor((up - E.lower1[d]) / h);
+        E.lowN[d] = lowN;
+        E.bins2[d] = upN - lowN + 1;
+        E.lower2[d] = E.lower1[d] + (double)lowN * h;
+    }
+    if (t == 0) {
+        out[0] = E;
+        QcmmGrid L;
+        const double lw[3] = {lx, ly, lz};
+        const int bn[3] = {bx, by, bz};
+        for (int d = 0; d < 3; d++) {
+            L.lower1[d] = L.lower2[d] = lw[d];
+            L.bins1[d] = L.bins2[d] = bn[d];
+            L.lowN[d] = 0;
+        }
+        L.bad = 0;
+        out[1] = L;
+    }
+}
+
+void launch_qcmm_grids(const QcmmGeom &G, const unsigned char *flags, const double *lower, const int *bins, double h,
+                       QcmmGrid *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qcmm_grids, dim3(1), dim3(256), 0, s, G, flags, lower[0], lower[1], lower[2], bins[0],
+                       bins[1], bins[2], h, out);
+}
+
 void launch_cross_list(const CrossListParams &P, int pass, hipStream_t s)
 {
     if (P.nrows <= 0) return;

@@ -51,13 +51,35 @@ def key_checksum(i, j, n):
     return np.array([k.size, int(np.sum(k * w, dtype=np.uint64))], np.uint64)
 
 
-def run_case(name, nwater, box, coupling, link_ratio, fixed_fn=None, seed=20261207):
+def edge_coordinates(s, first_mm, after_qc, cell=6.75):
+    """Shift the system to the positive octant and move, per axis, one MM atom to the next whole number of cells
+    above the largest coordinate: that atom lies on the upper face of the reference grid and so off it.  On x it is
+    the MM atom after_qc that follows a pure QC atom (the chain, atoms below first_mm, is first moved next to that
+    face so that the atom has pairs with it); on y and z the MM atom (index first_mm up) with the largest
+    coordinate."""
+    x = s.coordinates3 - s.coordinates3.min(axis=0)
+    x[:first_mm, 0] += x[:, 0].max() - 4.0 - x[:first_mm, 0].max()
+    x = x - x.min(axis=0)
+    for d in range(3):
+        top = float(np.ceil(x[:, d].max() / cell)) * cell
+        k = after_qc if d == 0 else first_mm + int(np.argmax(x[first_mm:, d]))
+        x[k, d] = top
+    s.coordinates3 = np.ascontiguousarray(x)
+
+
+def run_case(name, nwater, box, coupling, link_ratio, fixed_fn=None, seed=20261207, grid_by_cell=True, edge=False):
     s, qd = S.qcmm_chain_in_water(nwater, box, seed)
+    if edge:
+        last_qc = int(np.max(qd["index"]))
+        edge_coordinates(s, last_qc + 1, last_qc + 1)
+        # The pure QC atom before the off-grid MM atom sits in a far cell, so that its cell differs from the one
+        # the reference searches around (that of the latest earlier atom of the MM selection).
+        s.coordinates3[last_qc] = [1.0, 1.0, 1.0]
     lf = np.where(qd["boundary"], 1.09 / 1.53 if link_ratio else 1.09, 0.0)
     qc = Q.QCAtoms(s.natoms, qd["index"], qd["boundary"], qd["partner"], lf, qd["mmp_off"], qd["mmp"], link_ratio)
     fixed = fixed_fn(s.natoms) if fixed_fn else None
     sx, sy = Q.electrostatic_spline_points("Delta/Gaussian", 0.0, 2.0 * Q.BOHRS_TO_ANGSTROMS, 0.0, 8.0, 12.0)
-    ref = Q.ReferenceQCMM(s, qc, coupling, sx, sy, fixed=fixed)
+    ref = Q.ReferenceQCMM(s, qc, coupling, sx, sy, fixed=fixed, use_grid_by_cell=grid_by_cell)
     qcq = np.round(np.linspace(-0.6, 0.7, qc.nqc), 3)
     rng = np.random.default_rng(seed + 1)
     x0 = s.coordinates3
@@ -96,7 +118,7 @@ def run_case(name, nwater, box, coupling, link_ratio, fixed_fn=None, seed=202612
                coupling=int(coupling), fixed=(fixed if fixed is not None else np.zeros(0, bool)).astype(np.uint8),
                spline_x=sx, spline_y=sy, spline_h=h, qc_charges=qcq, energies=np.array(energies),
                rebuilt=np.array(rebuilt), potentials=np.array(pots), grads_e=np.array(ges), grads_q=np.array(gqs),
-               use_grid=use_grid, **first)
+               use_grid=use_grid, grid_by_cell=grid_by_cell, **first)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print(name, s.natoms, "grid" if use_grid else "direct", "rebuilt", rebuilt,
           {w: first["keys_" + w].size for w in ("el", "lj", "el14", "lj14")}, "mmmm", int(first["mmmm_checksum"][0]))
@@ -116,7 +138,20 @@ def main():
     run_case("qcmm_RC_ratio", 500, 25.0, 1, True)
     run_case("qcmm_RD_fixed", 500, 25.0, 2, False, fixed_waters)
     run_case("qcmm_MM_direct", 100, 15.0, 0, True)
+    pointcell_cases()
+
+
+def pointcell_cases():
+    """Boxes whose extent is a whole number of cells, with QC atoms: under the point/cell generator, MM atoms on the
+    grid's upper faces search around the cell of the latest earlier atom of the MM selection (the pure QC atoms
+    decode no cell); under both generators such atoms are in no cell of the cross lists' grids."""
+    run_case("qcmm_MM_pointcell_edge", 500, 25.0, 0, False, grid_by_cell=False, edge=True)
+    run_case("qcmm_MM_cellcell_edge", 500, 25.0, 0, False, grid_by_cell=True, edge=True)
+    run_case("qcmm_RC_pointcell_edge", 500, 25.0, 1, True, grid_by_cell=False, edge=True)
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["pointcell"]:
+        pointcell_cases()
+    else:
+        main()

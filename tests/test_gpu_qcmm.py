@@ -2,7 +2,8 @@
 
 Against the reference pDynamo C library's outputs (tests/golden/qcmm_*.npz, written by
 make_golden_qcmm.py: MM, RC and RD coupling, link distances and ratios, fixed atoms, grid and
-direct search), for both precisions:
+direct search, and boxes whose extent is a whole number of cells, so that atoms fall off the
+grids, under both grid searches), for both precisions:
 
 * exactly: the QC/MM electrostatic, LJ and 1-4 lists, the MM/MM list of the MM selection (size
   and key checksum) and its 1-4 list, the QC atom coordinates, the MM sites and their charges, the
@@ -33,6 +34,7 @@ def _state(fx, precision):
     from pdynamo_amd import _native
     o = _native.default_options()
     o.precision = _native.MIXED if precision == "mixed" else _native.PLAIN_FP64
+    o.useGridByCell = 1 if bool(fx.get("grid_by_cell", True)) else 0
     fixed = fx["fixed"] if fx["fixed"].size else None
     st = _native.NativeState(o, fx["charges"], fx["ljtypes"], int(fx["ntypes"]), fx["tableA"], fx["tableB"],
                              exclusions=fx["exclusions"], interactions14=fx["interactions14"], fixed=fixed)

@@ -61,7 +61,8 @@ def test_qcmm_oracle_matches_reference(Q, path):
     qc = fixture_qc(Q, fx, s.natoms)
     sp = Q.CubicSpline(fx["spline_x"], fx["spline_y"])
     assert np.abs(sp.h - fx["spline_h"]).max() <= 1e-13 * np.abs(fx["spline_h"]).max()
-    orc = Q.QCMMOracle(s, qc, int(fx["coupling"]), sp, fixed=fixture_fixed(fx))
+    orc = Q.QCMMOracle(s, qc, int(fx["coupling"]), sp, fixed=fixture_fixed(fx),
+                       use_grid_by_cell=bool(fx.get("grid_by_cell", True)))
     for k, xyz in enumerate(fx["coords"]):
         e, pot, ge, gq, up = orc.call(xyz, fx["qc_charges"])
         assert up == bool(fx["rebuilt"][k])
