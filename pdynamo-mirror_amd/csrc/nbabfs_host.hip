@@ -461,6 +461,7 @@ struct nbabfs_state {
     DBuf<float4> iblo, ibhi, icen;
     DBuf<float> ibox;
     DBuf<QcmmGrid> qgrid;        // QC/MM cross lists: grids of the electrostatic and LJ lists
+    DBuf<int4> qcolcell;         // QC/MM electrostatic list: cells of its columns
     DBuf<int4> icell;            // image lists: primary-grid search cells of the i atoms
     DBuf<ImageGrid> igrid;       // image lists: conforming grid of each image
     DBuf<int> istep_off, ibound, isoff, icounts, incand, ieoff, inits, iioff;
@@ -3186,6 +3187,8 @@ static int build_qcmm_lists(nbabfs_state *S, const double *d_xyz)
     if (S->useGrid) {
         CHK(S->qgrid.ensure(2));
         launch_qcmm_grids(G, S->freef.p, S->grid.lower, S->grid.bins, S->grid.h, S->qgrid.p, st);
+        CHK(S->qcolcell.ensure(std::max(std::max(S->n, S->nsites), 1)));
+        launch_qcmm_colcells(G, S->qgrid.p, S->grid.h, S->qcolcell.p, st);
     }
     for (int list = 0; list < 2; list++) {
         CrossListParams P{};
@@ -3194,6 +3197,8 @@ static int build_qcmm_lists(nbabfs_state *S, const double *d_xyz)
         P.flags = S->freef.p;
         if (S->useGrid && (list == 1 || S->nqc > 1)) {
             P.grid = S->qgrid.p + list;
+            // The LJ list's columns are the atoms on the main list's grid: their cells from the binning pass.
+            P.colcell = list == 1 ? S->cellv.p : S->qcolcell.p;
             P.stencil = S->stencil.p;
             P.u = S->u;
             P.h = S->grid.h;

@@ -233,6 +233,7 @@ struct CrossListParams {
     int col_atoms;                      // columns: atoms, else MM sites
     // Grid method (grid non-null): stencil flags over (2u+1)^3 offsets, cell size, point/cell search.
     const QcmmGrid *grid;
+    const int4 *colcell;                // each column's cell on the columns' grid, w = 1 on it (launch_qcmm_colcells)
     const unsigned char *stencil;
     int u, pointcell;
     double h;
@@ -257,6 +258,8 @@ void launch_cross_list(const CrossListParams &P, int pass, hipStream_t s);
 // lower/bins of the main list, for rows and columns).
 void launch_qcmm_grids(const QcmmGeom &G, const unsigned char *flags, const double *lower, const int *bins, double h,
                        QcmmGrid *out, hipStream_t s);
+// The cells of the electrostatic list's columns (MM sites, or atoms with MM coupling) on their grid (grid->lower2).
+void launch_qcmm_colcells(const QcmmGeom &G, const QcmmGrid *grid, double h, int4 *out, hipStream_t s);
 // chunks workgroups per QC atom; part: 2 per (QC atom, chunk), main and 1-4 sums.
 void launch_qcmm_potentials(const QcmmGeom &G, const SplineDev &S, const QcmmRows &L, const double *site_q,
                             double escale, double escale14, double r2off, int chunks, double *part, hipStream_t s);

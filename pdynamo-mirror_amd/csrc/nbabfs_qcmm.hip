@@ -257,8 +257,9 @@ __global__ __launch_bounds__(256) void k_cross_list(CrossListParams P)
             const double dx = xr[0] - xc[0], dy = xr[1] - xc[1], dz = xr[2] - xc[2];
             const bool near = (dx * dx + dy * dy + dz * dz) <= P.cutoff2;
             if (Gd) {
-                int kc[3];
-                const bool con = grid_cell(xc, Gd->lower2, P.h, Gd->bins2, kc);
+                const int4 cc = P.colcell[j];
+                const int kc[3] = {cc.x, cc.y, cc.z};
+                const bool con = cc.w != 0;
                 const int ox = kc[0] + Gd->lowN[0] - rc[0] + P.u, oy = kc[1] + Gd->lowN[1] - rc[1] + P.u,
                           oz = kc[2] + Gd->lowN[2] - rc[2] + P.u;
                 const bool inb = ox >= 0 && ox < wst && oy >= 0 && oy < wst && oz >= 0 && oz < wst;
@@ -558,6 +559,26 @@ __global__ __launch_bounds__(256) void k_qcmm_grids(QcmmGeom G, const unsigned c
         L.bad = 0;
         out[1] = L;
     }
+}
+
+__global__ __launch_bounds__(256) void k_qcmm_colcells(QcmmGeom G, const QcmmGrid *__restrict__ grid, double h,
+                                                       int4 *__restrict__ out)
+{
+    const bool atoms = G.site_a == nullptr;
+    const int ncol = atoms ? G.n : G.nsites;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= ncol) return;
+    double x[3];
+    site_coord(G, j, atoms, x);
+    int k[3];
+    const bool on = grid_cell(x, grid->lower2, h, grid->bins2, k);
+    out[j] = make_int4(k[0], k[1], k[2], on ? 1 : 0);
+}
+
+void launch_qcmm_colcells(const QcmmGeom &G, const QcmmGrid *grid, double h, int4 *out, hipStream_t s)
+{
+    const int ncol = G.site_a == nullptr ? G.n : G.nsites;
+    if (ncol > 0) hipLaunchKernelGGL(k_qcmm_colcells, dim3((ncol + 255) / 256), dim3(256), 0, s, G, grid, h, out);
 }
 
 void launch_qcmm_grids(const QcmmGeom &G, const unsigned char *flags, const double *lower, const int *bins, double h,
