@@ -30,6 +30,22 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 vector peak
 PEAK_FP64_TFLOPS = 78.6     # MI355X published FP64 vector peak
+# Special-function issue (v_rsq_f32 and the like), taken as a quarter of the FP32 lane rate (the rate of the
+# CDNA transcendental unit): 1,024 SIMDs x 4 lanes x 2.4 GHz.
+PEAK_SPECIAL_TOPS = 9.83
+
+
+def in_cutoff_terms(n_plain, n_switch, kms, ms):
+    """The in-cutoff (d <= 12 A) pair rate of a step and the reference's two special-function operations per
+    in-cutoff pair (1/r^2 and the square root) over the pair launch, against the special-function issue rate."""
+    n_in = n_plain + n_switch
+    ops = 2.0 * n_in
+    achieved = ops / (kms * 1e-3) / 1e12 if kms else None
+    return {"in_cutoff_pairs": n_in, "in_cutoff_pairs_per_s": n_in / (ms * 1e-3),
+            "special_ops_per_launch": ops, "special_achieved_Tops": achieved, "special_peak_Tops": PEAK_SPECIAL_TOPS,
+            "special_frac": (achieved / PEAK_SPECIAL_TOPS) if achieved else None}
+
+
 DTYPE = {"mixed": "f64 pair core + f32 rsqrt seed/force vectors + int64 fixed-point sums",
          "tiled_fp64": "f64 pair core and force vectors + int64 fixed-point sums", "fp64": "f64"}
 
@@ -366,7 +382,8 @@ def run_batch(args, systems, wdesc, workload):
                               "frac": achieved / peak, "traffic": traffic_from_summary(tpath),
                               "kernel": "k_force_plain" if args.precision == "fp64" else "k_force_tiled",
                               "kernel_ms": kms, "launches": min(args.steps, 20), "flops_per_launch": flops,
-                              "pairs_d_le_8": n_plain, "pairs_8_12": n_switch}
+                              "pairs_d_le_8": n_plain, "pairs_8_12": n_switch,
+                              **in_cutoff_terms(n_plain, n_switch, kms, ms)}
     batch.close()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline_batch(systems, npairs, args.cpu_image_calls)
@@ -619,7 +636,8 @@ def main():
                               "traffic": traffic_from_summary(tpath),
                               "kernel": "k_force_plain" if args.precision == "fp64" else "k_force_tiled",
                               "kernel_ms": kms, "launches": int(stats["forceKernelLaunches"]),
-                              "flops_per_launch": flops, "pairs_d_le_8": n_plain, "pairs_8_12": n_switch}
+                              "flops_per_launch": flops, "pairs_d_le_8": n_plain, "pairs_8_12": n_switch,
+                              **in_cutoff_terms(n_plain, n_switch, kms, ms)}
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(mm, args.cpu_calls, host_threads(), extra)
     if args.copy_calibration_mib > 0:
