@@ -461,6 +461,8 @@ struct nbabfs_state {
     DBuf<float4> iblo, ibhi, icen;
     DBuf<float> ibox;
     DBuf<QcmmGrid> qgrid;        // QC/MM cross lists: grids of the electrostatic and LJ lists
+    DBuf<double> qgpart;         // their per-workgroup boxes
+    DBuf<int> qoff;              // first list position of each (row, column chunk) of a cross list
     DBuf<int4> qcolcell;         // QC/MM electrostatic list: cells of its columns
     DBuf<int4> icell;            // image lists: primary-grid search cells of the i atoms
     DBuf<ImageGrid> igrid;       // image lists: conforming grid of each image
@@ -3177,16 +3179,14 @@ static int build_qcmm_lists(nbabfs_state *S, const double *d_xyz)
     hipStream_t st = S->stream;
     const QcmmGeom G = qcmm_geom(S, d_xyz);
     const double c2 = S->opt.listCutoff * S->opt.listCutoff;
-    const int rows_max = std::max(S->nqc, S->nlj) + 1;
-    CHK(S->qcnt.ensure(rows_max));
-    CHK(S->scantmp.ensure(scan_tmp_size(rows_max)));
     CHK(S->h_int.ensure(2));
     // Grid method: the cross lists search the grids of NBModelABFSState_GridInitialize (pMolecule-1.9.0/
     // extensions/csource/NBModelABFSState.c:207-226): the electrostatic list the QC coordinates' grid when there is
     // more than one QC atom, the LJ list the grid of all atoms (the main list's).
     if (S->useGrid) {
         CHK(S->qgrid.ensure(2));
-        launch_qcmm_grids(G, S->freef.p, S->grid.lower, S->grid.bins, S->grid.h, S->qgrid.p, st);
+        CHK(S->qgpart.ensure(12 * NB_QGRID_PARTS));
+        launch_qcmm_grids(G, S->freef.p, S->grid.lower, S->grid.bins, S->grid.h, S->qgpart.p, S->qgrid.p, st);
         CHK(S->qcolcell.ensure(std::max(std::max(S->n, S->nsites), 1)));
         launch_qcmm_colcells(G, S->qgrid.p, S->grid.h, S->qcolcell.p, st);
     }
@@ -3226,10 +3226,16 @@ static int build_qcmm_lists(nbabfs_state *S, const double *d_xyz)
         CHK(ptr.ensure(P.nrows + 1));
         long long total = 0;
         if (P.nrows > 0) {
-            launch_fill_int(S->qcnt.p, P.nrows + 1, 0, st);
+            // Counts per (row, column chunk) plus a zero, their exclusive scan, the row pointers from it.
+            const int nchunk = cross_list_chunks(P), m = P.nrows * nchunk + 1;
+            CHK(S->qcnt.ensure(m));
+            CHK(S->qoff.ensure(m));
+            CHK(S->scantmp.ensure(scan_tmp_size(m)));
+            launch_fill_int(S->qcnt.p, m, 0, st);
             P.count = S->qcnt.p;
             launch_cross_list(P, 0, st);
-            launch_scan(S->qcnt.p, ptr.p, P.nrows + 1, S->scantmp.p, st);
+            launch_scan(S->qcnt.p, S->qoff.p, m, S->scantmp.p, st);
+            launch_row_pointers(S->qoff.p, P.nrows, nchunk, ptr.p, st);
             HIPCHK(hipMemcpyAsync(S->h_int.p, ptr.p + P.nrows, sizeof(int), hipMemcpyDeviceToHost, st));
             S->h_int.p[1] = 0;
             if (P.grid)
@@ -3246,7 +3252,7 @@ static int build_qcmm_lists(nbabfs_state *S, const double *d_xyz)
         }
         CHK(idx.ensure(std::max<long long>(total, 1)));
         if (total > 0) {
-            P.ptr = ptr.p;
+            P.off = S->qoff.p;
             P.idx = idx.p;
             launch_cross_list(P, 1, st);
         }

@@ -238,10 +238,18 @@ struct CrossListParams {
     int u, pointcell;
     double h;
     double cutoff2;
-    int *count;
-    const int *ptr;
+    int *count;                         // count pass: accepted columns per (row, column chunk), row-major
+    const int *off;                     // fill pass: first list position of each (row, column chunk)
     int *idx;
 };
+
+// Columns per workgroup of the cross-list passes: each row is searched by ceil(columns / NB_XL_CHUNK) workgroups.
+#define NB_XL_CHUNK 2048
+inline int cross_list_chunks(const CrossListParams &P)
+{
+    const int ncol = P.col_atoms ? P.G.n : P.G.nsites;
+    return ncol > 0 ? (ncol + NB_XL_CHUNK - 1) / NB_XL_CHUNK : 1;
+}
 
 struct SplineDev {                      // CubicSpline: abscissae x (= r^2), ordinates y, second derivatives h
     const double *x, *y, *h;
@@ -253,11 +261,15 @@ struct QcmmRows {                       // main and 1-4 lists of the QC atoms (C
 };
 
 void launch_cross_list(const CrossListParams &P, int pass, hipStream_t s);
+// Row pointers ptr[r] = off[r * nchunk] (r = 0 .. nrows) of the exclusive scan of the per-(row, chunk) counts.
+void launch_row_pointers(const int *off, int nrows, int nchunk, int *ptr, hipStream_t s);
 // The grids of the two QC/MM cross lists into out[0] (electrostatic: the QC coordinates' grid and the grid of the MM
 // sites, or of the MM selection's atoms with MM coupling, conforming to it) and out[1] (LJ: the grid of all atoms,
 // lower/bins of the main list, for rows and columns).
+// part: scratch of 12 * NB_QGRID_PARTS doubles (the per-workgroup boxes).
+#define NB_QGRID_PARTS 64
 void launch_qcmm_grids(const QcmmGeom &G, const unsigned char *flags, const double *lower, const int *bins, double h,
-                       QcmmGrid *out, hipStream_t s);
+                       double *part, QcmmGrid *out, hipStream_t s);
 // The cells of the electrostatic list's columns (MM sites, or atoms with MM coupling) on their grid (grid->lower2).
 void launch_qcmm_colcells(const QcmmGeom &G, const QcmmGrid *grid, double h, int4 *out, hipStream_t s);
 // chunks workgroups per QC atom; part: 2 per (QC atom, chunk), main and 1-4 sums.

@@ -142,10 +142,11 @@ __device__ __forceinline__ void block_sum_n(double *v, int k, double *sh)
 }
 
 // ------------------------------------------------------------------------------------------------
-// Cross lists.  One workgroup per row; the columns (atoms or MM sites) are
-// tested in tiles of 256 in ascending order and the accepted ones are
-// compacted in that order, so a row's list is sorted.  Pass 0 counts, pass 1
-// fills at ptr[row].
+// Cross lists.  Workgroup (row, chunk) tests the columns (atoms or MM sites)
+// of its chunk of NB_XL_CHUNK in tiles of 256 in ascending order and compacts
+// the accepted ones in that order; the chunks of a row are placed in chunk
+// order, so a row's list is sorted.  Pass 0 counts per (row, chunk), pass 1
+// fills at off[row * chunks + chunk] (the exclusive scan of the counts).
 //
 // Predicate (PairListGenerator.c:28-85): the column in the AND selection
 // (atom flag bit 1 clear: not a pure QC atom), the OR test on the free flags
@@ -181,7 +182,7 @@ template <int PASS>
 __global__ __launch_bounds__(256) void k_cross_list(CrossListParams P)
 {
 #pragma clang fp contract(off)
-    const int r = blockIdx.x;
+    const int r = blockIdx.x, chunk = blockIdx.y, nchunk = gridDim.y;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const QcmmGeom &G = P.G;
     double xr[3];
@@ -239,13 +240,14 @@ __global__ __launch_bounds__(256) void k_cross_list(CrossListParams P)
     }
     const int wst = 2 * P.u + 1;
     const int ncol = P.col_atoms ? G.n : G.nsites;
+    const int cbeg = chunk * NB_XL_CHUNK, cend = min(ncol, cbeg + NB_XL_CHUNK);
     __shared__ int sW[4];
     __shared__ int sBase;
-    if (tid == 0) sBase = PASS ? P.ptr[r] : 0;
+    if (tid == 0) sBase = PASS ? P.off[(size_t)r * nchunk + chunk] : 0;
     __syncthreads();
-    for (int base = 0; base < ncol; base += 256) {
+    for (int base = cbeg; base < cend; base += 256) {
         const int j = base + tid;
-        bool ok = j < ncol;
+        bool ok = j < cend;
         if (ok && P.col_atoms && P.flags) {
             const int fj = P.flags[j];
             ok = !(fj & 2);
@@ -288,7 +290,14 @@ __global__ __launch_bounds__(256) void k_cross_list(CrossListParams P)
         if (tid == 0) sBase += total;
         __syncthreads();
     }
-    if (!PASS && tid == 0) P.count[r] = sBase;
+    if (!PASS && tid == 0) P.count[(size_t)r * nchunk + chunk] = sBase;
+}
+
+__global__ __launch_bounds__(256) void k_row_pointers(const int *__restrict__ off, int nrows, int nchunk,
+                                                      int *__restrict__ ptr)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r <= nrows) ptr[r] = off[(size_t)r * nchunk];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -468,40 +477,39 @@ __global__ __launch_bounds__(256) void k_qcmm_coords(QcmmGeom G, double *qcxyz, 
 // ------------------------------------------------------------------------------------------------
 // Launchers.
 // ------------------------------------------------------------------------------------------------
-// The two grids of the QC/MM cross lists (one workgroup).  Electrostatic list: the QC coordinates' grid
-// (Coordinates3_MakeGrid, pCore-1.9.0/extensions/csource/Coordinates3.c:1111-1158, no selection) for the rows and,
-// for the columns, the grid conforming to it over the box of the MM selection's atoms (MM coupling: the atoms are
-// the columns, the box leaves out the pure QC atoms) or of the MM sites (Coordinates3_MakeConformingGrid,
-// Coordinates3.c:998-1072).  LJ list: the main list's grid for both.
-__global__ __launch_bounds__(256) void k_qcmm_grids(QcmmGeom G, const unsigned char *__restrict__ flags, double lx,
-                                                    double ly, double lz, int bx, int by, int bz, double h,
-                                                    QcmmGrid *__restrict__ out)
+// Min and max per axis over the workgroup's threads (every thread gets the result).
+__device__ __forceinline__ void block_box(double *mn, double *mx, double (*smn)[256], double (*smx)[256])
 {
-#pragma clang fp contract(off)
-    __shared__ double smn[3][256], smx[3][256];
     const int t = threadIdx.x;
-    auto reduce = [&](double *mn, double *mx) {
-        for (int d = 0; d < 3; d++) {
-            smn[d][t] = mn[d];
-            smx[d][t] = mx[d];
-        }
+    for (int d = 0; d < 3; d++) {
+        smn[d][t] = mn[d];
+        smx[d][t] = mx[d];
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o)
+            for (int d = 0; d < 3; d++) {
+                smn[d][t] = fmin(smn[d][t], smn[d][t + o]);
+                smx[d][t] = fmax(smx[d][t], smx[d][t + o]);
+            }
         __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if (t < o)
-                for (int d = 0; d < 3; d++) {
-                    smn[d][t] = fmin(smn[d][t], smn[d][t + o]);
-                    smx[d][t] = fmax(smx[d][t], smx[d][t + o]);
-                }
-            __syncthreads();
-        }
-        for (int d = 0; d < 3; d++) {
-            mn[d] = smn[d][0];
-            mx[d] = smx[d][0];
-        }
-        __syncthreads();
-    };
+    }
+    for (int d = 0; d < 3; d++) {
+        mn[d] = smn[d][0];
+        mx[d] = smx[d][0];
+    }
+    __syncthreads();
+}
+
+// Per-workgroup boxes (min x, y, z, max x, y, z) of the QC coordinates (part[12 b + 0..5]) and of the electrostatic
+// list's columns (part[12 b + 6..11]: the MM sites, or the MM selection's atoms with MM coupling).
+__global__ __launch_bounds__(256) void k_qcmm_box_parts(QcmmGeom G, const unsigned char *__restrict__ flags,
+                                                        double *__restrict__ part)
+{
+    __shared__ double smn[3][256], smx[3][256];
+    const int t = threadIdx.x, b = blockIdx.x, stride = gridDim.x * 256;
     double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (int q = t; q < G.nqc; q += 256) {
+    for (int q = b * 256 + t; q < G.nqc; q += stride) {
         double x[3];
         qc_coord(G, q, x);
         for (int d = 0; d < 3; d++) {
@@ -509,7 +517,55 @@ __global__ __launch_bounds__(256) void k_qcmm_grids(QcmmGeom G, const unsigned c
             mx[d] = fmax(mx[d], x[d]);
         }
     }
-    reduce(mn, mx);
+    block_box(mn, mx, smn, smx);
+    if (t == 0)
+        for (int d = 0; d < 3; d++) {
+            part[12 * b + d] = mn[d];
+            part[12 * b + 3 + d] = mx[d];
+        }
+    const bool atoms = G.site_a == nullptr;
+    const int ncol = atoms ? G.n : G.nsites;
+    for (int d = 0; d < 3; d++) {
+        mn[d] = DBL_MAX;
+        mx[d] = -DBL_MAX;
+    }
+    for (int j = b * 256 + t; j < ncol; j += stride) {
+        if (atoms && flags && (flags[j] & 2)) continue;
+        double x[3];
+        site_coord(G, j, atoms, x);
+        for (int d = 0; d < 3; d++) {
+            mn[d] = fmin(mn[d], x[d]);
+            mx[d] = fmax(mx[d], x[d]);
+        }
+    }
+    block_box(mn, mx, smn, smx);
+    if (t == 0)
+        for (int d = 0; d < 3; d++) {
+            part[12 * b + 6 + d] = mn[d];
+            part[12 * b + 9 + d] = mx[d];
+        }
+}
+
+// The two grids of the QC/MM cross lists from the boxes (one workgroup; min and max are exact, so the boxes do not
+// depend on the partition).  Electrostatic list: the QC coordinates' grid (Coordinates3_MakeGrid,
+// pCore-1.9.0/extensions/csource/Coordinates3.c:1111-1158, no selection) for the rows and, for the columns, the grid
+// conforming to it over the box of the MM selection's atoms (MM coupling: the atoms are the columns, the box leaves
+// out the pure QC atoms) or of the MM sites (Coordinates3_MakeConformingGrid, Coordinates3.c:998-1072).  LJ list:
+// the main list's grid for both.
+__global__ __launch_bounds__(256) void k_qcmm_grids(const double *__restrict__ part, int nparts, double lx,
+                                                    double ly, double lz, int bx, int by, int bz, double h,
+                                                    QcmmGrid *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    __shared__ double smn[3][256], smx[3][256];
+    const int t = threadIdx.x;
+    double mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int b = t; b < nparts; b += 256)
+        for (int d = 0; d < 3; d++) {
+            mn[d] = fmin(mn[d], part[12 * b + d]);
+            mx[d] = fmax(mx[d], part[12 * b + 3 + d]);
+        }
+    block_box(mn, mx, smn, smx);
     QcmmGrid E;
     E.bad = 0;
     for (int d = 0; d < 3; d++) {
@@ -521,22 +577,16 @@ __global__ __launch_bounds__(256) void k_qcmm_grids(QcmmGeom G, const unsigned c
         E.lower1[d] = mid - 0.5 * (cells * h);
         if (E.bins1[d] <= 0) E.bad = 1;
     }
-    const bool atoms = G.site_a == nullptr;
-    const int ncol = atoms ? G.n : G.nsites;
     for (int d = 0; d < 3; d++) {
         mn[d] = DBL_MAX;
         mx[d] = -DBL_MAX;
     }
-    for (int j = t; j < ncol; j += 256) {
-        if (atoms && flags && (flags[j] & 2)) continue;
-        double x[3];
-        site_coord(G, j, atoms, x);
+    for (int b = t; b < nparts; b += 256)
         for (int d = 0; d < 3; d++) {
-            mn[d] = fmin(mn[d], x[d]);
-            mx[d] = fmax(mx[d], x[d]);
+            mn[d] = fmin(mn[d], part[12 * b + 6 + d]);
+            mx[d] = fmax(mx[d], part[12 * b + 9 + d]);
         }
-    }
-    reduce(mn, mx);
+    block_box(mn, mx, smn, smx);
     for (int d = 0; d < 3; d++) {
         const double lo = mn[d];
         const double up = (mx[d] - lo) + lo;
@@ -582,17 +632,26 @@ void launch_qcmm_colcells(const QcmmGeom &G, const QcmmGrid *grid, double h, int
 }
 
 void launch_qcmm_grids(const QcmmGeom &G, const unsigned char *flags, const double *lower, const int *bins, double h,
-                       QcmmGrid *out, hipStream_t s)
+                       double *part, QcmmGrid *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_qcmm_grids, dim3(1), dim3(256), 0, s, G, flags, lower[0], lower[1], lower[2], bins[0],
+    const int ncol = G.site_a == nullptr ? G.n : G.nsites;
+    const int nparts = std::max(1, std::min(NB_QGRID_PARTS, (std::max(ncol, G.nqc) + 1023) / 1024));
+    hipLaunchKernelGGL(k_qcmm_box_parts, dim3(nparts), dim3(256), 0, s, G, flags, part);
+    hipLaunchKernelGGL(k_qcmm_grids, dim3(1), dim3(256), 0, s, part, nparts, lower[0], lower[1], lower[2], bins[0],
                        bins[1], bins[2], h, out);
 }
 
 void launch_cross_list(const CrossListParams &P, int pass, hipStream_t s)
 {
     if (P.nrows <= 0) return;
-    if (pass == 0) hipLaunchKernelGGL(k_cross_list<0>, dim3(P.nrows), dim3(256), 0, s, P);
-    else hipLaunchKernelGGL(k_cross_list<1>, dim3(P.nrows), dim3(256), 0, s, P);
+    const dim3 grid(P.nrows, cross_list_chunks(P));
+    if (pass == 0) hipLaunchKernelGGL(k_cross_list<0>, grid, dim3(256), 0, s, P);
+    else hipLaunchKernelGGL(k_cross_list<1>, grid, dim3(256), 0, s, P);
+}
+
+void launch_row_pointers(const int *off, int nrows, int nchunk, int *ptr, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_row_pointers, dim3((nrows + 1 + 255) / 256), dim3(256), 0, s, off, nrows, nchunk, ptr);
 }
 
 void launch_qcmm_potentials(const QcmmGeom &G, const SplineDev &S, const QcmmRows &L, const double *site_q,
