@@ -299,6 +299,13 @@ def run_batch(args, systems, wdesc, workload):
         states = states_for(precision)
         batch = _native.NativeBatch(states, use_graph=True)
         settle(lambda: batch.energy(xp, gp), args.settle)
+        tuned = None
+        if not args.no_autotune and not args.entries_per_item and precision != "fp64":
+            # After the clock settling: the tuner's timings compare cuts, not clock states.
+            epi, akms = batch.autotune(xp)
+            tuned = {"entriesPerItem": epi, "kernel_ms": akms}
+            # The tuner's host-side re-cuts leave the GPU idle now and then: settle the clocks again.
+            settle(lambda: batch.energy(xp, gp), args.settle)
         for _ in range(warmup):
             batch.energy(xp, gp)
         _native.synchronize()
@@ -316,9 +323,9 @@ def run_batch(args, systems, wdesc, workload):
                 batch.energy(xp, gp)
             kms, nk = batch.kernel_timing()
             batch.set_timing(False)
-        return ms, kms, e, states, batch
+        return ms, kms, e, states, batch, tuned
 
-    ms, kms, e, states, batch = timed_batch(args.precision, args.steps, args.warmup)
+    ms, kms, e, states, batch, tuned = timed_batch(args.precision, args.steps, args.warmup)
     stats = [st.stats() for st in states]
     npairs = int(sum(s_["npairs"] for s_ in stats))
 
@@ -347,7 +354,7 @@ def run_batch(args, systems, wdesc, workload):
         for other in ("tiled_fp64", "fp64", "mixed"):
             if other == args.precision:
                 continue
-            ms_o, kms_o, _, st_o, b_o = timed_batch(other, max(5, args.steps // 5), 2)
+            ms_o, kms_o, _, st_o, b_o, _ = timed_batch(other, max(5, args.steps // 5), 2)
             b_o.close()
             for st in st_o:
                 st.close()
@@ -365,7 +372,7 @@ def run_batch(args, systems, wdesc, workload):
         "config": {"clock_settle_s": args.settle, "workload": wdesc, "systems": len(systems), "atoms": int(offs[-1]), "list_pairs": npairs,
                    "min_atoms": int(min(sizes)), "max_atoms": int(max(sizes)), "precision": args.precision,
                    "work_items": int(sum(s_["workItems"] for s_ in stats)),
-                   "cluster_entries": int(sum(s_["entries"] for s_ in stats)),
+                   "cluster_entries": int(sum(s_["entries"] for s_ in stats)), "autotuned": tuned,
                    "energies_kJ_mol_system0": [float(v) for v in e[0]], **cmp},
     }
     if not args.no_roofline and kms:
@@ -549,6 +556,11 @@ def main():
     _native.synchronize()
     tuned = None
     if not args.no_autotune and args.entries_per_item == 0 and args.waves_per_cu < 0:
+        # After the clock settling: the tuner's timings compare cuts, not clock states.
+        def tune_settle():
+            st.update(xs[0].ptr)
+            st.energy_async(xs[0].ptr, g.ptr)
+        settle(tune_settle, args.settle)
         epi, wpc, kms = st.autotune(xs[0].ptr)
         tuned = {"entriesPerItem": epi, "wavesPerCU": wpc, "kernel_ms": kms}
     counter = [0]

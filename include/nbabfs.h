@@ -329,6 +329,15 @@ int nbabfs_batch_energy(nbabfs_batch *batch, const double *const *d_xyz, double 
  * Outputs may be NULL. */
 int nbabfs_batch_info(nbabfs_batch *batch, long long *graph_captures, long long *launches, long long *layouts,
                       double *force_kernel_ms, long long *timed_launches);
+/* Batch work-item autotuner (no counterpart in the reference, which has no
+ * work items): after one batched call for d_xyz (lists built or updated as by
+ * nbabfs_batch_energy), times the batched pair launch for each candidate
+ * entriesPerItem (every state re-cut alike, one work item per wavefront) and
+ * keeps the fastest in every state's options.  No gradient array is written;
+ * the pair lists are unchanged, only their cut into work items.  Outputs may
+ * be NULL. */
+int nbabfs_batch_autotune(nbabfs_batch *batch, const double *const *d_xyz, int *best_entries_per_item,
+                          double *best_kernel_ms);
 /* Timing of the batched pair launch (no counterpart in the reference): while
  * enable != 0 the batch's calls enqueue their passes directly instead of
  * replaying the graph, with the runtime's dispatch start/stop events around

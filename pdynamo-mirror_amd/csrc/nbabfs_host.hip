@@ -8,6 +8,7 @@
 #include "nbabfs_kernels.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1842,18 +1843,33 @@ static int autotune(nbabfs_state *S, const double *d_xyz, int *best_epi, int *be
     double best = 1e300;
     int bi = -1;
     int rc = NBABFS_STATUS_CONTINUE;
-    for (size_t c = 0; c < sizeof(cand) / sizeof(cand[0]) && rc == NBABFS_STATUS_CONTINUE; c++) {
+    const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
+    std::vector<double> tbest(nc, 1e300);
+    // Two sweeps, the second in reverse order, best of the two per candidate: a drift of the device clocks over
+    // the sweep (they ramp up under load) then favours no end of the candidate list.
+    for (int sweep = 0; sweep < 2 * nc && rc == NBABFS_STATUS_CONTINUE; sweep++) {
+        const int c = sweep < nc ? sweep : 2 * nc - 1 - sweep;
         S->opt.entriesPerItem = cand[c][0];
         S->opt.wavesPerCU = cand[c][1];
         rc = recut_items(S, cand[c][0]);
         // The launches below read the descriptor: refresh it after the new cut.
         if (rc == NBABFS_STATUS_CONTINUE) rc = ensure_desc(S, d_xyz, S->last_grad);
-        // Each launch is timed alone; persistent queues are reset before it.
-        for (int r = -1; r < reps && rc == NBABFS_STATUS_CONTINUE; r++) {
+        // About 20 ms of untimed launches first (the re-cut left the GPU idle for a while and its clocks ramp up
+        // again under load), then each launch timed alone; persistent queues are reset before every launch.
+        const auto w0 = std::chrono::steady_clock::now();
+        for (int w = 0; w < 4000 && rc == NBABFS_STATUS_CONTINUE; w++) {
             if (S->opt.wavesPerCU > 0) launch_fill_int(S->heads.p, 8 * 32, 0, st);
-            if (r >= 0 && hipEventRecord(ev[2 * r], st) != hipSuccess) rc = fail(NBABFS_STATUS_LOGIC_ERROR, "event record failed");
+            rc = launch_force(S, ForceGuard{nullptr, 0});
+            if ((w & 15) == 15) {
+                if (hipStreamSynchronize(st) != hipSuccess) rc = fail(NBABFS_STATUS_LOGIC_ERROR, "synchronisation failed");
+                if (std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(20)) break;
+            }
+        }
+        for (int r = 0; r < reps && rc == NBABFS_STATUS_CONTINUE; r++) {
+            if (S->opt.wavesPerCU > 0) launch_fill_int(S->heads.p, 8 * 32, 0, st);
+            if (hipEventRecord(ev[2 * r], st) != hipSuccess) rc = fail(NBABFS_STATUS_LOGIC_ERROR, "event record failed");
             if (rc == NBABFS_STATUS_CONTINUE) rc = launch_force(S, ForceGuard{nullptr, 0});
-            if (rc == NBABFS_STATUS_CONTINUE && r >= 0 && hipEventRecord(ev[2 * r + 1], st) != hipSuccess)
+            if (rc == NBABFS_STATUS_CONTINUE && hipEventRecord(ev[2 * r + 1], st) != hipSuccess)
                 rc = fail(NBABFS_STATUS_LOGIC_ERROR, "event record failed");
         }
         if (rc != NBABFS_STATUS_CONTINUE) break;
@@ -1867,11 +1883,13 @@ static int autotune(nbabfs_state *S, const double *d_xyz, int *best_epi, int *be
             if (hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]) != hipSuccess) rc = fail(NBABFS_STATUS_LOGIC_ERROR, "event timing failed");
             tot += ms;
         }
-        if (rc == NBABFS_STATUS_CONTINUE && tot / reps < best) {
-            best = tot / reps;
-            bi = (int)c;
-        }
+        if (rc == NBABFS_STATUS_CONTINUE) tbest[c] = std::min(tbest[c], tot / reps);
     }
+    for (int c = 0; c < nc; c++)
+        if (tbest[c] < best) {
+            best = tbest[c];
+            bi = c;
+        }
     for (int k = 0; k < 2 * reps; k++) (void)hipEventDestroy(ev[k]);
     S->timing = was_timing;
     S->opt = keep;
@@ -2873,6 +2891,80 @@ extern "C" int nbabfs_batch_energy(nbabfs_batch *B, const double *const *d_xyz, 
         return call_status((int)B->h_energies.p[NB_NENERGIES * (size_t)bad + NB_ESTATUS], what);
     }
     return NBABFS_STATUS_CONTINUE;
+}
+
+// Batch work-item autotuner: times the batched pair launch (dispatch events, as with timing on) on the systems'
+// lists for each entriesPerItem candidate, every state re-cut alike (one work item per wave), and keeps the fastest
+// in every state.  The calls pass no gradient arrays (nothing of the caller's is written) and leave the call counts
+// (the states' numberOfCalls, the batch's launches) as they were.
+static int batch_autotune(nbabfs_batch *B, const double *const *d_xyz, int *best_epi, double *best_ms)
+{
+    static const int cand[] = {8, 12, 16, 20, 24, 32, 40, 48};
+    const int reps = 8;
+    const int ns = (int)B->st.size();
+    std::vector<double> e(4 * (size_t)ns);
+    std::vector<long long> calls(ns);
+    for (int k = 0; k < ns; k++) calls[k] = B->st[k]->calls;
+    const bool was_timing = B->timing;
+    const long long launches = B->n_launches;
+    const int keep = B->st[0]->opt.entriesPerItem;
+    int rc = nbabfs_batch_energy(B, d_xyz, nullptr, e.data(), nullptr);
+    const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
+    std::vector<double> tbest(nc, 1e300);
+    // Two sweeps, the second in reverse order, best of the two per candidate: a drift of the device clocks over
+    // the sweep (they ramp up under load) then favours no end of the candidate list.
+    for (int sweep = 0; sweep < 2 * nc && rc == NBABFS_STATUS_CONTINUE && B->mixed; sweep++) {
+        const int c = sweep < nc ? sweep : 2 * nc - 1 - sweep;
+        for (auto *s : B->st) {
+            s->opt.entriesPerItem = cand[c];
+            s->opt.wavesPerCU = 0;
+            rc = recut_items(s, cand[c]);
+            if (rc != NBABFS_STATUS_CONTINUE) break;
+        }
+        B->timing = true;
+        // Untimed calls for about 20 ms first (the first re-lays the batch out): the re-cuts left the GPU idle
+        // for a while and its clocks ramp up again under load.
+        const auto w0 = std::chrono::steady_clock::now();
+        for (int w = 0; w < 1000 && rc == NBABFS_STATUS_CONTINUE; w++) {
+            rc = nbabfs_batch_energy(B, d_xyz, nullptr, e.data(), nullptr);
+            if (std::chrono::steady_clock::now() - w0 > std::chrono::milliseconds(20)) break;
+        }
+        B->force_ms = 0.0;
+        B->force_launches = 0;
+        for (int r = 0; r < reps && rc == NBABFS_STATUS_CONTINUE; r++)
+            rc = nbabfs_batch_energy(B, d_xyz, nullptr, e.data(), nullptr);
+        if (rc == NBABFS_STATUS_CONTINUE && B->force_launches > 0)
+            tbest[c] = std::min(tbest[c], B->force_ms / B->force_launches);
+    }
+    double best = 1e300;
+    int bi = -1;
+    for (int c = 0; c < nc; c++)
+        if (tbest[c] < best) {
+            best = tbest[c];
+            bi = c;
+        }
+    const int pick = bi >= 0 ? cand[bi] : keep;
+    for (auto *s : B->st) {
+        s->opt.entriesPerItem = pick;
+        const int r2 = recut_items(s, pick);
+        if (rc == NBABFS_STATUS_CONTINUE) rc = r2;
+    }
+    B->timing = was_timing;
+    B->force_ms = 0.0;
+    B->force_launches = 0;
+    for (int k = 0; k < ns; k++) B->st[k]->calls = calls[k];
+    B->n_launches = launches;
+    if (rc != NBABFS_STATUS_CONTINUE) return rc;
+    if (best_epi) *best_epi = pick;
+    if (best_ms) *best_ms = bi >= 0 ? best : 0.0;
+    return NBABFS_STATUS_CONTINUE;
+}
+
+extern "C" int nbabfs_batch_autotune(nbabfs_batch *B, const double *const *d_xyz, int *best_entries_per_item,
+                                     double *best_kernel_ms)
+{
+    if (!B || !d_xyz) return fail(NBABFS_STATUS_INVALID_ARGUMENT, "null argument");
+    return batch_autotune(B, d_xyz, best_entries_per_item, best_kernel_ms);
 }
 
 extern "C" int nbabfs_batch_info(nbabfs_batch *B, long long *graph_captures, long long *launches, long long *layouts,

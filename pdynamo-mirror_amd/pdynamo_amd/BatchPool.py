@@ -86,6 +86,19 @@ class SGOFProcessPoolGPU:
                 g[k][...] = G[k]
         return [float(v) for v in e.sum(axis=1)]
 
+    def Autotune(self, x):
+        """Choose the work-item cut (entriesPerItem) of the batched pair launch by timing it on the images x[k]
+        (the library's batch autotuner; an extension: the reference has no work items).  No gradient is written."""
+        m = len(x)
+        if m == 0:
+            return None
+        X = np.concatenate([np.asarray(xi, np.float64).reshape(self.natoms, 3) for xi in x])
+        self._resize(m)
+        self.x.upload(X)
+        stride = 24 * self.natoms
+        epi, ms = self.batch.autotune([self.x.ptr + k * stride for k in range(m)])
+        return {"entriesPerItem": epi, "kernelMs": ms}
+
     def Energies(self):
         """Per-image energy terms of the last call as (MM/MM Elect., LJ, 1-4 Elect., 1-4 LJ) rows."""
         return self.lastEnergies

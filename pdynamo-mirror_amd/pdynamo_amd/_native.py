@@ -28,7 +28,7 @@ EXPORTED = ["nbabfs_default_options", "nbabfs_init", "nbabfs_last_error", "nbabf
             "nbabfs_images", "nbabfs_export_image_pairs", "nbabfs_set_qc_atoms", "nbabfs_set_qcmm_spline",
             "nbabfs_qcmm_terms", "nbabfs_qcmm_potentials", "nbabfs_qcmm_gradients_host", "nbabfs_export_qcmm_pairs",
             "nbabfs_qcmm_sites", "nbabfs_set_gradient_mode", "nbabfs_qcmm_potentials_async",
-            "nbabfs_qcmm_gradients_async", "nbabfs_batch_info", "nbabfs_batch_set_timing"]
+            "nbabfs_qcmm_gradients_async", "nbabfs_batch_info", "nbabfs_batch_set_timing", "nbabfs_batch_autotune"]
 
 
 class CLibraryError(Exception):
@@ -92,6 +92,7 @@ def load():
     L.nbabfs_batch_destroy.argtypes = [vp]
     L.nbabfs_batch_destroy.restype = None
     L.nbabfs_batch_energy.argtypes = [vp, vp, vp, vp, vp]
+    L.nbabfs_batch_autotune.argtypes = [vp, vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.nbabfs_device_alloc.argtypes = [C.c_size_t, C.POINTER(vp)]
     L.nbabfs_device_free.argtypes = [vp]
     L.nbabfs_memcpy_htod.argtypes = [vp, vp, vp, C.c_size_t]
@@ -471,6 +472,16 @@ class NativeBatch:
     def set_timing(self, enable):
         """Time the batched pair launch with dispatch events (calls run without the graph meanwhile)."""
         check(load().nbabfs_batch_set_timing(self.h, 1 if enable else 0))
+
+    def autotune(self, d_xyz_ptrs):
+        """Time the batched pair launch per entriesPerItem on the systems' lists for d_xyz_ptrs and keep the
+        fastest in every state.  Returns (entriesPerItem, pair launch ms)."""
+        if not hasattr(self, "_arrays"):
+            self._arrays = {}
+        xa = self._pointer_array(d_xyz_ptrs, 0)
+        epi, ms = C.c_int(), C.c_double()
+        check(load().nbabfs_batch_autotune(self.h, xa, C.byref(epi), C.byref(ms)))
+        return epi.value, ms.value
 
     def kernel_timing(self):
         """(mean ms of the batched pair launch, timed launches) since timing was enabled."""

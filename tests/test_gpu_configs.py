@@ -202,3 +202,34 @@ def test_uneven_batch_partial_rebuild(oracle_mod):
             assert info == {"graph_captures": 2, "launches": 3, "layouts": 2}, info
     finally:
         B.close()
+
+
+@pytest.mark.timeout(600)
+def test_batch_autotune(oracle_mod):
+    """The batch work-item autotuner (nbabfs_batch_autotune) re-cuts every state of the batch alike and keeps their
+    lists: the pair sets are unchanged, no gradient array is written by it, the states' call counts and the batch's
+    launch count are as before, and the next batched call matches every system's oracle (both tiled paths)."""
+    from pdynamo_amd import synthetic
+    systems = [synthetic.random_small_system(n, b, sd) for n, b, sd in [(600, 35.0, 7), (2500, 55.0, 8), (1200, 40.0, 9)]]
+    oracles = [oracle_mod.OracleABFS(m, nthreads=NTHREADS) for m in systems]
+    B = _Batches(systems)
+    try:
+        xyzs = [m.coordinates3 for m in systems]
+        assert _batch_step(B, oracles, xyzs, "before autotune").all()
+        for precision in ("mixed", "tiled_fp64"):
+            batch, sts, G = B.batches[precision], B.states[precision], B.G[precision]
+            pairs = [s.export_pairs(0) for s in sts]
+            calls = [s.stats()["numberOfCalls"] for s in sts]
+            launches = batch.info()["launches"]
+            G.upload(np.full((int(B.offs[-1]), 3), 7.0))
+            epi, ms = batch.autotune([B.X[precision].ptr + 24 * int(a) for a in B.offs[:-1]])
+            print("batch autotune", precision, epi, ms, flush=True)
+            assert epi in (8, 12, 16, 20, 24, 32, 40, 48) and ms > 0.0, (epi, ms)
+            assert np.all(G.numpy() == 7.0)
+            assert [s.stats()["numberOfCalls"] for s in sts] == calls
+            assert batch.info()["launches"] == launches
+            for s, p in zip(sts, pairs):
+                assert np.array_equal(s.export_pairs(0), p)
+        assert not _batch_step(B, oracles, xyzs, "after autotune").any()
+    finally:
+        B.close()
