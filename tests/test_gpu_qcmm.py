@@ -149,3 +149,54 @@ def test_qcmm_device_variants(path):
     assert np.array_equal(dp.numpy(), ph)
     assert np.array_equal(dg.numpy(), gh)
     st.close()
+
+
+@pytest.mark.parametrize("coupling", ["MM", "RC"])
+def test_qcmm_many_columns(oracle_mod, coupling):
+    """A QC/MM system with more columns than one workgroup of the cross-list passes takes (the chain and 1,500
+    waters, 4,517 atoms: three chunks of 2,048 columns per row): the four QC/MM lists bit for bit, the six energy
+    terms, the energy gradient, the potentials and the electrostatic gradient against the QC/MM oracle."""
+    import qcmm_oracle as Q
+    from pdynamo_amd import _native, synthetic
+    from pdynamo_amd.NBModelABFS import NBModelABFS
+    mm, qd = synthetic.qcmm_chain_in_water(1500, 35.8, qc_waters=8)
+    assert mm.natoms > 2 * 2048
+    lf = np.where(qd["boundary"], 1.09, 0.0)
+    model = NBModelABFS(qcmmCoupling=coupling)
+    model.CheckPairwiseInteractions()
+    cp = {"MM": _native.QCMM_MM, "RC": _native.QCMM_RC}[coupling]
+    st = _native.NativeState(model.NativeOptions(), mm.charges, mm.ljtypes, mm.ntypes, mm.tableA, mm.tableB,
+                             exclusions=mm.exclusions, interactions14=mm.interactions14)
+    st.set_qc_atoms(qd["index"], qd["boundary"], qd["partner"], lf, False, qd["mmp_off"], qd["mmp"], cp)
+    qi = model.qcmmPairwiseInteraction
+    st.set_qcmm_spline(qi.splineX, qi.splineY)
+    qc = Q.QCAtoms(mm.natoms, qd["index"], qd["boundary"], qd["partner"], lf, qd["mmp_off"], qd["mmp"], False)
+    orc = Q.QCMMOracle(mm, qc, cp, Q.CubicSpline(qi.splineX, qi.splineY))
+    xyz = mm.coordinates3
+    nqc = qd["index"].size
+    qcq = np.linspace(-0.8, 0.8, nqc)
+    rterms, rpot, rge, rgq, _ = orc.call(xyz, qcq)
+    print("qcmm many columns", coupling, mm.natoms, nqc, flush=True)
+    g = np.zeros_like(xyz)
+    e, present, rebuilt = st.energy_host(xyz, g)
+    qe, qp = st.qcmm_terms()
+    L = orc.lists
+    key = 1 << 20
+    assert np.array_equal(_keys20(st.export_qcmm_pairs("el")), Q.QCMMLists.keys(L.el, range(nqc), key))
+    assert np.array_equal(_keys20(st.export_qcmm_pairs("el14")), Q.QCMMLists.keys(L.el14, range(nqc), key))
+    assert np.array_equal(_keys20(st.export_qcmm_pairs("lj")), Q.QCMMLists.keys(L.lj, L.lj_rows, key))
+    assert np.array_equal(_keys20(st.export_qcmm_pairs("lj14")), Q.QCMMLists.keys(L.lj14, L.lj_rows, key))
+    got = dict(zip(QCMM_LABELS, list(e) + list(qe)))
+    have = dict(zip(QCMM_LABELS, [present[0], present[0], present[1], present[1], qp[0], qp[1]]))
+    etol, gtol = TOL["mixed"]
+    scale = max(abs(sum(rterms.values())), 1.0)
+    for label in QCMM_LABELS:
+        assert bool(have[label]) == (label in rterms), label
+        if label in rterms:
+            assert abs(got[label] - rterms[label]) <= etol * scale, (label, got[label], rterms[label])
+    assert rel_rms(g, rge) <= gtol
+    assert np.abs(st.qcmm_potentials() - rpot).max() <= 1e-12
+    gq = np.zeros_like(xyz)
+    st.qcmm_gradients(qcq, gq)
+    assert rel_rms(gq, rgq) <= 1e-10
+    st.close()
