@@ -502,6 +502,33 @@ def test_sgof_pool_matches_system_energy():
     pool.Finalize()
 
 
+def test_sgof_pool_autotune():
+    """SGOFProcessPoolGPU.Autotune (the batch work-item autotuner) writes no gradient, keeps the images' lists, and
+    the pool's next call still gives each image the energy and gradients of a System.Energy call, up to the
+    rounding of the other cut (energies 1e-9, gradients 1e-6 relative RMS)."""
+    from pdynamo_amd import NBModelABFS, SGOFProcessPoolGPU, System, synthetic
+    mm = synthetic.random_small_system(1500, 45.0, 6)
+    s = System.FromSynthetic(mm)
+    s.DefineNBModel(NBModelABFS())
+    rng = np.random.default_rng(4)
+    images = [mm.coordinates3 + rng.normal(0.0, 0.1, mm.coordinates3.shape) for _ in range(4)]
+    pool = SGOFProcessPoolGPU.FromSystem(s)
+    pool.FunctionGradients(images, None)
+    choice = pool.Autotune(images)
+    assert choice["entriesPerItem"] in (8, 12, 16, 20, 24, 32, 40, 48) and choice["kernelMs"] > 0.0, choice
+    g = [np.zeros((mm.natoms, 3)) for _ in images]
+    f = pool.FunctionGradients(images, g)
+    assert not pool.rebuilt.any()
+    for k, x in enumerate(images):
+        t = System.FromSynthetic(mm.with_coordinates(x))
+        t.DefineNBModel(NBModelABFS())
+        e = t.Energy(log=None, doGradients=True)
+        assert abs(f[k] - e) <= 1e-9 * abs(e), (k, f[k], e)
+        ref = t.configuration.gradients3
+        assert np.linalg.norm(g[k] - ref) <= 1e-6 * np.linalg.norm(ref)
+    pool.Finalize()
+
+
 @pytest.mark.parametrize("seed", range(10))
 def test_randomized_configurations(oracle_mod, seed):
     """Seeded random systems, box shapes and options (cutoffs, dielectric, 1-4 scaling), both paths: the
